@@ -1,0 +1,179 @@
+"""Python host side of the MI355X query engine: ctypes bindings over libgdb.so.
+
+This mirrors the reference's per-pipeline entry points (runQ6Benchmark,
+runQ1Benchmark, ... in reference src/infra.h:874-922, dispatched by name from
+src/main.cpp:161-188): one function per pipeline over tables resident in HBM.
+All computation happens in libgdb.so's HIP kernels; if the library is missing
+or no GPU is visible, calls raise instead of computing anything on the CPU.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .abi import (GdbError, Q1Params, Q1Result, Q3Params, Q3Result, Q6Params, Q6Result,
+                  Q9Params, Q9Result, Q13Params, Q13Result, Timing)
+
+__all__ = ["Engine", "Table", "GdbError", "lib", "generate", "LIB_PATH"]
+
+LIB_PATH = abi.LIB_PATH
+
+
+def lib():
+    """The loaded libgdb.so (raises GdbError if it has not been built)."""
+    return abi.load()
+
+
+class Table:
+    """A columnar table resident in HBM (gdb_table_create / gdb_table_free)."""
+
+    def __init__(self, engine, columns, nrows=None):
+        self.engine = engine
+        self._keep = []
+        descs = (abi.ColDesc * len(columns))()
+        n = None
+        for i, (name, arr) in enumerate(columns.items()):
+            arr = np.ascontiguousarray(arr)
+            if arr.dtype == np.int32:
+                typ, width, rows = abi.GDB_I32, 4, len(arr)
+            elif arr.dtype == np.int64:
+                typ, width, rows = abi.GDB_I64, 8, len(arr)
+            elif arr.dtype == np.uint8 and arr.ndim == 1:
+                typ, width, rows = abi.GDB_U8, 1, len(arr)
+            elif arr.dtype == np.uint8 and arr.ndim == 2:
+                typ, width, rows = abi.GDB_BYTES, arr.shape[1], arr.shape[0]
+            else:
+                raise GdbError(abi.GDB_E_INVALID, f"column {name}: unsupported dtype {arr.dtype}")
+            if n is None:
+                n = rows
+            elif rows != n:
+                raise GdbError(abi.GDB_E_INVALID, f"column {name} has {rows} rows, expected {n}")
+            self._keep.append(arr)
+            descs[i].name = name.encode()
+            descs[i].type = typ
+            descs[i].width = width
+            descs[i].host_data = arr.ctypes.data_as(C.c_void_p)
+        if nrows is not None:
+            n = nrows
+        self.nrows = n or 0
+        self.names = list(columns)
+        self._t = C.c_void_p()
+        abi.check(lib().gdb_table_create(engine._ctx, descs, len(columns), self.nrows,
+                                         C.byref(self._t)))
+        self._keep = []
+
+    @property
+    def handle(self):
+        return self._t
+
+    def free(self):
+        if self._t:
+            lib().gdb_table_free(self._t)
+            self._t = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Engine:
+    """One GPU context (gdb_open / gdb_close)."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        self._ctx = C.c_void_p()
+        abi.check(lib().gdb_open(device, hbm_budget, C.byref(self._ctx)))
+
+    def close(self):
+        if self._ctx:
+            lib().gdb_close(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def device_info(self):
+        info = abi.DeviceInfo()
+        abi.check(lib().gdb_device_info_get(self._ctx, C.byref(info)))
+        return info
+
+    def hbm_used(self):
+        return lib().gdb_hbm_used(self._ctx)
+
+    def table(self, columns):
+        return Table(self, columns)
+
+    # ---- pipelines ---------------------------------------------------------
+    def q6(self, lineitem, date_lo=731, date_hi=1096, disc_lo=5, disc_hi=7, qty_lt=24,
+           timing=None):
+        p = Q6Params(date_lo, date_hi, disc_lo, disc_hi, qty_lt)
+        r = Q6Result()
+        abi.check(lib().gdb_q6(self._ctx, lineitem.handle, C.byref(p), C.byref(r),
+                               abi.tref(timing)))
+        return dict(revenue=r.revenue, rows=r.rows)
+
+    def q1(self, lineitem, cutoff=2435, timing=None):
+        p = Q1Params(cutoff)
+        r = Q1Result()
+        abi.check(lib().gdb_q1(self._ctx, lineitem.handle, C.byref(p), C.byref(r),
+                               abi.tref(timing)))
+        return [abi.q1_row_dict(r.rows[i]) for i in range(r.nrows)]
+
+    def q3(self, customer, orders, lineitem, segment=1, date=1169, top_k=10, timing=None):
+        p = Q3Params(segment, date, top_k)
+        r = Q3Result()
+        abi.check(lib().gdb_q3(self._ctx, customer.handle, orders.handle, lineitem.handle,
+                               C.byref(p), C.byref(r), abi.tref(timing)))
+        rows = [dict(orderkey=x.orderkey, revenue=x.revenue, orderdate=x.orderdate,
+                     shippriority=x.shippriority) for x in r.rows[:r.nrows]]
+        return dict(groups=r.groups, rows=rows)
+
+    def q9(self, part, supplier, partsupp, orders, lineitem, pattern="green", timing=None):
+        p = Q9Params(pattern.encode())
+        r = Q9Result()
+        abi.check(lib().gdb_q9(self._ctx, part.handle, supplier.handle, partsupp.handle,
+                               orders.handle, lineitem.handle, C.byref(p), C.byref(r),
+                               abi.tref(timing)))
+        return [dict(nationkey=x.nationkey, year=x.year, profit=x.profit)
+                for x in r.rows[:r.nrows]]
+
+    def q13(self, customer, orders, word1="special", word2="requests", timing=None):
+        p = Q13Params(word1.encode(), word2.encode())
+        r = Q13Result()
+        abi.check(lib().gdb_q13(self._ctx, customer.handle, orders.handle, C.byref(p),
+                                C.byref(r), abi.tref(timing)))
+        return [dict(c_count=x.c_count, custdist=x.custdist) for x in r.rows[:r.nrows]]
+
+    # ---- primitives --------------------------------------------------------
+    def select_lt(self, table, column, value, bitmap=True, rowids=True, timing=None):
+        n = table.nrows
+        bm = np.zeros((n + 63) // 64, dtype=np.uint64) if bitmap else None
+        ids = np.zeros(max(n, 1), dtype=np.uint32) if rowids else None
+        cnt = C.c_uint64()
+        abi.check(lib().gdb_select_lt(
+            self._ctx, table.handle, column.encode(), value,
+            bm.ctypes.data_as(C.POINTER(C.c_uint64)) if bm is not None else None,
+            ids.ctypes.data_as(C.POINTER(C.c_uint32)) if ids is not None else None,
+            C.byref(cnt), abi.tref(timing)))
+        return dict(count=cnt.value, bitmap=bm,
+                    rowids=ids[:cnt.value] if ids is not None else None)
+
+    def sum(self, table, column, timing=None):
+        s = C.c_int64()
+        abi.check(lib().gdb_sum(self._ctx, table.handle, column.encode(), C.byref(s),
+                                abi.tref(timing)))
+        return s.value
+
+    def join_count(self, build, build_col, probe, probe_col, timing=None):
+        m = C.c_uint64()
+        abi.check(lib().gdb_join_count(self._ctx, build.handle, build_col.encode(), probe.handle,
+                                       probe_col.encode(), C.byref(m), abi.tref(timing)))
+        return m.value
+
+
+from . import generate  # noqa: E402  (after Engine: generate imports abi only)

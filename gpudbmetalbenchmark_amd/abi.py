@@ -1,0 +1,187 @@
+"""ctypes declarations of include/gdb.h (structs, status codes, prototypes)."""
+import ctypes as C
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libgdb.so")
+
+GDB_OK, GDB_E_INVALID, GDB_E_NOMEM, GDB_E_HIP, GDB_E_ADMISSION = 0, -1, -2, -3, -4
+GDB_E_UNSUPPORTED, GDB_E_NODEVICE, GDB_E_TIMEOUT, GDB_E_CANCELLED, GDB_E_NOTFOUND = (
+    -5, -6, -7, -8, -9)
+GDB_I32, GDB_I64, GDB_U8, GDB_BYTES = 1, 2, 3, 4
+
+
+class GdbError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"gdb error {code}: {msg}")
+        self.code = code
+
+
+class ColDesc(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("type", C.c_int), ("width", C.c_int),
+                ("host_data", C.c_void_p)]
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("arch", C.c_char * 32), ("cu_count", C.c_int),
+                ("xcd_count", C.c_int), ("hbm_bytes", C.c_uint64), ("l2_bytes", C.c_uint64),
+                ("lds_bytes_per_cu", C.c_int), ("wave_size", C.c_int),
+                ("hbm_peak_gbps", C.c_double)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("gpu_ms", C.c_double), ("main_kernel_ms", C.c_double), ("post_ms", C.c_double),
+                ("bytes", C.c_uint64), ("main_kernel_bytes", C.c_uint64)]
+
+
+class Q6Params(C.Structure):
+    _fields_ = [("date_lo", C.c_int32), ("date_hi", C.c_int32), ("disc_lo", C.c_int32),
+                ("disc_hi", C.c_int32), ("qty_lt", C.c_int32)]
+
+
+class Q6Result(C.Structure):
+    _fields_ = [("revenue", C.c_int64), ("rows", C.c_uint64)]
+
+
+class Q1Params(C.Structure):
+    _fields_ = [("cutoff", C.c_int32)]
+
+
+class Q1Row(C.Structure):
+    _fields_ = [("returnflag", C.c_char), ("linestatus", C.c_char), ("sum_qty", C.c_int64),
+                ("sum_base_price", C.c_int64), ("sum_disc_price", C.c_int64),
+                ("sum_charge_lo", C.c_uint64), ("sum_charge_hi", C.c_int64),
+                ("sum_disc", C.c_int64), ("count", C.c_int64), ("avg_qty", C.c_double),
+                ("avg_price", C.c_double), ("avg_disc", C.c_double)]
+
+
+class Q1Result(C.Structure):
+    _fields_ = [("nrows", C.c_int32), ("rows", Q1Row * 6)]
+
+
+class Q3Params(C.Structure):
+    _fields_ = [("segment", C.c_int32), ("date", C.c_int32), ("top_k", C.c_int32)]
+
+
+class Q3Row(C.Structure):
+    _fields_ = [("orderkey", C.c_int32), ("orderdate", C.c_int32), ("shippriority", C.c_int32),
+                ("pad", C.c_int32), ("revenue", C.c_int64)]
+
+
+class Q3Result(C.Structure):
+    _fields_ = [("nrows", C.c_int32), ("groups", C.c_int64), ("rows", Q3Row * 100)]
+
+
+class Q9Params(C.Structure):
+    _fields_ = [("pattern", C.c_char * 32)]
+
+
+class Q9Row(C.Structure):
+    _fields_ = [("nationkey", C.c_int32), ("year", C.c_int32), ("profit", C.c_int64)]
+
+
+class Q9Result(C.Structure):
+    _fields_ = [("nrows", C.c_int32), ("rows", Q9Row * 400)]
+
+
+class Q13Params(C.Structure):
+    _fields_ = [("word1", C.c_char * 32), ("word2", C.c_char * 32)]
+
+
+class Q13Row(C.Structure):
+    _fields_ = [("c_count", C.c_int32), ("pad", C.c_int32), ("custdist", C.c_int64)]
+
+
+class Q13Result(C.Structure):
+    _fields_ = [("nrows", C.c_int32), ("rows", Q13Row * 512)]
+
+
+def q1_row_dict(r):
+    charge = (int(r.sum_charge_hi) << 64) | int(r.sum_charge_lo)
+    return dict(returnflag=r.returnflag.decode(), linestatus=r.linestatus.decode(),
+                sum_qty=r.sum_qty, sum_base_price=r.sum_base_price,
+                sum_disc_price=r.sum_disc_price, sum_charge=charge, sum_disc=r.sum_disc,
+                count=r.count, avg_qty=r.avg_qty, avg_price=r.avg_price, avg_disc=r.avg_disc)
+
+
+P = C.c_void_p
+I32P, I64P, U8P = C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_uint8)
+U32P, U64P = C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)
+
+# name -> (restype, argtypes); every symbol include/gdb.h declares.
+PROTOTYPES = {
+    "gdb_strerror": (C.c_char_p, [C.c_int]),
+    "gdb_last_error": (C.c_char_p, []),
+    "gdb_open": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "gdb_close": (None, [P]),
+    "gdb_device_info_get": (C.c_int, [P, C.POINTER(DeviceInfo)]),
+    "gdb_hbm_used": (C.c_uint64, [P]),
+    "gdb_debug_fail_allocs": (None, [P, C.c_int]),
+    "gdb_table_create": (C.c_int, [P, C.POINTER(ColDesc), C.c_int, C.c_uint64,
+                                   C.POINTER(C.c_void_p)]),
+    "gdb_table_free": (None, [P]),
+    "gdb_table_rows": (C.c_uint64, [P]),
+    "gdb_table_bytes": (C.c_uint64, [P]),
+    "gdb_q6": (C.c_int, [P, P, C.POINTER(Q6Params), C.POINTER(Q6Result), C.POINTER(Timing)]),
+    "gdb_q1": (C.c_int, [P, P, C.POINTER(Q1Params), C.POINTER(Q1Result), C.POINTER(Timing)]),
+    "gdb_q3": (C.c_int, [P, P, P, P, C.POINTER(Q3Params), C.POINTER(Q3Result),
+                         C.POINTER(Timing)]),
+    "gdb_q9": (C.c_int, [P, P, P, P, P, P, C.POINTER(Q9Params), C.POINTER(Q9Result),
+                         C.POINTER(Timing)]),
+    "gdb_q13": (C.c_int, [P, P, P, C.POINTER(Q13Params), C.POINTER(Q13Result),
+                          C.POINTER(Timing)]),
+    "gdb_select_lt": (C.c_int, [P, P, C.c_char_p, C.c_int32, U64P, U32P, U64P,
+                                C.POINTER(Timing)]),
+    "gdb_sum": (C.c_int, [P, P, C.c_char_p, I64P, C.POINTER(Timing)]),
+    "gdb_join_count": (C.c_int, [P, P, C.c_char_p, P, C.c_char_p, U64P, C.POINTER(Timing)]),
+    "gdb_gen_lineitem_flat": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, I32P, I32P, I32P,
+                                        I32P, I64P, U8P, U8P, I32P, C.c_int]),
+    "gdb_gen_customer": (C.c_int, [C.c_uint64, C.c_uint64, I32P, U8P, C.c_int]),
+    "gdb_gen_orders": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, I32P, I32P,
+                                 I32P, I32P, U8P, C.c_int]),
+    "gdb_gen_lineitem_rows": (C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64]),
+    "gdb_gen_lineitem_joined": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
+                                          C.c_uint64, I32P, I32P, I32P, I32P, I32P, I64P, I32P,
+                                          C.c_int]),
+    "gdb_gen_part": (C.c_int, [C.c_uint64, C.c_uint64, I32P, U8P, C.c_int]),
+    "gdb_gen_partsupp": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, I32P, I32P, I64P,
+                                   C.c_int]),
+    "gdb_gen_supplier": (C.c_int, [C.c_uint64, C.c_uint64, I32P, I32P, C.c_int]),
+    "gdb_nation_name": (C.c_char_p, [C.c_int]),
+    "gdb_segment_name": (C.c_char_p, [C.c_int]),
+    "gdb_year_of_day": (C.c_int32, [C.c_int32]),
+}
+
+_lib = None
+
+
+def load(path=None):
+    """Loads libgdb.so once and declares every prototype. Raises GdbError when
+    the library has not been built: there is no fallback implementation."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise GdbError(GDB_E_UNSUPPORTED,
+                       f"{path} is missing: build it with python -m gpudbmetalbenchmark_amd.build")
+    lib = C.CDLL(path)
+    for name, (res, args) in PROTOTYPES.items():
+        fn = getattr(lib, name, None)
+        if fn is None:  # reported by the export test; calling it raises AttributeError
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != GDB_OK:
+        lib = load()
+        raise GdbError(rc, f"{lib.gdb_strerror(rc).decode()}: {lib.gdb_last_error().decode()}")
+    return rc
+
+
+def tref(t):
+    return C.byref(t) if t is not None else None

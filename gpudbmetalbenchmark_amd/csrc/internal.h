@@ -1,0 +1,103 @@
+// Engine-internal declarations shared by the runtime and the pipeline files.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gdb.h"
+
+namespace gdb {
+
+void set_error(const std::string& msg);
+
+#define GDB_HIP(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      ::gdb::set_error(std::string(#call) + ": " + hipGetErrorString(e_));              \
+      return GDB_E_HIP;                                                                 \
+    }                                                                                   \
+  } while (0)
+
+#define GDB_TRY(expr)             \
+  do {                            \
+    int rc_ = (expr);             \
+    if (rc_ != GDB_OK) return rc_; \
+  } while (0)
+
+int fail(int code, const std::string& msg);
+
+struct Column {
+  std::string name;
+  int type = 0;
+  int width = 0;       // bytes per value
+  void* dev = nullptr;
+  size_t bytes = 0;
+  int64_t minv = 0, maxv = -1;  // value range of integer columns (empty: max < min)
+};
+
+// Launch shape chosen per kernel; the autotuner may overwrite the defaults.
+struct Tuning {
+  int block = 256;          // threads per workgroup
+  int blocks_per_cu = 0;    // 0: occupancy-derived residency
+  int unroll = 2;           // vector groups per thread per loop trip
+};
+
+// Device scratch that grows on demand and is reused across calls (partials,
+// maps, hash tables). Owned by the context; released by gdb_close.
+struct Arena {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace gdb
+
+struct gdb_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t ev[6] = {};
+  gdb_device_info info{};
+  uint64_t budget = 0;
+  uint64_t used = 0;
+  int fail_allocs = 0;
+  std::mutex mu;
+  std::vector<gdb::Arena> arenas;  // indexed by purpose slot
+  void* host_pinned = nullptr;     // small pinned staging for results
+  size_t host_pinned_bytes = 0;
+  gdb::Tuning tune[16];
+};
+
+struct gdb_table {
+  gdb_ctx* ctx = nullptr;
+  uint64_t nrows = 0;
+  std::vector<gdb::Column> cols;
+};
+
+namespace gdb {
+
+// Device allocation accounted against the context's HBM budget.
+int dev_alloc(gdb_ctx* ctx, void** p, size_t bytes);
+void dev_free(gdb_ctx* ctx, void* p, size_t bytes);
+// Scratch slot `slot` of at least `bytes` (contents undefined).
+int arena(gdb_ctx* ctx, int slot, size_t bytes, void** out);
+// Pinned host staging of at least `bytes`.
+int pinned(gdb_ctx* ctx, size_t bytes, void** out);
+// Column lookup with type check; sets the error message on failure.
+int column(const gdb_table* t, const char* name, int type, const Column** out);
+// Number of workgroups that keep every CU busy for `kernel` at `block` threads
+// and `lds` bytes of dynamic LDS.
+int resident_blocks(gdb_ctx* ctx, const void* kernel, int block, size_t lds, int per_cu_cap);
+
+enum ArenaSlot {
+  A_PARTIALS = 0, A_BITMAP = 1, A_MAP = 2, A_HT = 3, A_AGG = 4, A_AUX = 5, A_OUT = 6,
+  A_AUX2 = 7, A_STREAM0 = 8, A_STREAM1 = 9, A_NSLOTS = 16
+};
+enum TuneSlot { T_Q6 = 0, T_Q1 = 1, T_SELECT = 2, T_SUM = 3, T_Q3 = 4, T_Q9 = 5, T_Q13 = 6, T_JOIN = 7 };
+
+// Event helpers around a pipeline (ev[0], ev[5]) and its dominant kernel (ev[1], ev[2]).
+int timing_finish(gdb_ctx* ctx, gdb_timing* t, uint64_t bytes, uint64_t main_bytes);
+
+}  // namespace gdb

@@ -1,0 +1,343 @@
+// Context, resident tables, scratch arenas, errors and timing: the HIP layer
+// that replaces the reference's Metal device/buffer/pipeline helpers
+// (reference src/infra.h:177-189, :499-531; src/main.cpp:76-156).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "internal.h"
+
+namespace gdb {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg) {
+  set_error(msg);
+  return code;
+}
+
+int dev_alloc(gdb_ctx* ctx, void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  if (ctx->fail_allocs > 0) {
+    --ctx->fail_allocs;
+    return fail(GDB_E_NOMEM, "injected device allocation failure");
+  }
+  if (ctx->budget && ctx->used + bytes > ctx->budget)
+    return fail(GDB_E_NOMEM, "HBM budget exceeded: " + std::to_string(ctx->used + bytes) +
+                                 " > " + std::to_string(ctx->budget) + " bytes");
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return fail(GDB_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  ctx->used += bytes;
+  return GDB_OK;
+}
+
+void dev_free(gdb_ctx* ctx, void* p, size_t bytes) {
+  if (!p) return;
+  (void)hipFree(p);
+  if (bytes == 0) bytes = 256;
+  ctx->used -= std::min<uint64_t>(ctx->used, bytes);
+}
+
+int arena(gdb_ctx* ctx, int slot, size_t bytes, void** out) {
+  if (slot < 0 || slot >= A_NSLOTS) return fail(GDB_E_INVALID, "bad arena slot");
+  Arena& a = ctx->arenas[slot];
+  if (a.bytes < bytes) {
+    if (a.ptr) {
+      GDB_HIP(hipStreamSynchronize(ctx->stream));
+      dev_free(ctx, a.ptr, a.bytes);
+      a.ptr = nullptr;
+      a.bytes = 0;
+    }
+    size_t want = std::max<size_t>(bytes, 4096);
+    want = (want + 4095) & ~size_t(4095);
+    GDB_TRY(dev_alloc(ctx, &a.ptr, want));
+    a.bytes = want;
+  }
+  *out = a.ptr;
+  return GDB_OK;
+}
+
+int pinned(gdb_ctx* ctx, size_t bytes, void** out) {
+  if (ctx->host_pinned_bytes < bytes) {
+    if (ctx->host_pinned) {
+      GDB_HIP(hipStreamSynchronize(ctx->stream));
+      (void)hipHostFree(ctx->host_pinned);
+    }
+    ctx->host_pinned = nullptr;
+    ctx->host_pinned_bytes = 0;
+    size_t want = std::max<size_t>(bytes, 65536);
+    if (hipHostMalloc(&ctx->host_pinned, want, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      ctx->host_pinned = nullptr;
+      return fail(GDB_E_NOMEM, "hipHostMalloc failed");
+    }
+    ctx->host_pinned_bytes = want;
+  }
+  *out = ctx->host_pinned;
+  return GDB_OK;
+}
+
+int column(const gdb_table* t, const char* name, int type, const Column** out) {
+  if (!t) return fail(GDB_E_INVALID, "null table");
+  if (!name) return fail(GDB_E_INVALID, "null column name");
+  for (const Column& c : t->cols) {
+    if (c.name == name) {
+      if (type && c.type != type)
+        return fail(GDB_E_INVALID, std::string("column ") + name + " has the wrong type");
+      *out = &c;
+      return GDB_OK;
+    }
+  }
+  return fail(GDB_E_INVALID, std::string("table has no column ") + name);
+}
+
+int resident_blocks(gdb_ctx* ctx, const void* kernel, int block, size_t lds, int per_cu_cap) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds) != hipSuccess ||
+      per_cu <= 0) {
+    (void)hipGetLastError();
+    per_cu = 2;
+  }
+  if (per_cu_cap > 0) per_cu = std::min(per_cu, per_cu_cap);
+  return std::max(1, per_cu * ctx->info.cu_count);
+}
+
+int timing_finish(gdb_ctx* ctx, gdb_timing* t, uint64_t bytes, uint64_t main_bytes) {
+  GDB_HIP(hipEventSynchronize(ctx->ev[5]));
+  if (t) {
+    float a = 0.f, b = 0.f;
+    GDB_HIP(hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[5]));
+    GDB_HIP(hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]));
+    t->gpu_ms = a;
+    t->main_kernel_ms = b;
+    t->bytes = bytes;
+    t->main_kernel_bytes = main_bytes;
+  }
+  return GDB_OK;
+}
+
+// Min / max of an integer host column, split over threads for large columns.
+template <typename T>
+static void value_range(const T* v, uint64_t n, int64_t* lo, int64_t* hi) {
+  if (n == 0) {
+    *lo = 0;
+    *hi = -1;
+    return;
+  }
+  const int T_ = n > (1u << 22) ? (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency())) : 1;
+  std::vector<int64_t> mn(T_, INT64_MAX), mx(T_, INT64_MIN);
+  const uint64_t per = (n + T_ - 1) / T_;
+  auto body = [&](int t) {
+    const uint64_t a = (uint64_t)t * per, b = std::min(n, a + per);
+    int64_t l = INT64_MAX, h = INT64_MIN;
+    for (uint64_t i = a; i < b; ++i) {
+      l = std::min<int64_t>(l, (int64_t)v[i]);
+      h = std::max<int64_t>(h, (int64_t)v[i]);
+    }
+    mn[t] = l;
+    mx[t] = h;
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < T_; ++t) th.emplace_back(body, t);
+  body(0);
+  for (auto& x : th) x.join();
+  *lo = *std::min_element(mn.begin(), mn.end());
+  *hi = *std::max_element(mx.begin(), mx.end());
+}
+
+}  // namespace gdb
+
+using namespace gdb;
+
+extern "C" {
+
+const char* gdb_strerror(int code) {
+  switch (code) {
+    case GDB_OK: return "ok";
+    case GDB_E_INVALID: return "invalid argument";
+    case GDB_E_NOMEM: return "out of memory";
+    case GDB_E_HIP: return "HIP runtime error";
+    case GDB_E_ADMISSION: return "request exceeds the HBM admission budget";
+    case GDB_E_UNSUPPORTED: return "unsupported input";
+    case GDB_E_NODEVICE: return "no GPU device";
+    case GDB_E_TIMEOUT: return "timed out";
+    case GDB_E_CANCELLED: return "cancelled";
+    case GDB_E_NOTFOUND: return "not found";
+    default: return "unknown error";
+  }
+}
+
+const char* gdb_last_error(void) { return g_last_error.c_str(); }
+
+int gdb_open(int device, uint64_t hbm_budget_bytes, gdb_ctx** out) {
+  if (!out) return fail(GDB_E_INVALID, "null output pointer");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    return fail(GDB_E_NODEVICE, "no HIP device visible");
+  }
+  if (device < 0 || device >= n) return fail(GDB_E_INVALID, "device index out of range");
+  GDB_HIP(hipSetDevice(device));
+  gdb_ctx* ctx = new gdb_ctx();
+  ctx->device = device;
+  ctx->arenas.resize(A_NSLOTS);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    delete ctx;
+    return fail(GDB_E_HIP, "hipGetDeviceProperties failed");
+  }
+  std::strncpy(ctx->info.name, prop.name, sizeof(ctx->info.name) - 1);
+  std::strncpy(ctx->info.arch, prop.gcnArchName, sizeof(ctx->info.arch) - 1);
+  ctx->info.cu_count = prop.multiProcessorCount;
+  int xcc = 1;
+  if (hipDeviceGetAttribute(&xcc, hipDeviceAttributeNumberOfXccs, device) != hipSuccess) {
+    (void)hipGetLastError();
+    xcc = 1;
+  }
+  ctx->info.xcd_count = xcc;
+  ctx->info.hbm_bytes = prop.totalGlobalMem;
+  ctx->info.l2_bytes = (uint64_t)prop.l2CacheSize;
+  ctx->info.lds_bytes_per_cu = (int)prop.maxSharedMemoryPerMultiProcessor;
+  ctx->info.wave_size = prop.warpSize;
+  ctx->info.hbm_peak_gbps = 8000.0;  // MI355X HBM3E published peak
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+    (void)hipGetLastError();
+    free_b = prop.totalGlobalMem;
+  }
+  const uint64_t reserve = 4ull << 30;
+  ctx->budget = hbm_budget_bytes ? hbm_budget_bytes : (free_b > reserve ? free_b - reserve : free_b);
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return fail(GDB_E_HIP, "stream creation failed");
+  }
+  for (auto& e : ctx->ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete ctx;
+      return fail(GDB_E_HIP, "event creation failed");
+    }
+  }
+  *out = ctx;
+  return GDB_OK;
+}
+
+void gdb_close(gdb_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
+  for (Arena& a : ctx->arenas) dev_free(ctx, a.ptr, a.bytes);
+  if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
+  delete ctx;
+}
+
+int gdb_device_info_get(gdb_ctx* ctx, gdb_device_info* out) {
+  if (!ctx || !out) return fail(GDB_E_INVALID, "null argument");
+  *out = ctx->info;
+  return GDB_OK;
+}
+
+uint64_t gdb_hbm_used(gdb_ctx* ctx) { return ctx ? ctx->used : 0; }
+
+void gdb_debug_fail_allocs(gdb_ctx* ctx, int count) {
+  if (ctx) ctx->fail_allocs = count;
+}
+
+int gdb_table_create(gdb_ctx* ctx, const gdb_col_desc* cols, int ncols, uint64_t nrows,
+                     gdb_table** out) {
+  if (!ctx || !out || (ncols > 0 && !cols) || ncols < 0)
+    return fail(GDB_E_INVALID, "null argument");
+  *out = nullptr;
+  if (nrows >= (1ull << 32)) return fail(GDB_E_UNSUPPORTED, "tables are limited to 2^32-1 rows");
+  GDB_HIP(hipSetDevice(ctx->device));
+  gdb_table* t = new gdb_table();
+  t->ctx = ctx;
+  t->nrows = nrows;
+  for (int i = 0; i < ncols; ++i) {
+    const gdb_col_desc& d = cols[i];
+    Column c;
+    if (!d.name) {
+      gdb_table_free(t);
+      return fail(GDB_E_INVALID, "column without a name");
+    }
+    c.name = d.name;
+    c.type = d.type;
+    switch (d.type) {
+      case GDB_I32: c.width = 4; break;
+      case GDB_I64: c.width = 8; break;
+      case GDB_U8: c.width = 1; break;
+      case GDB_BYTES: c.width = d.width; break;
+      default: gdb_table_free(t); return fail(GDB_E_INVALID, "bad column type for " + c.name);
+    }
+    if (c.width <= 0) {
+      gdb_table_free(t);
+      return fail(GDB_E_INVALID, "bad column width for " + c.name);
+    }
+    // Pad every column to a 64-byte multiple so dwordx4 loads of the last
+    // vector group stay inside the allocation; the pad is zeroed.
+    c.bytes = (size_t)nrows * (size_t)c.width;
+    size_t alloc = ((c.bytes + 63) & ~size_t(63)) + 64;
+    int rc = dev_alloc(ctx, &c.dev, alloc);
+    if (rc != GDB_OK) {
+      gdb_table_free(t);
+      return rc;
+    }
+    if (d.host_data && d.type == GDB_I32)
+      value_range((const int32_t*)d.host_data, nrows, &c.minv, &c.maxv);
+    else if (d.host_data && d.type == GDB_I64)
+      value_range((const int64_t*)d.host_data, nrows, &c.minv, &c.maxv);
+    t->cols.push_back(c);
+    Column& cc = t->cols.back();
+    cc.bytes = alloc;  // accounted size
+    if (hipMemsetAsync(cc.dev, 0, alloc, ctx->stream) != hipSuccess ||
+        (c.bytes && d.host_data &&
+         hipMemcpyAsync(cc.dev, d.host_data, (size_t)nrows * c.width, hipMemcpyHostToDevice,
+                        ctx->stream) != hipSuccess)) {
+      gdb_table_free(t);
+      return fail(GDB_E_HIP, "upload of column " + c.name + " failed");
+    }
+  }
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    gdb_table_free(t);
+    return fail(GDB_E_HIP, "upload synchronisation failed");
+  }
+  *out = t;
+  return GDB_OK;
+}
+
+void gdb_table_free(gdb_table* t) {
+  if (!t) return;
+  if (t->ctx) {
+    (void)hipStreamSynchronize(t->ctx->stream);
+    for (Column& c : t->cols) dev_free(t->ctx, c.dev, c.bytes);
+  }
+  delete t;
+}
+
+uint64_t gdb_table_rows(const gdb_table* t) { return t ? t->nrows : 0; }
+
+uint64_t gdb_table_bytes(const gdb_table* t) {
+  uint64_t b = 0;
+  if (t)
+    for (const Column& c : t->cols) b += (uint64_t)t->nrows * c.width;
+  return b;
+}
+
+}  // extern "C"

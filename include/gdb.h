@@ -1,0 +1,224 @@
+/*
+ * gdb.h — C ABI of the MI355X (gfx950) query engine, libgdb.so.
+ *
+ * The reference drives each pipeline through a Metal buffer/pipeline layer:
+ * runQ*Benchmark(MTL::Device*, MTL::CommandQueue*, MTL::Library*)
+ * (reference src/infra.h:874-922), createPipeline (src/infra.h:178-186),
+ * device->newBuffer(..., StorageModeShared) and commit/waitUntilCompleted
+ * (e.g. src/q6.cpp:42-47, :89-90). This header replaces that layer: columnar
+ * tables are copied once into HBM and stay resident, every pipeline is one
+ * synchronous call on a resident table, and timing comes from hipEvents.
+ *
+ * Conventions
+ *  - Every call returns GDB_OK (0) or a negative GDB_E_* code; no C++ exception
+ *    crosses the ABI. gdb_last_error() gives a message for the calling thread.
+ *  - Integer and money columns are int32 / int64 fixed point (cents, hundredths),
+ *    so every sum and count is exact.
+ *  - A context is used by one host thread at a time; the serving calls
+ *    (gdb_serve_*) are the exception and are thread-safe.
+ *  - Day numbers count from 1992-01-01 = day 0.
+ */
+#ifndef GDB_H
+#define GDB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  GDB_OK = 0,
+  GDB_E_INVALID = -1,     /* bad argument, unknown column, malformed parameters */
+  GDB_E_NOMEM = -2,       /* device or pinned-host allocation failed */
+  GDB_E_HIP = -3,         /* a HIP runtime call failed */
+  GDB_E_ADMISSION = -4,   /* request footprint exceeds the HBM budget */
+  GDB_E_UNSUPPORTED = -5, /* shape the engine does not handle */
+  GDB_E_NODEVICE = -6,    /* no GPU visible */
+  GDB_E_TIMEOUT = -7,     /* serving watchdog fired */
+  GDB_E_CANCELLED = -8,   /* request cancelled before it ran */
+  GDB_E_NOTFOUND = -9     /* unknown pipeline name / request id */
+};
+
+/* Replaces the stderr prints of createPipeline (reference src/infra.h:181-184). */
+const char* gdb_strerror(int code);
+const char* gdb_last_error(void);
+
+/* ---- context (replaces MTL::CreateSystemDefaultDevice + newCommandQueue,
+ *      reference src/main.cpp:76-89) ------------------------------------- */
+typedef struct gdb_ctx gdb_ctx;
+typedef struct gdb_table gdb_table;
+
+typedef struct {
+  char name[64];
+  char arch[32];
+  int cu_count;
+  int xcd_count;           /* 8 on MI355X */
+  uint64_t hbm_bytes;
+  uint64_t l2_bytes;
+  int lds_bytes_per_cu;
+  int wave_size;
+  double hbm_peak_gbps;    /* published peak used for roofline fractions */
+} gdb_device_info;
+
+/* hbm_budget_bytes = 0 means "all free HBM minus 4 GiB". */
+int gdb_open(int device, uint64_t hbm_budget_bytes, gdb_ctx** out);
+void gdb_close(gdb_ctx* ctx);
+int gdb_device_info_get(gdb_ctx* ctx, gdb_device_info* out);
+/* Bytes of HBM currently held by resident tables and arenas. */
+uint64_t gdb_hbm_used(gdb_ctx* ctx);
+/* Test hook: the next `count` device allocations fail with GDB_E_NOMEM. */
+void gdb_debug_fail_allocs(gdb_ctx* ctx, int count);
+
+/* ---- resident columnar tables (replaces newBuffer(ptr, bytes, Shared),
+ *      reference src/q6.cpp:42-47, and the per-query column vectors) -------- */
+enum { GDB_I32 = 1, GDB_I64 = 2, GDB_U8 = 3, GDB_BYTES = 4 /* fixed width */ };
+
+typedef struct {
+  const char* name;      /* e.g. "l_shipdate" (TPC-H column names) */
+  int type;              /* GDB_I32 / GDB_I64 / GDB_U8 / GDB_BYTES */
+  int width;             /* bytes per value for GDB_BYTES (55, 80), else ignored */
+  const void* host_data; /* nrows values; copied to HBM by gdb_table_create */
+} gdb_col_desc;
+
+int gdb_table_create(gdb_ctx* ctx, const gdb_col_desc* cols, int ncols, uint64_t nrows,
+                     gdb_table** out);
+void gdb_table_free(gdb_table* t);
+uint64_t gdb_table_rows(const gdb_table* t);
+uint64_t gdb_table_bytes(const gdb_table* t);
+
+/* ---- timing (replaces GPUEndTime()-GPUStartTime(), reference
+ *      src/q6.cpp:92-94, and printTimingSummary, src/infra.h:511-516) ------- */
+typedef struct {
+  double gpu_ms;          /* hipEvent time of the whole pipeline on its stream */
+  double main_kernel_ms;  /* hipEvent time of the dominant kernel alone */
+  double post_ms;         /* host finalisation (sort, format) */
+  uint64_t bytes;         /* algorithmic bytes read by the pipeline's scans */
+  uint64_t main_kernel_bytes;
+} gdb_timing;
+
+/* ---- pipelines (drop-in for runQ6/Q1/Q3/Q9/Q13Benchmark,
+ *      reference src/infra.h:874-922) --------------------------------------- */
+
+/* Q6 filter-scan-sum (reference src/q6.cpp:8-120, kernels/q6.metal:12-73,
+ * sql/q6.sql). Table columns: l_shipdate, l_discount, l_quantity (int32),
+ * l_extendedprice (int64 cents). Predicate: date_lo <= shipdate < date_hi,
+ * disc_lo <= discount <= disc_hi, quantity < qty_lt.
+ * revenue = SUM(extprice * discount) in cents*hundredths (1e-4 dollars). */
+typedef struct { int32_t date_lo, date_hi, disc_lo, disc_hi, qty_lt; } gdb_q6_params;
+typedef struct { int64_t revenue; uint64_t rows; } gdb_q6_result;
+int gdb_q6(gdb_ctx* ctx, const gdb_table* lineitem, const gdb_q6_params* p, gdb_q6_result* out,
+           gdb_timing* timing);
+
+/* Q1 grouped aggregation (reference src/q1.cpp:8-173, kernels/q1.metal:39-121,
+ * sql/q1.sql). Columns: l_shipdate, l_quantity, l_discount, l_tax (int32),
+ * l_extendedprice (int64), l_returnflag, l_linestatus (uint8 chars).
+ * Filter shipdate <= cutoff; groups (returnflag in {A,N,R}) x (linestatus in
+ * {F,O}); rows with other codes belong to no group. Output rows are the
+ * non-empty groups in (returnflag, linestatus) order. Units: sum_disc_price is
+ * cents*hundredths, sum_charge cents*1e-4 held as a 128-bit (hi, lo) pair. */
+typedef struct { int32_t cutoff; } gdb_q1_params;
+typedef struct {
+  char returnflag, linestatus;
+  int64_t sum_qty, sum_base_price, sum_disc_price;
+  uint64_t sum_charge_lo; int64_t sum_charge_hi;
+  int64_t sum_disc, count;
+  double avg_qty, avg_price, avg_disc; /* exact sums / count, in SQL units */
+} gdb_q1_row;
+typedef struct { int32_t nrows; gdb_q1_row rows[6]; } gdb_q1_result;
+int gdb_q1(gdb_ctx* ctx, const gdb_table* lineitem, const gdb_q1_params* p, gdb_q1_result* out,
+           gdb_timing* timing);
+
+/* Q3 join + group + top-k (reference src/q3.cpp:8-173, kernels/q3.metal:43-340,
+ * src/infra.h:538-564, sql/q3.sql). customer: c_custkey, c_mktsegment (uint8
+ * code 0..4); orders: o_orderkey, o_custkey, o_orderdate, o_shippriority;
+ * lineitem: l_orderkey, l_shipdate, l_discount (int32), l_extendedprice (int64).
+ * Filter segment, orderdate < date, shipdate > date; revenue =
+ * SUM(extprice*(100-discount)) per orderkey; top k by revenue desc, orderdate
+ * asc, orderkey asc. `groups` = number of qualifying orderkeys. */
+typedef struct { int32_t segment, date, top_k; } gdb_q3_params;
+typedef struct { int32_t orderkey; int32_t orderdate; int32_t shippriority; int32_t pad; int64_t revenue; } gdb_q3_row;
+typedef struct { int32_t nrows; int64_t groups; gdb_q3_row rows[100]; } gdb_q3_result;
+int gdb_q3(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* orders,
+           const gdb_table* lineitem, const gdb_q3_params* p, gdb_q3_result* out,
+           gdb_timing* timing);
+
+/* Q9 multi-way join with substring filter (reference src/q9.cpp:8-241,
+ * kernels/q9.metal:44-289, src/infra.h:733-776, sql/q9.sql). part: p_partkey,
+ * p_name (55-byte field); supplier: s_suppkey, s_nationkey; partsupp:
+ * ps_partkey, ps_suppkey (int32), ps_supplycost (int64 cents); orders:
+ * o_orderkey, o_orderdate; lineitem: l_partkey, l_suppkey, l_orderkey,
+ * l_quantity, l_discount (int32), l_extendedprice (int64).
+ * amount = extprice*(100-discount) - supplycost*quantity*100 (cents*hundredths),
+ * grouped by (nation, year(orderdate)), ordered nation name asc, year desc. */
+typedef struct { char pattern[32]; } gdb_q9_params; /* substring, e.g. "green" */
+typedef struct { int32_t nationkey; int32_t year; int64_t profit; } gdb_q9_row;
+typedef struct { int32_t nrows; gdb_q9_row rows[25 * 16]; } gdb_q9_result;
+int gdb_q9(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supplier,
+           const gdb_table* partsupp, const gdb_table* orders, const gdb_table* lineitem,
+           const gdb_q9_params* p, gdb_q9_result* out, gdb_timing* timing);
+
+/* Q13 left-outer count + histogram (reference src/q13.cpp:9-116,
+ * kernels/q13.metal:33-167, src/infra.h:779-799, sql/q13.sql). customer:
+ * c_custkey; orders: o_custkey (int32), o_comment (80-byte field, NUL padded).
+ * An order counts when its comment does NOT match %word1%word2%. Output rows
+ * (c_count, custdist) ordered custdist desc, c_count desc; no bin cap. */
+typedef struct { char word1[32]; char word2[32]; } gdb_q13_params;
+typedef struct { int32_t c_count; int32_t pad; int64_t custdist; } gdb_q13_row;
+typedef struct { int32_t nrows; gdb_q13_row rows[512]; } gdb_q13_result;
+int gdb_q13(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* orders,
+            const gdb_q13_params* p, gdb_q13_result* out, gdb_timing* timing);
+
+/* ---- operator primitives (drop-in for the `selection`, `aggregation`,
+ *      `join` microbenchmarks, reference src/microbenchmarks.cpp:51-251,
+ *      kernels/microbenchmarks.metal:158-294) --------------------------------- */
+
+/* Predicate scan `column < value` over an int32 column. Emits a selection
+ * bitmap (bit r of word r/64 = row r passes; host buffer of ceil(n/64) u64, may
+ * be NULL) and, if rowids != NULL, the passing row ids in ascending order (host
+ * buffer of n u32). *count = number of passing rows. */
+int gdb_select_lt(gdb_ctx* ctx, const gdb_table* t, const char* column, int32_t value,
+                  uint64_t* bitmap, uint32_t* rowids, uint64_t* count, gdb_timing* timing);
+/* SUM of an int32 or int64 column, exact in int64. */
+int gdb_sum(gdb_ctx* ctx, const gdb_table* t, const char* column, int64_t* sum,
+            gdb_timing* timing);
+/* Equi-join count: number of (probe row, build row) pairs with equal keys,
+ * build keys unique (reference hash_join_build/probe). */
+int gdb_join_count(gdb_ctx* ctx, const gdb_table* build, const char* build_col,
+                   const gdb_table* probe, const char* probe_col, uint64_t* matches,
+                   gdb_timing* timing);
+
+/* ---- seeded synthetic tables (replace the dbgen .tbl loaders, reference
+ *      src/infra.h:231-468). Host-side; callers own the output arrays. ------ */
+int gdb_gen_lineitem_flat(uint64_t seed, uint64_t row0, uint64_t n, int32_t* shipdate,
+                          int32_t* quantity, int32_t* discount, int32_t* tax, int64_t* extprice,
+                          uint8_t* returnflag, uint8_t* linestatus, int32_t* receiptdate,
+                          int nthreads);
+int gdb_gen_customer(uint64_t seed, uint64_t n, int32_t* custkey, uint8_t* segment, int nthreads);
+int gdb_gen_orders(uint64_t seed, uint64_t order0, uint64_t n, uint64_t ncustomers,
+                   int32_t* orderkey, int32_t* custkey, int32_t* orderdate,
+                   int32_t* shippriority, uint8_t* comment /* n*80 or NULL */, int nthreads);
+/* Rows of the joined fact table generated by orders [order0, order0+n). */
+uint64_t gdb_gen_lineitem_rows(uint64_t seed, uint64_t order0, uint64_t n);
+int gdb_gen_lineitem_joined(uint64_t seed, uint64_t order0, uint64_t n, uint64_t nparts,
+                            uint64_t nsuppliers, int32_t* orderkey, int32_t* partkey,
+                            int32_t* suppkey, int32_t* quantity, int32_t* discount,
+                            int64_t* extprice, int32_t* shipdate, int nthreads);
+int gdb_gen_part(uint64_t seed, uint64_t n, int32_t* partkey, uint8_t* name /* n*55 */,
+                 int nthreads);
+int gdb_gen_partsupp(uint64_t seed, uint64_t nparts, uint64_t nsuppliers, int32_t* partkey,
+                     int32_t* suppkey, int64_t* supplycost, int nthreads);
+int gdb_gen_supplier(uint64_t seed, uint64_t n, int32_t* suppkey, int32_t* nationkey,
+                     int nthreads);
+/* Name of nation key 0..24 / market segment code 0..4 (TPC-H). */
+const char* gdb_nation_name(int nationkey);
+const char* gdb_segment_name(int code);
+/* Calendar: year of a day number (day 0 = 1992-01-01). */
+int32_t gdb_year_of_day(int32_t day);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GDB_H */

@@ -1,0 +1,140 @@
+"""GPU parity of the scan pipelines (Q6, Q1) and the selection / aggregation
+primitives against the sqlite3 goldens and the oracle, bit-exact."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from gpudbmetalbenchmark_amd import GdbError, Timing, generate
+from gpudbmetalbenchmark_amd.abi import GDB_E_UNSUPPORTED
+
+pytestmark = pytest.mark.gpu
+
+Q6_COLS = ("l_shipdate", "l_discount", "l_quantity", "l_extendedprice")
+Q1_COLS = ("l_shipdate", "l_quantity", "l_discount", "l_tax", "l_extendedprice",
+           "l_returnflag", "l_linestatus")
+
+
+def table(engine, cols, names):
+    return engine.table({k: cols[k] for k in names})
+
+
+def np_cols(d):
+    out = {}
+    for k, v in d.items():
+        if k == "l_extendedprice":
+            out[k] = np.array(v, np.int64)
+        elif k in ("l_returnflag", "l_linestatus"):
+            out[k] = np.array(v, np.uint8)
+        else:
+            out[k] = np.array(v, np.int32)
+    return out
+
+
+def q1_equal(got, exp, tol=1e-6):
+    assert len(got) == len(exp)
+    for g, e in zip(got, exp):
+        for k in ("returnflag", "linestatus", "sum_qty", "sum_base_price", "sum_disc_price",
+                  "sum_disc", "count"):
+            assert g[k] == e[k], k
+        assert g["sum_charge"] == int(e["sum_charge"])
+        for k in ("avg_qty", "avg_price", "avg_disc"):
+            assert g[k] == pytest.approx(e[k], rel=tol, abs=0), k
+
+
+def test_q6_golden(engine, golden, flat):
+    t = table(engine, flat, Q6_COLS)
+    for case in golden["q6"]:
+        assert engine.q6(t, **case["params"]) == dict(revenue=case["revenue"],
+                                                      rows=case["rows"]), case["params"]
+
+
+def test_q6_edge_and_tails(engine, golden):
+    e = golden["edge"]["q6"]
+    full = np_cols(e["table"])
+    for n in range(0, len(full["l_shipdate"]) + 1):  # every tail length 0..3 and empty
+        cols = {k: v[:n] for k, v in full.items()}
+        t = engine.table(cols)
+        p = e["results"][0]["params"]
+        assert engine.q6(t, **p) == O.q6(cols, **p), n
+    t = engine.table(full)
+    assert engine.q6(t, **e["results"][0]["params"]) == dict(
+        revenue=e["results"][0]["revenue"], rows=e["results"][0]["rows"])
+
+
+def test_q1_golden(engine, golden, flat):
+    t = table(engine, flat, Q1_COLS)
+    for case in golden["q1"]:
+        q1_equal(engine.q1(t, case["cutoff"]), case["rows"])
+
+
+def test_q1_edge(engine, golden):
+    e = golden["edge"]["q1"]
+    t = engine.table(np_cols(e["table"]))
+    for case in e["results"]:
+        q1_equal(engine.q1(t, case["cutoff"]), case["rows"])
+
+
+def test_q1_rejects_out_of_domain_rows(engine):
+    cols = generate.lineitem_flat(3, 1000)
+    cols["l_quantity"][17] = 500  # packed per-lane counters need quantity <= 127
+    t = table(engine, cols, Q1_COLS)
+    with pytest.raises(GdbError) as ei:
+        engine.q1(t)
+    assert ei.value.code == GDB_E_UNSUPPORTED
+
+
+def test_config0_q6_6m_rows(engine):
+    li = generate.lineitem_flat(42, 6_000_000)
+    t = table(engine, li, Q6_COLS)
+    tm = Timing()
+    got = engine.q6(t, timing=tm)
+    assert got == O.q6(li, nthreads=8)
+    assert got["rows"] > 0 and tm.main_kernel_ms > 0
+    for p in (dict(date_lo=0, date_hi=2526, disc_lo=0, disc_hi=10, qty_lt=51),
+              dict(date_lo=1461, date_hi=1826, disc_lo=2, disc_hi=4, qty_lt=25)):
+        assert engine.q6(t, **p) == O.q6(li, **p, nthreads=8)
+
+
+def test_config1_q1_60m_rows(engine):
+    li = generate.lineitem_flat(42, 60_000_000)
+    t = table(engine, li, Q1_COLS)
+    got = engine.q1(t, 2435)
+    q1_equal(got, O.q1(li, 2435, nthreads=8), tol=1e-12)
+    assert sum(r["count"] for r in got) == int(np.count_nonzero(li["l_shipdate"] <= 2435))
+    again = engine.q1(t, 2435)
+    assert again == got  # run-to-run bit-exact
+
+
+def test_permutation_invariance(engine, flat):
+    perm = np.random.default_rng(1).permutation(len(flat["l_shipdate"]))
+    shuf = {k: v[perm] for k, v in flat.items()}
+    a, b = table(engine, flat, Q1_COLS), table(engine, shuf, Q1_COLS)
+    assert engine.q1(a) == engine.q1(b)
+    assert engine.q6(a) == engine.q6(b)
+
+
+def test_select_sum_golden(engine, golden, joinset):
+    li = joinset["lineitem"]
+    t = engine.table({k: li[k] for k in ("l_partkey", "l_quantity", "l_extendedprice")})
+    pr = golden["primitives"]
+    for c in pr["select"]:
+        r = engine.select_lt(t, c["column"], c["value"])
+        ref = O.select_lt(li[c["column"]], c["value"])
+        assert r["count"] == c["count"]
+        assert np.array_equal(r["bitmap"], ref["bitmap"])
+        assert np.array_equal(r["rowids"], np.flatnonzero(li[c["column"]] < c["value"]))
+    for c in pr["sum"]:
+        assert engine.sum(t, c["column"]) == c["sum"]
+
+
+@pytest.mark.parametrize("n", [0, 1, 63, 64, 65, 1023, 1024, 1025, 4097, 3_000_001])
+def test_select_sizes(engine, n):
+    rng = np.random.default_rng(n)
+    col = rng.integers(-1000, 1000, n, dtype=np.int32)
+    t = engine.table({"x": col})
+    r = engine.select_lt(t, "x", 17)
+    ref = O.select_lt(col, 17)
+    assert r["count"] == ref["count"]
+    assert np.array_equal(r["bitmap"], ref["bitmap"])
+    assert np.array_equal(r["rowids"], np.flatnonzero(col < 17))
+    assert engine.sum(t, "x") == int(col.astype(np.int64).sum())
