@@ -26,9 +26,11 @@ def lib():
 
 
 class Table:
-    """A columnar table resident in HBM (gdb_table_create / gdb_table_free)."""
+    """A columnar table resident in HBM (gdb_table_create), or, with host=True,
+    a host-resident table streamed in chunks by the pipelines
+    (gdb_table_create_host; the numpy arrays are referenced, not copied)."""
 
-    def __init__(self, engine, columns, nrows=None):
+    def __init__(self, engine, columns, nrows=None, host=False):
         self.engine = engine
         self._keep = []
         descs = (abi.ColDesc * len(columns))()
@@ -59,9 +61,11 @@ class Table:
         self.nrows = n or 0
         self.names = list(columns)
         self._t = C.c_void_p()
-        abi.check(lib().gdb_table_create(engine._ctx, descs, len(columns), self.nrows,
-                                         C.byref(self._t)))
-        self._keep = []
+        self.host = host
+        create = lib().gdb_table_create_host if host else lib().gdb_table_create
+        abi.check(create(engine._ctx, descs, len(columns), self.nrows, C.byref(self._t)))
+        if not host:
+            self._keep = []
 
     @property
     def handle(self):
@@ -84,12 +88,32 @@ class Engine:
 
     def __init__(self, device=0, hbm_budget=0):
         self._ctx = C.c_void_p()
+        self._pinned = []
         abi.check(lib().gdb_open(device, hbm_budget, C.byref(self._ctx)))
 
     def close(self):
         if self._ctx:
+            for p in self._pinned:
+                lib().gdb_host_free(self._ctx, p)
+            self._pinned = []
             lib().gdb_close(self._ctx)
             self._ctx = C.c_void_p()
+
+    def pinned_empty(self, shape, dtype):
+        """A numpy array in pinned host memory (gdb_host_alloc), freed by close()."""
+        dtype = np.dtype(dtype)
+        n = int(np.prod(shape)) if np.ndim(shape) else int(shape)
+        p = C.c_void_p()
+        abi.check(lib().gdb_host_alloc(self._ctx, max(1, n * dtype.itemsize), C.byref(p)))
+        self._pinned.append(p)
+        buf = (C.c_char * max(1, n * dtype.itemsize)).from_address(p.value)
+        return np.frombuffer(buf, dtype=dtype, count=n).reshape(shape)
+
+    def set_chunk_rows(self, rows):
+        abi.check(lib().gdb_set_chunk_rows(self._ctx, rows))
+
+    def host_table(self, columns):
+        return Table(self, columns, host=True)
 
     def __enter__(self):
         return self

@@ -31,7 +31,8 @@ class DeviceInfo(C.Structure):
 
 class Timing(C.Structure):
     _fields_ = [("gpu_ms", C.c_double), ("main_kernel_ms", C.c_double), ("post_ms", C.c_double),
-                ("bytes", C.c_uint64), ("main_kernel_bytes", C.c_uint64)]
+                ("bytes", C.c_uint64), ("main_kernel_bytes", C.c_uint64),
+                ("h2d_bytes", C.c_uint64), ("chunks", C.c_int32), ("pad", C.c_int32)]
 
 
 class Q6Params(C.Structure):
@@ -122,6 +123,12 @@ PROTOTYPES = {
     "gdb_table_free": (None, [P]),
     "gdb_table_rows": (C.c_uint64, [P]),
     "gdb_table_bytes": (C.c_uint64, [P]),
+    "gdb_host_alloc": (C.c_int, [P, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "gdb_host_free": (None, [P, P]),
+    "gdb_table_create_host": (C.c_int, [P, C.POINTER(ColDesc), C.c_int, C.c_uint64,
+                                        C.POINTER(C.c_void_p)]),
+    "gdb_table_is_host": (C.c_int, [P]),
+    "gdb_set_chunk_rows": (C.c_int, [P, C.c_uint64]),
     "gdb_q6": (C.c_int, [P, P, C.POINTER(Q6Params), C.POINTER(Q6Result), C.POINTER(Timing)]),
     "gdb_q1": (C.c_int, [P, P, C.POINTER(Q1Params), C.POINTER(Q1Result), C.POINTER(Timing)]),
     "gdb_q3": (C.c_int, [P, P, P, P, C.POINTER(Q3Params), C.POINTER(Q3Result),

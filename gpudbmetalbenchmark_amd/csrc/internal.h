@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -36,6 +37,7 @@ struct Column {
   void* dev = nullptr;
   size_t bytes = 0;
   int64_t minv = 0, maxv = -1;  // value range of integer columns (empty: max < min)
+  const void* host = nullptr;   // host tables: caller memory streamed in chunks
 };
 
 // Launch shape chosen per kernel; the autotuner may overwrite the defaults.
@@ -68,11 +70,15 @@ struct gdb_ctx {
   void* host_pinned = nullptr;     // small pinned staging for results
   size_t host_pinned_bytes = 0;
   gdb::Tuning tune[16];
+  uint64_t chunk_rows = 0;         // streamed chunk size, 0 = adaptive
+  hipEvent_t sev[4] = {};          // copy-done / slot-free events of the two slots
+  std::vector<hipEvent_t> chunk_ev;  // per-chunk compute start/end
 };
 
 struct gdb_table {
   gdb_ctx* ctx = nullptr;
   uint64_t nrows = 0;
+  bool host = false;  // columns live in host memory and are streamed
   std::vector<gdb::Column> cols;
 };
 
@@ -97,7 +103,26 @@ enum ArenaSlot {
 };
 enum TuneSlot { T_Q6 = 0, T_Q1 = 1, T_SELECT = 2, T_SUM = 3, T_Q3 = 4, T_Q9 = 5, T_Q13 = 6, T_JOIN = 7 };
 
+// Launches of a pipeline's consuming stage over one batch of rows: `dev[i]`
+// is the device address of column i for these rows.
+using Consume = std::function<int(const void* const* dev, uint32_t rows)>;
+
+struct RunStats {
+  double kernel_ms = 0;   // streamed: summed per-chunk compute time
+  uint64_t h2d_bytes = 0;
+  int chunks = 0;         // 0: the table was resident
+};
+
+// Runs `fn` once over a resident table (bracketed by ev[1], ev[2]) or, for a
+// host table, chunk by chunk: two device slots per column, copies on the copy
+// stream overlapped with compute on the engine stream, event hand-off both ways.
+int run_over(gdb_ctx* ctx, const gdb_table* t, const Column* const* cols, int ncols,
+             const Consume& fn, RunStats* rs);
+// Rejects host tables where a pipeline needs the table resident.
+int require_resident(const gdb_table* t, const char* what);
+
 // Event helpers around a pipeline (ev[0], ev[5]) and its dominant kernel (ev[1], ev[2]).
-int timing_finish(gdb_ctx* ctx, gdb_timing* t, uint64_t bytes, uint64_t main_bytes);
+int timing_finish(gdb_ctx* ctx, gdb_timing* t, uint64_t bytes, uint64_t main_bytes,
+                  const RunStats* rs = nullptr);
 
 }  // namespace gdb

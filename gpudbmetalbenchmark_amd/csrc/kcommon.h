@@ -59,8 +59,9 @@ __device__ __forceinline__ uint32_t ld_u8x4(const uint8_t* p, uint32_t vec) {
 }
 
 // Finaliser: block k of the grid sums column k of a k-major partials array
-// ([K][nparts]) into out[k]. Launch with 256 threads.
-template <int TAG = 0>
+// ([K][nparts]) into out[k] (ACC = 1: adds to out[k], for streamed chunks).
+// Launch with 256 threads.
+template <int ACC = 0>
 __global__ __launch_bounds__(256) void sum_columns(const int64_t* __restrict__ partials,
                                                    uint32_t nparts, int64_t* __restrict__ out) {
   __shared__ int64_t red[4];
@@ -68,7 +69,7 @@ __global__ __launch_bounds__(256) void sum_columns(const int64_t* __restrict__ p
   int64_t s = 0;
   for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) s += col[i];
   s = block_sum(s, red);
-  if (threadIdx.x == 0) out[blockIdx.x] = s;
+  if (threadIdx.x == 0) out[blockIdx.x] = ACC ? out[blockIdx.x] + s : s;
 }
 
 }  // namespace gdbk

@@ -151,51 +151,69 @@ __global__ __launch_bounds__(256) void q1_scan(Q1Cols c, uint32_t n, int32_t cut
 extern "C" int gdb_q1(gdb_ctx* ctx, const gdb_table* li, const gdb_q1_params* p,
                       gdb_q1_result* out, gdb_timing* timing) {
   if (!ctx || !li || !p || !out) return fail(GDB_E_INVALID, "null argument");
-  const Column *cs, *cq, *cd, *ct, *cp, *crf, *cls;
-  GDB_TRY(column(li, "l_shipdate", GDB_I32, &cs));
-  GDB_TRY(column(li, "l_quantity", GDB_I32, &cq));
-  GDB_TRY(column(li, "l_discount", GDB_I32, &cd));
-  GDB_TRY(column(li, "l_tax", GDB_I32, &ct));
-  GDB_TRY(column(li, "l_extendedprice", GDB_I64, &cp));
-  GDB_TRY(column(li, "l_returnflag", GDB_U8, &crf));
-  GDB_TRY(column(li, "l_linestatus", GDB_U8, &cls));
+  const Column* c[7];
+  GDB_TRY(column(li, "l_shipdate", GDB_I32, &c[0]));
+  GDB_TRY(column(li, "l_quantity", GDB_I32, &c[1]));
+  GDB_TRY(column(li, "l_discount", GDB_I32, &c[2]));
+  GDB_TRY(column(li, "l_tax", GDB_I32, &c[3]));
+  GDB_TRY(column(li, "l_extendedprice", GDB_I64, &c[4]));
+  GDB_TRY(column(li, "l_returnflag", GDB_U8, &c[5]));
+  GDB_TRY(column(li, "l_linestatus", GDB_U8, &c[6]));
   GDB_HIP(hipSetDevice(ctx->device));
-  const uint32_t n = (uint32_t)li->nrows;
+  const uint64_t n = li->nrows;
   const Tuning& tu = ctx->tune[T_Q1];
-  const int block = (tu.block == 64 || tu.block == 128 || tu.block == 256) ? tu.block : 128;
+  const int block = (tu.block == 64 || tu.block == 128 || tu.block == 256) ? tu.block : 256;
   const size_t lds = (size_t)(block / 64) * NBIN * NVAL * WAVE * sizeof(unsigned long long);
   const void* kfn = tu.unroll >= 4   ? (const void*)q1_scan<4>
                     : tu.unroll == 1 ? (const void*)q1_scan<1>
                                      : (const void*)q1_scan<2>;
-  int blocks = resident_blocks(ctx, kfn, block, lds, tu.blocks_per_cu);
-  // Keep every lane under MAX_ROWS_PER_LANE rows (packed counter capacity).
-  const uint64_t need = ((uint64_t)n + (uint64_t)block * MAX_ROWS_PER_LANE - 1) /
-                        ((uint64_t)block * MAX_ROWS_PER_LANE);
-  blocks = std::max<int>(blocks, (int)need);
-  blocks = std::max(1, std::min<int>(blocks, std::max<int>((int)need, (int)((n / 4 + block - 1) / block))));
+  const int resident = resident_blocks(ctx, kfn, block, lds, tu.blocks_per_cu);
+  // Worst case per launch: enough blocks that no lane sees more than
+  // MAX_ROWS_PER_LANE rows (the packed counters' capacity).
+  const uint64_t max_launch_rows = li->host ? std::min<uint64_t>(n, 1ull << 32) : n;
+  const uint64_t need_max = (max_launch_rows + (uint64_t)block * MAX_ROWS_PER_LANE - 1) /
+                            ((uint64_t)block * MAX_ROWS_PER_LANE);
+  const int max_blocks = std::max<int>(resident, (int)need_max);
   int64_t *partials, *dout, *host;
-  GDB_TRY(arena(ctx, A_PARTIALS, sizeof(int64_t) * K * blocks, (void**)&partials));
+  GDB_TRY(arena(ctx, A_PARTIALS, sizeof(int64_t) * K * max_blocks, (void**)&partials));
   GDB_TRY(arena(ctx, A_OUT, sizeof(int64_t) * K, (void**)&dout));
   GDB_TRY(pinned(ctx, sizeof(int64_t) * K, (void**)&host));
-  const Q1Cols c{(const int32_t*)cs->dev, (const int32_t*)cq->dev, (const int32_t*)cd->dev,
-                 (const int32_t*)ct->dev, (const int64_t*)cp->dev, (const uint8_t*)crf->dev,
-                 (const uint8_t*)cls->dev};
   hipStream_t s = ctx->stream;
+  int launches = 0;
+  auto consume = [&](const void* const* d, uint32_t rows) -> int {
+    const uint64_t need = ((uint64_t)rows + (uint64_t)block * MAX_ROWS_PER_LANE - 1) /
+                          ((uint64_t)block * MAX_ROWS_PER_LANE);
+    int blocks = std::max<int>(resident, (int)need);
+    blocks = std::max(1, std::min<int>(blocks, std::max<int>((int)need, (int)((rows / 4 + block - 1) / block))));
+    const Q1Cols q{(const int32_t*)d[0], (const int32_t*)d[1], (const int32_t*)d[2],
+                   (const int32_t*)d[3], (const int64_t*)d[4], (const uint8_t*)d[5],
+                   (const uint8_t*)d[6]};
+    if (tu.unroll >= 4)
+      q1_scan<4><<<blocks, block, lds, s>>>(q, rows, p->cutoff, partials);
+    else if (tu.unroll == 1)
+      q1_scan<1><<<blocks, block, lds, s>>>(q, rows, p->cutoff, partials);
+    else
+      q1_scan<2><<<blocks, block, lds, s>>>(q, rows, p->cutoff, partials);
+    GDB_HIP(hipGetLastError());
+    if (launches++ == 0)
+      sum_columns<0><<<K, 256, 0, s>>>(partials, (uint32_t)blocks, dout);
+    else
+      sum_columns<1><<<K, 256, 0, s>>>(partials, (uint32_t)blocks, dout);
+    GDB_HIP(hipGetLastError());
+    return GDB_OK;
+  };
+  RunStats rs;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_HIP(hipEventRecord(ctx->ev[1], s));
-  if (tu.unroll >= 4)
-    q1_scan<4><<<blocks, block, lds, s>>>(c, n, p->cutoff, partials);
-  else if (tu.unroll == 1)
-    q1_scan<1><<<blocks, block, lds, s>>>(c, n, p->cutoff, partials);
-  else
-    q1_scan<2><<<blocks, block, lds, s>>>(c, n, p->cutoff, partials);
-  GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[2], s));
-  sum_columns<<<K, 256, 0, s>>>(partials, (uint32_t)blocks, dout);
-  GDB_HIP(hipGetLastError());
+  if (n == 0) {
+    GDB_HIP(hipMemsetAsync(dout, 0, sizeof(int64_t) * K, s));
+    GDB_HIP(hipEventRecord(ctx->ev[1], s));
+    GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  } else {
+    GDB_TRY(run_over(ctx, li, c, 7, consume, &rs));
+  }
   GDB_HIP(hipMemcpyAsync(host, dout, sizeof(int64_t) * K, hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
-  GDB_TRY(timing_finish(ctx, timing, 26ull * n, 26ull * n));
+  GDB_TRY(timing_finish(ctx, timing, 26ull * n, 26ull * n, &rs));
   if (host[K - 1] != 0)
     return fail(GDB_E_UNSUPPORTED,
                 "Q1 input outside the exact domain (quantity 0..127, discount 0..100, tax "
@@ -220,6 +238,5 @@ extern "C" int gdb_q1(gdb_ctx* ctx, const gdb_table* li, const gdb_q1_params* p,
     r.avg_price = (double)r.sum_base_price / 100.0 / (double)r.count;
     r.avg_disc = (double)r.sum_disc / 100.0 / (double)r.count;
   }
-  if (timing) timing->post_ms = 0;
   return GDB_OK;
 }

@@ -113,16 +113,24 @@ int resident_blocks(gdb_ctx* ctx, const void* kernel, int block, size_t lds, int
   return std::max(1, per_cu * ctx->info.cu_count);
 }
 
-int timing_finish(gdb_ctx* ctx, gdb_timing* t, uint64_t bytes, uint64_t main_bytes) {
+int timing_finish(gdb_ctx* ctx, gdb_timing* t, uint64_t bytes, uint64_t main_bytes,
+                  const RunStats* rs) {
   GDB_HIP(hipEventSynchronize(ctx->ev[5]));
   if (t) {
     float a = 0.f, b = 0.f;
     GDB_HIP(hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[5]));
-    GDB_HIP(hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]));
+    if (rs && rs->chunks > 0) {
+      b = (float)rs->kernel_ms;
+    } else {
+      GDB_HIP(hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]));
+    }
     t->gpu_ms = a;
     t->main_kernel_ms = b;
     t->bytes = bytes;
     t->main_kernel_bytes = main_bytes;
+    t->h2d_bytes = rs ? rs->h2d_bytes : 0;
+    t->chunks = rs ? rs->chunks : 0;
+    t->post_ms = 0;
   }
   return GDB_OK;
 }
@@ -230,6 +238,12 @@ int gdb_open(int device, uint64_t hbm_budget_bytes, gdb_ctx** out) {
       return fail(GDB_E_HIP, "event creation failed");
     }
   }
+  for (auto& e : ctx->sev) {
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      delete ctx;
+      return fail(GDB_E_HIP, "event creation failed");
+    }
+  }
   *out = ctx;
   return GDB_OK;
 }
@@ -242,6 +256,10 @@ void gdb_close(gdb_ctx* ctx) {
   for (Arena& a : ctx->arenas) dev_free(ctx, a.ptr, a.bytes);
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
   for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : ctx->sev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : ctx->chunk_ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
@@ -324,11 +342,75 @@ int gdb_table_create(gdb_ctx* ctx, const gdb_col_desc* cols, int ncols, uint64_t
 
 void gdb_table_free(gdb_table* t) {
   if (!t) return;
-  if (t->ctx) {
+  if (t->ctx && !t->host) {
     (void)hipStreamSynchronize(t->ctx->stream);
     for (Column& c : t->cols) dev_free(t->ctx, c.dev, c.bytes);
   }
   delete t;
+}
+
+int gdb_table_create_host(gdb_ctx* ctx, const gdb_col_desc* cols, int ncols, uint64_t nrows,
+                          gdb_table** out) {
+  if (!ctx || !out || (ncols > 0 && !cols) || ncols < 0)
+    return fail(GDB_E_INVALID, "null argument");
+  *out = nullptr;
+  if (nrows >= (1ull << 40)) return fail(GDB_E_UNSUPPORTED, "host tables are limited to 2^40 rows");
+  gdb_table* t = new gdb_table();
+  t->ctx = ctx;
+  t->nrows = nrows;
+  t->host = true;
+  for (int i = 0; i < ncols; ++i) {
+    const gdb_col_desc& d = cols[i];
+    Column c;
+    if (!d.name || (!d.host_data && nrows)) {
+      delete t;
+      return fail(GDB_E_INVALID, "host column without a name or data");
+    }
+    c.name = d.name;
+    c.type = d.type;
+    c.width = d.type == GDB_I32 ? 4 : d.type == GDB_I64 ? 8 : d.type == GDB_U8 ? 1
+              : d.type == GDB_BYTES ? d.width : 0;
+    if (c.width <= 0) {
+      delete t;
+      return fail(GDB_E_INVALID, "bad column type or width for " + c.name);
+    }
+    c.host = d.host_data;
+    if (d.type == GDB_I32) value_range((const int32_t*)d.host_data, nrows, &c.minv, &c.maxv);
+    else if (d.type == GDB_I64) value_range((const int64_t*)d.host_data, nrows, &c.minv, &c.maxv);
+    t->cols.push_back(c);
+  }
+  *out = t;
+  return GDB_OK;
+}
+
+int gdb_table_is_host(const gdb_table* t) { return t && t->host ? 1 : 0; }
+
+int gdb_host_alloc(gdb_ctx* ctx, uint64_t bytes, void** out) {
+  if (!ctx || !out) return fail(GDB_E_INVALID, "null argument");
+  *out = nullptr;
+  GDB_HIP(hipSetDevice(ctx->device));
+  if (hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    *out = nullptr;
+    return fail(GDB_E_NOMEM, "hipHostMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
+  return GDB_OK;
+}
+
+void gdb_host_free(gdb_ctx* ctx, void* p) {
+  if (!p) return;
+  if (ctx) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->copy_stream);
+  }
+  (void)hipHostFree(p);
+}
+
+int gdb_set_chunk_rows(gdb_ctx* ctx, uint64_t rows) {
+  if (!ctx) return fail(GDB_E_INVALID, "null context");
+  if (rows && rows < 1024) return fail(GDB_E_INVALID, "chunks must hold at least 1024 rows");
+  ctx->chunk_rows = rows;
+  return GDB_OK;
 }
 
 uint64_t gdb_table_rows(const gdb_table* t) { return t ? t->nrows : 0; }

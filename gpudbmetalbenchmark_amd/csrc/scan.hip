@@ -201,45 +201,56 @@ extern "C" {
 int gdb_q6(gdb_ctx* ctx, const gdb_table* li, const gdb_q6_params* p, gdb_q6_result* out,
            gdb_timing* timing) {
   if (!ctx || !li || !p || !out) return fail(GDB_E_INVALID, "null argument");
-  const Column *cs, *cd, *cq, *cp;
-  GDB_TRY(column(li, "l_shipdate", GDB_I32, &cs));
-  GDB_TRY(column(li, "l_discount", GDB_I32, &cd));
-  GDB_TRY(column(li, "l_quantity", GDB_I32, &cq));
-  GDB_TRY(column(li, "l_extendedprice", GDB_I64, &cp));
+  const Column* c[4];
+  GDB_TRY(column(li, "l_shipdate", GDB_I32, &c[0]));
+  GDB_TRY(column(li, "l_discount", GDB_I32, &c[1]));
+  GDB_TRY(column(li, "l_quantity", GDB_I32, &c[2]));
+  GDB_TRY(column(li, "l_extendedprice", GDB_I64, &c[3]));
   GDB_HIP(hipSetDevice(ctx->device));
-  const uint32_t n = (uint32_t)li->nrows;
+  const uint64_t n = li->nrows;
   const Tuning& tu = ctx->tune[T_Q6];
   const int block = 256;
-  int blocks = resident_blocks(ctx, (const void*)q6_scan<2>, block, 0, tu.blocks_per_cu);
-  blocks = std::max(1, std::min<int>(blocks, (int)((n / 4 + block - 1) / block)));
-  int64_t *partials, *dout;
-  GDB_TRY(arena(ctx, A_PARTIALS, sizeof(int64_t) * 2 * blocks, (void**)&partials));
+  const int resident = resident_blocks(ctx, (const void*)q6_scan<2>, block, 0, tu.blocks_per_cu);
+  int64_t *partials, *dout, *host;
+  GDB_TRY(arena(ctx, A_PARTIALS, sizeof(int64_t) * 2 * resident, (void**)&partials));
   GDB_TRY(arena(ctx, A_OUT, 64, (void**)&dout));
-  int64_t* host;
   GDB_TRY(pinned(ctx, 64, (void**)&host));
   const Q6P kp{p->date_lo, p->date_hi, p->disc_lo, p->disc_hi, p->qty_lt};
   hipStream_t s = ctx->stream;
+  int launches = 0;
+  auto consume = [&](const void* const* d, uint32_t rows) -> int {
+    const int blocks = std::max(1, std::min<int>(resident, (int)((rows / 4 + block - 1) / block)));
+    const int32_t *sp = (const int32_t*)d[0], *dp = (const int32_t*)d[1], *qp = (const int32_t*)d[2];
+    const int64_t* pp = (const int64_t*)d[3];
+    if (tu.unroll >= 4)
+      q6_scan<4><<<blocks, block, 0, s>>>(sp, dp, qp, pp, rows, kp, partials);
+    else if (tu.unroll == 1)
+      q6_scan<1><<<blocks, block, 0, s>>>(sp, dp, qp, pp, rows, kp, partials);
+    else
+      q6_scan<2><<<blocks, block, 0, s>>>(sp, dp, qp, pp, rows, kp, partials);
+    GDB_HIP(hipGetLastError());
+    // partials is [2][blocks]; the finaliser reads column k at k*blocks.
+    if (launches++ == 0)
+      sum_columns<0><<<2, 256, 0, s>>>(partials, (uint32_t)blocks, dout);
+    else
+      sum_columns<1><<<2, 256, 0, s>>>(partials, (uint32_t)blocks, dout);
+    GDB_HIP(hipGetLastError());
+    return GDB_OK;
+  };
+  RunStats rs;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_HIP(hipEventRecord(ctx->ev[1], s));
-  const int32_t *sp = (const int32_t*)cs->dev, *dp = (const int32_t*)cd->dev,
-                *qp = (const int32_t*)cq->dev;
-  const int64_t* pp = (const int64_t*)cp->dev;
-  if (tu.unroll >= 4)
-    q6_scan<4><<<blocks, block, 0, s>>>(sp, dp, qp, pp, n, kp, partials);
-  else if (tu.unroll == 1)
-    q6_scan<1><<<blocks, block, 0, s>>>(sp, dp, qp, pp, n, kp, partials);
-  else
-    q6_scan<2><<<blocks, block, 0, s>>>(sp, dp, qp, pp, n, kp, partials);
-  GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[2], s));
-  sum_columns<<<2, 256, 0, s>>>(partials, (uint32_t)blocks, dout);
-  GDB_HIP(hipGetLastError());
+  if (n == 0) {
+    GDB_HIP(hipMemsetAsync(dout, 0, 16, s));
+    GDB_HIP(hipEventRecord(ctx->ev[1], s));
+    GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  } else {
+    GDB_TRY(run_over(ctx, li, c, 4, consume, &rs));
+  }
   GDB_HIP(hipMemcpyAsync(host, dout, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
-  GDB_TRY(timing_finish(ctx, timing, 20ull * n, 20ull * n));
+  GDB_TRY(timing_finish(ctx, timing, 20ull * n, 20ull * n, &rs));
   out->revenue = host[0];
   out->rows = (uint64_t)host[1];
-  if (timing) timing->post_ms = 0;
   return GDB_OK;
 }
 
@@ -249,6 +260,7 @@ int gdb_sum(gdb_ctx* ctx, const gdb_table* t, const char* name, int64_t* sum, gd
   GDB_TRY(column(t, name, 0, &c));
   if (c->type != GDB_I32 && c->type != GDB_I64)
     return fail(GDB_E_INVALID, "gdb_sum needs an int32 or int64 column");
+  GDB_TRY(require_resident(t, "gdb_sum's table"));
   GDB_HIP(hipSetDevice(ctx->device));
   const uint32_t n = (uint32_t)t->nrows;
   const int block = 256;
@@ -282,6 +294,7 @@ int gdb_select_lt(gdb_ctx* ctx, const gdb_table* t, const char* name, int32_t va
   if (!ctx || !t || !count) return fail(GDB_E_INVALID, "null argument");
   const Column* c;
   GDB_TRY(column(t, name, GDB_I32, &c));
+  GDB_TRY(require_resident(t, "gdb_select_lt's table"));
   GDB_HIP(hipSetDevice(ctx->device));
   const uint32_t n = (uint32_t)t->nrows;
   const int block = 256;

@@ -20,12 +20,13 @@ def _p(a, t):
     return a.ctypes.data_as(C.POINTER(t)) if a is not None else None
 
 
-def lineitem_flat(seed, n, row0=0, receipt=False):
+def lineitem_flat(seed, n, row0=0, receipt=False, empty=np.empty):
+    """`empty(n, dtype)` allocates each column (e.g. Engine.pinned_empty)."""
     lib = abi.load()
-    cols = dict(l_shipdate=np.empty(n, np.int32), l_quantity=np.empty(n, np.int32),
-                l_discount=np.empty(n, np.int32), l_tax=np.empty(n, np.int32),
-                l_extendedprice=np.empty(n, np.int64), l_returnflag=np.empty(n, np.uint8),
-                l_linestatus=np.empty(n, np.uint8))
+    cols = dict(l_shipdate=empty(n, np.int32), l_quantity=empty(n, np.int32),
+                l_discount=empty(n, np.int32), l_tax=empty(n, np.int32),
+                l_extendedprice=empty(n, np.int64), l_returnflag=empty(n, np.uint8),
+                l_linestatus=empty(n, np.uint8))
     rc = np.empty(n, np.int32) if receipt else None
     abi.check(lib.gdb_gen_lineitem_flat(
         seed, row0, n, _p(cols["l_shipdate"], C.c_int32), _p(cols["l_quantity"], C.c_int32),
@@ -45,11 +46,12 @@ def customer(seed, n):
     return c
 
 
-def orders(seed, n, ncustomers, comments=False, order0=0):
+def orders(seed, n, ncustomers, comments=False, order0=0, empty=np.empty):
+    """`empty(shape, dtype)` allocates each column (e.g. Engine.pinned_empty)."""
     lib = abi.load()
-    o = dict(o_orderkey=np.empty(n, np.int32), o_custkey=np.empty(n, np.int32),
-             o_orderdate=np.empty(n, np.int32), o_shippriority=np.empty(n, np.int32))
-    com = np.empty((n, 80), np.uint8) if comments else None
+    o = dict(o_orderkey=empty(n, np.int32), o_custkey=empty(n, np.int32),
+             o_orderdate=empty(n, np.int32), o_shippriority=empty(n, np.int32))
+    com = empty((n, 80), np.uint8) if comments else None
     abi.check(lib.gdb_gen_orders(seed, order0, n, ncustomers, _p(o["o_orderkey"], C.c_int32),
                                  _p(o["o_custkey"], C.c_int32), _p(o["o_orderdate"], C.c_int32),
                                  _p(o["o_shippriority"], C.c_int32), _p(com, C.c_uint8),
@@ -59,13 +61,14 @@ def orders(seed, n, ncustomers, comments=False, order0=0):
     return o
 
 
-def lineitem_joined(seed, norders, nparts, nsuppliers, order0=0):
+def lineitem_joined(seed, norders, nparts, nsuppliers, order0=0, empty=np.empty):
+    """`empty(n, dtype)` allocates each column (e.g. Engine.pinned_empty)."""
     lib = abi.load()
     n = lib.gdb_gen_lineitem_rows(seed, order0, norders)
-    li = dict(l_orderkey=np.empty(n, np.int32), l_partkey=np.empty(n, np.int32),
-              l_suppkey=np.empty(n, np.int32), l_quantity=np.empty(n, np.int32),
-              l_discount=np.empty(n, np.int32), l_extendedprice=np.empty(n, np.int64),
-              l_shipdate=np.empty(n, np.int32))
+    li = dict(l_orderkey=empty(n, np.int32), l_partkey=empty(n, np.int32),
+              l_suppkey=empty(n, np.int32), l_quantity=empty(n, np.int32),
+              l_discount=empty(n, np.int32), l_extendedprice=empty(n, np.int64),
+              l_shipdate=empty(n, np.int32))
     abi.check(lib.gdb_gen_lineitem_joined(
         seed, order0, norders, nparts, nsuppliers, _p(li["l_orderkey"], C.c_int32),
         _p(li["l_partkey"], C.c_int32), _p(li["l_suppkey"], C.c_int32),

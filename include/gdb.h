@@ -97,7 +97,28 @@ typedef struct {
   double post_ms;         /* host finalisation (sort, format) */
   uint64_t bytes;         /* algorithmic bytes read by the pipeline's scans */
   uint64_t main_kernel_bytes;
+  uint64_t h2d_bytes;     /* bytes copied host->device for a streamed table */
+  int32_t chunks;         /* chunks a streamed table was cut into (0: resident) */
+  int32_t pad;
 } gdb_timing;
+
+/* ---- host-resident tables streamed in chunks (replaces chunkedStreamLoop
+ *      and ChunkConfig, reference src/infra.h:37-57, :805-869, used by the
+ *      runQ*BenchmarkSF100 entry points, src/infra.h:874-922) ----------------
+ * A host table references caller memory (no copy; keep it alive until
+ * gdb_table_free). Passed as the fact table of gdb_q6/q1/q3/q9 or the orders
+ * table of gdb_q13, it is copied in chunks through two device slots per column
+ * on a copy stream while the previous chunk is processed; every other input
+ * must be resident. Memory from gdb_host_alloc is pinned, so copies run at
+ * full PCIe rate; pageable memory works but copies more slowly. */
+int gdb_host_alloc(gdb_ctx* ctx, uint64_t bytes, void** out);
+void gdb_host_free(gdb_ctx* ctx, void* p);
+int gdb_table_create_host(gdb_ctx* ctx, const gdb_col_desc* cols, int ncols, uint64_t nrows,
+                          gdb_table** out);
+int gdb_table_is_host(const gdb_table* t);
+/* Rows per streamed chunk; 0 = adaptive (1/8 of the HBM budget over two
+ * slots, clamped to 1M..64M rows; the reference's ChunkConfig rule). */
+int gdb_set_chunk_rows(gdb_ctx* ctx, uint64_t rows);
 
 /* ---- pipelines (drop-in for runQ6/Q1/Q3/Q9/Q13Benchmark,
  *      reference src/infra.h:874-922) --------------------------------------- */

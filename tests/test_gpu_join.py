@@ -1,0 +1,144 @@
+"""GPU parity of the join pipelines (Q3, Q9, Q13) and the join primitive
+against the sqlite3 goldens, hand-built edge cases, and the oracle at the
+benchmark configurations' full sizes. Integer results must be bit-exact."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from gpudbmetalbenchmark_amd import Timing, generate
+
+pytestmark = pytest.mark.gpu
+
+FULL = dict(seed=11, customers=1_500_000, orders=15_000_000, parts=2_000_000, suppliers=100_000)
+
+
+def np_cols(d):
+    out = {}
+    for k, v in d.items():
+        if k in ("l_extendedprice", "ps_supplycost"):
+            out[k] = np.array(v, np.int64)
+        elif k in ("c_mktsegment", "o_comment", "p_name"):
+            out[k] = np.array(v, np.uint8)
+        else:
+            out[k] = np.array(v, np.int32)
+    return out
+
+
+def upload(engine, js):
+    return {name: engine.table(cols) for name, cols in js.items()}
+
+
+@pytest.fixture(scope="module")
+def small(engine, joinset):
+    t = upload(engine, joinset)
+    yield t
+    for x in t.values():
+        x.free()
+
+
+@pytest.fixture(scope="module")
+def full(engine):
+    js = generate.join_set(FULL["seed"], FULL["customers"], FULL["orders"], FULL["parts"],
+                           FULL["suppliers"])
+    t = upload(engine, js)
+    yield js, t
+    for x in t.values():
+        x.free()
+
+
+# ------------------------------------------------------------------------ Q3
+def test_q3_golden(engine, golden, small):
+    for case in golden["q3"]:
+        r = engine.q3(small["customer"], small["orders"], small["lineitem"], case["segment"],
+                      case["date"], case["top_k"])
+        assert r["groups"] == case["groups"]
+        assert r["rows"] == case["rows"], (case["segment"], case["date"])
+
+
+def test_q3_edge_ties(engine, golden):
+    e = golden["edge"]["q3"]
+    c, o, li = (engine.table(np_cols(e[k])) for k in ("customer", "orders", "lineitem"))
+    for case in e["results"]:
+        r = engine.q3(c, o, li, case["segment"], case["date"], case["top_k"])
+        assert r["rows"] == case["rows"] and r["groups"] == case["groups"]
+
+
+def test_q3_full_config(engine, full):
+    js, t = full
+    tm = Timing()
+    for seg, date, k in ((1, 1169, 10), (3, 400, 100), (0, 2300, 7)):
+        got = engine.q3(t["customer"], t["orders"], t["lineitem"], seg, date, k, timing=tm)
+        exp = O.q3(js["customer"], js["orders"], js["lineitem"], seg, date, k, nthreads=16)
+        assert got == exp, (seg, date, k)
+    again = engine.q3(t["customer"], t["orders"], t["lineitem"], 3, 400, 100)
+    assert again == engine.q3(t["customer"], t["orders"], t["lineitem"], 3, 400, 100)
+    assert tm.main_kernel_ms > 0
+
+
+# ------------------------------------------------------------------------ Q9
+def test_q9_golden(engine, golden, small):
+    s = small
+    for case in golden["q9"]:
+        r = engine.q9(s["part"], s["supplier"], s["partsupp"], s["orders"], s["lineitem"],
+                      case["pattern"])
+        assert r == case["rows"], case["pattern"]
+
+
+def test_q9_full_config(engine, full):
+    js, t = full
+    for pat in ("green", "ivory"):
+        got = engine.q9(t["part"], t["supplier"], t["partsupp"], t["orders"], t["lineitem"], pat)
+        exp = O.q9(js["part"], js["supplier"], js["partsupp"], js["orders"], js["lineitem"], pat,
+                   nthreads=16)
+        assert got == exp, pat
+        assert len(got) == 25 * 7
+
+
+# ----------------------------------------------------------------------- Q13
+def test_q13_golden(engine, golden, small):
+    for case in golden["q13"]:
+        r = engine.q13(small["customer"], small["orders"], case["word1"], case["word2"])
+        assert r == case["rows"]
+
+
+def test_q13_edge_patterns(engine, golden):
+    e = golden["edge"]["q13"]
+    c, o = engine.table(np_cols(e["customer"])), engine.table(np_cols(e["orders"]))
+    for case in e["results"]:
+        assert engine.q13(c, o, case["word1"], case["word2"]) == case["rows"]
+
+
+def test_q13_full_config(engine, full):
+    js, t = full
+    got = engine.q13(t["customer"], t["orders"])
+    assert got == O.q13(js["customer"], js["orders"], nthreads=16)
+    assert sum(r["custdist"] for r in got) == FULL["customers"]
+
+
+def test_q13_random_fields_against_oracle(engine):
+    rng = np.random.default_rng(5)
+    n = 20000
+    alphabet = np.frombuffer(b"specialrqut ", np.uint8)
+    com = rng.choice(alphabet, (n, 80)).astype(np.uint8)
+    cut = rng.integers(0, 81, n)
+    com[np.arange(80)[None, :] >= cut[:, None]] = 0
+    cust = dict(c_custkey=np.arange(1, 301, dtype=np.int32))
+    od = dict(o_custkey=rng.integers(1, 320, n).astype(np.int32), o_comment=com)
+    c, o = engine.table(cust), engine.table(od)
+    for w1, w2 in (("spe", "qu"), ("a", "a"), ("l", "t"), ("ecia", "ue")):
+        assert engine.q13(c, o, w1, w2) == O.q13(cust, od, w1, w2), (w1, w2)
+
+
+# -------------------------------------------------------------- join primitive
+def test_join_count_golden(engine, golden, small):
+    for c in golden["primitives"]["join"]:
+        assert engine.join_count(small["orders"], c["build"], small["lineitem"],
+                                 c["probe"]) == c["matches"]
+
+
+def test_join_count_duplicates(engine):
+    rng = np.random.default_rng(9)
+    b = rng.integers(-5000, 5000, 30001).astype(np.int32)
+    p = rng.integers(-6000, 6000, 1_000_003).astype(np.int32)
+    tb, tp = engine.table({"k": b}), engine.table({"k": p})
+    assert engine.join_count(tb, "k", tp, "k") == O.join_count(b, p)
