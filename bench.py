@@ -1,15 +1,20 @@
-"""Benchmark: one step = one run of a pipeline over tables resident in HBM.
+"""Benchmark: one step = one run of a pipeline over one batch of synthetic input.
 
 Default workload: Q1 low-cardinality grouped aggregation over a 60,000,000-row
 synthetic fact table (7 columns, 26 B/row), the configuration the headline
 scan-bandwidth metric is quoted on. `value` is algorithmic bytes per second of
-whole-pipeline time (scan + finaliser + 37-value readback), inputs resident.
+whole-pipeline time (every kernel, the finaliser and the result readback),
+with the inputs resident in HBM when the timed region starts.
 
-  python bench.py [--steps K] [--warmup W] [--pipeline q1|q6|q3|q9|q13]
+  python bench.py [--steps K] [--warmup W]
+                  [--pipeline q1|q6|q3|q9|q13|select|sum|join|all] [--scale sf10|sf1|sf100]
 
-Prints one JSON line. `roofline` covers the dominant kernel, timed with
-hipEvents on the engine's stream inside each call; `cpu_baseline` times the
-CPU oracle on a bounded sample of the same workload on this host's cores.
+--scale sf100 streams the fact table (orders for Q13) from pinned host memory in
+chunks (the 600M-row variant); its `value` is then host-to-result throughput.
+Prints one JSON line per pipeline (one for the default run). `roofline` covers
+the pipeline's dominant kernel, timed with hipEvents on the engine stream inside
+every call; `cpu_baseline` times the CPU oracle (oracle/oracle.c, all host
+threads) on the same tables.
 """
 import argparse
 import json
@@ -22,14 +27,21 @@ sys.path.insert(0, ROOT)
 
 METRIC = "scan GB/s (% HBM peak) per pipeline; served QPS at p99, batched vs serial"
 HBM_PEAK_GBPS = 8000.0
+PCIE_PEAK_GBPS = 64.0  # PCIe Gen5 x16, one direction, theoretical
+
+# (customers, orders, parts, suppliers) of the joined tables per scale; the
+# fact table has 1..7 rows per order (~4x orders).
+JOIN_SCALE = {"sf1": (150_000, 1_500_000, 200_000, 10_000),
+              "sf10": (1_500_000, 15_000_000, 2_000_000, 100_000),
+              "sf100": (15_000_000, 150_000_000, 20_000_000, 1_000_000)}
+FLAT_ROWS = {"sf1": 6_000_000, "sf10": 60_000_000, "sf100": 600_000_000}
 
 
 def load_traffic(name):
-    """HBM bytes per launch of `name` measured by rocprofv3 --pmc (FETCH_SIZE +
-    WRITE_SIZE, scaled by the copy-kernel calibration), if a summary exists."""
-    p = os.path.join(ROOT, "profiles", "traffic.json")
+    """HBM bytes per launch of kernel `name` from rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration), if it was measured."""
     try:
-        with open(p) as f:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
             return json.load(f).get(name, {}).get("bytes_per_launch")
     except (OSError, ValueError):
         return None
@@ -40,93 +52,175 @@ def cpu_threads():
     return max(1, min(n or 1, 16))
 
 
-def run(args):
+class Workload:
+    """setup() uploads tables; step(timing) runs the pipeline once; oracle()
+    runs the CPU restatement once on the same data."""
+
+    def __init__(self, name, kernel, workload, rows, step, oracle, oracle_bytes):
+        self.name, self.kernel, self.workload, self.rows = name, kernel, workload, rows
+        self.step, self.oracle, self.oracle_bytes = step, oracle, oracle_bytes
+
+
+def build_workloads(eng, names, scale, O):
+    from gpudbmetalbenchmark_amd import generate
+    stream = scale == "sf100"
+    alloc = eng.pinned_empty if stream else None
+    th = cpu_threads()
+    out = {}
+    flat_needed = [n for n in names if n in ("q1", "q6")]
+    if flat_needed:
+        rows = FLAT_ROWS[scale]
+        li = generate.lineitem_flat(42, rows, **({"empty": alloc} if alloc else {}))
+        for name in flat_needed:
+            if name == "q1":
+                t = (eng.host_table if stream else eng.table)(
+                    {k: li[k] for k in ("l_shipdate", "l_quantity", "l_discount", "l_tax",
+                                        "l_extendedprice", "l_returnflag", "l_linestatus")})
+                out[name] = Workload(
+                    name, "q1_scan",
+                    f"Q1 grouped aggregation, {rows}-row synthetic fact table"
+                    f"{' streamed from pinned host memory' if stream else ''}, 6 groups x 8 aggregates",
+                    rows, lambda tm, t=t: eng.q1(t, 2435, timing=tm),
+                    lambda li=li: O.q1(li, 2435, nthreads=th), 26 * rows)
+            else:
+                t = (eng.host_table if stream else eng.table)(
+                    {k: li[k] for k in ("l_shipdate", "l_discount", "l_quantity",
+                                        "l_extendedprice")})
+                note = (" (120 MB: resident in the 256 MB Infinity Cache across steps)"
+                        if rows == FLAT_ROWS["sf1"] else "")
+                out[name] = Workload(
+                    name, "q6_scan",
+                    f"Q6 filter-scan-sum, {rows}-row synthetic fact table{note}"
+                    f"{' streamed from pinned host memory' if stream else ''}",
+                    rows, lambda tm, t=t: eng.q6(t, timing=tm),
+                    lambda li=li: O.q6(li, nthreads=th), 20 * rows)
+    join_needed = [n for n in names if n in ("q3", "q9", "q13", "select", "sum", "join")]
+    if join_needed:
+        nc, no, npart, ns = JOIN_SCALE[scale]
+        seed = 11
+        kw = {"empty": alloc} if alloc else {}
+        js = dict(customer=generate.customer(seed, nc),
+                  orders=generate.orders(seed, no, nc, comments="q13" in names, **kw),
+                  lineitem=generate.lineitem_joined(seed, no, npart, ns, **kw),
+                  part=generate.part(seed, npart), partsupp=generate.partsupp(seed, npart, ns),
+                  supplier=generate.supplier(seed, ns))
+        nl = len(js["lineitem"]["l_orderkey"])
+        dims = {k: eng.table(js[k]) for k in ("customer", "part", "partsupp", "supplier")}
+        orders_res = eng.table({k: v for k, v in js["orders"].items() if k != "o_comment"})
+        fact = (eng.host_table if stream else eng.table)(js["lineitem"])
+        if "q3" in names:
+            out["q3"] = Workload(
+                "q3", "q3_probe",
+                f"Q3 join+group+top-10: customer {nc}, orders {no}, fact {nl} rows"
+                f"{' (fact streamed from pinned host memory)' if stream else ''}",
+                nl, lambda tm: eng.q3(dims["customer"], orders_res, fact, 1, 1169, 10, timing=tm),
+                lambda: O.q3(js["customer"], js["orders"], js["lineitem"], 1, 1169, 10, nthreads=th),
+                5 * nc + 16 * no + 20 * nl)
+        if "q9" in names:
+            out["q9"] = Workload(
+                "q9", "q9_probe",
+                f"Q9 multi-way join, part {npart} ('green'), partsupp {4 * npart}, supplier {ns}, "
+                f"orders {no}, fact {nl} rows"
+                f"{' (fact streamed from pinned host memory)' if stream else ''}",
+                nl, lambda tm: eng.q9(dims["part"], dims["supplier"], dims["partsupp"], orders_res,
+                                      fact, "green", timing=tm),
+                lambda: O.q9(js["part"], js["supplier"], js["partsupp"], js["orders"],
+                             js["lineitem"], "green", nthreads=th),
+                28 * nl + 59 * npart + 16 * 4 * npart + 8 * ns + 8 * no)
+        if "q13" in names:
+            otab = (eng.host_table if stream else eng.table)(
+                {k: js["orders"][k] for k in ("o_custkey", "o_comment")})
+            out["q13"] = Workload(
+                "q13", "q13_count",
+                f"Q13 left-outer count: orders {no} (80-byte comments), customer {nc}"
+                f"{' (orders streamed from pinned host memory)' if stream else ''}",
+                no, lambda tm: eng.q13(dims["customer"], otab, timing=tm),
+                lambda: O.q13(js["customer"], js["orders"], nthreads=th), 84 * no + 4 * nc)
+        if not stream:
+            li = js["lineitem"]
+            if "select" in names:
+                out["select"] = Workload(
+                    "select", "select_lt_bitmap",
+                    f"selection l_partkey < {npart // 2} over {nl} rows (device bitmap + count)", nl,
+                    lambda tm: eng.select_lt(fact, "l_partkey", npart // 2, bitmap=False,
+                                             rowids=False, timing=tm),
+                    lambda: O.select_lt(li["l_partkey"], npart // 2), 4 * nl)
+            if "sum" in names:
+                out["sum"] = Workload(
+                    "sum", "sum_scan", f"SUM(l_quantity) over {nl} rows", nl,
+                    lambda tm: eng.sum(fact, "l_quantity", timing=tm),
+                    lambda: O.sum_col(li["l_quantity"]), 4 * nl)
+            if "join" in names:
+                out["join"] = Workload(
+                    "join", "join_probe", f"join count orders({no}) x lineitem({nl}) on orderkey",
+                    nl, lambda tm: eng.join_count(orders_res, "o_orderkey", fact, "l_orderkey",
+                                                  timing=tm),
+                    lambda: O.join_count(js["orders"]["o_orderkey"], li["l_orderkey"]),
+                    4 * no + 4 * nl)
+    return out
+
+
+def run_one(w, args, info, O):
     import numpy as np
-    from gpudbmetalbenchmark_amd import Engine, Timing, generate
-
-    eng = Engine(0)
-    info = eng.device_info()
-    pipe = args.pipeline
-    tables = {}
-    if pipe in ("q1", "q6"):
-        n = args.rows or (60_000_000)
-        li = generate.lineitem_flat(42, n)
-        names = (("l_shipdate", "l_quantity", "l_discount", "l_tax", "l_extendedprice",
-                  "l_returnflag", "l_linestatus") if pipe == "q1" else
-                 ("l_shipdate", "l_discount", "l_quantity", "l_extendedprice"))
-        tables["li"] = eng.table({k: li[k] for k in names})
-        host = li
-        if pipe == "q1":
-            step = lambda tm: eng.q1(tables["li"], 2435, timing=tm)  # noqa: E731
-            kernel = "q1_scan"
-            workload = f"Q1 grouped aggregation, {n}-row synthetic fact table, 6 groups x 8 aggregates"
-        else:
-            step = lambda tm: eng.q6(tables["li"], timing=tm)  # noqa: E731
-            kernel = "q6_scan"
-            workload = f"Q6 filter-scan-sum, {n}-row synthetic fact table"
-        rows = n
-    else:
-        raise SystemExit(f"unknown pipeline {pipe}")
-
+    from gpudbmetalbenchmark_amd import Timing
     tm = Timing()
     for _ in range(args.warmup):
-        step(tm)
-    kernel_ms = []
-    gpu_bytes = 0
+        w.step(tm)
+    kernel_ms, h2d = [], 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step(tm)
+        w.step(tm)
         kernel_ms.append(tm.main_kernel_ms)
-        gpu_bytes = tm.bytes
+        h2d = tm.h2d_bytes
     elapsed = time.perf_counter() - t0
-    ms_per_step = elapsed * 1000.0 / args.steps
-    value = gpu_bytes * args.steps / elapsed / 1e9
+    bytes_step = tm.bytes
+    value = bytes_step * args.steps / elapsed / 1e9
     kavg = float(np.mean(kernel_ms))
-    achieved = tm.main_kernel_bytes / (kavg / 1000.0) / 1e9
-
+    achieved = tm.main_kernel_bytes / (kavg / 1000.0) / 1e9 if kavg > 0 else 0.0
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "GB/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64",
-        "data": "synthetic (seeded generator, seed 42)",
-        "config": {"workload": workload, "pipeline": pipe, "rows": rows,
-                   "bytes_per_step": gpu_bytes, "device": info.name.decode(),
-                   "cu_count": info.cu_count, "pct_hbm_peak": round(100 * value / HBM_PEAK_GBPS, 2)},
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1000.0 / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int64",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": w.workload, "pipeline": w.name, "scale": args.scale,
+                   "rows": w.rows, "bytes_per_step": bytes_step,
+                   "device": (info.name or info.arch).decode(errors="replace"),
+                   "cu_count": info.cu_count,
+                   "pct_hbm_peak": round(100 * value / HBM_PEAK_GBPS, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                     "traffic": load_traffic(kernel), "kernel": kernel,
-                     "kernel_ms_avg": round(kavg, 5)},
+                     "traffic": load_traffic(w.kernel), "kernel": w.kernel,
+                     "kernel_ms_avg": round(kavg, 5),
+                     "bytes_per_launch": tm.main_kernel_bytes},
     }
+    if tm.chunks:
+        out["streaming"] = {"chunks": tm.chunks, "h2d_bytes": h2d,
+                            "h2d_gbps": round(h2d * args.steps / elapsed / 1e9, 2),
+                            "pcie_peak_gbps": PCIE_PEAK_GBPS,
+                            "pcie_frac": round(h2d * args.steps / elapsed / 1e9 / PCIE_PEAK_GBPS, 4)}
+    if args.hbm_probe:
+        out["hbm_probe"] = args.hbm_probe
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(pipe, host, args.cpu_seconds)
-    for t in tables.values():
-        t.free()
-    eng.close()
-    print(json.dumps(out), flush=True)
+        out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
+    return out
 
 
-def cpu_baseline(pipe, li, seconds):
-    """The oracle over a bounded row sample of the same table, all host threads,
-    repeated for about `seconds`; reported in the metric's unit (GB/s)."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle as O
+def cpu_baseline(w, seconds):
+    """The oracle over the same tables, all host threads (at most 16), repeated
+    for about `seconds`; reported in the metric's unit (GB/s)."""
     threads = cpu_threads()
-    sample = min(len(li["l_shipdate"]), 20_000_000)
-    part = {k: v[:sample] for k, v in li.items()}
-    per_row = 26 if pipe == "q1" else 20
-    fn = (lambda: O.q1(part, 2435, nthreads=threads)) if pipe == "q1" else \
-        (lambda: O.q6(part, nthreads=threads))
-    fn()
+    w.oracle()
     reps, t0 = 0, time.perf_counter()
     while True:
-        fn()
+        w.oracle()
         reps += 1
         el = time.perf_counter() - t0
         if el >= seconds:
             break
-    return {"value": round(sample * per_row * reps / el / 1e9, 3), "unit": "GB/s",
+    return {"value": round(w.oracle_bytes * reps / el / 1e9, 3), "unit": "GB/s",
             "cores": threads, "kind": "port",
-            "sample": f"first {sample} rows of the same table, {reps} runs in {el:.1f} s "
+            "sample": f"the full {w.rows}-row workload, {reps} runs in {el:.1f} s "
                       f"(oracle/oracle.c, {threads} threads)"}
 
 
@@ -136,15 +230,32 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--pipeline", default="q1")
-    ap.add_argument("--rows", type=int, default=0)
+    ap.add_argument("--scale", default="sf10", choices=["sf1", "sf10", "sf100"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe", action="store_true", help="also measure plain read/copy GB/s")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
-    run(args)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    from gpudbmetalbenchmark_amd import Engine
+    names = (["q1", "q6", "q3", "q9", "q13", "select", "sum", "join"]
+             if args.pipeline == "all" else args.pipeline.split(","))
+    if args.scale == "sf100":
+        names = [n for n in names if n in ("q1", "q6", "q3", "q9", "q13")]
+    eng = Engine(0)
+    info = eng.device_info()
+    args.hbm_probe = None
+    if args.probe:
+        r, c = eng.bandwidth_probe(1 << 31, 10)
+        args.hbm_probe = {"read_gbps": round(r, 1), "copy_gbps": round(c, 1)}
+    wl = build_workloads(eng, names, args.scale, O)
+    for n in names:
+        print(json.dumps(run_one(wl[n], args, info, O)), flush=True)
+    eng.close()
 
 
 if __name__ == "__main__":

@@ -8,6 +8,7 @@ or no GPU is visible, calls raise instead of computing anything on the CPU.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -64,6 +65,7 @@ class Table:
         self.host = host
         create = lib().gdb_table_create_host if host else lib().gdb_table_create
         abi.check(create(engine._ctx, descs, len(columns), self.nrows, C.byref(self._t)))
+        engine._tables.add(self)
         if not host:
             self._keep = []
 
@@ -72,9 +74,11 @@ class Table:
         return self._t
 
     def free(self):
-        if self._t:
+        """Releases the table; a no-op once freed or once its engine is closed
+        (gdb_close releases the context the table belongs to)."""
+        if self._t and self.engine._ctx:
             lib().gdb_table_free(self._t)
-            self._t = C.c_void_p()
+        self._t = C.c_void_p()
 
     def __del__(self):
         try:
@@ -89,10 +93,13 @@ class Engine:
     def __init__(self, device=0, hbm_budget=0):
         self._ctx = C.c_void_p()
         self._pinned = []
+        self._tables = weakref.WeakSet()
         abi.check(lib().gdb_open(device, hbm_budget, C.byref(self._ctx)))
 
     def close(self):
         if self._ctx:
+            for t in list(self._tables):
+                t.free()
             for p in self._pinned:
                 lib().gdb_host_free(self._ctx, p)
             self._pinned = []
@@ -128,6 +135,33 @@ class Engine:
 
     def hbm_used(self):
         return lib().gdb_hbm_used(self._ctx)
+
+    def autotune(self, pipeline, tables, reps=3):
+        """Times the candidate launch shapes of `pipeline` ('q6', 'q1', 'q3',
+        'q9', 'q13') on `tables` (argument order of the pipeline) and keeps the
+        fastest for this engine; returns it as a dict."""
+        arr = (C.c_void_p * len(tables))(*[t.handle.value for t in tables])
+        out = abi.Tuning()
+        abi.check(lib().gdb_autotune(self._ctx, abi.PIPE_IDS[pipeline], arr, len(tables), reps,
+                                     C.byref(out)))
+        return dict(pipeline=pipeline, block=out.block, blocks_per_cu=out.blocks_per_cu,
+                    unroll=out.unroll, ms=out.ms, candidates=out.candidates)
+
+    def get_tuning(self, pipeline):
+        out = abi.Tuning()
+        abi.check(lib().gdb_get_tuning(self._ctx, abi.PIPE_IDS[pipeline], C.byref(out)))
+        return dict(pipeline=pipeline, block=out.block, blocks_per_cu=out.blocks_per_cu,
+                    unroll=out.unroll)
+
+    def set_tuning(self, pipeline, block, blocks_per_cu, unroll):
+        t = abi.Tuning(abi.PIPE_IDS[pipeline], block, blocks_per_cu, unroll, 0.0, 0, 0)
+        abi.check(lib().gdb_set_tuning(self._ctx, C.byref(t)))
+
+    def bandwidth_probe(self, nbytes=1 << 30, reps=10):
+        """(read GB/s, copy GB/s) of plain streaming kernels over nbytes."""
+        r, c = C.c_double(), C.c_double()
+        abi.check(lib().gdb_bandwidth_probe(self._ctx, nbytes, reps, C.byref(r), C.byref(c)))
+        return r.value, c.value
 
     def table(self, columns):
         return Table(self, columns)

@@ -97,6 +97,15 @@ class Q13Result(C.Structure):
     _fields_ = [("nrows", C.c_int32), ("rows", Q13Row * 512)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("pipeline", C.c_int32), ("block", C.c_int32), ("blocks_per_cu", C.c_int32),
+                ("unroll", C.c_int32), ("ms", C.c_double), ("candidates", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+PIPE_IDS = {"q6": 0, "q1": 1, "q3": 4, "q9": 5, "q13": 6}
+
+
 def q1_row_dict(r):
     charge = (int(r.sum_charge_hi) << 64) | int(r.sum_charge_lo)
     return dict(returnflag=r.returnflag.decode(), linestatus=r.linestatus.decode(),
@@ -118,6 +127,8 @@ PROTOTYPES = {
     "gdb_device_info_get": (C.c_int, [P, C.POINTER(DeviceInfo)]),
     "gdb_hbm_used": (C.c_uint64, [P]),
     "gdb_debug_fail_allocs": (None, [P, C.c_int]),
+    "gdb_bandwidth_probe": (C.c_int, [P, C.c_uint64, C.c_int, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double)]),
     "gdb_table_create": (C.c_int, [P, C.POINTER(ColDesc), C.c_int, C.c_uint64,
                                    C.POINTER(C.c_void_p)]),
     "gdb_table_free": (None, [P]),
@@ -141,6 +152,10 @@ PROTOTYPES = {
                                 C.POINTER(Timing)]),
     "gdb_sum": (C.c_int, [P, P, C.c_char_p, I64P, C.POINTER(Timing)]),
     "gdb_join_count": (C.c_int, [P, P, C.c_char_p, P, C.c_char_p, U64P, C.POINTER(Timing)]),
+    "gdb_autotune": (C.c_int, [P, C.c_int, C.POINTER(C.c_void_p), C.c_int, C.c_int,
+                               C.POINTER(Tuning)]),
+    "gdb_get_tuning": (C.c_int, [P, C.c_int, C.POINTER(Tuning)]),
+    "gdb_set_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
     "gdb_gen_lineitem_flat": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, I32P, I32P, I32P,
                                         I32P, I64P, U8P, U8P, I32P, C.c_int]),
     "gdb_gen_customer": (C.c_int, [C.c_uint64, C.c_uint64, I32P, U8P, C.c_int]),

@@ -1,0 +1,431 @@
+// gdb_bench — the C++ host driver: `gdb_bench [sf1|sf10|sf100] [query|all]`,
+// the CLI of the reference's src/main.cpp:48-72 with its name registry
+// (src/main.cpp:158-203), running the pipelines through libgdb.so.
+//
+// Scales: sf1 = 6M fact rows, sf10 = 60M, sf100 = 600M streamed from pinned host
+// memory (the reference's runQ*BenchmarkSF100 entry points). Tables come from
+// the seeded generator instead of dbgen .tbl files. Output keeps the reference's
+// contract parsed by scripts/benchmark_gpu.sh:54-66: a `Qn | N rows` header and
+// the four printTimingSummary lines (src/infra.h:511-516); "CPU Parsing (.tbl)"
+// reports the generation + upload time. Each pipeline runs 2 warm-ups and 10
+// timed runs; the median GPU time is reported (the reference times one run).
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "gdb.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+double ms_since(Clock::time_point t) {
+  return std::chrono::duration<double, std::milli>(Clock::now() - t).count();
+}
+
+struct Scale {
+  std::string name;
+  uint64_t flat_rows, customers, orders, parts, suppliers;
+  bool stream;
+};
+
+gdb_ctx* g_ctx = nullptr;
+
+void die(int rc, const char* what) {
+  fprintf(stderr, "%s failed: %s (%s)\n", what, gdb_strerror(rc), gdb_last_error());
+  exit(1);
+}
+#define CHECK(x)                     \
+  do {                               \
+    int rc_ = (x);                   \
+    if (rc_ != GDB_OK) die(rc_, #x); \
+  } while (0)
+
+// Host column storage: pinned for streamed scales, plain heap otherwise.
+struct Buf {
+  void* p = nullptr;
+  bool pinned = false;
+  ~Buf() {
+    if (pinned) gdb_host_free(g_ctx, p);
+    else free(p);
+  }
+};
+std::vector<Buf*> g_bufs;
+template <typename T>
+T* alloc(uint64_t n, bool pin) {
+  Buf* b = new Buf();
+  b->pinned = pin;
+  if (pin) CHECK(gdb_host_alloc(g_ctx, n * sizeof(T) + 64, &b->p));
+  else b->p = malloc(n * sizeof(T) + 64);
+  if (!b->p) die(GDB_E_NOMEM, "malloc");
+  g_bufs.push_back(b);
+  return (T*)b->p;
+}
+
+gdb_table* make_table(const std::vector<gdb_col_desc>& cols, uint64_t n, bool host) {
+  gdb_table* t = nullptr;
+  if (host) CHECK(gdb_table_create_host(g_ctx, cols.data(), (int)cols.size(), n, &t));
+  else CHECK(gdb_table_create(g_ctx, cols.data(), (int)cols.size(), n, &t));
+  return t;
+}
+
+void print_timing(const char* q, uint64_t rows, double parse_ms, double gpu_ms, double post_ms,
+                  uint64_t bytes) {
+  printf("\n%s | %llu rows\n", q, (unsigned long long)rows);
+  printf("  CPU Parsing (.tbl): %10.2f ms\n", parse_ms);
+  printf("  GPU Execution:      %10.2f ms\n", gpu_ms);
+  printf("  CPU Post Process:   %10.2f ms\n", post_ms);
+  printf("  Total Execution:    %10.2f ms  (GPU + CPU post)\n", gpu_ms + post_ms);
+  printf("  Bandwidth:          %10.2f GB/s\n", bytes / (gpu_ms / 1000.0) / 1e9);
+}
+
+// Runs fn 2 + 10 times; returns the median timing of the timed runs.
+gdb_timing timed(const std::function<void(gdb_timing*)>& fn) {
+  std::vector<gdb_timing> ts;
+  for (int i = 0; i < 12; ++i) {
+    gdb_timing t{};
+    fn(&t);
+    if (i >= 2) ts.push_back(t);
+  }
+  std::sort(ts.begin(), ts.end(), [](const gdb_timing& a, const gdb_timing& b) { return a.gpu_ms < b.gpu_ms; });
+  return ts[ts.size() / 2];
+}
+
+std::string money4(int64_t v) {  // value in units of 1e-4
+  char buf[64];
+  const bool neg = v < 0;
+  const uint64_t a = neg ? (uint64_t)(-(v + 1)) + 1 : (uint64_t)v;
+  snprintf(buf, sizeof buf, "%s%llu.%04llu", neg ? "-" : "", (unsigned long long)(a / 10000),
+           (unsigned long long)(a % 10000));
+  return buf;
+}
+
+// ------------------------------------------------------------------ flat
+struct Flat {
+  uint64_t n;
+  int32_t *ship, *qty, *disc, *tax;
+  int64_t* price;
+  uint8_t *rf, *ls;
+  double gen_ms;
+};
+Flat make_flat(const Scale& s) {
+  Flat f{};
+  f.n = s.flat_rows;
+  auto t0 = Clock::now();
+  f.ship = alloc<int32_t>(f.n, s.stream);
+  f.qty = alloc<int32_t>(f.n, s.stream);
+  f.disc = alloc<int32_t>(f.n, s.stream);
+  f.tax = alloc<int32_t>(f.n, s.stream);
+  f.price = alloc<int64_t>(f.n, s.stream);
+  f.rf = alloc<uint8_t>(f.n, s.stream);
+  f.ls = alloc<uint8_t>(f.n, s.stream);
+  CHECK(gdb_gen_lineitem_flat(42, 0, f.n, f.ship, f.qty, f.disc, f.tax, f.price, f.rf, f.ls,
+                              nullptr, 0));
+  f.gen_ms = ms_since(t0);
+  return f;
+}
+
+void run_q6(const Scale& s, Flat& f) {
+  printf("--- Running TPC-H Query 6 Benchmark ---\n");
+  auto t0 = Clock::now();
+  gdb_table* t = make_table({{"l_shipdate", GDB_I32, 0, f.ship}, {"l_discount", GDB_I32, 0, f.disc},
+                             {"l_quantity", GDB_I32, 0, f.qty},
+                             {"l_extendedprice", GDB_I64, 0, f.price}},
+                            f.n, s.stream);
+  const double load = f.gen_ms + ms_since(t0);
+  gdb_q6_params p{731, 1096, 5, 7, 24};
+  gdb_q6_result r{};
+  gdb_timing tm = timed([&](gdb_timing* x) { CHECK(gdb_q6(g_ctx, t, &p, &r, x)); });
+  printf("TPC-H Query 6 Result:\nTotal Revenue: $%s\n", money4(r.revenue).c_str());
+  print_timing(s.stream ? "SF100 Q6" : "Q6", f.n, load, tm.gpu_ms, tm.post_ms, tm.bytes);
+  gdb_table_free(t);
+}
+
+void run_q1(const Scale& s, Flat& f) {
+  printf("--- Running TPC-H Query 1 Benchmark ---\n");
+  auto t0 = Clock::now();
+  gdb_table* t = make_table(
+      {{"l_shipdate", GDB_I32, 0, f.ship}, {"l_quantity", GDB_I32, 0, f.qty},
+       {"l_discount", GDB_I32, 0, f.disc}, {"l_tax", GDB_I32, 0, f.tax},
+       {"l_extendedprice", GDB_I64, 0, f.price}, {"l_returnflag", GDB_U8, 0, f.rf},
+       {"l_linestatus", GDB_U8, 0, f.ls}},
+      f.n, s.stream);
+  const double load = f.gen_ms + ms_since(t0);
+  gdb_q1_params p{2435};
+  gdb_q1_result r{};
+  gdb_timing tm = timed([&](gdb_timing* x) { CHECK(gdb_q1(g_ctx, t, &p, &r, x)); });
+  printf("TPC-H Query 1 Results:\n");
+  printf("| rf | ls |      sum_qty |   sum_base_price |     sum_disc_price |          sum_charge |"
+         " avg_qty | avg_price | avg_disc |    count |\n");
+  for (int i = 0; i < r.nrows; ++i) {
+    const gdb_q1_row& q = r.rows[i];
+    const __int128 ch = ((__int128)q.sum_charge_hi << 64) | q.sum_charge_lo;
+    const long double charge = (long double)ch / 1e6L;
+    printf("| %c  | %c  | %12lld | %16.2f | %18.4f | %19.6Lf | %7.2f | %9.2f | %8.2f | %8lld |\n",
+           q.returnflag, q.linestatus, (long long)q.sum_qty, q.sum_base_price / 100.0,
+           q.sum_disc_price / 1e4, charge, q.avg_qty, q.avg_price, q.avg_disc,
+           (long long)q.count);
+  }
+  print_timing(s.stream ? "SF100 Q1" : "Q1", f.n, load, tm.gpu_ms, tm.post_ms, tm.bytes);
+  gdb_table_free(t);
+}
+
+// ------------------------------------------------------------------ joined set
+struct Joined {
+  uint64_t nc, no, nl, np, ns;
+  int32_t *c_custkey, *o_orderkey, *o_custkey, *o_orderdate, *o_prio;
+  uint8_t *c_seg, *o_comment;
+  int32_t *l_ok, *l_pk, *l_sk, *l_qty, *l_disc, *l_ship;
+  int64_t* l_price;
+  int32_t *p_partkey, *ps_pk, *ps_sk, *s_sk, *s_nat;
+  uint8_t* p_name;
+  int64_t* ps_cost;
+  double gen_ms;
+};
+Joined make_joined(const Scale& s, bool comments) {
+  Joined j{};
+  j.nc = s.customers, j.no = s.orders, j.np = s.parts, j.ns = s.suppliers;
+  auto t0 = Clock::now();
+  const uint64_t seed = 11;
+  j.c_custkey = alloc<int32_t>(j.nc, false);
+  j.c_seg = alloc<uint8_t>(j.nc, false);
+  CHECK(gdb_gen_customer(seed, j.nc, j.c_custkey, j.c_seg, 0));
+  j.o_orderkey = alloc<int32_t>(j.no, false);
+  j.o_custkey = alloc<int32_t>(j.no, s.stream);
+  j.o_orderdate = alloc<int32_t>(j.no, false);
+  j.o_prio = alloc<int32_t>(j.no, false);
+  j.o_comment = comments ? alloc<uint8_t>(j.no * 80, s.stream) : nullptr;
+  CHECK(gdb_gen_orders(seed, 0, j.no, j.nc, j.o_orderkey, j.o_custkey, j.o_orderdate, j.o_prio,
+                       j.o_comment, 0));
+  j.nl = gdb_gen_lineitem_rows(seed, 0, j.no);
+  j.l_ok = alloc<int32_t>(j.nl, s.stream);
+  j.l_pk = alloc<int32_t>(j.nl, s.stream);
+  j.l_sk = alloc<int32_t>(j.nl, s.stream);
+  j.l_qty = alloc<int32_t>(j.nl, s.stream);
+  j.l_disc = alloc<int32_t>(j.nl, s.stream);
+  j.l_price = alloc<int64_t>(j.nl, s.stream);
+  j.l_ship = alloc<int32_t>(j.nl, s.stream);
+  CHECK(gdb_gen_lineitem_joined(seed, 0, j.no, j.np, j.ns, j.l_ok, j.l_pk, j.l_sk, j.l_qty,
+                                j.l_disc, j.l_price, j.l_ship, 0));
+  j.p_partkey = alloc<int32_t>(j.np, false);
+  j.p_name = alloc<uint8_t>(j.np * 55, false);
+  CHECK(gdb_gen_part(seed, j.np, j.p_partkey, j.p_name, 0));
+  j.ps_pk = alloc<int32_t>(j.np * 4, false);
+  j.ps_sk = alloc<int32_t>(j.np * 4, false);
+  j.ps_cost = alloc<int64_t>(j.np * 4, false);
+  CHECK(gdb_gen_partsupp(seed, j.np, j.ns, j.ps_pk, j.ps_sk, j.ps_cost, 0));
+  j.s_sk = alloc<int32_t>(j.ns, false);
+  j.s_nat = alloc<int32_t>(j.ns, false);
+  CHECK(gdb_gen_supplier(seed, j.ns, j.s_sk, j.s_nat, 0));
+  j.gen_ms = ms_since(t0);
+  return j;
+}
+
+void run_q3(const Scale& s, Joined& j) {
+  printf("--- Running TPC-H Query 3 Benchmark ---\n");
+  auto t0 = Clock::now();
+  gdb_table* c = make_table({{"c_custkey", GDB_I32, 0, j.c_custkey},
+                             {"c_mktsegment", GDB_U8, 0, j.c_seg}}, j.nc, false);
+  gdb_table* o = make_table({{"o_orderkey", GDB_I32, 0, j.o_orderkey},
+                             {"o_custkey", GDB_I32, 0, j.o_custkey},
+                             {"o_orderdate", GDB_I32, 0, j.o_orderdate},
+                             {"o_shippriority", GDB_I32, 0, j.o_prio}}, j.no, false);
+  gdb_table* l = make_table({{"l_orderkey", GDB_I32, 0, j.l_ok}, {"l_shipdate", GDB_I32, 0, j.l_ship},
+                             {"l_discount", GDB_I32, 0, j.l_disc},
+                             {"l_extendedprice", GDB_I64, 0, j.l_price}}, j.nl, s.stream);
+  const double load = j.gen_ms + ms_since(t0);
+  gdb_q3_params p{1, 1169, 10};
+  gdb_q3_result r{};
+  gdb_timing tm = timed([&](gdb_timing* x) { CHECK(gdb_q3(g_ctx, c, o, l, &p, &r, x)); });
+  printf("\nTPC-H Q3 Results (Top 10):\n");
+  printf("+----------+------------------+------------+--------------+\n");
+  printf("| orderkey |          revenue | orderdate  | shippriority |\n");
+  printf("+----------+------------------+------------+--------------+\n");
+  for (int i = 0; i < r.nrows; ++i)
+    printf("| %8d | %16s | %10d | %12d |\n", r.rows[i].orderkey, money4(r.rows[i].revenue).c_str(),
+           r.rows[i].orderdate, r.rows[i].shippriority);
+  printf("+----------+------------------+------------+--------------+\n");
+  printf("Total results found: %lld\n", (long long)r.groups);
+  print_timing(s.stream ? "SF100 Q3" : "Q3", j.nl, load, tm.gpu_ms, tm.post_ms, tm.bytes);
+  gdb_table_free(c);
+  gdb_table_free(o);
+  gdb_table_free(l);
+}
+
+void run_q9(const Scale& s, Joined& j) {
+  printf("--- Running TPC-H Query 9 Benchmark ---\n");
+  auto t0 = Clock::now();
+  gdb_table* pa = make_table({{"p_partkey", GDB_I32, 0, j.p_partkey},
+                              {"p_name", GDB_BYTES, 55, j.p_name}}, j.np, false);
+  gdb_table* su = make_table({{"s_suppkey", GDB_I32, 0, j.s_sk},
+                              {"s_nationkey", GDB_I32, 0, j.s_nat}}, j.ns, false);
+  gdb_table* ps = make_table({{"ps_partkey", GDB_I32, 0, j.ps_pk}, {"ps_suppkey", GDB_I32, 0, j.ps_sk},
+                              {"ps_supplycost", GDB_I64, 0, j.ps_cost}}, j.np * 4, false);
+  gdb_table* o = make_table({{"o_orderkey", GDB_I32, 0, j.o_orderkey},
+                             {"o_orderdate", GDB_I32, 0, j.o_orderdate}}, j.no, false);
+  gdb_table* l = make_table({{"l_partkey", GDB_I32, 0, j.l_pk}, {"l_suppkey", GDB_I32, 0, j.l_sk},
+                             {"l_orderkey", GDB_I32, 0, j.l_ok}, {"l_quantity", GDB_I32, 0, j.l_qty},
+                             {"l_discount", GDB_I32, 0, j.l_disc},
+                             {"l_extendedprice", GDB_I64, 0, j.l_price}}, j.nl, s.stream);
+  const double load = j.gen_ms + ms_since(t0);
+  gdb_q9_params p{};
+  strcpy(p.pattern, "green");
+  static gdb_q9_result r;
+  gdb_timing tm = timed([&](gdb_timing* x) { CHECK(gdb_q9(g_ctx, pa, su, ps, o, l, &p, &r, x)); });
+  printf("\nTPC-H Query 9 Results (Top 15):\n");
+  printf("+----------------+------+-------------------+\n");
+  printf("| Nation         | Year |            Profit |\n");
+  printf("+----------------+------+-------------------+\n");
+  for (int i = 0; i < r.nrows && i < 15; ++i)
+    printf("| %-14s | %4d | %17s |\n", gdb_nation_name(r.rows[i].nationkey), r.rows[i].year,
+           money4(r.rows[i].profit).c_str());
+  printf("+----------------+------+-------------------+\n");
+  printf("Total results found: %d\n", r.nrows);
+  print_timing(s.stream ? "SF100 Q9" : "Q9", j.nl, load, tm.gpu_ms, tm.post_ms, tm.bytes);
+  gdb_table_free(pa);
+  gdb_table_free(su);
+  gdb_table_free(ps);
+  gdb_table_free(o);
+  gdb_table_free(l);
+}
+
+void run_q13(const Scale& s, Joined& j) {
+  printf("--- Running TPC-H Query 13 Benchmark ---\n");
+  auto t0 = Clock::now();
+  gdb_table* c = make_table({{"c_custkey", GDB_I32, 0, j.c_custkey}}, j.nc, false);
+  gdb_table* o = make_table({{"o_custkey", GDB_I32, 0, j.o_custkey},
+                             {"o_comment", GDB_BYTES, 80, j.o_comment}}, j.no, s.stream);
+  const double load = j.gen_ms + ms_since(t0);
+  gdb_q13_params p{};
+  strcpy(p.word1, "special");
+  strcpy(p.word2, "requests");
+  static gdb_q13_result r;
+  gdb_timing tm = timed([&](gdb_timing* x) { CHECK(gdb_q13(g_ctx, c, o, &p, &r, x)); });
+  printf("\nTPC-H Query 13 Results:\n+---------+----------+\n| c_count | custdist |\n+---------+----------+\n");
+  for (int i = 0; i < r.nrows; ++i) printf("| %7d | %8lld |\n", r.rows[i].c_count, (long long)r.rows[i].custdist);
+  printf("+---------+----------+\n");
+  print_timing(s.stream ? "SF100 Q13" : "Q13", j.no, load, tm.gpu_ms, tm.post_ms, tm.bytes);
+  gdb_table_free(c);
+  gdb_table_free(o);
+}
+
+// ------------------------------------------------------------------ primitives
+void run_selection(const Scale&, Joined& j) {
+  printf("--- Running Selection Benchmark ---\n");
+  gdb_table* t = make_table({{"l_partkey", GDB_I32, 0, j.l_pk}}, j.nl, false);
+  for (int32_t v : {1000, 10000, 50000}) {
+    uint64_t cnt = 0;
+    gdb_timing tm = timed([&](gdb_timing* x) { CHECK(gdb_select_lt(g_ctx, t, "l_partkey", v, nullptr, nullptr, &cnt, x)); });
+    printf("--- Filter Value: < %d ---\n", v);
+    printf("Selectivity: %g%% (%llu rows matched)\n", 100.0 * cnt / std::max<uint64_t>(1, j.nl),
+           (unsigned long long)cnt);
+    printf("GPU execution time: %g ms\n", tm.gpu_ms);
+    printf("Effective Bandwidth: %g GB/s\n\n", tm.bytes / (tm.main_kernel_ms / 1000.0) / 1e9);
+  }
+  gdb_table_free(t);
+}
+
+void run_aggregation(const Scale&, Joined& j) {
+  printf("--- Running Aggregation Benchmark ---\n");
+  gdb_table* t = make_table({{"l_quantity", GDB_I32, 0, j.l_qty}}, j.nl, false);
+  int64_t sum = 0;
+  gdb_timing tm = timed([&](gdb_timing* x) { CHECK(gdb_sum(g_ctx, t, "l_quantity", &sum, x)); });
+  printf("Final SUM(l_quantity): %lld\n", (long long)sum);
+  printf("GPU execution time: %g ms\n", tm.gpu_ms);
+  printf("Effective Bandwidth: %g GB/s\n\n", tm.bytes / (tm.gpu_ms / 1000.0) / 1e9);
+  gdb_table_free(t);
+}
+
+void run_join(const Scale&, Joined& j) {
+  printf("--- Running Join Benchmark ---\n");
+  gdb_table* b = make_table({{"o_orderkey", GDB_I32, 0, j.o_orderkey}}, j.no, false);
+  gdb_table* p = make_table({{"l_orderkey", GDB_I32, 0, j.l_ok}}, j.nl, false);
+  uint64_t m = 0;
+  gdb_timing tm = timed([&](gdb_timing* x) { CHECK(gdb_join_count(g_ctx, b, "o_orderkey", p, "l_orderkey", &m, x)); });
+  printf("Join complete. Found %llu total matches.\n", (unsigned long long)m);
+  printf("Probe Phase GPU time: %g ms\n", tm.main_kernel_ms);
+  printf("Total Join GPU time: %g ms\n\n", tm.gpu_ms);
+  gdb_table_free(b);
+  gdb_table_free(p);
+}
+
+void help() {
+  printf("Usage: gdb_bench [sf1|sf10|sf100] [query]\n");
+  printf("  scales: sf1 (6M fact rows), sf10 (60M), sf100 (600M, streamed from pinned host memory)\n");
+  printf("  queries: all, q1, q3, q6, q9, q13, selection, aggregation, join\n");
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  std::string query = "all", scale = "sf1";
+  for (int i = 1; i < argc; ++i) {
+    std::string a = argv[i];
+    if (a == "help" || a == "--help" || a == "-h") {
+      help();
+      return 0;
+    }
+    if (a == "sf1" || a == "sf10" || a == "sf100") {
+      scale = a;
+      continue;
+    }
+    query = a;
+  }
+  const std::vector<Scale> scales = {{"sf1", 6000000, 150000, 1500000, 200000, 10000, false},
+                                     {"sf10", 60000000, 1500000, 15000000, 2000000, 100000, false},
+                                     {"sf100", 600000000, 15000000, 150000000, 20000000, 1000000, true}};
+  const Scale s = *std::find_if(scales.begin(), scales.end(), [&](const Scale& x) { return x.name == scale; });
+  const std::vector<std::string> flat_q = {"q1", "q6"};
+  const std::vector<std::string> join_q = {"q3", "q9", "q13", "selection", "aggregation", "join"};
+  const std::vector<std::string> all_q = {"selection", "aggregation", "join", "q1", "q3", "q6", "q9", "q13"};
+  std::vector<std::string> run;
+  if (query == "all") {
+    run = all_q;
+    if (s.stream) run = {"q1", "q3", "q6", "q9", "q13"};
+  } else if (std::find(all_q.begin(), all_q.end(), query) != all_q.end()) {
+    run = {query};
+  } else {
+    fprintf(stderr, "Unknown query: %s\nUse 'help' to see available options.\n", query.c_str());
+    return 1;
+  }
+  if (s.stream) {
+    for (const auto& q : run)
+      if (q == "selection" || q == "aggregation" || q == "join") {
+        fprintf(stderr, "Unknown query for sf100: %s\n", q.c_str());
+        return 1;
+      }
+  }
+  CHECK(gdb_open(0, 0, &g_ctx));
+  gdb_device_info info{};
+  CHECK(gdb_device_info_get(g_ctx, &info));
+  printf("Device: %s (%s), %d CUs, %.0f GB HBM\n", info.name, info.arch, info.cu_count,
+         info.hbm_bytes / 1e9);
+  bool need_flat = false, need_join = false, need_comments = false;
+  for (const auto& q : run) {
+    need_flat |= std::find(flat_q.begin(), flat_q.end(), q) != flat_q.end();
+    need_join |= std::find(join_q.begin(), join_q.end(), q) != join_q.end();
+    need_comments |= q == "q13";
+  }
+  Flat f{};
+  Joined j{};
+  if (need_flat) f = make_flat(s);
+  if (need_join) j = make_joined(s, need_comments);
+  for (const auto& q : run) {
+    if (q == "q1") run_q1(s, f);
+    else if (q == "q6") run_q6(s, f);
+    else if (q == "q3") run_q3(s, j);
+    else if (q == "q9") run_q9(s, j);
+    else if (q == "q13") run_q13(s, j);
+    else if (q == "selection") run_selection(s, j);
+    else if (q == "aggregation") run_aggregation(s, j);
+    else if (q == "join") run_join(s, j);
+    printf("\n");
+  }
+  for (Buf* b : g_bufs) delete b;
+  gdb_close(g_ctx);
+  return 0;
+}

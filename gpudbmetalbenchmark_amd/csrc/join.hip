@@ -3,7 +3,7 @@
 // src/microbenchmarks.cpp:135-251).
 //
 // Build: open-addressing table (key, multiplicity) with capacity nextpow2(2*nb),
-// Fibonacci hashing, linear probing, CAS insert. Probe: dwordx4 loads of 4 probe
+// a locality-preserving fold hash, linear probing, CAS insert. Probe: dwordx4 loads of 4 probe
 // keys per lane, each matched key adds its multiplicity; per-block partial
 // counts are summed by the finaliser. Keys may repeat on both sides.
 #include <hip/hip_runtime.h>
@@ -20,8 +20,14 @@ namespace {
 
 constexpr int32_t EMPTY = INT32_MIN;
 
-__device__ __forceinline__ uint32_t jhash(int32_t k, uint32_t shift) {
-  return ((uint32_t)k * 0x9E3779B1u) >> shift;
+// Keys below the capacity map to themselves and higher bits are folded in, so
+// keys arriving in (near) key order touch neighbouring slots and the CAS
+// inserts and probes coalesce (the reference hashes by key modulo size,
+// kernels/microbenchmarks.metal:240).
+__device__ __forceinline__ uint32_t jhash(int32_t key, uint32_t shift) {
+  const uint32_t k = (uint32_t)key;
+  const uint32_t lg = 32 - shift;
+  return (k ^ (k >> lg) ^ (k >> (2 * lg < 32 ? 2 * lg : 31))) & ((1u << lg) - 1);
 }
 
 __global__ __launch_bounds__(256) void join_clear(int32_t* __restrict__ keys,

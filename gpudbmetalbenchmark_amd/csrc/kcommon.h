@@ -48,6 +48,25 @@ __device__ __forceinline__ int64_t block_sum(int64_t v, int64_t* red) {
 }
 
 // Global-load helpers: dwordx4 vector loads of 4 int32 rows / 2 int64 rows.
+// The *nt variants are nontemporal (streamed once, not kept in the caches):
+// on gfx950 a grid-stride read with 4 such loads in flight per lane reaches
+// ~88% of HBM peak, against ~74% for plain loads.
+typedef int v4i_t __attribute__((ext_vector_type(4)));
+typedef long long v2l_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ int4 ld4nt(const int32_t* p, uint32_t vec) {
+  const v4i_t v = __builtin_nontemporal_load(reinterpret_cast<const v4i_t*>(p) + vec);
+  return make_int4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ longlong2 ld2nt(const int64_t* p, uint32_t idx2) {
+  const v2l_t v = __builtin_nontemporal_load(reinterpret_cast<const v2l_t*>(p) + idx2);
+  longlong2 r;
+  r.x = v.x;
+  r.y = v.y;
+  return r;
+}
+__device__ __forceinline__ uint32_t ld_u8x4nt(const uint8_t* p, uint32_t vec) {
+  return __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(p) + vec);
+}
 __device__ __forceinline__ int4 ld4(const int32_t* p, uint32_t vec) {
   return reinterpret_cast<const int4*>(p)[vec];
 }
@@ -56,6 +75,45 @@ __device__ __forceinline__ longlong2 ld2(const int64_t* p, uint32_t idx2) {
 }
 __device__ __forceinline__ uint32_t ld_u8x4(const uint8_t* p, uint32_t vec) {
   return reinterpret_cast<const uint32_t*>(p)[vec];
+}
+
+// Exclusive prefix sum of v over the block (blockDim a multiple of 64, at most
+// 1024); *total receives the block sum. `scratch` holds 17 words of LDS. Every
+// thread of the block must call it.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* scratch,
+                                                    uint32_t* total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const uint32_t incl = wave_incl_scan(v);
+  if (lane == 63) scratch[w] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int i = 0; i < nw; ++i) {
+      const uint32_t t = scratch[i];
+      scratch[i] = run;
+      run += t;
+    }
+    scratch[16] = run;
+  }
+  __syncthreads();
+  const uint32_t r = scratch[w] + incl - v;
+  *total = scratch[16];
+  __syncthreads();
+  return r;
+}
+
+// Monotone bucket of an int64 in [0, 4096): log-linear with 32 sub-buckets per
+// power of two (about 3% wide), negatives mirrored below the non-negatives.
+__device__ __forceinline__ uint32_t log_bucket_mag(uint64_t x) {
+  if (x == 0) return 0;
+  const int e = 63 - __clzll((long long)x);
+  const uint32_t m = e >= 5 ? (uint32_t)(x >> (e - 5)) & 31u : (uint32_t)(x << (5 - e)) & 31u;
+  return 1u + (uint32_t)e * 32u + m;  // 1 .. 2048
+}
+__device__ __forceinline__ uint32_t value_bucket(int64_t v) {
+  if (v >= 0) return 2047u + log_bucket_mag((uint64_t)v);  // 2047 .. 4095
+  const uint64_t mag = v == INT64_MIN ? (1ull << 63) : (uint64_t)(-v);
+  return 2047u - min(2047u, log_bucket_mag(mag));          // 0 .. 2046
 }
 
 // Finaliser: block k of the grid sums column k of a k-major partials array

@@ -79,9 +79,9 @@ struct Q1Cols {
 
 __device__ __forceinline__ void q1_group(const Q1Cols& c, uint32_t g, int32_t cutoff,
                                          unsigned long long* lane_tab, uint32_t& bad) {
-  const int4 s = ld4(c.ship, g), q = ld4(c.qty, g), d = ld4(c.disc, g), t = ld4(c.tax, g);
-  const longlong2 a = ld2(c.price, 2 * g), b = ld2(c.price, 2 * g + 1);
-  const uint32_t rf = ld_u8x4(c.rf, g), ls = ld_u8x4(c.ls, g);
+  const int4 s = ld4nt(c.ship, g), q = ld4nt(c.qty, g), d = ld4nt(c.disc, g), t = ld4nt(c.tax, g);
+  const longlong2 a = ld2nt(c.price, 2 * g), b = ld2nt(c.price, 2 * g + 1);
+  const uint32_t rf = ld_u8x4nt(c.rf, g), ls = ld_u8x4nt(c.ls, g);
   q1_row({s.x, q.x, d.x, t.x, a.x, rf & 0xff, ls & 0xff}, cutoff, lane_tab, bad);
   q1_row({s.y, q.y, d.y, t.y, a.y, (rf >> 8) & 0xff, (ls >> 8) & 0xff}, cutoff, lane_tab, bad);
   q1_row({s.z, q.z, d.z, t.z, b.x, (rf >> 16) & 0xff, (ls >> 16) & 0xff}, cutoff, lane_tab, bad);

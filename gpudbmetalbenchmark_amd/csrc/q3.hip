@@ -12,8 +12,10 @@
 //   probe    fact rows with shipdate > D and obit[orderkey] add
 //            extprice*(100-discount) into their order's slot; discount and
 //            price are loaded only for lanes with a surviving row
-//   topk     per-block LDS top-k over the table, then one block over the
-//            block winners (revenue desc, orderdate asc, orderkey asc)
+//   topk     a 4096-bucket log-linear histogram of the groups' revenues finds
+//            the bucket holding the k-th best; only groups at or above it are
+//            collected, and one block ranks them (revenue desc, orderdate asc,
+//            orderkey asc) with the LDS top-k of topk.h
 // The orders bitmap (maxkey/8 bytes) is the reference's bitmap pre-filter
 // (kernels/q3.metal:72-74) moved onto the probe side, where it rejects ~99%
 // of fact rows before any hash-table access; the table holds only qualifying
@@ -34,15 +36,20 @@ namespace {
 
 constexpr int32_t EMPTY = INT32_MIN;
 constexpr int TOPK_BUF = 2048;
-constexpr int TOPK_BLOCKS = 256;
+constexpr uint32_t CAND_CAP = 65536;  // candidate list entries
 
 __device__ __forceinline__ uint32_t ht_cap(uint32_t count) {
   uint32_t c = 1024;
   while (c < 2 * count) c <<= 1;
   return c;
 }
+// Folds the key's high bits onto its low bits: keys below the capacity map to
+// themselves, so orders appended and probed in key order touch neighbouring
+// slots (the reference hashes by key modulo size, kernels/q3.metal:189).
 __device__ __forceinline__ uint32_t ht_hash(int32_t key, uint32_t cap) {
-  return ((uint32_t)key * 0x9E3779B1u) & (cap - 1);
+  const uint32_t k = (uint32_t)key;
+  const int lg = __ffs((int)cap) - 1;
+  return (k ^ (k >> lg) ^ (k >> (2 * lg < 32 ? 2 * lg : 31))) & (cap - 1);
 }
 __device__ __forceinline__ bool bit_test(const uint32_t* __restrict__ bm, uint32_t k) {
   return (bm[k >> 5] >> (k & 31)) & 1u;
@@ -59,7 +66,8 @@ struct Q3Scratch {
   int32_t* ht_q;      // slot -> list index
   int64_t* ht_rev;
   uint32_t* ht_hit;
-  TopEntry* cand;     // TOPK_BLOCKS * k block winners
+  uint32_t* hist;     // 4096 revenue buckets
+  TopEntry* cand;     // CAND_CAP candidates
   TopEntry* out;      // k final
 };
 
@@ -84,8 +92,12 @@ __global__ __launch_bounds__(256) void q3_customer(const int32_t* __restrict__ c
   if (seg[i] == (uint8_t)segment && ck <= maxck) atomicOr(&cbm[ck >> 5], 1u << (ck & 31));
 }
 
-// One row per thread; the append to the compact list is wave-aggregated
-// (one atomic per wave, ballot rank for the slot).
+// Tiles of 16 x 256 rows, thread t owning rows [16t, 16t+16) of the tile
+// (four dwordx4 loads per column). The qualifying rows of a tile are appended
+// with one atomic per tile and a block-wide scan for the slots, so the list
+// comes out in row order: for orders stored in key order the table inserts and
+// bitmap updates that follow hit neighbouring addresses and coalesce.
+constexpr int OTILE_PER_THREAD = 16;
 __global__ __launch_bounds__(256) void q3_orders(const int32_t* __restrict__ okey,
                                                  const int32_t* __restrict__ ocust,
                                                  const int32_t* __restrict__ odate,
@@ -97,28 +109,54 @@ __global__ __launch_bounds__(256) void q3_orders(const int32_t* __restrict__ oke
                                                  int32_t* __restrict__ qkey,
                                                  int32_t* __restrict__ qdate,
                                                  int32_t* __restrict__ qprio) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool q = false;
-  int32_t k = 0, d = 0;
-  if (i < n) {
-    d = odate[i];
-    const uint32_t ck = (uint32_t)ocust[i];
-    k = okey[i];
-    q = d < date && ck <= maxck && (uint32_t)k <= maxok && bit_test(cbm, ck);
-  }
-  const uint64_t mask = __ballot(q);
-  if (mask == 0) return;
-  const int lane = threadIdx.x & 63;
-  const int leader = __ffsll((unsigned long long)mask) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(&counters[0], (uint32_t)__popcll(mask));
-  base = __shfl(base, leader, 64);
-  if (q) {
-    const uint32_t pos = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1));
-    qkey[pos] = k;
-    qdate[pos] = d;
-    qprio[pos] = oprio[i];
-    atomicOr(&obm[(uint32_t)k >> 5], 1u << ((uint32_t)k & 31));
+  __shared__ uint32_t scratch[17];
+  __shared__ uint32_t base_sh;
+  const uint32_t tile_rows = OTILE_PER_THREAD * blockDim.x;
+  const uint32_t ntiles = (n + tile_rows - 1) / tile_rows;
+  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint32_t r0 = t * tile_rows + threadIdx.x * OTILE_PER_THREAD;
+    uint32_t mask = 0;
+    if (r0 + OTILE_PER_THREAD <= n) {
+#pragma unroll
+      for (int v = 0; v < OTILE_PER_THREAD / 4; ++v) {
+        const int4 d4 = ld4(odate, r0 / 4 + v), c4 = ld4(ocust, r0 / 4 + v), k4 = ld4(okey, r0 / 4 + v);
+        const int32_t ds[4] = {d4.x, d4.y, d4.z, d4.w}, cs[4] = {c4.x, c4.y, c4.z, c4.w},
+                      ks[4] = {k4.x, k4.y, k4.z, k4.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const uint32_t ck = (uint32_t)cs[j];
+          if (ds[j] < date && ck <= maxck && (uint32_t)ks[j] <= maxok && bit_test(cbm, ck))
+            mask |= 1u << (v * 4 + j);
+        }
+      }
+    } else {
+      for (int j = 0; j < OTILE_PER_THREAD; ++j) {
+        const uint32_t r = r0 + j;
+        if (r < n) {
+          const uint32_t ck = (uint32_t)ocust[r];
+          if (odate[r] < date && ck <= maxck && (uint32_t)okey[r] <= maxok && bit_test(cbm, ck))
+            mask |= 1u << j;
+        }
+      }
+    }
+    uint32_t total;
+    uint32_t pos = block_excl_scan((uint32_t)__popc(mask), scratch, &total);
+    if (total == 0) continue;
+    if (threadIdx.x == 0) base_sh = atomicAdd(&counters[0], total);
+    __syncthreads();
+    pos += base_sh;
+    while (mask) {
+      const int j = __ffs((int)mask) - 1;
+      mask &= mask - 1;
+      const uint32_t r = r0 + j;
+      const int32_t k = okey[r];
+      qkey[pos] = k;
+      qdate[pos] = odate[r];
+      qprio[pos] = oprio[r];
+      ++pos;
+      atomicOr(&obm[(uint32_t)k >> 5], 1u << ((uint32_t)k & 31));
+    }
+    __syncthreads();  // base_sh is rewritten by the next tile
   }
 }
 
@@ -240,58 +278,136 @@ __global__ __launch_bounds__(256) void q3_probe(Q3Fact f, uint32_t n, int32_t da
   }
 }
 
-// Stage 1: block b ranks the hit slots of its share of [0, cap).
-__global__ __launch_bounds__(256) void q3_topk_slots(const uint32_t* __restrict__ counters_in,
+// Top-k, step 1: histogram of the groups' revenue buckets (LDS per block,
+// merged by global atomics) and the group count.
+__global__ __launch_bounds__(256) void q3_topk_hist(const uint32_t* __restrict__ counters_in,
+                                                    const int32_t* __restrict__ ht_key,
+                                                    const int64_t* __restrict__ ht_rev,
+                                                    const uint32_t* __restrict__ ht_hit,
+                                                    uint32_t* __restrict__ hist,
+                                                    uint32_t* __restrict__ counters) {
+  __shared__ uint32_t lh[4096];
+  __shared__ int64_t red[4];
+  for (int i = threadIdx.x; i < 4096; i += blockDim.x) lh[i] = 0u;
+  __syncthreads();
+  const uint32_t cap = ht_cap(counters_in[0]);
+  const uint32_t stride = gridDim.x * blockDim.x;
+  uint32_t hits = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += stride) {
+    if (ht_key[i] != EMPTY && ht_hit[i]) {
+      atomicAdd(&lh[value_bucket(ht_rev[i])], 1u);
+      ++hits;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 4096; i += blockDim.x)
+    if (lh[i]) atomicAdd(&hist[i], lh[i]);
+  const int64_t bh = block_sum((int64_t)hits, red);
+  if (threadIdx.x == 0 && bh) atomicAdd(&counters[1], (uint32_t)bh);
+}
+
+// Step 2 (one block): the highest bucket b such that buckets >= b hold at
+// least k groups (0 when there are fewer than k groups).
+__global__ __launch_bounds__(256) void q3_topk_threshold(const uint32_t* __restrict__ hist, int k,
+                                                         uint32_t* __restrict__ counters) {
+  __shared__ uint32_t part[256];
+  // Thread t sums buckets [16t, 16t+16); then a serial suffix walk on thread 0.
+  uint32_t s = 0;
+  for (int i = 0; i < 16; ++i) s += hist[threadIdx.x * 16 + i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0, b = 0;
+    for (int t = 255; t >= 0; --t) {
+      if (acc + part[t] >= (uint32_t)k) {
+        for (int i = 15; i >= 0; --i) {
+          acc += hist[t * 16 + i];
+          if (acc >= (uint32_t)k) {
+            b = (uint32_t)(t * 16 + i);
+            break;
+          }
+        }
+        break;
+      }
+      acc += part[t];
+    }
+    counters[2] = b;
+    counters[3] = 0u;  // candidate count
+  }
+}
+
+// Step 3: groups in buckets >= threshold -> candidate list (tile-aggregated append).
+__global__ __launch_bounds__(256) void q3_topk_collect(const int32_t* __restrict__ ht_key,
+                                                       const int32_t* __restrict__ ht_q,
+                                                       const int64_t* __restrict__ ht_rev,
+                                                       const uint32_t* __restrict__ ht_hit,
+                                                       const int32_t* __restrict__ qdate,
+                                                       const int32_t* __restrict__ qprio,
+                                                       uint32_t* __restrict__ counters,
+                                                       uint32_t cand_cap,
+                                                       TopEntry* __restrict__ cand) {
+  __shared__ uint32_t scratch[17];
+  __shared__ uint32_t base_sh;
+  const uint32_t cap = ht_cap(counters[0]);
+  const uint32_t thr = counters[2];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t span = (cap + stride - 1) / stride * stride;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < span; i += stride) {
+    bool q = false;
+    if (i < cap && ht_key[i] != EMPTY && ht_hit[i]) q = value_bucket(ht_rev[i]) >= thr;
+    uint32_t total;
+    const uint32_t pre = block_excl_scan(q ? 1u : 0u, scratch, &total);
+    if (total == 0) continue;
+    if (threadIdx.x == 0) base_sh = atomicAdd(&counters[3], total);
+    __syncthreads();
+    const uint32_t pos = base_sh + pre;
+    if (q && pos < cand_cap) {
+      const int32_t j = ht_q[i];
+      cand[pos] = TopEntry{ht_rev[i], qdate[j], ht_key[i], qprio[j], 0};
+    }
+    __syncthreads();
+  }
+}
+
+// Step 4 (one block): rank the candidates; if more than fit the list (heavy
+// ties in the threshold bucket), rank the qualifying table slots directly.
+__global__ __launch_bounds__(256) void q3_topk_final(const TopEntry* __restrict__ cand,
+                                                     uint32_t cand_cap,
+                                                     const uint32_t* __restrict__ counters,
                                                      const int32_t* __restrict__ ht_key,
                                                      const int32_t* __restrict__ ht_q,
                                                      const int64_t* __restrict__ ht_rev,
                                                      const uint32_t* __restrict__ ht_hit,
                                                      const int32_t* __restrict__ qdate,
                                                      const int32_t* __restrict__ qprio, int k,
-                                                     uint32_t* __restrict__ counters,
-                                                     TopEntry* __restrict__ cand) {
-  __shared__ TopEntry buf[TOPK_BUF];
-  __shared__ int nbuf;
-  __shared__ int64_t red[4];
-  BlockTopK<TOPK_BUF> tk{buf, &nbuf, k};
-  tk.init();
-  const uint32_t cap = ht_cap(counters_in[0]);
-  const uint32_t chunk = (cap + gridDim.x - 1) / gridDim.x;
-  const uint32_t begin = blockIdx.x * chunk, end = min(cap, begin + chunk);
-  uint32_t hits = 0;
-  for (uint32_t base = begin; base < end; base += blockDim.x) {
-    const uint32_t i = base + threadIdx.x;
-    bool valid = false;
-    TopEntry e;
-    if (i < end && ht_key[i] != EMPTY && ht_hit[i]) {
-      const int32_t q = ht_q[i];
-      e = TopEntry{ht_rev[i], qdate[q], ht_key[i], qprio[q], 0};
-      valid = true;
-      ++hits;
-    }
-    tk.push(valid, e);
-  }
-  tk.finish(cand + (size_t)blockIdx.x * k);
-  const int64_t bh = block_sum((int64_t)hits, red);
-  if (threadIdx.x == 0 && bh) atomicAdd(&counters[1], (uint32_t)bh);
-}
-
-// Stage 2: one block ranks the nc block winners.
-__global__ __launch_bounds__(256) void q3_topk_merge(const TopEntry* __restrict__ cand, uint32_t nc,
-                                                     int k, TopEntry* __restrict__ out) {
+                                                     TopEntry* __restrict__ out) {
   __shared__ TopEntry buf[TOPK_BUF];
   __shared__ int nbuf;
   BlockTopK<TOPK_BUF> tk{buf, &nbuf, k};
   tk.init();
-  for (uint32_t base = 0; base < nc; base += blockDim.x) {
-    const uint32_t i = base + threadIdx.x;
-    bool valid = false;
-    TopEntry e;
-    if (i < nc) {
-      e = cand[i];
-      valid = e.rev != INT64_MIN;
+  const uint32_t nc = counters[3];
+  if (nc <= cand_cap) {
+    for (uint32_t base = 0; base < nc; base += blockDim.x) {
+      const uint32_t i = base + threadIdx.x;
+      TopEntry e;
+      const bool valid = i < nc;
+      if (valid) e = cand[i];
+      tk.push(valid, e);
     }
-    tk.push(valid, e);
+  } else {
+    const uint32_t cap = ht_cap(counters[0]);
+    const uint32_t thr = counters[2];
+    for (uint32_t base = 0; base < cap; base += blockDim.x) {
+      const uint32_t i = base + threadIdx.x;
+      TopEntry e;
+      bool valid = false;
+      if (i < cap && ht_key[i] != EMPTY && ht_hit[i] && value_bucket(ht_rev[i]) >= thr) {
+        const int32_t j = ht_q[i];
+        e = TopEntry{ht_rev[i], qdate[j], ht_key[i], qprio[j], 0};
+        valid = true;
+      }
+      tk.push(valid, e);
+    }
   }
   tk.finish(out);
 }
@@ -333,12 +449,13 @@ extern "C" int gdb_q3(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* 
   Q3Scratch s{};
   {
     char* b;
-    const size_t bm_bytes = ((size_t)ncw + now_ + 16) * 4;
+    const size_t bm_bytes = ((size_t)ncw + now_ + 16 + 4096) * 4;
     const size_t list_bytes = (size_t)std::max<uint32_t>(no, 1) * 12;
     GDB_TRY(arena(ctx, A_BITMAP, bm_bytes + list_bytes + 256, (void**)&b));
     s.cbm = (uint32_t*)b;
     s.obm = s.cbm + ncw;
     s.counters = s.obm + now_;
+    s.hist = s.counters + 16;
     s.qkey = (int32_t*)(b + ((bm_bytes + 255) & ~size_t(255)));
     s.qdate = s.qkey + std::max<uint32_t>(no, 1);
     s.qprio = s.qdate + std::max<uint32_t>(no, 1);
@@ -349,9 +466,9 @@ extern "C" int gdb_q3(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* 
     s.ht_q = s.ht_key + capmax;
     s.ht_hit = (uint32_t*)(s.ht_q + capmax);
     char* a;
-    GDB_TRY(arena(ctx, A_AUX, sizeof(TopEntry) * ((size_t)TOPK_BLOCKS * k + k) + 64, (void**)&a));
+    GDB_TRY(arena(ctx, A_AUX, sizeof(TopEntry) * ((size_t)CAND_CAP + k) + 64, (void**)&a));
     s.cand = (TopEntry*)a;
-    s.out = s.cand + (size_t)TOPK_BLOCKS * k;
+    s.out = s.cand + CAND_CAP;
   }
   TopEntry* host;
   GDB_TRY(pinned(ctx, sizeof(TopEntry) * k + 64, (void**)&host));
@@ -360,13 +477,13 @@ extern "C" int gdb_q3(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* 
   hipStream_t st = ctx->stream;
   const int B = 256;
   GDB_HIP(hipEventRecord(ctx->ev[0], st));
-  q3_clear<<<std::min<uint32_t>(2048, (ncw + now_ + B - 1) / B + 1), B, 0, st>>>(
-      s.cbm, ncw, s.obm, now_, s.counters);
+  q3_clear<<<std::min<uint32_t>(2048, (ncw + now_ + 16 + 4096 + B - 1) / B + 1), B, 0, st>>>(
+      s.cbm, ncw, s.obm, now_ + 16 + 4096, s.counters);
   if (nc)
     q3_customer<<<(nc + B - 1) / B, B, 0, st>>>((const int32_t*)cck->dev, (const uint8_t*)cseg->dev,
                                                  nc, p->segment, maxck, s.cbm);
   if (no)
-    q3_orders<<<(no + B - 1) / B, B, 0, st>>>(
+    q3_orders<<<std::min<uint32_t>(ctx->info.cu_count * 8, (no + 16 * B - 1) / (16 * B)), B, 0, st>>>(
         (const int32_t*)cok->dev, (const int32_t*)coc->dev, (const int32_t*)cod->dev,
         (const int32_t*)cop->dev, no, p->date, maxck, maxok, s.cbm, s.obm, s.counters, s.qkey,
         s.qdate, s.qprio);
@@ -393,9 +510,13 @@ extern "C" int gdb_q3(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* 
     GDB_HIP(hipEventRecord(ctx->ev[1], st));
     GDB_HIP(hipEventRecord(ctx->ev[2], st));
   }
-  q3_topk_slots<<<TOPK_BLOCKS, B, 0, st>>>(s.counters, s.ht_key, s.ht_q, s.ht_rev, s.ht_hit, s.qdate,
-                                           s.qprio, k, s.counters, s.cand);
-  q3_topk_merge<<<1, B, 0, st>>>(s.cand, (uint32_t)TOPK_BLOCKS * k, k, s.out);
+  q3_topk_hist<<<fill_blocks, B, 0, st>>>(s.counters, s.ht_key, s.ht_rev, s.ht_hit, s.hist,
+                                          s.counters);
+  q3_topk_threshold<<<1, B, 0, st>>>(s.hist, k, s.counters);
+  q3_topk_collect<<<fill_blocks, B, 0, st>>>(s.ht_key, s.ht_q, s.ht_rev, s.ht_hit, s.qdate, s.qprio,
+                                             s.counters, CAND_CAP, s.cand);
+  q3_topk_final<<<1, B, 0, st>>>(s.cand, CAND_CAP, s.counters, s.ht_key, s.ht_q, s.ht_rev,
+                                 s.ht_hit, s.qdate, s.qprio, k, s.out);
   GDB_HIP(hipGetLastError());
   GDB_HIP(hipMemcpyAsync(host, s.out, sizeof(TopEntry) * k, hipMemcpyDeviceToHost, st));
   GDB_HIP(hipMemcpyAsync(host_cnt, s.counters, 8, hipMemcpyDeviceToHost, st));

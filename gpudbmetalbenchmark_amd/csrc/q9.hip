@@ -48,8 +48,14 @@ __device__ __forceinline__ uint32_t ps_cap(uint32_t count) {
 __device__ __forceinline__ uint64_t ps_key(int32_t pk, int32_t sk) {
   return ((uint64_t)(uint32_t)pk << 32) | (uint32_t)sk;
 }
+// Neighbouring partkeys map to neighbouring slots (the list is inserted in
+// partkey order, so the CAS inserts coalesce); the suppkey and the key's high
+// bits are folded in to spread a part's suppliers and large keys.
 __device__ __forceinline__ uint32_t ps_hash(uint64_t k, uint32_t cap) {
-  return (uint32_t)((k * 0x9E3779B97F4A7C15ull) >> 32) & (cap - 1);
+  const uint32_t pk = (uint32_t)(k >> 32), sk = (uint32_t)k;
+  const int lg = __ffs((int)cap) - 1;
+  const uint32_t x = pk * 4u + (sk & 3u);
+  return (x ^ (x >> lg) ^ (sk * 0x9E3779B1u >> (32 - lg))) & (cap - 1);
 }
 
 // Year of a day number (day 0 = 1992-01-01) by the days-from-civil inverse.
@@ -147,6 +153,8 @@ __global__ __launch_bounds__(256) void q9_orders(const int32_t* __restrict__ ok,
   if (i < n && (uint32_t)ok[i] <= maxok) oyear[ok[i]] = (uint8_t)(year_of_day(od[i]) - YEAR0);
 }
 
+// Partsupp rows whose part passed -> compact list in row order, one atomic per
+// tile of 16 x 256 rows (thread t owns rows [16t, 16t+16) of the tile).
 __global__ __launch_bounds__(256) void q9_partsupp(const int32_t* __restrict__ pk,
                                                    const int32_t* __restrict__ sk,
                                                    const int64_t* __restrict__ cost, uint32_t n,
@@ -155,25 +163,36 @@ __global__ __launch_bounds__(256) void q9_partsupp(const int32_t* __restrict__ p
                                                    int32_t* __restrict__ lpk,
                                                    int32_t* __restrict__ lsk,
                                                    int64_t* __restrict__ lcost) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool q = false;
-  uint32_t p = 0;
-  if (i < n) {
-    p = (uint32_t)pk[i];
-    q = p <= maxpk && ((pbm[p >> 5] >> (p & 31)) & 1u);
-  }
-  const uint64_t mask = __ballot(q);
-  if (mask == 0) return;
-  const int lane = threadIdx.x & 63;
-  const int leader = __ffsll((unsigned long long)mask) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(&counters[0], (uint32_t)__popcll(mask));
-  base = __shfl(base, leader, 64);
-  if (q) {
-    const uint32_t pos = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1));
-    lpk[pos] = (int32_t)p;
-    lsk[pos] = sk[i];
-    lcost[pos] = cost[i];
+  __shared__ uint32_t scratch[17];
+  __shared__ uint32_t base_sh;
+  const uint32_t tile_rows = 16 * blockDim.x;
+  const uint32_t ntiles = (n + tile_rows - 1) / tile_rows;
+  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint32_t r0 = t * tile_rows + threadIdx.x * 16;
+    uint32_t mask = 0;
+    for (int j = 0; j < 16; ++j) {
+      const uint32_t r = r0 + j;
+      if (r < n) {
+        const uint32_t p = (uint32_t)pk[r];
+        if (p <= maxpk && ((pbm[p >> 5] >> (p & 31)) & 1u)) mask |= 1u << j;
+      }
+    }
+    uint32_t total;
+    uint32_t pos = block_excl_scan((uint32_t)__popc(mask), scratch, &total);
+    if (total == 0) continue;
+    if (threadIdx.x == 0) base_sh = atomicAdd(&counters[0], total);
+    __syncthreads();
+    pos += base_sh;
+    while (mask) {
+      const int j = __ffs((int)mask) - 1;
+      mask &= mask - 1;
+      const uint32_t r = r0 + j;
+      lpk[pos] = pk[r];
+      lsk[pos] = sk[r];
+      lcost[pos] = cost[r];
+      ++pos;
+    }
+    __syncthreads();
   }
 }
 
@@ -242,50 +261,86 @@ __device__ __forceinline__ void q9_row(int32_t pk, int32_t sk, int32_t ok, int32
   lhit[slot] = 1u;
 }
 
-template <int U>
+// Orders LDS traffic within one wave: writes by some lanes become visible to
+// reads by other lanes of the same wave after this.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int RING = 512;  // per-wave queue of surviving row ids
+
+// Fact rows are tested 4 per lane on l_partkey alone; the survivors (about 5%)
+// are queued per wave in LDS and processed 64 at a time, one row per lane, so
+// the dependent chain (nation map, partsupp table, year map, LDS add) runs
+// with full waves instead of once per row slot with most lanes idle.
 __global__ __launch_bounds__(256) void q9_probe(Q9Fact f, uint32_t n, uint32_t maxpk,
                                                 uint32_t maxsk, uint32_t maxok, Q9Scratch s) {
   __shared__ int64_t lagg[NNAT * NYEAR];
   __shared__ uint32_t lhit[NNAT * NYEAR];
+  __shared__ uint32_t ring[4][RING];
   for (int i = threadIdx.x; i < NNAT * NYEAR; i += blockDim.x) {
     lagg[i] = 0;
     lhit[i] = 0u;
   }
   __syncthreads();
   const uint32_t cap = ps_cap(s.counters[0]);
-  const uint32_t nvec = n >> 2;
-  const uint32_t nthr = gridDim.x * blockDim.x;
-  auto group = [&](uint32_t g) {
-    const int4 p4 = ld4(f.pk, g);
-    const int32_t ps[4] = {p4.x, p4.y, p4.z, p4.w};
-    bool pass[4];
-    bool any = false;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      pass[j] = (uint32_t)ps[j] <= maxpk && ((s.pbm[(uint32_t)ps[j] >> 5] >> (ps[j] & 31)) & 1u);
-      any |= pass[j];
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t* q = ring[w];
+  uint32_t head = 0, tail = 0;  // wave-uniform
+  auto drain = [&](bool all) {
+    while (tail - head >= 64 || (all && tail != head)) {
+      const uint32_t avail = min(64u, tail - head);
+      if (lane < avail) {
+        const uint32_t r = q[(head + lane) & (RING - 1)];
+        q9_row(f.pk[r], f.sk[r], f.ok[r], f.qty[r], f.disc[r], f.price[r], cap, maxsk, maxok, s,
+               lagg, lhit);
+      }
+      head += avail;
     }
-    if (!any) return;
-    const int4 s4 = ld4(f.sk, g), o4 = ld4(f.ok, g), q4 = ld4(f.qty, g), d4 = ld4(f.disc, g);
-    const longlong2 a = ld2(f.price, 2 * g), b = ld2(f.price, 2 * g + 1);
-    if (pass[0]) q9_row(ps[0], s4.x, o4.x, q4.x, d4.x, a.x, cap, maxsk, maxok, s, lagg, lhit);
-    if (pass[1]) q9_row(ps[1], s4.y, o4.y, q4.y, d4.y, a.y, cap, maxsk, maxok, s, lagg, lhit);
-    if (pass[2]) q9_row(ps[2], s4.z, o4.z, q4.z, d4.z, b.x, cap, maxsk, maxok, s, lagg, lhit);
-    if (pass[3]) q9_row(ps[3], s4.w, o4.w, q4.w, d4.w, b.y, cap, maxsk, maxok, s, lagg, lhit);
   };
-  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  for (; g + (U - 1) * nthr < nvec; g += U * nthr) {
+  const uint32_t nvec = n >> 2;
+  const uint32_t wstride = gridDim.x * blockDim.x;           // vector groups per grid step
+  const uint32_t wbase = blockIdx.x * blockDim.x + w * 64;   // first group of this wave
+  for (uint32_t g0 = wbase; g0 < nvec; g0 += wstride) {
+    const uint32_t g = g0 + lane;
+    uint32_t m = 0;
+    if (g < nvec) {
+      const int4 p4 = ld4nt(f.pk, g);
+      const int32_t ps[4] = {p4.x, p4.y, p4.z, p4.w};
 #pragma unroll
-    for (int u = 0; u < U; ++u) group(g + u * nthr);
+      for (int j = 0; j < 4; ++j)
+        if ((uint32_t)ps[j] <= maxpk && ((s.pbm[(uint32_t)ps[j] >> 5] >> (ps[j] & 31)) & 1u))
+          m |= 1u << j;
+    }
+    const uint32_t c = (uint32_t)__popc(m);
+    const uint32_t incl = wave_incl_scan(c);
+    const uint32_t total = __shfl(incl, 63, 64);
+    uint32_t pos = tail + incl - c;
+    while (m) {
+      const int j = __ffs((int)m) - 1;
+      m &= m - 1;
+      q[(pos++) & (RING - 1)] = g * 4 + (uint32_t)j;
+    }
+    tail += total;
+    wave_lds_sync();
+    drain(false);
+    wave_lds_sync();
   }
-  for (; g < nvec; g += nthr) group(g);
-  if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
-    const uint32_t r = nvec * 4 + threadIdx.x;
-    const uint32_t p = (uint32_t)f.pk[r];
-    if (p <= maxpk && ((s.pbm[p >> 5] >> (p & 31)) & 1u))
-      q9_row(f.pk[r], f.sk[r], f.ok[r], f.qty[r], f.disc[r], f.price[r], cap, maxsk, maxok, s,
-             lagg, lhit);
+  if (blockIdx.x == 0 && w == 0) {  // wave-uniform: the last n % 4 rows
+    const uint32_t r = nvec * 4 + lane;
+    bool pass = false;
+    if (lane < (n & 3)) {
+      const uint32_t p = (uint32_t)f.pk[r];
+      pass = p <= maxpk && ((s.pbm[p >> 5] >> (p & 31)) & 1u);
+    }
+    const uint64_t b = __ballot(pass);
+    if (pass) q[(tail + (uint32_t)__popcll(b & ((1ull << lane) - 1))) & (RING - 1)] = r;
+    tail += (uint32_t)__popcll(b);
   }
+  wave_lds_sync();
+  drain(true);
   __syncthreads();
   for (int i = threadIdx.x; i < NNAT * NYEAR; i += blockDim.x) {
     if (lhit[i]) {
@@ -402,7 +457,7 @@ extern "C" int gdb_q9(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supp
     q9_orders<<<(no + B - 1) / B, B, 0, st>>>((const int32_t*)ook->dev, (const int32_t*)ood->dev, no,
                                                maxok, s.oyear);
   if (nps)
-    q9_partsupp<<<(nps + B - 1) / B, B, 0, st>>>(
+    q9_partsupp<<<std::min<uint32_t>(ctx->info.cu_count * 8, (nps + 16 * B - 1) / (16 * B)), B, 0, st>>>(
         (const int32_t*)pspk->dev, (const int32_t*)pssk->dev, (const int64_t*)psc->dev, nps, maxpk,
         s.pbm, s.counters, s.lpk, s.lsk, s.lcost);
   const int fill_blocks = ctx->info.cu_count * 4;
@@ -410,12 +465,12 @@ extern "C" int gdb_q9(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supp
   q9_ps_insert<<<fill_blocks, B, 0, st>>>(s.counters, s.lpk, s.lsk, s.lcost, s.ht_key, s.ht_cost);
   GDB_HIP(hipGetLastError());
   const int president =
-      resident_blocks(ctx, (const void*)q9_probe<2>, B, 0, ctx->tune[T_Q9].blocks_per_cu);
+      resident_blocks(ctx, (const void*)q9_probe, B, 0, ctx->tune[T_Q9].blocks_per_cu);
   auto consume = [&](const void* const* d, uint32_t rows) -> int {
     const Q9Fact f{(const int32_t*)d[0], (const int32_t*)d[1], (const int32_t*)d[2],
                    (const int32_t*)d[3], (const int32_t*)d[4], (const int64_t*)d[5]};
     const int pblocks = std::max(1, std::min<int>(president, (int)((rows / 4 + B - 1) / B)));
-    q9_probe<2><<<pblocks, B, 0, st>>>(f, rows, maxpk, maxsk, maxok, s);
+    q9_probe<<<pblocks, B, 0, st>>>(f, rows, maxpk, maxsk, maxok, s);
     GDB_HIP(hipGetLastError());
     return GDB_OK;
   };

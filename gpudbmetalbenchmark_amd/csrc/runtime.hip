@@ -220,6 +220,10 @@ int gdb_open(int device, uint64_t hbm_budget_bytes, gdb_ctx** out) {
   ctx->info.lds_bytes_per_cu = (int)prop.maxSharedMemoryPerMultiProcessor;
   ctx->info.wave_size = prop.warpSize;
   ctx->info.hbm_peak_gbps = 8000.0;  // MI355X HBM3E published peak
+  // Launch-shape defaults (measured on MI355X: nontemporal dwordx4 streams run
+  // best with 4 groups in flight per lane at 2 workgroups of 256 per CU).
+  ctx->tune[T_Q6] = Tuning{256, 2, 4};
+  ctx->tune[T_Q1] = Tuning{256, 2, 4};
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
     (void)hipGetLastError();

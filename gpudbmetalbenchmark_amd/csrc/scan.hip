@@ -35,8 +35,8 @@ __device__ __forceinline__ void q6_group(const int32_t* __restrict__ ship,
                                          const int32_t* __restrict__ qty,
                                          const int64_t* __restrict__ price, uint32_t g,
                                          const Q6P& p, int64_t& rev, uint32_t& cnt) {
-  const int4 s = ld4(ship, g), d = ld4(disc, g), q = ld4(qty, g);
-  const longlong2 a = ld2(price, 2 * g), b = ld2(price, 2 * g + 1);
+  const int4 s = ld4nt(ship, g), d = ld4nt(disc, g), q = ld4nt(qty, g);
+  const longlong2 a = ld2nt(price, 2 * g), b = ld2nt(price, 2 * g + 1);
   q6_row(s.x, d.x, q.x, a.x, p, rev, cnt);
   q6_row(s.y, d.y, q.y, a.y, p, rev, cnt);
   q6_row(s.z, d.z, q.z, b.x, p, rev, cnt);

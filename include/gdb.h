@@ -69,6 +69,12 @@ void gdb_close(gdb_ctx* ctx);
 int gdb_device_info_get(gdb_ctx* ctx, gdb_device_info* out);
 /* Bytes of HBM currently held by resident tables and arenas. */
 uint64_t gdb_hbm_used(gdb_ctx* ctx);
+/* Achievable HBM bandwidth: a plain streaming read and a plain copy over
+ * `bytes` (rounded down to 4 KiB), averaged over `reps` runs after one warm-up.
+ * copy_gbps counts read + write bytes. No counterpart in the reference: it is
+ * the calibration kernel for counter-based traffic measurement. */
+int gdb_bandwidth_probe(gdb_ctx* ctx, uint64_t bytes, int reps, double* read_gbps,
+                        double* copy_gbps);
 /* Test hook: the next `count` device allocations fail with GDB_E_NOMEM. */
 void gdb_debug_fail_allocs(gdb_ctx* ctx, int count);
 
@@ -210,6 +216,30 @@ int gdb_sum(gdb_ctx* ctx, const gdb_table* t, const char* column, int64_t* sum,
 int gdb_join_count(gdb_ctx* ctx, const gdb_table* build, const char* build_col,
                    const gdb_table* probe, const char* probe_col, uint64_t* matches,
                    gdb_timing* timing);
+
+/* ---- autotuning (no counterpart in the reference: it hard-codes 2048 x 1024
+ *      threadgroups, e.g. src/q6.cpp:72-74) ---------------------------------
+ * A tuning entry is the launch shape of a pipeline's dominant kernel:
+ * threads per workgroup, workgroups per CU (0 = occupancy limit) and vector
+ * groups of 4 rows in flight per lane. gdb_autotune times every candidate on
+ * the given tables (median of `reps` runs after one warm-up), keeps the fastest
+ * for the context and returns it. Tables are passed in the pipeline's argument
+ * order (q6/q1: lineitem; q3: customer, orders, lineitem; q9: part, supplier,
+ * partsupp, orders, lineitem; q13: customer, orders). */
+enum { GDB_PIPE_Q6 = 0, GDB_PIPE_Q1 = 1, GDB_PIPE_Q3 = 4, GDB_PIPE_Q9 = 5, GDB_PIPE_Q13 = 6 };
+typedef struct {
+  int32_t pipeline;
+  int32_t block;
+  int32_t blocks_per_cu;
+  int32_t unroll;
+  double ms;           /* median pipeline time of the chosen shape */
+  int32_t candidates;  /* shapes timed */
+  int32_t pad;
+} gdb_tuning;
+int gdb_autotune(gdb_ctx* ctx, int pipeline, const gdb_table* const* tables, int ntables,
+                 int reps, gdb_tuning* best);
+int gdb_get_tuning(gdb_ctx* ctx, int pipeline, gdb_tuning* out);
+int gdb_set_tuning(gdb_ctx* ctx, const gdb_tuning* t);
 
 /* ---- seeded synthetic tables (replace the dbgen .tbl loaders, reference
  *      src/infra.h:231-468). Host-side; callers own the output arrays. ------ */
