@@ -224,6 +224,66 @@ def cpu_baseline(w, seconds):
                       f"(oracle/oracle.c, {threads} threads)"}
 
 
+def run_serve(args, info):
+    """Config 4: Poisson arrivals drawn 50/30/20 from the Q6 / Q1 / Q3 templates
+    against device-resident tables (60M-row flat fact table; 1.5M customers,
+    15M orders, ~60M joined fact rows). Serial plain execution is swept first;
+    its highest sustained rate sets the p99 bound, then the batched loop is
+    swept and its highest sustained rate within that p99 is `value`."""
+    from gpudbmetalbenchmark_amd import Engine, Server, generate
+    eng = Engine(0)
+    li = generate.lineitem_flat(42, FLAT_ROWS["sf10"])
+    flat = eng.table({k: li[k] for k in ("l_shipdate", "l_quantity", "l_discount", "l_tax",
+                                          "l_extendedprice", "l_returnflag", "l_linestatus")})
+    del li
+    nc, no, npart, ns = JOIN_SCALE["sf10"]
+    js = dict(customer=generate.customer(11, nc), orders=generate.orders(11, no, nc),
+              lineitem=generate.lineitem_joined(11, no, npart, ns))
+    t = {k: eng.table(v) for k, v in js.items()}
+    del js
+    dur = args.serve_seconds
+
+    def sweep(mode, rates, p99_bound=None):
+        best, points = None, []
+        with Server(eng, flat, t["customer"], t["orders"], t["lineitem"], mode=mode,
+                    max_batch=64, slots=2) as srv:
+            srv.loadgen(rate=min(rates), seconds=0.3, seed=999)  # warm-up
+            for i, r in enumerate(rates):
+                st = srv.loadgen(rate=r, seconds=dur, seed=100 + i)
+                ok = st["qps"] >= 0.9 * r and st["completed"] == st["submitted"]
+                ok = ok and (p99_bound is None or st["p99_ms"] <= p99_bound)
+                points.append(dict(rate=r, qps=round(st["qps"], 1), p50_ms=round(st["p50_ms"], 3),
+                                   p99_ms=round(st["p99_ms"], 3),
+                                   mean_batch=round(st["mean_batch"], 2), sustained=ok))
+                print(json.dumps({"serve_point": mode, **points[-1]}), file=sys.stderr, flush=True)
+                if ok:
+                    best = points[-1]
+                elif best is not None and st["qps"] < 0.8 * r:
+                    break
+        return best, points
+
+    s_best, s_pts = sweep("serial", [500, 1000, 1500, 2000, 3000, 4000, 5000, 6000])
+    bound = s_best["p99_ms"] if s_best else None
+    base = s_best["qps"] if s_best else 500.0
+    rates = [int(base * m) for m in (2, 5, 10, 15, 20, 30, 40, 60)]
+    b_best, b_pts = sweep("batched", rates, bound)
+    out = {
+        "metric": "served QPS at p99 (batched vs serial), config-4 mix", "value":
+        b_best["qps"] if b_best else 0.0, "unit": "req/s", "n_gpus": 1, "steps": len(b_pts),
+        "warmup": 1, "ms_per_step": dur * 1000.0, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int64", "data": "synthetic (seeded generator)",
+        "config": {"workload": "Poisson serving mix 50/30/20 Q6/Q1/Q3, 60M-row tables resident",
+                   "device": (info.name or info.arch).decode(errors="replace"),
+                   "serial_best": s_best, "batched_best": b_best, "p99_bound_ms": bound,
+                   "speedup_qps": round(b_best["qps"] / s_best["qps"], 2) if (b_best and s_best) else None,
+                   "serial_points": s_pts, "batched_points": b_pts},
+    }
+    for x in list(t.values()) + [flat]:
+        x.free()
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -234,11 +294,19 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe", action="store_true", help="also measure plain read/copy GB/s")
+    ap.add_argument("--serve-seconds", type=float, default=1.5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
+    if args.pipeline == "serve":
+        from gpudbmetalbenchmark_amd import Engine
+        e = Engine(0)
+        info = e.device_info()
+        e.close()
+        run_serve(args, info)
+        return
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
     from gpudbmetalbenchmark_amd import Engine

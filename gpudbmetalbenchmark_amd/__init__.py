@@ -16,7 +16,7 @@ from . import abi
 from .abi import (GdbError, Q1Params, Q1Result, Q3Params, Q3Result, Q6Params, Q6Result,
                   Q9Params, Q9Result, Q13Params, Q13Result, Timing)
 
-__all__ = ["Engine", "Table", "GdbError", "lib", "generate", "LIB_PATH"]
+__all__ = ["Engine", "Table", "Server", "GdbError", "lib", "generate", "request", "LIB_PATH"]
 
 LIB_PATH = abi.LIB_PATH
 
@@ -232,6 +232,102 @@ class Engine:
         abi.check(lib().gdb_join_count(self._ctx, build.handle, build_col.encode(), probe.handle,
                                        probe_col.encode(), C.byref(m), abi.tref(timing)))
         return m.value
+
+
+def stats_dict(st):
+    return {k: getattr(st, k) for k, _ in abi.ServeStats._fields_}
+
+
+def response_dict(r):
+    out = dict(id=r.id, kind=("q6", "q1", "q3")[r.kind], status=r.status,
+               latency_ms=r.latency_ms, batch_size=r.batch_size)
+    if r.kind == abi.REQ_Q6:
+        out["result"] = dict(revenue=r.q6.revenue, rows=r.q6.rows)
+    elif r.kind == abi.REQ_Q1:
+        out["result"] = [abi.q1_row_dict(r.q1.rows[i]) for i in range(r.q1.nrows)]
+    else:
+        out["result"] = dict(groups=r.q3.groups, rows=[
+            dict(orderkey=x.orderkey, revenue=x.revenue, orderdate=x.orderdate,
+                 shippriority=x.shippriority) for x in r.q3.rows[:r.q3.nrows]])
+    return out
+
+
+def request(kind, **p):
+    """A gdb_request: request('q6', date_lo=.., ...), request('q1', cutoff=..),
+    request('q3', segment=.., date=.., top_k=..)."""
+    r = abi.Request()
+    if kind == "q6":
+        r.kind = abi.REQ_Q6
+        r.q6 = Q6Params(p["date_lo"], p["date_hi"], p["disc_lo"], p["disc_hi"], p["qty_lt"])
+    elif kind == "q1":
+        r.kind = abi.REQ_Q1
+        r.q1 = Q1Params(p["cutoff"])
+    else:
+        r.kind = abi.REQ_Q3
+        r.q3 = Q3Params(p["segment"], p["date"], p.get("top_k", 10))
+    return r
+
+
+def loadgen_request(seed, i, q3_share=0.2):
+    """The i-th request of the built-in load generator for `seed`, as a dict."""
+    r = abi.Request()
+    lib().gdb_loadgen_request(seed, i, q3_share, C.byref(r))
+    if r.kind == abi.REQ_Q6:
+        return "q6", dict(date_lo=r.q6.date_lo, date_hi=r.q6.date_hi, disc_lo=r.q6.disc_lo,
+                          disc_hi=r.q6.disc_hi, qty_lt=r.q6.qty_lt)
+    if r.kind == abi.REQ_Q1:
+        return "q1", dict(cutoff=r.q1.cutoff)
+    return "q3", dict(segment=r.q3.segment, date=r.q3.date, top_k=r.q3.top_k)
+
+
+class Server:
+    """A serving loop on an engine (gdb_serve_start ... gdb_serve_stop)."""
+
+    def __init__(self, engine, lineitem_flat=None, customer=None, orders=None, lineitem=None,
+                 mode="batched", max_batch=64, slots=2, hbm_budget=0):
+        cfg = abi.ServeCfg(
+            lineitem_flat.handle.value if lineitem_flat else None,
+            customer.handle.value if customer else None,
+            orders.handle.value if orders else None,
+            lineitem.handle.value if lineitem else None,
+            abi.SERVE_BATCHED if mode == "batched" else abi.SERVE_SERIAL,
+            max_batch, slots, 0, hbm_budget)
+        self._s = C.c_void_p()
+        abi.check(lib().gdb_serve_start(engine._ctx, C.byref(cfg), C.byref(self._s)))
+
+    def submit(self, req):
+        i = C.c_uint64()
+        abi.check(lib().gdb_submit(self._s, C.byref(req), C.byref(i)))
+        return i.value
+
+    def poll(self, max_n=256, timeout_ms=100):
+        buf = (abi.Response * max_n)()
+        n = lib().gdb_poll(self._s, buf, max_n, timeout_ms)
+        if n < 0:
+            abi.check(n)
+        return [response_dict(buf[i]) for i in range(n)]
+
+    def stats(self):
+        st = abi.ServeStats()
+        abi.check(lib().gdb_serve_stats_get(self._s, C.byref(st)))
+        return stats_dict(st)
+
+    def loadgen(self, rate, seconds, seed=1, q3_share=0.2):
+        st = abi.ServeStats()
+        abi.check(lib().gdb_serve_loadgen(self._s, rate, seconds, seed, q3_share, C.byref(st)))
+        return stats_dict(st)
+
+    def stop(self):
+        if self._s:
+            rc = lib().gdb_serve_stop(self._s)
+            self._s = C.c_void_p()
+            abi.check(rc)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.stop()
 
 
 from . import generate  # noqa: E402  (after Engine: generate imports abi only)

@@ -103,6 +103,33 @@ class Tuning(C.Structure):
                 ("pad", C.c_int32)]
 
 
+class ServeCfg(C.Structure):
+    _fields_ = [("lineitem_flat", C.c_void_p), ("customer", C.c_void_p), ("orders", C.c_void_p),
+                ("lineitem", C.c_void_p), ("mode", C.c_int32), ("max_batch", C.c_int32),
+                ("slots", C.c_int32), ("pad", C.c_int32), ("hbm_budget", C.c_uint64)]
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pad", C.c_int32), ("q6", Q6Params), ("q1", Q1Params),
+                ("q3", Q3Params)]
+
+
+class Response(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("kind", C.c_int32), ("status", C.c_int32),
+                ("latency_ms", C.c_double), ("batch_size", C.c_int32), ("pad", C.c_int32),
+                ("q6", Q6Result), ("q1", Q1Result), ("q3", Q3Result)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
+                ("batches", C.c_uint64), ("qps", C.c_double), ("p50_ms", C.c_double),
+                ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
+                ("mean_batch", C.c_double), ("inflight_peak_bytes", C.c_uint64)]
+
+
+REQ_Q6, REQ_Q1, REQ_Q3 = 0, 1, 2
+SERVE_BATCHED, SERVE_SERIAL = 0, 1
+
 PIPE_IDS = {"q6": 0, "q1": 1, "q3": 4, "q9": 5, "q13": 6}
 
 
@@ -156,6 +183,14 @@ PROTOTYPES = {
                                C.POINTER(Tuning)]),
     "gdb_get_tuning": (C.c_int, [P, C.c_int, C.POINTER(Tuning)]),
     "gdb_set_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
+    "gdb_serve_start": (C.c_int, [P, C.POINTER(ServeCfg), C.POINTER(C.c_void_p)]),
+    "gdb_submit": (C.c_int, [P, C.POINTER(Request), U64P]),
+    "gdb_poll": (C.c_int, [P, C.POINTER(Response), C.c_int, C.c_int]),
+    "gdb_serve_stats_get": (C.c_int, [P, C.POINTER(ServeStats)]),
+    "gdb_serve_stop": (C.c_int, [P]),
+    "gdb_serve_loadgen": (C.c_int, [P, C.c_double, C.c_double, C.c_uint64, C.c_double,
+                                    C.POINTER(ServeStats)]),
+    "gdb_loadgen_request": (None, [C.c_uint64, C.c_uint64, C.c_double, C.POINTER(Request)]),
     "gdb_gen_lineitem_flat": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, I32P, I32P, I32P,
                                         I32P, I64P, U8P, U8P, I32P, C.c_int]),
     "gdb_gen_customer": (C.c_int, [C.c_uint64, C.c_uint64, I32P, U8P, C.c_int]),

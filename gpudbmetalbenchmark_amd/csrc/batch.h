@@ -48,8 +48,10 @@ struct FlatPlan {
 int interval_of(const std::vector<int32_t>& b, int32_t v);
 
 // Builds the plan for a set of requests; returns false if it does not fit.
+// With fill_tables = false only the boundaries and sizes are computed (the
+// cheap fit check used while a batch is being formed).
 bool plan_flat_batch(const std::vector<gdb_q6_params>& q6, const std::vector<gdb_q1_params>& q1,
-                     FlatPlan* plan, size_t lds_limit = FB_LDS_LIMIT);
+                     FlatPlan* plan, size_t lds_limit = FB_LDS_LIMIT, bool fill_tables = true);
 
 // Device state of one flat batch slot: sized for the largest plan, reused.
 struct FlatSlot {

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cstring>
 
 #include "batch.h"
@@ -24,7 +25,7 @@ static void uniq(std::vector<int32_t>* v) {
 }
 
 bool plan_flat_batch(const std::vector<gdb_q6_params>& q6, const std::vector<gdb_q1_params>& q1,
-                     FlatPlan* plan, size_t lds_limit) {
+                     FlatPlan* plan, size_t lds_limit, bool fill_tables) {
   FlatPlan& p = *plan;
   p.date_b.clear();
   p.disc_b.clear();
@@ -49,7 +50,7 @@ bool plan_flat_batch(const std::vector<gdb_q6_params>& q6, const std::vector<gdb
     if (v < 0 || v > FB_LUT_SMALL) return false;
   if (!p.date_b.empty() && (int64_t)p.date_b.back() - p.date_b.front() >= FB_LUT_DAYS) return false;
   FlatBatchDesc& d = p.desc;
-  memset(&d, 0, sizeof(d));
+  memset(&d, 0, offsetof(FlatBatchDesc, date_lut));
   d.has_q6 = !q6.empty();
   d.has_q1 = !q1.empty();
   d.n_date = (int)p.date_b.size() + 1;
@@ -60,14 +61,23 @@ bool plan_flat_batch(const std::vector<gdb_q6_params>& q6, const std::vector<gdb
   d.q1_cells = d.has_q1 ? d.n_date * 6 : 0;
   d.lut_lo = p.date_b.empty() ? 0 : p.date_b.front();
   d.n_date_b = (int)p.date_b.size();
-  for (int i = 0; i < FB_LUT_DAYS; ++i) d.date_lut[i] = (uint8_t)interval_of(p.date_b, d.lut_lo + i);
-  for (int i = 0; i < FB_LUT_SMALL; ++i) {
-    d.disc_lut[i] = (uint8_t)interval_of(p.disc_b, i);
-    d.qty_lut[i] = (uint8_t)interval_of(p.qty_b, i);
-  }
   p.ncells = d.q6_cells * 2 + d.q1_cells * 6;
   p.lds_bytes = (size_t)p.ncells * 8 + FB_LUT_DAYS + 2 * FB_LUT_SMALL;
-  return p.lds_bytes <= std::min<size_t>(lds_limit, FB_LDS_LIMIT);
+  if (p.lds_bytes > std::min<size_t>(lds_limit, FB_LDS_LIMIT)) return false;
+  if (fill_tables) {
+    // Linear sweeps: entry i holds the number of boundaries <= its value.
+    auto sweep = [](const std::vector<int32_t>& b, int32_t lo, int n, uint8_t* lut) {
+      size_t k = 0;
+      for (int i = 0; i < n; ++i) {
+        while (k < b.size() && b[k] <= lo + i) ++k;
+        lut[i] = (uint8_t)k;
+      }
+    };
+    sweep(p.date_b, d.lut_lo, FB_LUT_DAYS, d.date_lut);
+    sweep(p.disc_b, 0, FB_LUT_SMALL, d.disc_lut);
+    sweep(p.qty_b, 0, FB_LUT_SMALL, d.qty_lut);
+  }
+  return true;
 }
 
 namespace {

@@ -1,0 +1,476 @@
+// Q3 for the serving loop: many (segment, date) requests answered in one
+// launch sequence from resident join indexes (no reference counterpart; the
+// reference rebuilds its maps for every run, src/q3.cpp:80-142).
+//
+// Indexes, built once per table set and kept in HBM:
+//   cseg   custkey -> segment code (0xFF: no such customer)
+//   byday  order rows sorted by orderdate (counting sort) + per-day offsets
+//   csr    for every order row, its fact rows' (shipdate, extprice*(100-disc))
+//          contiguous (CSR by order row, built through orderkey -> order row)
+//   maxlag the largest shipdate - orderdate over all fact rows
+// A request (s, D) can only gain revenue from orders dated in
+// [D - maxlag, D): any earlier order's rows all ship by D. Its groups are the
+// orders of that window in segment s with at least one row shipped after D;
+// revenue sums those rows. Per request the groups are ranked with the same
+// bucket-histogram threshold + LDS top-k as gdb_q3 (revenue desc, orderdate
+// asc, orderkey asc), so every result is identical to gdb_q3's.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "batch.h"
+#include "kcommon.h"
+#include "scan_util.h"
+#include "topk.h"
+
+using namespace gdbk;
+
+namespace gdb {
+
+struct Q3Slot {
+  // tables
+  const int32_t *okey = nullptr, *ocust = nullptr, *odate = nullptr, *oprio = nullptr;
+  uint32_t no = 0, nl = 0;
+  int32_t dmin = 0, dmax = -1, maxlag = 0;
+  // indexes
+  uint8_t* cseg = nullptr;        // [maxck+1]
+  uint32_t ncseg = 0;
+  uint32_t* byday = nullptr;      // [no] order rows sorted by date
+  uint32_t* dayoff = nullptr;     // [ndays+1]
+  std::vector<uint32_t> h_dayoff; // host copy
+  uint32_t* csr_off = nullptr;    // [no+1]
+  int32_t* csr_ship = nullptr;    // [nl]
+  int64_t* csr_rev = nullptr;     // [nl]
+  // batch scratch (sized for Q3B_MAX requests over the largest windows)
+  Q3BatchDesc* d_desc = nullptr;
+  Q3BatchDesc* h_desc = nullptr;
+  uint32_t* d_win = nullptr;      // [Q3B_MAX][4]: range begin, end, output base, capacity
+  uint32_t* h_win = nullptr;
+  uint32_t* counters = nullptr;   // [Q3B_MAX][4]: groups, threshold, candidates, -
+  uint32_t* hist = nullptr;       // [Q3B_MAX][4096]
+  TopEntry* groups = nullptr;     // group arrays of all requests of a batch
+  size_t groups_cap = 0;
+  TopEntry* cand = nullptr;       // [Q3B_MAX][CAND]
+  TopEntry* out = nullptr;        // [Q3B_MAX][100]
+  TopEntry* h_out = nullptr;
+  uint32_t* h_counters = nullptr;
+  uint64_t bytes = 0;
+  std::vector<std::pair<void*, size_t>> allocs;
+};
+
+namespace {
+
+constexpr uint32_t Q3B_CAND = 8192;
+constexpr int Q3B_TOPK = 100;
+
+int alloc(gdb_ctx* ctx, Q3Slot* s, void** p, size_t bytes) {
+  GDB_TRY(dev_alloc(ctx, p, bytes));
+  s->allocs.push_back({*p, bytes});
+  s->bytes += bytes;
+  return GDB_OK;
+}
+
+// ---- index builds (once) --------------------------------------------------
+__global__ void k_fill_u8(uint8_t* p, uint32_t n, uint8_t v) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_i32(int32_t* p, uint32_t n, int32_t v) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_cseg(const int32_t* ck, const uint8_t* seg, uint32_t n, uint8_t* cseg) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (ck[i] >= 0) cseg[ck[i]] = seg[i];
+}
+__global__ void k_omap(const int32_t* okey, uint32_t n, int32_t* omap) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (okey[i] >= 0) omap[okey[i]] = (int32_t)i;
+}
+// Per fact row: its order row (or -1) and the per-order row counts; the
+// largest ship - orderdate lag.
+__global__ void k_fact_count(const int32_t* lkey, const int32_t* lship, uint32_t nl,
+                             const int32_t* omap, uint32_t maxok, const int32_t* odate,
+                             int32_t* frow, uint32_t* cnt, int32_t* maxlag) {
+  int32_t lag = INT32_MIN;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
+    const uint32_t k = (uint32_t)lkey[i];
+    const int32_t o = k <= maxok ? omap[k] : -1;
+    frow[i] = o;
+    if (o >= 0) {
+      atomicAdd(&cnt[o], 1u);
+      lag = max(lag, lship[i] - odate[o]);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) lag = max(lag, __shfl_xor(lag, off, 64));
+  if ((threadIdx.x & 63) == 0 && lag != INT32_MIN) atomicMax(maxlag, lag);
+}
+__global__ void k_fact_scatter(const int32_t* frow, const int32_t* lship, const int32_t* ldisc,
+                               const int64_t* lprice, uint32_t nl, uint32_t* cursor,
+                               int32_t* cship, int64_t* crev) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
+    const int32_t o = frow[i];
+    if (o < 0) continue;
+    const uint32_t p = atomicAdd(&cursor[o], 1u);
+    cship[p] = lship[i];
+    crev[p] = lprice[i] * (int64_t)(100 - ldisc[i]);
+  }
+}
+__global__ void k_day_count(const int32_t* odate, uint32_t n, int32_t dmin, uint32_t* cnt) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    atomicAdd(&cnt[odate[i] - dmin], 1u);
+}
+__global__ void k_day_scatter(const int32_t* odate, uint32_t n, int32_t dmin, uint32_t* cursor,
+                              uint32_t* byday) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    byday[atomicAdd(&cursor[odate[i] - dmin], 1u)] = i;
+}
+
+// ---- batch kernels ----------------------------------------------------------
+__global__ void q3b_clear(uint32_t* counters, uint32_t* hist, int n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (uint32_t)n * 4096; i += gridDim.x * blockDim.x)
+    hist[i] = 0u;
+  if (blockIdx.x == 0)
+    for (uint32_t i = threadIdx.x; i < (uint32_t)n * 4; i += blockDim.x) counters[i] = 0u;
+}
+
+// Groups of request blockIdx.y: orders of its window in its segment with a
+// fact row shipped after its date. Tile-aggregated append to its group range.
+__global__ __launch_bounds__(256) void q3b_groups(const Q3BatchDesc* __restrict__ desc,
+                                                  const uint32_t* __restrict__ win,
+                                                  const uint32_t* __restrict__ byday,
+                                                  const int32_t* __restrict__ okey,
+                                                  const int32_t* __restrict__ ocust,
+                                                  const int32_t* __restrict__ odate,
+                                                  const int32_t* __restrict__ oprio,
+                                                  const uint8_t* __restrict__ cseg, uint32_t ncseg,
+                                                  const uint32_t* __restrict__ csr_off,
+                                                  const int32_t* __restrict__ csr_ship,
+                                                  const int64_t* __restrict__ csr_rev,
+                                                  uint32_t* __restrict__ counters,
+                                                  uint32_t* __restrict__ hist,
+                                                  TopEntry* __restrict__ groups) {
+  __shared__ uint32_t scratch[17];
+  __shared__ uint32_t base_sh;
+  const int r = blockIdx.y;
+  if (r >= desc->n) return;
+  const int32_t seg = desc->segment[r], D = desc->date[r];
+  const uint32_t begin = win[r * 4], end = win[r * 4 + 1], obase = win[r * 4 + 2];
+  const uint32_t span = end - begin;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t rounds = (span + stride - 1) / stride;
+  for (uint32_t it = 0; it < rounds; ++it) {
+    const uint32_t i = begin + it * stride + blockIdx.x * blockDim.x + threadIdx.x;
+    bool q = false;
+    TopEntry e;
+    if (i < end) {
+      const uint32_t o = byday[i];
+      const uint32_t ck = (uint32_t)ocust[o];
+      if (ck < ncseg && cseg[ck] == (uint8_t)seg && odate[o] < D) {
+        int64_t rev = 0;
+        bool any = false;
+        for (uint32_t p = csr_off[o]; p < csr_off[o + 1]; ++p) {
+          if (csr_ship[p] > D) {
+            rev += csr_rev[p];
+            any = true;
+          }
+        }
+        if (any) {
+          q = true;
+          e = TopEntry{rev, odate[o], okey[o], oprio[o], 0};
+        }
+      }
+    }
+    uint32_t total;
+    const uint32_t pre = block_excl_scan(q ? 1u : 0u, scratch, &total);
+    if (total == 0) continue;
+    if (threadIdx.x == 0) base_sh = atomicAdd(&counters[r * 4], total);
+    __syncthreads();
+    if (q) {
+      groups[obase + base_sh + pre] = e;
+      atomicAdd(&hist[r * 4096 + value_bucket(e.rev)], 1u);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void q3b_threshold(const Q3BatchDesc* __restrict__ desc,
+                                                     const uint32_t* __restrict__ hist,
+                                                     uint32_t* __restrict__ counters) {
+  __shared__ uint32_t part[256];
+  const int r = blockIdx.x;
+  if (r >= desc->n) return;
+  const uint32_t* h = hist + r * 4096;
+  const uint32_t k = (uint32_t)desc->k[r];
+  uint32_t s = 0;
+  for (int i = 0; i < 16; ++i) s += h[threadIdx.x * 16 + i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0, b = 0;
+    for (int t = 255; t >= 0; --t) {
+      if (acc + part[t] >= k) {
+        for (int i = 15; i >= 0; --i) {
+          acc += h[t * 16 + i];
+          if (acc >= k) {
+            b = (uint32_t)(t * 16 + i);
+            break;
+          }
+        }
+        break;
+      }
+      acc += part[t];
+    }
+    counters[r * 4 + 1] = b;
+  }
+}
+
+// One block per request: candidates (bucket >= threshold) into LDS top-k.
+__global__ __launch_bounds__(256) void q3b_topk(const Q3BatchDesc* __restrict__ desc,
+                                                const uint32_t* __restrict__ win,
+                                                const uint32_t* __restrict__ counters,
+                                                const TopEntry* __restrict__ groups,
+                                                TopEntry* __restrict__ out) {
+  __shared__ TopEntry buf[2048];
+  __shared__ int nbuf;
+  const int r = blockIdx.x;
+  if (r >= desc->n) return;
+  BlockTopK<2048> tk{buf, &nbuf, desc->k[r]};
+  tk.init();
+  const uint32_t ng = counters[r * 4], thr = counters[r * 4 + 1];
+  const TopEntry* g = groups + win[r * 4 + 2];
+  for (uint32_t base = 0; base < ng; base += blockDim.x) {
+    const uint32_t i = base + threadIdx.x;
+    TopEntry e;
+    bool valid = false;
+    if (i < ng) {
+      e = g[i];
+      valid = value_bucket(e.rev) >= thr;
+    }
+    tk.push(valid, e);
+  }
+  tk.finish(out + (size_t)r * Q3B_TOPK);
+}
+
+}  // namespace
+
+int q3_slot_create(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* orders,
+                   const gdb_table* lineitem, Q3Slot** out) {
+  *out = nullptr;
+  const Column *cck, *cseg, *cok, *coc, *cod, *cop, *clk, *cls, *cld, *clp;
+  GDB_TRY(column(customer, "c_custkey", GDB_I32, &cck));
+  GDB_TRY(column(customer, "c_mktsegment", GDB_U8, &cseg));
+  GDB_TRY(column(orders, "o_orderkey", GDB_I32, &cok));
+  GDB_TRY(column(orders, "o_custkey", GDB_I32, &coc));
+  GDB_TRY(column(orders, "o_orderdate", GDB_I32, &cod));
+  GDB_TRY(column(orders, "o_shippriority", GDB_I32, &cop));
+  GDB_TRY(column(lineitem, "l_orderkey", GDB_I32, &clk));
+  GDB_TRY(column(lineitem, "l_shipdate", GDB_I32, &cls));
+  GDB_TRY(column(lineitem, "l_discount", GDB_I32, &cld));
+  GDB_TRY(column(lineitem, "l_extendedprice", GDB_I64, &clp));
+  GDB_TRY(require_resident(customer, "serving customer"));
+  GDB_TRY(require_resident(orders, "serving orders"));
+  GDB_TRY(require_resident(lineitem, "serving lineitem"));
+  if ((customer->nrows && cck->minv < 0) || (orders->nrows && (cok->minv < 0 || cod->maxv - cod->minv > (1 << 20))))
+    return fail(GDB_E_UNSUPPORTED, "serving Q3 needs non-negative keys and a date span under 2^20 days");
+  Q3Slot* s = new Q3Slot();
+  auto bail = [&](int rc) {
+    q3_slot_free(ctx, s);
+    return rc;
+  };
+  s->okey = (const int32_t*)cok->dev;
+  s->ocust = (const int32_t*)coc->dev;
+  s->odate = (const int32_t*)cod->dev;
+  s->oprio = (const int32_t*)cop->dev;
+  s->no = (uint32_t)orders->nrows;
+  s->nl = (uint32_t)lineitem->nrows;
+  s->dmin = orders->nrows ? (int32_t)cod->minv : 0;
+  s->dmax = orders->nrows ? (int32_t)cod->maxv : -1;
+  const uint32_t maxck = (uint32_t)std::max<int64_t>(0, cck->maxv);
+  const uint32_t maxok = (uint32_t)std::max<int64_t>(0, cok->maxv);
+  const uint32_t ndays = (uint32_t)std::max(0, s->dmax - s->dmin + 1);
+  s->ncseg = maxck + 1;
+  hipStream_t st = ctx->stream;
+  const int G = ctx->info.cu_count * 8, B = 256;
+  int32_t *omap = nullptr, *frow = nullptr, *d_lag = nullptr;
+  uint32_t *cnt = nullptr, *tsum = nullptr, *total = nullptr;
+  int rc;
+  if ((rc = alloc(ctx, s, (void**)&s->cseg, s->ncseg)) ||
+      (rc = alloc(ctx, s, (void**)&s->byday, (size_t)std::max(1u, s->no) * 4)) ||
+      (rc = alloc(ctx, s, (void**)&s->dayoff, (size_t)(ndays + 1) * 4 + 64)) ||
+      (rc = alloc(ctx, s, (void**)&s->csr_off, (size_t)(s->no + 1) * 4 + 64)) ||
+      (rc = alloc(ctx, s, (void**)&s->csr_ship, (size_t)std::max(1u, s->nl) * 4)) ||
+      (rc = alloc(ctx, s, (void**)&s->csr_rev, (size_t)std::max(1u, s->nl) * 8)))
+    return bail(rc);
+  // Build temporaries (freed at the end of the build).
+  std::vector<std::pair<void*, size_t>> tmp;
+  auto talloc = [&](void** p, size_t bytes) {
+    int r = dev_alloc(ctx, p, bytes);
+    if (r == GDB_OK) tmp.push_back({*p, bytes});
+    return r;
+  };
+  auto tfree = [&]() {
+    (void)hipStreamSynchronize(st);
+    for (auto& x : tmp) dev_free(ctx, x.first, x.second);
+    tmp.clear();
+  };
+  const uint32_t nscan = std::max(s->no, ndays) + 1;
+  if ((rc = talloc((void**)&omap, ((size_t)maxok + 1) * 4)) ||
+      (rc = talloc((void**)&frow, (size_t)std::max(1u, s->nl) * 4)) ||
+      (rc = talloc((void**)&cnt, (size_t)nscan * 4 + 64)) ||
+      (rc = talloc((void**)&tsum, (size_t)scan_tile_sums_needed(nscan) * 4 + 64)) ||
+      (rc = talloc((void**)&total, 64)) || (rc = talloc((void**)&d_lag, 64))) {
+    tfree();
+    return bail(rc);
+  }
+  // customer segments, order map
+  k_fill_u8<<<G, B, 0, st>>>(s->cseg, s->ncseg, 0xFF);
+  if (customer->nrows)
+    k_cseg<<<G, B, 0, st>>>((const int32_t*)cck->dev, (const uint8_t*)cseg->dev,
+                            (uint32_t)customer->nrows, s->cseg);
+  k_fill_i32<<<G, B, 0, st>>>(omap, maxok + 1, -1);
+  if (s->no) k_omap<<<G, B, 0, st>>>(s->okey, s->no, omap);
+  // CSR of fact rows by order row
+  k_fill_i32<<<G, B, 0, st>>>((int32_t*)cnt, s->no + 1, 0);
+  k_fill_i32<<<1, 64, 0, st>>>(d_lag, 1, 0);
+  if (s->nl)
+    k_fact_count<<<G, B, 0, st>>>((const int32_t*)clk->dev, (const int32_t*)cls->dev, s->nl, omap,
+                                  maxok, s->odate, frow, cnt, d_lag);
+  if ((rc = exclusive_scan_u32(ctx, cnt, s->no + 1, s->csr_off, total, tsum, st))) {
+    tfree();
+    return bail(rc);
+  }
+  (void)hipMemcpyAsync(cnt, s->csr_off, (size_t)(s->no + 1) * 4, hipMemcpyDeviceToDevice, st);
+  if (s->nl)
+    k_fact_scatter<<<G, B, 0, st>>>(frow, (const int32_t*)cls->dev, (const int32_t*)cld->dev,
+                                    (const int64_t*)clp->dev, s->nl, cnt, s->csr_ship, s->csr_rev);
+  // orders by day
+  k_fill_i32<<<G, B, 0, st>>>((int32_t*)cnt, ndays + 1, 0);
+  if (s->no) k_day_count<<<G, B, 0, st>>>(s->odate, s->no, s->dmin, cnt);
+  if ((rc = exclusive_scan_u32(ctx, cnt, ndays + 1, s->dayoff, total, tsum, st))) {
+    tfree();
+    return bail(rc);
+  }
+  (void)hipMemcpyAsync(cnt, s->dayoff, (size_t)(ndays + 1) * 4, hipMemcpyDeviceToDevice, st);
+  if (s->no) k_day_scatter<<<G, B, 0, st>>>(s->odate, s->no, s->dmin, cnt, s->byday);
+  if (hipGetLastError() != hipSuccess) {
+    tfree();
+    return bail(fail(GDB_E_HIP, "Q3 index build launch failed"));
+  }
+  s->h_dayoff.resize(ndays + 1);
+  int32_t lag = 0;
+  if (hipMemcpyAsync(s->h_dayoff.data(), s->dayoff, (size_t)(ndays + 1) * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(&lag, d_lag, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess) {
+    tfree();
+    return bail(fail(GDB_E_HIP, "Q3 index build failed"));
+  }
+  s->maxlag = std::max(0, lag);
+  tfree();
+  // batch scratch: group arrays sized for the Q3B_MAX largest windows
+  uint64_t widest = 0;
+  for (uint32_t d = 0; d < ndays; ++d) {
+    const uint32_t lo = d, hi = std::min<uint32_t>(ndays, d + (uint32_t)s->maxlag + 1);
+    widest = std::max<uint64_t>(widest, (uint64_t)s->h_dayoff[hi] - s->h_dayoff[lo]);
+  }
+  s->groups_cap = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)widest * Q3B_MAX, (uint64_t)s->no * Q3B_MAX));
+  if ((rc = alloc(ctx, s, (void**)&s->d_desc, sizeof(Q3BatchDesc))) ||
+      (rc = alloc(ctx, s, (void**)&s->d_win, Q3B_MAX * 16)) ||
+      (rc = alloc(ctx, s, (void**)&s->counters, Q3B_MAX * 16)) ||
+      (rc = alloc(ctx, s, (void**)&s->hist, (size_t)Q3B_MAX * 4096 * 4)) ||
+      (rc = alloc(ctx, s, (void**)&s->groups, s->groups_cap * sizeof(TopEntry))) ||
+      (rc = alloc(ctx, s, (void**)&s->out, (size_t)Q3B_MAX * Q3B_TOPK * sizeof(TopEntry))))
+    return bail(rc);
+  if (hipHostMalloc((void**)&s->h_desc, sizeof(Q3BatchDesc), 0) != hipSuccess ||
+      hipHostMalloc((void**)&s->h_win, Q3B_MAX * 16, 0) != hipSuccess ||
+      hipHostMalloc((void**)&s->h_out, (size_t)Q3B_MAX * Q3B_TOPK * sizeof(TopEntry), 0) != hipSuccess ||
+      hipHostMalloc((void**)&s->h_counters, Q3B_MAX * 16, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    return bail(fail(GDB_E_NOMEM, "pinned allocation for the Q3 batch slot failed"));
+  }
+  *out = s;
+  return GDB_OK;
+}
+
+void q3_slot_free(gdb_ctx* ctx, Q3Slot* s) {
+  if (!s) return;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& a : s->allocs) dev_free(ctx, a.first, a.second);
+  if (s->h_desc) (void)hipHostFree(s->h_desc);
+  if (s->h_win) (void)hipHostFree(s->h_win);
+  if (s->h_out) (void)hipHostFree(s->h_out);
+  if (s->h_counters) (void)hipHostFree(s->h_counters);
+  delete s;
+}
+
+uint64_t q3_slot_bytes(const Q3Slot* s) { return s ? s->bytes : 0; }
+
+int q3_batch_launch(gdb_ctx* ctx, Q3Slot* s, const std::vector<gdb_q3_params>& reqs,
+                    hipStream_t st) {
+  const int n = (int)reqs.size();
+  if (n < 1 || n > Q3B_MAX) return fail(GDB_E_INVALID, "Q3 batch size must be 1..64");
+  Q3BatchDesc& d = *s->h_desc;
+  memset(&d, 0, sizeof(d));
+  d.n = n;
+  uint64_t obase = 0;
+  uint32_t maxspan = 1;
+  const uint32_t ndays = (uint32_t)std::max(0, s->dmax - s->dmin + 1);
+  for (int i = 0; i < n; ++i) {
+    const gdb_q3_params& p = reqs[i];
+    if (p.top_k < 1 || p.top_k > Q3B_TOPK) return fail(GDB_E_INVALID, "top_k must be in 1..100");
+    d.k[i] = p.top_k;
+    d.segment[i] = p.segment;
+    d.date[i] = p.date;
+    // window [date - maxlag, date) in day offsets, clamped to the table's days
+    const int64_t lo = std::max<int64_t>((int64_t)p.date - s->maxlag, s->dmin) - s->dmin;
+    const int64_t hi = std::min<int64_t>((int64_t)p.date, (int64_t)s->dmax + 1) - s->dmin;
+    uint32_t b = 0, e = 0;
+    if (hi > lo && ndays) {
+      b = s->h_dayoff[(size_t)std::max<int64_t>(0, lo)];
+      e = s->h_dayoff[(size_t)std::min<int64_t>(ndays, hi)];
+    }
+    s->h_win[i * 4 + 0] = b;
+    s->h_win[i * 4 + 1] = e;
+    s->h_win[i * 4 + 2] = (uint32_t)obase;
+    s->h_win[i * 4 + 3] = e - b;
+    obase += e - b;
+    maxspan = std::max(maxspan, e - b);
+  }
+  if (obase > s->groups_cap) return fail(GDB_E_ADMISSION, "Q3 batch exceeds its group arena");
+  GDB_HIP(hipMemcpyAsync(s->d_desc, s->h_desc, sizeof(Q3BatchDesc), hipMemcpyHostToDevice, st));
+  GDB_HIP(hipMemcpyAsync(s->d_win, s->h_win, n * 16, hipMemcpyHostToDevice, st));
+  q3b_clear<<<64, 256, 0, st>>>(s->counters, s->hist, n);
+  const uint32_t gx = std::max<uint32_t>(1, std::min<uint32_t>((maxspan + 255) / 256,
+                                                               (uint32_t)ctx->info.cu_count * 4 / n + 1));
+  q3b_groups<<<dim3(gx, n), 256, 0, st>>>(s->d_desc, s->d_win, s->byday, s->okey, s->ocust, s->odate,
+                                          s->oprio, s->cseg, s->ncseg, s->csr_off, s->csr_ship,
+                                          s->csr_rev, s->counters, s->hist, s->groups);
+  q3b_threshold<<<n, 256, 0, st>>>(s->d_desc, s->hist, s->counters);
+  q3b_topk<<<n, 256, 0, st>>>(s->d_desc, s->d_win, s->counters, s->groups, s->out);
+  GDB_HIP(hipGetLastError());
+  GDB_HIP(hipMemcpyAsync(s->h_out, s->out, (size_t)n * Q3B_TOPK * sizeof(TopEntry),
+                         hipMemcpyDeviceToHost, st));
+  GDB_HIP(hipMemcpyAsync(s->h_counters, s->counters, n * 16, hipMemcpyDeviceToHost, st));
+  return GDB_OK;
+}
+
+int q3_batch_results(Q3Slot* s, const std::vector<gdb_q3_params>& reqs,
+                     std::vector<gdb_q3_result>* out) {
+  out->assign(reqs.size(), gdb_q3_result{});
+  for (size_t i = 0; i < reqs.size(); ++i) {
+    gdb_q3_result& r = (*out)[i];
+    r.groups = s->h_counters[i * 4];
+    const TopEntry* e = s->h_out + i * Q3B_TOPK;
+    for (int j = 0; j < reqs[i].top_k; ++j) {
+      if (e[j].rev == INT64_MIN) break;
+      gdb_q3_row& row = r.rows[r.nrows++];
+      row.orderkey = e[j].key;
+      row.orderdate = e[j].date;
+      row.shippriority = e[j].prio;
+      row.revenue = e[j].rev;
+    }
+  }
+  return GDB_OK;
+}
+
+}  // namespace gdb

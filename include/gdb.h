@@ -217,6 +217,75 @@ int gdb_join_count(gdb_ctx* ctx, const gdb_table* build, const char* build_col,
                    const gdb_table* probe, const char* probe_col, uint64_t* matches,
                    gdb_timing* timing);
 
+/* ---- serving (no counterpart in the reference, which runs one query per
+ *      command buffer and has no request loop) ----------------------------------
+ * A server owns the context while it runs (do not call other gdb_* functions
+ * on the same context meanwhile). Requests are parameterised Q6 / Q1 / Q3
+ * templates over the configured tables; gdb_submit and gdb_poll are
+ * thread-safe. In GDB_SERVE_BATCHED mode a scheduler thread forms batches
+ * continuously: pending flat-table requests (Q6, Q1) share one fused pass over
+ * the fact table, pending Q3 requests run as one launch over resident join
+ * indexes; up to `slots` batches are in flight on their own streams, and a
+ * batch is admitted only while the HBM footprint of the in-flight batches
+ * stays within `hbm_budget`. GDB_SERVE_SERIAL runs each request alone with the
+ * plain pipeline (gdb_q6 / gdb_q1 / gdb_q3), one at a time. */
+enum { GDB_SERVE_BATCHED = 0, GDB_SERVE_SERIAL = 1 };
+enum { GDB_REQ_Q6 = 0, GDB_REQ_Q1 = 1, GDB_REQ_Q3 = 2 };
+typedef struct gdb_server gdb_server;
+typedef struct {
+  const gdb_table* lineitem_flat;  /* Q6 / Q1 fact table (may be a host table) */
+  const gdb_table* customer;       /* Q3 tables (resident), or NULL for no Q3 */
+  const gdb_table* orders;
+  const gdb_table* lineitem;
+  int32_t mode;                    /* GDB_SERVE_BATCHED / GDB_SERVE_SERIAL */
+  int32_t max_batch;               /* requests per batch, 1..64 */
+  int32_t slots;                   /* batches in flight, 1..4 */
+  int32_t pad;
+  uint64_t hbm_budget;             /* bytes for in-flight batch scratch; 0 = no limit */
+} gdb_serve_cfg;
+typedef struct {
+  int32_t kind;                    /* GDB_REQ_* */
+  int32_t pad;
+  gdb_q6_params q6;
+  gdb_q1_params q1;
+  gdb_q3_params q3;
+} gdb_request;
+typedef struct {
+  uint64_t id;
+  int32_t kind;
+  int32_t status;                  /* GDB_OK or an error code */
+  double latency_ms;               /* submit -> result available */
+  int32_t batch_size;              /* requests that shared this request's pass */
+  int32_t pad;
+  gdb_q6_result q6;
+  gdb_q1_result q1;
+  gdb_q3_result q3;
+} gdb_response;
+typedef struct {
+  uint64_t submitted, completed, rejected, batches;
+  double qps;                      /* completed / (last completion - first submit) */
+  double p50_ms, p99_ms, mean_ms, max_ms;
+  double mean_batch;
+  uint64_t inflight_peak_bytes;
+} gdb_serve_stats;
+int gdb_serve_start(gdb_ctx* ctx, const gdb_serve_cfg* cfg, gdb_server** out);
+/* Thread-safe; GDB_E_ADMISSION if the request alone exceeds the budget. */
+int gdb_submit(gdb_server* s, const gdb_request* req, uint64_t* id);
+/* Up to max completed responses; waits up to timeout_ms for the first. */
+int gdb_poll(gdb_server* s, gdb_response* out, int max, int timeout_ms);
+int gdb_serve_stats_get(gdb_server* s, gdb_serve_stats* out);
+/* Drains in-flight work, stops the scheduler and frees the server. */
+int gdb_serve_stop(gdb_server* s);
+/* Open-loop load: Poisson arrivals at `rate` requests/s for `seconds`, drawn
+ * 50/30/20 from Q6/Q1/Q3 with seeded parameters (window start U[0,2160],
+ * discount band centre U[2,9] -> [c-1, c+1], quantity cap 24 or 25, segment
+ * U{0..4}; Q1 cutoff = start + 365, Q3 date = start + 182, top 10). Waits
+ * for every response; fills stats for this run. q3_share = 0 drops Q3. */
+int gdb_serve_loadgen(gdb_server* s, double rate, double seconds, uint64_t seed, double q3_share,
+                      gdb_serve_stats* stats);
+/* The request the load generator issues as its i-th request for `seed`. */
+void gdb_loadgen_request(uint64_t seed, uint64_t i, double q3_share, gdb_request* out);
+
 /* ---- autotuning (no counterpart in the reference: it hard-codes 2048 x 1024
  *      threadgroups, e.g. src/q6.cpp:72-74) ---------------------------------
  * A tuning entry is the launch shape of a pipeline's dominant kernel:

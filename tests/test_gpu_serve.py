@@ -1,0 +1,110 @@
+"""GPU: the serving loop. Every response in a batched mix equals the same
+request run alone through the plain pipeline (bit-exact); batching actually
+happens; admission rejects requests over the HBM budget; the load generator
+reports sane statistics."""
+import pytest
+
+from gpudbmetalbenchmark_amd import GdbError, Server, generate, loadgen_request, request
+from gpudbmetalbenchmark_amd.abi import GDB_E_ADMISSION
+
+pytestmark = pytest.mark.gpu
+
+FLAT_COLS = ("l_shipdate", "l_quantity", "l_discount", "l_tax", "l_extendedprice",
+             "l_returnflag", "l_linestatus")
+
+
+@pytest.fixture(scope="module")
+def tables(engine):
+    li = generate.lineitem_flat(31, 6_000_007)
+    js = generate.join_set(32, 150_000, 1_500_000, 200_000, 10_000, comments=False)
+    flat = engine.table({k: li[k] for k in FLAT_COLS})
+    t = {k: engine.table(js[k]) for k in ("customer", "orders", "lineitem")}
+    return flat, t
+
+
+def alone(engine, flat, t, kind, p):
+    if kind == "q6":
+        return engine.q6(flat, **p)
+    if kind == "q1":
+        return engine.q1(flat, **p)
+    return engine.q3(t["customer"], t["orders"], t["lineitem"], p["segment"], p["date"], p["top_k"])
+
+
+def same(kind, got, exp):
+    if kind == "q1":
+        strip = lambda rows: [{k: r[k] for k in r if not k.startswith("avg")} for r in rows]  # noqa: E731
+        return strip(got) == strip(exp)
+    return got == exp
+
+
+def run_mix(engine, flat, t, reqs, **kw):
+    expected = [alone(engine, flat, t, k, p) for k, p in reqs]
+    srv = Server(engine, flat, t["customer"], t["orders"], t["lineitem"], **kw)
+    try:
+        ids = {srv.submit(request(k, **p)): i for i, (k, p) in enumerate(reqs)}
+        got = {}
+        while len(got) < len(ids):
+            for r in srv.poll(256, 1000):
+                got[ids[r["id"]]] = r
+        stats = srv.stats()
+    finally:
+        srv.stop()
+    return expected, got, stats
+
+
+@pytest.mark.parametrize("mode", ["batched", "serial"])
+def test_mix_results_equal_alone(engine, tables, mode):
+    flat, t = tables
+    reqs = [loadgen_request(7, i) for i in range(160)]
+    reqs += [("q6", dict(date_lo=0, date_hi=2526, disc_lo=0, disc_hi=10, qty_lt=51)),
+             ("q1", dict(cutoff=-1)), ("q3", dict(segment=4, date=0, top_k=100)),
+             ("q3", dict(segment=2, date=1169, top_k=1))]
+    expected, got, stats = run_mix(engine, flat, t, reqs, mode=mode)
+    for i, (k, p) in enumerate(reqs):
+        assert got[i]["status"] == 0
+        assert same(k, got[i]["result"], expected[i]), (i, k, p)
+    assert stats["completed"] == len(reqs)
+    if mode == "batched":
+        assert max(r["batch_size"] for r in got.values()) > 1
+        assert stats["batches"] < len(reqs)
+
+
+def test_admission_rejects_oversized_requests(engine, tables):
+    flat, t = tables
+    srv = Server(engine, flat, t["customer"], t["orders"], t["lineitem"], hbm_budget=1024)
+    try:
+        with pytest.raises(GdbError) as e:
+            srv.submit(request("q6", date_lo=731, date_hi=1096, disc_lo=5, disc_hi=7, qty_lt=24))
+        assert e.value.code == GDB_E_ADMISSION
+        assert srv.stats()["rejected"] == 1
+    finally:
+        srv.stop()
+
+
+def test_loadgen_stats(engine, tables):
+    flat, t = tables
+    with Server(engine, flat, t["customer"], t["orders"], t["lineitem"]) as srv:
+        st = srv.loadgen(rate=2000, seconds=0.5, seed=3)
+    assert st["completed"] == st["submitted"] == 1000
+    assert 0 < st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
+    assert st["qps"] > 500 and st["mean_batch"] >= 1
+
+
+def test_streamed_flat_table_serving(engine):
+    li = generate.lineitem_flat(5, 3_000_001)
+    host = engine.host_table({k: li[k] for k in FLAT_COLS})
+    res = engine.table({k: li[k] for k in FLAT_COLS})
+    reqs = [loadgen_request(11, i, q3_share=0.0) for i in range(40)]
+    expected = [engine.q6(res, **p) if k == "q6" else engine.q1(res, **p) for k, p in reqs]
+    engine.set_chunk_rows(1 << 20)
+    try:
+        with Server(engine, lineitem_flat=host) as srv:
+            ids = {srv.submit(request(k, **p)): i for i, (k, p) in enumerate(reqs)}
+            got = {}
+            while len(got) < len(ids):
+                for r in srv.poll(256, 1000):
+                    got[ids[r["id"]]] = r
+    finally:
+        engine.set_chunk_rows(0)
+    for i, (k, p) in enumerate(reqs):
+        assert same(k, got[i]["result"], expected[i]), (i, k, p)
