@@ -71,6 +71,9 @@ struct gdb_ctx {
   size_t host_pinned_bytes = 0;
   gdb::Tuning tune[16];
   uint64_t chunk_rows = 0;         // streamed chunk size, 0 = adaptive
+  uint8_t q3_epoch = 0;            // tag of the current Q3 run in its orderkey byte map
+  void* q3_map = nullptr;          // the byte map the tags refer to
+  size_t q3_map_bytes = 0;
   hipEvent_t sev[4] = {};          // copy-done / slot-free events of the two slots
   std::vector<hipEvent_t> chunk_ev;  // per-chunk compute start/end
 };
@@ -99,7 +102,7 @@ int resident_blocks(gdb_ctx* ctx, const void* kernel, int block, size_t lds, int
 
 enum ArenaSlot {
   A_PARTIALS = 0, A_BITMAP = 1, A_MAP = 2, A_HT = 3, A_AGG = 4, A_AUX = 5, A_OUT = 6,
-  A_AUX2 = 7, A_STREAM0 = 8, A_STREAM1 = 9, A_NSLOTS = 16
+  A_AUX2 = 7, A_STREAM0 = 8, A_STREAM1 = 9, A_Q3BYTE = 10, A_Q3MAP = 11, A_NSLOTS = 16
 };
 enum TuneSlot { T_Q6 = 0, T_Q1 = 1, T_SELECT = 2, T_SUM = 3, T_Q3 = 4, T_Q9 = 5, T_Q13 = 6, T_JOIN = 7 };
 

@@ -53,6 +53,13 @@ __device__ __forceinline__ int64_t block_sum(int64_t v, int64_t* red) {
 // ~88% of HBM peak, against ~74% for plain loads.
 typedef int v4i_t __attribute__((ext_vector_type(4)));
 typedef long long v2l_t __attribute__((ext_vector_type(2)));
+// Orders a wave's own LDS writes before its later LDS reads by other lanes.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ int4 ld4nt(const int32_t* p, uint32_t vec) {
   const v4i_t v = __builtin_nontemporal_load(reinterpret_cast<const v4i_t*>(p) + vec);
   return make_int4(v.x, v.y, v.z, v.w);

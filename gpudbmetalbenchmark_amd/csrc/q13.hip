@@ -1,10 +1,10 @@
 // Q13 left-outer count + histogram (reference kernels/q13.metal:33-167, host
 // src/q13.cpp:9-116, post-processing src/infra.h:779-799, SQL sql/q13.sql).
 //
-//   match     (dominant) tiles of 256 orders: their 80-byte comments are copied
-//             global->LDS with contiguous dwordx4 loads (the tile is one
-//             20 KiB run), then each lane searches its comment in LDS for
-//             word1 ... word2 (strmatch.h). The custkeys of orders whose
+//   match     (dominant) each lane loads one 80-byte comment into registers
+//             (five nontemporal dwordx4 loads) and searches it for
+//             word1 ... word2 with exact SWAR candidate masks over
+//             compile-time register indices (strmatch.h). The custkeys of orders whose
 //             comment does NOT match go to the block's own output region, with
 //             a per-block histogram over custkey buckets of 16384 keys
 //   scan      exclusive scan of the [bucket][block] counts (scan_util)
@@ -39,10 +39,64 @@ namespace {
 
 constexpr int TILE = 256;
 constexpr int CW = 80;         // comment width
-constexpr int SLOT = 84;       // LDS bytes per comment: 21 dwords, odd stride (no bank conflicts)
 constexpr int NBINS = 1024;
 constexpr int BUCKET_SHIFT = 14;                 // 16384 custkeys per bucket (64 KiB of LDS counters)
 constexpr uint32_t MAX_BUCKETS = 4096;
+
+bool make_pattern(const char* s, PatternW* p) {
+  const size_t n = strnlen(s, 32);
+  if (n == 0 || n > 31) return false;
+  memset(p, 0, sizeof(*p));
+  uint8_t b[32] = {0};
+  memcpy(b, s, n);
+  memcpy(p->w, b, 32);
+  p->len = (int)n;
+  p->nw = (int)(n + 3) / 4;
+  const int rem = (int)n - 4 * (p->nw - 1);
+  p->last_mask = rem == 4 ? 0xFFFFFFFFu : ((1u << (8 * rem)) - 1u);
+  p->c0 = 0x01010101u * b[0];
+  p->c1 = 0x01010101u * (n > 1 ? b[1] : b[0]);
+  p->m0 = n >= 4 ? 0xFFFFFFFFu : p->last_mask;
+  return true;
+}
+
+// Comment matchers: a functor on the 80-byte field held in 20 dwords followed
+// by 8 zero dwords, true if the field is LIKE '%w1%w2%'.
+template <bool FULL0>
+struct GenericMatch {  // any words of 1..31 bytes
+  PatternW w1, w2;
+  __device__ __forceinline__ bool operator()(const uint32_t* d) const {
+    return like_two<20, FULL0>(d, w1, w2);
+  }
+};
+
+template <int LA, int LB>
+struct FixedMatch {  // words of LA and LB bytes, 4..8 each
+  uint32_t a_head, a_tail, b_head, b_tail;
+  __device__ __forceinline__ bool operator()(const uint32_t* d) const {
+    return like_two_fixed<20, LA, LB>(d, a_head, a_tail, b_head, b_tail);
+  }
+};
+
+// Loads comment row r (80 bytes, 16-byte aligned) into registers and tests
+// NOT LIKE '%w1%w2%'.
+template <class M>
+__device__ __forceinline__ bool q13_counts_row(const uint8_t* __restrict__ comment, uint32_t r,
+                                               const M& match) {
+  uint32_t d[28];
+  const int32_t* base = reinterpret_cast<const int32_t*>(comment);
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    const int4 v = ld4nt(base, r * 5 + q);
+    d[4 * q] = (uint32_t)v.x;
+    d[4 * q + 1] = (uint32_t)v.y;
+    d[4 * q + 2] = (uint32_t)v.z;
+    d[4 * q + 3] = (uint32_t)v.w;
+  }
+#pragma unroll
+  for (int q = 20; q < 28; ++q) d[q] = 0u;
+  return !match(d);
+}
 
 __global__ __launch_bounds__(256) void q13_clear(uint32_t* __restrict__ counts, uint32_t ncounts,
                                                  uint32_t* __restrict__ hist) {
@@ -53,96 +107,82 @@ __global__ __launch_bounds__(256) void q13_clear(uint32_t* __restrict__ counts, 
   }
 }
 
+template <class M>
 __global__ __launch_bounds__(TILE) void q13_count(const uint8_t* __restrict__ comment,
                                                   const int32_t* __restrict__ custkey, uint32_t n,
-                                                  Pattern w1, Pattern w2, uint32_t maxck,
+                                                  M match, uint32_t maxck,
                                                   uint32_t* __restrict__ counts) {
-  __shared__ __align__(16) uint8_t tile[TILE * SLOT + 16];
-  const uint32_t ntiles = (n + TILE - 1) / TILE;
-  uint8_t* mine = tile + threadIdx.x * SLOT;
-  // Zero the 4 pad bytes after every comment once; the copies never touch them.
-  *reinterpret_cast<uint32_t*>(mine + CW) = 0u;
-  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const uint32_t row0 = t * TILE;
-    const uint32_t rows = min((uint32_t)TILE, n - row0);
-    const uint32_t chunks = rows * (CW / 16);
-    const int4* src = reinterpret_cast<const int4*>(comment + (size_t)row0 * CW);
-    __syncthreads();
-    for (uint32_t c = threadIdx.x; c < chunks; c += TILE) {
-      const int4 v = src[c];
-      const uint32_t r = c / (CW / 16), off = (c % (CW / 16)) * 16;
-      uint32_t* d = reinterpret_cast<uint32_t*>(tile + r * SLOT + off);
-      d[0] = (uint32_t)v.x;
-      d[1] = (uint32_t)v.y;
-      d[2] = (uint32_t)v.z;
-      d[3] = (uint32_t)v.w;
-    }
-    __syncthreads();
-    if (threadIdx.x < rows) {
-      const int len = field_len(mine, CW);
-      const int p1 = find_from(mine, len, 0, w1);
-      const bool matches = p1 >= 0 && find_from(mine, len, p1 + w1.len, w2) >= 0;
-      if (!matches) {
-        const uint32_t ck = (uint32_t)custkey[row0 + threadIdx.x];
-        if (ck <= maxck) atomicAdd(&counts[ck], 1u);
-      }
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+    if (q13_counts_row(comment, r, match)) {
+      const uint32_t ck = (uint32_t)custkey[r];
+      if (ck <= maxck) atomicAdd(&counts[ck], 1u);
     }
   }
 }
 
-// Copies tile t (256 comments) into LDS; returns the rows in the tile.
-__device__ __forceinline__ uint32_t q13_load_tile(const uint8_t* __restrict__ comment, uint32_t n,
-                                                  uint32_t t, uint8_t* tile) {
-  const uint32_t row0 = t * TILE;
-  const uint32_t rows = min((uint32_t)TILE, n - row0);
-  const uint32_t chunks = rows * (CW / 16);
-  const int4* src = reinterpret_cast<const int4*>(comment + (size_t)row0 * CW);
-  for (uint32_t c = threadIdx.x; c < chunks; c += TILE) {
-    const int4 v = ld4nt(reinterpret_cast<const int32_t*>(src), c);
-    const uint32_t r = c / (CW / 16), off = (c % (CW / 16)) * 16;
-    uint32_t* d = reinterpret_cast<uint32_t*>(tile + r * SLOT + off);
-    d[0] = (uint32_t)v.x;
-    d[1] = (uint32_t)v.y;
-    d[2] = (uint32_t)v.z;
-    d[3] = (uint32_t)v.w;
-  }
-  return rows;
-}
+// Pass A: block b owns rows [b*tpb*TILE, (b+1)*tpb*TILE); the custkeys of its
+// counted orders go to region b (keys + b*tpb*TILE) in row order, with a
+// per-bucket histogram written k-major to hist[k*G + b]. A wave's 64 comments
+// (5120 contiguous bytes) are read with coalesced dwordx4 loads into the
+// wave's own LDS area and each lane then reads its 80-byte row from there:
+// lane-strided 80-byte loads straight from HBM touch ~40 cache lines per
+// instruction and ran at a third of the coalesced rate.
+constexpr int WROWS = 64;                  // rows per wave and tile
+constexpr int WCHUNKS = WROWS * CW / 16;   // 16-byte chunks per wave and tile (320)
 
-// Pass A: block b owns tiles [b*tpb, (b+1)*tpb); the custkeys of its counted
-// orders go to region b (keys + b*tpb*TILE), with a per-bucket histogram
-// written k-major to hist[k*G + b].
+template <class M>
 __global__ __launch_bounds__(TILE) void q13_match(const uint8_t* __restrict__ comment,
                                                   const int32_t* __restrict__ custkey, uint32_t n,
-                                                  Pattern w1, Pattern w2, uint32_t maxck,
+                                                  M match, uint32_t maxck,
                                                   uint32_t tpb, uint32_t nb,
                                                   uint32_t* __restrict__ keys,
                                                   uint32_t* __restrict__ region_count,
                                                   uint32_t* __restrict__ hist) {
-  __shared__ __align__(16) uint8_t tile[TILE * SLOT + 16];
-  __shared__ uint32_t lhist[MAX_BUCKETS];
+  extern __shared__ uint32_t lhist[];  // nb buckets (dynamic: small for small customer tables)
   __shared__ uint32_t scratch[17];
+  __shared__ int4 stage[TILE / WROWS][WCHUNKS];
   for (uint32_t i = threadIdx.x; i < nb; i += TILE) lhist[i] = 0u;
-  uint8_t* mine = tile + threadIdx.x * SLOT;
-  *reinterpret_cast<uint32_t*>(mine + CW) = 0u;
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int4* ws = stage[wave];
+  const int32_t* base = reinterpret_cast<const int32_t*>(comment);
+  const uint64_t nchunks = (uint64_t)n * (CW / 16);  // 16-byte chunks in the column
   const uint32_t ntiles = (n + TILE - 1) / TILE;
   const uint32_t t0 = blockIdx.x * tpb, t1 = min(ntiles, t0 + tpb);
   uint32_t* out = keys + (size_t)blockIdx.x * tpb * TILE;
   uint32_t count = 0;  // block-uniform
   for (uint32_t t = t0; t < t1; ++t) {
-    __syncthreads();
-    const uint32_t rows = q13_load_tile(comment, n, t, tile);
-    __syncthreads();
+    const uint32_t row0 = t * TILE + wave * WROWS;  // first row of this wave
+    const uint64_t c0 = (uint64_t)row0 * (CW / 16);
+    const int32_t* wbase = base + c0 * 4;
+    int4 v[WCHUNKS / 64];
+#pragma unroll
+    for (int q = 0; q < WCHUNKS / 64; ++q) {
+      const uint32_t c = q * 64 + lane;
+      v[q] = c0 + c < nchunks ? ld4nt(wbase, c) : int4{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int q = 0; q < WCHUNKS / 64; ++q) ws[q * 64 + lane] = v[q];
+    wave_lds_sync();
+    const uint32_t r = row0 + lane;
+    uint32_t d[28];
+#pragma unroll
+    for (int q = 0; q < CW / 16; ++q) {
+      const int4 x = ws[lane * (CW / 16) + q];
+      d[4 * q] = (uint32_t)x.x;
+      d[4 * q + 1] = (uint32_t)x.y;
+      d[4 * q + 2] = (uint32_t)x.z;
+      d[4 * q + 3] = (uint32_t)x.w;
+    }
+#pragma unroll
+    for (int q = 20; q < 28; ++q) d[q] = 0u;
+    wave_lds_sync();  // the area is rewritten by the next tile
     bool q = false;
     uint32_t ck = 0;
-    if (threadIdx.x < rows) {
-      const int len = field_len(mine, CW);
-      const int p1 = find_from(mine, len, 0, w1);
-      const bool matches = p1 >= 0 && find_from(mine, len, p1 + w1.len, w2) >= 0;
-      if (!matches) {
-        ck = (uint32_t)custkey[t * TILE + threadIdx.x];
-        q = ck <= maxck;
-      }
+    if (r < n && !match(d)) {
+      ck = (uint32_t)custkey[r];
+      q = ck <= maxck;
     }
     uint32_t total;
     const uint32_t pos = block_excl_scan(q ? 1u : 0u, scratch, &total);
@@ -217,13 +257,24 @@ __global__ __launch_bounds__(256) void q13_histogram(const int32_t* __restrict__
     if (lh[i]) atomicAdd(&hist[i], lh[i]);
 }
 
-bool make_pattern(const char* s, Pattern* p) {
-  const size_t n = strnlen(s, 32);
-  if (n == 0 || n > 31) return false;
-  memset(p, 0, sizeof(*p));
-  memcpy(p->b, s, n);
-  p->len = (int)n;
-  return true;
+// Little-endian dword of the pattern bytes [at, at + 4).
+uint32_t pattern_dword(const PatternW& p, int at) {
+  const uint8_t* b = reinterpret_cast<const uint8_t*>(p.w);
+  return (uint32_t)b[at] | (uint32_t)b[at + 1] << 8 | (uint32_t)b[at + 2] << 16 |
+         (uint32_t)b[at + 3] << 24;
+}
+
+// Calls launch(FixedMatch<LA, LB>) for the words' lengths if both are 4..8.
+template <int LA, int LB, class F>
+bool dispatch_fixed(const PatternW& a, const PatternW& b, F& launch) {
+  if (a.len == LA && b.len == LB) {
+    launch(FixedMatch<LA, LB>{pattern_dword(a, 0), pattern_dword(a, LA - 4), pattern_dword(b, 0),
+                              pattern_dword(b, LB - 4)});
+    return true;
+  }
+  if constexpr (LB < 8) return dispatch_fixed<LA, LB + 1>(a, b, launch);
+  else if constexpr (LA < 8) return dispatch_fixed<LA + 1, 4>(a, b, launch);
+  else return false;
 }
 
 }  // namespace
@@ -231,7 +282,7 @@ bool make_pattern(const char* s, Pattern* p) {
 extern "C" int gdb_q13(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* orders,
                        const gdb_q13_params* prm, gdb_q13_result* out, gdb_timing* timing) {
   if (!ctx || !customer || !orders || !prm || !out) return fail(GDB_E_INVALID, "null argument");
-  Pattern w1, w2;
+  PatternW w1, w2;
   if (!make_pattern(prm->word1, &w1) || !make_pattern(prm->word2, &w2))
     return fail(GDB_E_INVALID, "Q13 words must be 1..31 bytes");
   const Column *cck, *ock, *ocom;
@@ -248,7 +299,8 @@ extern "C" int gdb_q13(gdb_ctx* ctx, const gdb_table* customer, const gdb_table*
   const uint32_t maxck = (uint32_t)std::max<int64_t>(0, cck->maxv);
   const uint32_t ncounts = maxck + 1;
   const uint32_t nb = (maxck >> BUCKET_SHIFT) + 1;
-  const bool partitioned = !orders->host && nb <= MAX_BUCKETS;
+  const bool partitioned = !orders->host && nb <= MAX_BUCKETS && no < (1ull << 31);
+  const bool full0 = w1.len >= 4 && w2.len >= 4;
   uint32_t *counts, *hist, *overflow, *host;
   {
     char* b;
@@ -266,7 +318,8 @@ extern "C" int gdb_q13(gdb_ctx* ctx, const gdb_table* customer, const gdb_table*
   if (partitioned) {
     const uint32_t n = (uint32_t)no;
     const uint32_t ntiles = (n + TILE - 1) / TILE;
-    uint32_t G = (uint32_t)std::max(1, resident_blocks(ctx, (const void*)q13_match, TILE, 0,
+    const size_t hist_lds = (size_t)nb * 4;
+    uint32_t G = (uint32_t)std::max(1, resident_blocks(ctx, (const void*)q13_match<FixedMatch<7, 8>>, TILE, hist_lds,
                                                        ctx->tune[T_Q13].blocks_per_cu));
     G = std::max<uint32_t>(1, std::min<uint32_t>(G, ntiles));
     const uint32_t tpb = std::max<uint32_t>(1, (ntiles + G - 1) / G);
@@ -284,8 +337,15 @@ extern "C" int gdb_q13(gdb_ctx* ctx, const gdb_table* customer, const gdb_table*
     total = tsum + ntsum;
     q13_clear<<<5, 256, 0, st>>>(counts, 0, hist);
     GDB_HIP(hipEventRecord(ctx->ev[1], st));
-    q13_match<<<G, TILE, 0, st>>>((const uint8_t*)ocom->dev, (const int32_t*)ock->dev, n, w1, w2,
-                                  maxck, tpb, nb, keys, rcount, bh);
+    auto launch = [&](auto match) {
+      q13_match<decltype(match)><<<G, TILE, hist_lds, st>>>((const uint8_t*)ocom->dev,
+                                                     (const int32_t*)ock->dev, n, match, maxck,
+                                                     tpb, nb, keys, rcount, bh);
+    };
+    if (!dispatch_fixed<4, 4>(w1, w2, launch)) {
+      if (full0) launch(GenericMatch<true>{w1, w2});
+      else launch(GenericMatch<false>{w1, w2});
+    }
     GDB_HIP(hipGetLastError());
     GDB_HIP(hipEventRecord(ctx->ev[2], st));
     GDB_TRY(exclusive_scan_u32(ctx, bh, nhist, bh, total, tsum, st));
@@ -296,11 +356,16 @@ extern "C" int gdb_q13(gdb_ctx* ctx, const gdb_table* customer, const gdb_table*
     q13_clear<<<std::min<uint32_t>(4096, (ncounts + NBINS + 4 + 255) / 256), 256, 0, st>>>(
         counts, ncounts, hist);
     const int cresident =
-        resident_blocks(ctx, (const void*)q13_count, TILE, 0, ctx->tune[T_Q13].blocks_per_cu);
+        resident_blocks(ctx, (const void*)q13_count<GenericMatch<true>>, TILE, 0,
+                        ctx->tune[T_Q13].blocks_per_cu);
     auto consume = [&](const void* const* d, uint32_t rows) -> int {
       const int blocks = std::max(1, std::min<int>(cresident, (int)((rows + TILE - 1) / TILE)));
-      q13_count<<<blocks, TILE, 0, st>>>((const uint8_t*)d[0], (const int32_t*)d[1], rows, w1,
-                                          w2, maxck, counts);
+      if (full0)
+        q13_count<<<blocks, TILE, 0, st>>>((const uint8_t*)d[0], (const int32_t*)d[1], rows,
+                                           GenericMatch<true>{w1, w2}, maxck, counts);
+      else
+        q13_count<<<blocks, TILE, 0, st>>>((const uint8_t*)d[0], (const int32_t*)d[1], rows,
+                                           GenericMatch<false>{w1, w2}, maxck, counts);
       GDB_HIP(hipGetLastError());
       return GDB_OK;
     };

@@ -2,24 +2,26 @@
 // host src/q3.cpp:8-173, CPU top-10 src/infra.h:538-564, SQL sql/q3.sql).
 //
 // Plan (all on the engine stream, no host round trip until the k results):
-//   clear    zero the two bitmaps and the counters
-//   customer bit[custkey] = (segment == s)                         (188 KB bitmap)
-//   orders   qualifying = orderdate < D && bit[custkey]:
-//            set obit[orderkey], append (key, date, prio) to a compact list
-//   ht_clear / ht_insert
-//            open-addressing table over the qualifying orders, capacity
-//            nextpow2(2*count) computed on the device from the list count
-//   probe    fact rows with shipdate > D and obit[orderkey] add
-//            extprice*(100-discount) into their order's slot; discount and
-//            price are loaded only for lanes with a surviving row
-//   topk     a 4096-bucket log-linear histogram of the groups' revenues finds
-//            the bucket holding the k-th best; only groups at or above it are
-//            collected, and one block ranks them (revenue desc, orderdate asc,
-//            orderkey asc) with the LDS top-k of topk.h
-// The orders bitmap (maxkey/8 bytes) is the reference's bitmap pre-filter
-// (kernels/q3.metal:72-74) moved onto the probe side, where it rejects ~99%
-// of fact rows before any hash-table access; the table holds only qualifying
-// orders, so it stays resident in the 256 MB Infinity Cache.
+//   clear     zero the customer bitmap, the counters and the revenue histogram
+//   customer  bit[custkey] = (segment == s)                        (188 KB bitmap)
+//   orders    qualifying = orderdate < D && bit[custkey]: append (key, date,
+//             prio) to a list in row order, tag[orderkey] = this run's epoch,
+//             slot[orderkey] = list position (plain stores, no atomics)
+//   acc_clear zero the per-list-entry revenue and hit flags
+//   probe     fact rows with shipdate > D and tag[orderkey] == epoch add
+//             extprice*(100-discount) into their order's list entry; discount
+//             and price are loaded only for lanes with a surviving row, and a
+//             lane's consecutive rows of one order are added once
+//   topk      a 4096-bucket log-linear histogram of the groups' revenues finds
+//             the bucket holding the k-th best; only groups at or above it are
+//             collected, and one block ranks them (revenue desc, orderdate asc,
+//             orderkey asc) with the LDS top-k of topk.h
+// The reference's direct map orders_map[orderkey] (kernels/q3.metal:59-78) and
+// its reset before every run become the byte tags: a run only trusts entries
+// tagged with its own epoch, so neither map is cleared between runs (the
+// byte map is zeroed once every 255 runs). Device-scope global atomics run at
+// ~10 G/s on MI355X, so the build uses none; only the probe's revenue adds are
+// atomic (one per run of rows of one order, ~10^5 per run).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,52 +36,41 @@ using namespace gdbk;
 
 namespace {
 
-constexpr int32_t EMPTY = INT32_MIN;
 constexpr int TOPK_BUF = 2048;
 constexpr uint32_t CAND_CAP = 65536;  // candidate list entries
+constexpr int OTILE_PER_THREAD = 16;
 
-__device__ __forceinline__ uint32_t ht_cap(uint32_t count) {
-  uint32_t c = 1024;
-  while (c < 2 * count) c <<= 1;
-  return c;
-}
-// Folds the key's high bits onto its low bits: keys below the capacity map to
-// themselves, so orders appended and probed in key order touch neighbouring
-// slots (the reference hashes by key modulo size, kernels/q3.metal:189).
-__device__ __forceinline__ uint32_t ht_hash(int32_t key, uint32_t cap) {
-  const uint32_t k = (uint32_t)key;
-  const int lg = __ffs((int)cap) - 1;
-  return (k ^ (k >> lg) ^ (k >> (2 * lg < 32 ? 2 * lg : 31))) & (cap - 1);
-}
 __device__ __forceinline__ bool bit_test(const uint32_t* __restrict__ bm, uint32_t k) {
   return (bm[k >> 5] >> (k & 31)) & 1u;
 }
 
 struct Q3Scratch {
   uint32_t* cbm;      // customer bitmap
-  uint32_t* obm;      // qualifying-orders bitmap
-  uint32_t* counters; // [0] qualifying orders, [1] groups
-  int32_t* qkey;      // compact list of qualifying orders
+  uint32_t* counters; // [0] qualifying orders, [1] groups, [2] threshold, [3] candidates
+  uint32_t* hist;     // 4096 revenue buckets
+  int32_t* qkey;      // list of qualifying orders
   int32_t* qdate;
   int32_t* qprio;
-  int32_t* ht_key;
-  int32_t* ht_q;      // slot -> list index
-  int64_t* ht_rev;
-  uint32_t* ht_hit;
-  uint32_t* hist;     // 4096 revenue buckets
+  int64_t* rev;       // per list entry
+  uint8_t* hit;
+  uint8_t* tag;       // [maxok+1] epoch of the run that listed the key
+  int32_t* slot;      // [maxok+1] list position (valid where tag == epoch)
   TopEntry* cand;     // CAND_CAP candidates
   TopEntry* out;      // k final
 };
 
 __global__ __launch_bounds__(256) void q3_clear(uint32_t* __restrict__ a, uint32_t na,
-                                                uint32_t* __restrict__ b, uint32_t nb,
-                                                uint32_t* __restrict__ counters) {
+                                                uint32_t* __restrict__ b, uint32_t nb) {
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na + nb; i += stride) {
     if (i < na) a[i] = 0u;
     else b[i - na] = 0u;
   }
-  if (blockIdx.x == 0 && threadIdx.x < 4) counters[threadIdx.x] = 0u;
+}
+
+__global__ __launch_bounds__(256) void q3_fill_bytes(uint32_t* __restrict__ p, uint32_t nwords) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += stride) p[i] = 0u;
 }
 
 __global__ __launch_bounds__(256) void q3_customer(const int32_t* __restrict__ custkey,
@@ -92,121 +83,100 @@ __global__ __launch_bounds__(256) void q3_customer(const int32_t* __restrict__ c
   if (seg[i] == (uint8_t)segment && ck <= maxck) atomicOr(&cbm[ck >> 5], 1u << (ck & 31));
 }
 
-// Tiles of 16 x 256 rows, thread t owning rows [16t, 16t+16) of the tile
-// (four dwordx4 loads per column). The qualifying rows of a tile are appended
-// with one atomic per tile and a block-wide scan for the slots, so the list
-// comes out in row order: for orders stored in key order the table inserts and
-// bitmap updates that follow hit neighbouring addresses and coalesce.
-constexpr int OTILE_PER_THREAD = 16;
+// Tiles of 16 x 256 rows: thread t reads rows 4t..4t+3 of each of the tile's
+// four 1024-row quarters (coalesced dwordx4 loads, all twelve issued before
+// the first use), then the 16 customer-bitmap lookups together. Qualifying
+// rows are appended with one atomic per tile and a block-wide scan for the
+// slots, written from the values still held in registers. The list order
+// (tile, thread, quarter, row) is deterministic; nothing downstream depends on
+// it.
 __global__ __launch_bounds__(256) void q3_orders(const int32_t* __restrict__ okey,
                                                  const int32_t* __restrict__ ocust,
                                                  const int32_t* __restrict__ odate,
                                                  const int32_t* __restrict__ oprio, uint32_t n,
                                                  int32_t date, uint32_t maxck, uint32_t maxok,
-                                                 const uint32_t* __restrict__ cbm,
-                                                 uint32_t* __restrict__ obm,
-                                                 uint32_t* __restrict__ counters,
-                                                 int32_t* __restrict__ qkey,
-                                                 int32_t* __restrict__ qdate,
-                                                 int32_t* __restrict__ qprio) {
+                                                 uint8_t epoch, Q3Scratch s) {
+  constexpr int R = OTILE_PER_THREAD;
+  constexpr int V = R / 4;
   __shared__ uint32_t scratch[17];
   __shared__ uint32_t base_sh;
-  const uint32_t tile_rows = OTILE_PER_THREAD * blockDim.x;
+  const uint32_t tile_rows = R * blockDim.x;
   const uint32_t ntiles = (n + tile_rows - 1) / tile_rows;
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const uint32_t r0 = t * tile_rows + threadIdx.x * OTILE_PER_THREAD;
-    uint32_t mask = 0;
-    if (r0 + OTILE_PER_THREAD <= n) {
+    int32_t ks[R], cs[R], ds[R];
+    uint32_t rows[V];
 #pragma unroll
-      for (int v = 0; v < OTILE_PER_THREAD / 4; ++v) {
-        const int4 d4 = ld4(odate, r0 / 4 + v), c4 = ld4(ocust, r0 / 4 + v), k4 = ld4(okey, r0 / 4 + v);
-        const int32_t ds[4] = {d4.x, d4.y, d4.z, d4.w}, cs[4] = {c4.x, c4.y, c4.z, c4.w},
-                      ks[4] = {k4.x, k4.y, k4.z, k4.w};
+    for (int v = 0; v < V; ++v) rows[v] = t * tile_rows + v * 4 * blockDim.x + 4 * threadIdx.x;
+    if (t * tile_rows + tile_rows <= n) {
+      int4 k4[V], c4[V], d4[V];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const uint32_t ck = (uint32_t)cs[j];
-          if (ds[j] < date && ck <= maxck && (uint32_t)ks[j] <= maxok && bit_test(cbm, ck))
-            mask |= 1u << (v * 4 + j);
-        }
+      for (int v = 0; v < V; ++v) {
+        k4[v] = ld4(okey, rows[v] / 4);
+        c4[v] = ld4(ocust, rows[v] / 4);
+        d4[v] = ld4(odate, rows[v] / 4);
+      }
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        ks[4 * v] = k4[v].x; ks[4 * v + 1] = k4[v].y; ks[4 * v + 2] = k4[v].z; ks[4 * v + 3] = k4[v].w;
+        cs[4 * v] = c4[v].x; cs[4 * v + 1] = c4[v].y; cs[4 * v + 2] = c4[v].z; cs[4 * v + 3] = c4[v].w;
+        ds[4 * v] = d4[v].x; ds[4 * v + 1] = d4[v].y; ds[4 * v + 2] = d4[v].z; ds[4 * v + 3] = d4[v].w;
       }
     } else {
-      for (int j = 0; j < OTILE_PER_THREAD; ++j) {
-        const uint32_t r = r0 + j;
-        if (r < n) {
-          const uint32_t ck = (uint32_t)ocust[r];
-          if (odate[r] < date && ck <= maxck && (uint32_t)okey[r] <= maxok && bit_test(cbm, ck))
-            mask |= 1u << j;
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const uint32_t r = rows[v] + j;
+          const bool in = r < n;
+          ks[4 * v + j] = in ? okey[r] : 0;
+          cs[4 * v + j] = in ? ocust[r] : 0;
+          ds[4 * v + j] = in ? odate[r] : date;  // fails the date test
         }
       }
+    }
+    uint32_t words[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) words[j] = s.cbm[min((uint32_t)cs[j], maxck) >> 5];
+    uint32_t mask = 0;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const uint32_t ck = (uint32_t)cs[j];
+      const bool q = ds[j] < date && ck <= maxck && (uint32_t)ks[j] <= maxok &&
+                     ((words[j] >> (ck & 31)) & 1u);
+      mask |= (q ? 1u : 0u) << j;
     }
     uint32_t total;
     uint32_t pos = block_excl_scan((uint32_t)__popc(mask), scratch, &total);
     if (total == 0) continue;
-    if (threadIdx.x == 0) base_sh = atomicAdd(&counters[0], total);
+    if (threadIdx.x == 0) base_sh = atomicAdd(&s.counters[0], total);
     __syncthreads();
     pos += base_sh;
-    while (mask) {
-      const int j = __ffs((int)mask) - 1;
-      mask &= mask - 1;
-      const uint32_t r = r0 + j;
-      const int32_t k = okey[r];
-      qkey[pos] = k;
-      qdate[pos] = odate[r];
-      qprio[pos] = oprio[r];
-      ++pos;
-      atomicOr(&obm[(uint32_t)k >> 5], 1u << ((uint32_t)k & 31));
+    if (mask) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        if ((mask >> j) & 1u) {
+          const uint32_t k = (uint32_t)ks[j];
+          s.qkey[pos] = ks[j];
+          s.qdate[pos] = ds[j];
+          s.qprio[pos] = oprio[rows[j / 4] + (j & 3)];
+          s.tag[k] = epoch;
+          s.slot[k] = (int32_t)pos;
+          ++pos;
+        }
+      }
     }
     __syncthreads();  // base_sh is rewritten by the next tile
   }
 }
 
-__global__ __launch_bounds__(256) void q3_ht_clear(const uint32_t* __restrict__ counters,
-                                                   int32_t* __restrict__ ht_key) {
-  const uint32_t cap = ht_cap(counters[0]);
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += stride) ht_key[i] = EMPTY;
-}
-
-__global__ __launch_bounds__(256) void q3_ht_insert(const uint32_t* __restrict__ counters,
-                                                    const int32_t* __restrict__ qkey,
-                                                    int32_t* __restrict__ ht_key,
-                                                    int32_t* __restrict__ ht_q,
-                                                    int64_t* __restrict__ ht_rev,
-                                                    uint32_t* __restrict__ ht_hit) {
+__global__ __launch_bounds__(256) void q3_acc_clear(const uint32_t* __restrict__ counters,
+                                                    int64_t* __restrict__ rev,
+                                                    uint8_t* __restrict__ hit) {
   const uint32_t count = counters[0];
-  const uint32_t cap = ht_cap(count);
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += stride) {
-    const int32_t key = qkey[j];
-    uint32_t h = ht_hash(key, cap);
-    for (uint32_t probe = 0; probe < cap; ++probe) {
-      const int32_t prev = atomicCAS(&ht_key[h], EMPTY, key);
-      if (prev == EMPTY) {
-        ht_q[h] = (int32_t)j;
-        ht_rev[h] = 0;
-        ht_hit[h] = 0u;
-        break;
-      }
-      if (prev == key) break;  // duplicate orderkey: the first insert keeps the slot
-      h = (h + 1) & (cap - 1);
-    }
-  }
-}
-
-__device__ __forceinline__ void q3_flush(int32_t key, int64_t rev, uint32_t cap,
-                                         const int32_t* __restrict__ ht_key,
-                                         int64_t* __restrict__ ht_rev,
-                                         uint32_t* __restrict__ ht_hit) {
-  uint32_t h = ht_hash(key, cap);
-  for (uint32_t probe = 0; probe < cap; ++probe) {
-    const int32_t k = ht_key[h];
-    if (k == key) {
-      atomicAdd((unsigned long long*)&ht_rev[h], (unsigned long long)rev);
-      ht_hit[h] = 1u;
-      return;
-    }
-    if (k == EMPTY) return;
-    h = (h + 1) & (cap - 1);
+    rev[j] = 0;
+    hit[j] = 0;
   }
 }
 
@@ -215,19 +185,25 @@ struct Q3Fact {
   const int64_t* price;
 };
 
-__device__ __forceinline__ void q3_group(const Q3Fact& f, uint32_t g, int32_t date,
-                                         uint32_t maxok, const uint32_t* __restrict__ obm,
-                                         uint32_t cap, const int32_t* __restrict__ ht_key,
-                                         int64_t* __restrict__ ht_rev,
-                                         uint32_t* __restrict__ ht_hit) {
-  const int4 k4 = ld4(f.okey, g), s4 = ld4(f.ship, g);
+__device__ __forceinline__ void q3_flush(int32_t key, int64_t r, const Q3Scratch& s) {
+  const int32_t j = s.slot[(uint32_t)key];
+  atomicAdd((unsigned long long*)&s.rev[j], (unsigned long long)r);
+  s.hit[j] = 1;
+}
+
+// Rows of one 4-row group whose keys and ship dates are loaded and whose
+// order tags have been fetched: discount and price are loaded only if a row
+// survives, and a lane's consecutive rows of one order are added once.
+__device__ __forceinline__ void q3_finish(const Q3Fact& f, uint32_t g, int4 k4, int4 s4,
+                                          const uint8_t* tg, int32_t date, uint32_t maxok,
+                                          uint8_t epoch, const Q3Scratch& s) {
   const int32_t ks[4] = {k4.x, k4.y, k4.z, k4.w};
   const int32_t ss[4] = {s4.x, s4.y, s4.z, s4.w};
   bool pass[4];
   bool any = false;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    pass[j] = ss[j] > date && (uint32_t)ks[j] <= maxok && bit_test(obm, (uint32_t)ks[j]);
+    pass[j] = ss[j] > date && (uint32_t)ks[j] <= maxok && tg[j] == epoch;
     any |= pass[j];
   }
   if (!any) return;
@@ -245,83 +221,93 @@ __device__ __forceinline__ void q3_group(const Q3Fact& f, uint32_t g, int32_t da
     if (open && ks[j] == run_key) {
       run_rev += r;
     } else {
-      if (open) q3_flush(run_key, run_rev, cap, ht_key, ht_rev, ht_hit);
+      if (open) q3_flush(run_key, run_rev, s);
       run_key = ks[j];
       run_rev = r;
       open = true;
     }
   }
-  if (open) q3_flush(run_key, run_rev, cap, ht_key, ht_rev, ht_hit);
+  if (open) q3_flush(run_key, run_rev, s);
 }
 
+// U groups of 4 rows per thread and step: all key and ship-date loads are
+// issued first, then all order-tag gathers (keys are clamped into the map; a
+// clamped key fails the key test), then the groups are finished.
 template <int U>
 __global__ __launch_bounds__(256) void q3_probe(Q3Fact f, uint32_t n, int32_t date, uint32_t maxok,
-                                                const uint32_t* __restrict__ obm,
-                                                const uint32_t* __restrict__ counters,
-                                                const int32_t* __restrict__ ht_key,
-                                                int64_t* __restrict__ ht_rev,
-                                                uint32_t* __restrict__ ht_hit) {
-  const uint32_t cap = ht_cap(counters[0]);
+                                                uint8_t epoch, Q3Scratch s) {
   const uint32_t nvec = n >> 2;
   const uint32_t nthr = gridDim.x * blockDim.x;
   uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   for (; g + (U - 1) * nthr < nvec; g += U * nthr) {
+    int4 k4[U], s4[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) q3_group(f, g + u * nthr, date, maxok, obm, cap, ht_key, ht_rev, ht_hit);
+    for (int u = 0; u < U; ++u) {
+      k4[u] = ld4nt(f.okey, g + u * nthr);
+      s4[u] = ld4nt(f.ship, g + u * nthr);
+    }
+    uint8_t tg[U][4];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      tg[u][0] = s.tag[min((uint32_t)k4[u].x, maxok)];
+      tg[u][1] = s.tag[min((uint32_t)k4[u].y, maxok)];
+      tg[u][2] = s.tag[min((uint32_t)k4[u].z, maxok)];
+      tg[u][3] = s.tag[min((uint32_t)k4[u].w, maxok)];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) q3_finish(f, g + u * nthr, k4[u], s4[u], tg[u], date, maxok, epoch, s);
   }
-  for (; g < nvec; g += nthr) q3_group(f, g, date, maxok, obm, cap, ht_key, ht_rev, ht_hit);
+  for (; g < nvec; g += nthr) {
+    const int4 k4 = ld4nt(f.okey, g), s4 = ld4nt(f.ship, g);
+    const uint8_t tg[4] = {s.tag[min((uint32_t)k4.x, maxok)], s.tag[min((uint32_t)k4.y, maxok)],
+                           s.tag[min((uint32_t)k4.z, maxok)], s.tag[min((uint32_t)k4.w, maxok)]};
+    q3_finish(f, g, k4, s4, tg, date, maxok, epoch, s);
+  }
   if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
     const uint32_t r = nvec * 4 + threadIdx.x;
     const int32_t k = f.okey[r];
-    if (f.ship[r] > date && (uint32_t)k <= maxok && bit_test(obm, (uint32_t)k))
-      q3_flush(k, f.price[r] * (int64_t)(100 - f.disc[r]), cap, ht_key, ht_rev, ht_hit);
+    if (f.ship[r] > date && (uint32_t)k <= maxok && s.tag[(uint32_t)k] == epoch)
+      q3_flush(k, f.price[r] * (int64_t)(100 - f.disc[r]), s);
   }
 }
 
 // Top-k, step 1: histogram of the groups' revenue buckets (LDS per block,
 // merged by global atomics) and the group count.
-__global__ __launch_bounds__(256) void q3_topk_hist(const uint32_t* __restrict__ counters_in,
-                                                    const int32_t* __restrict__ ht_key,
-                                                    const int64_t* __restrict__ ht_rev,
-                                                    const uint32_t* __restrict__ ht_hit,
-                                                    uint32_t* __restrict__ hist,
-                                                    uint32_t* __restrict__ counters) {
+__global__ __launch_bounds__(256) void q3_topk_hist(Q3Scratch s) {
   __shared__ uint32_t lh[4096];
   __shared__ int64_t red[4];
   for (int i = threadIdx.x; i < 4096; i += blockDim.x) lh[i] = 0u;
   __syncthreads();
-  const uint32_t cap = ht_cap(counters_in[0]);
+  const uint32_t count = s.counters[0];
   const uint32_t stride = gridDim.x * blockDim.x;
   uint32_t hits = 0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += stride) {
-    if (ht_key[i] != EMPTY && ht_hit[i]) {
-      atomicAdd(&lh[value_bucket(ht_rev[i])], 1u);
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += stride) {
+    if (s.hit[j]) {
+      atomicAdd(&lh[value_bucket(s.rev[j])], 1u);
       ++hits;
     }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < 4096; i += blockDim.x)
-    if (lh[i]) atomicAdd(&hist[i], lh[i]);
+    if (lh[i]) atomicAdd(&s.hist[i], lh[i]);
   const int64_t bh = block_sum((int64_t)hits, red);
-  if (threadIdx.x == 0 && bh) atomicAdd(&counters[1], (uint32_t)bh);
+  if (threadIdx.x == 0 && bh) atomicAdd(&s.counters[1], (uint32_t)bh);
 }
 
 // Step 2 (one block): the highest bucket b such that buckets >= b hold at
 // least k groups (0 when there are fewer than k groups).
-__global__ __launch_bounds__(256) void q3_topk_threshold(const uint32_t* __restrict__ hist, int k,
-                                                         uint32_t* __restrict__ counters) {
+__global__ __launch_bounds__(256) void q3_topk_threshold(int k, Q3Scratch s) {
   __shared__ uint32_t part[256];
-  // Thread t sums buckets [16t, 16t+16); then a serial suffix walk on thread 0.
-  uint32_t s = 0;
-  for (int i = 0; i < 16; ++i) s += hist[threadIdx.x * 16 + i];
-  part[threadIdx.x] = s;
+  uint32_t v = 0;
+  for (int i = 0; i < 16; ++i) v += s.hist[threadIdx.x * 16 + i];
+  part[threadIdx.x] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t acc = 0, b = 0;
     for (int t = 255; t >= 0; --t) {
       if (acc + part[t] >= (uint32_t)k) {
         for (int i = 15; i >= 0; --i) {
-          acc += hist[t * 16 + i];
+          acc += s.hist[t * 16 + i];
           if (acc >= (uint32_t)k) {
             b = (uint32_t)(t * 16 + i);
             break;
@@ -331,85 +317,62 @@ __global__ __launch_bounds__(256) void q3_topk_threshold(const uint32_t* __restr
       }
       acc += part[t];
     }
-    counters[2] = b;
-    counters[3] = 0u;  // candidate count
+    s.counters[2] = b;
+    s.counters[3] = 0u;
   }
 }
 
 // Step 3: groups in buckets >= threshold -> candidate list (tile-aggregated append).
-__global__ __launch_bounds__(256) void q3_topk_collect(const int32_t* __restrict__ ht_key,
-                                                       const int32_t* __restrict__ ht_q,
-                                                       const int64_t* __restrict__ ht_rev,
-                                                       const uint32_t* __restrict__ ht_hit,
-                                                       const int32_t* __restrict__ qdate,
-                                                       const int32_t* __restrict__ qprio,
-                                                       uint32_t* __restrict__ counters,
-                                                       uint32_t cand_cap,
-                                                       TopEntry* __restrict__ cand) {
+__global__ __launch_bounds__(256) void q3_topk_collect(Q3Scratch s) {
   __shared__ uint32_t scratch[17];
   __shared__ uint32_t base_sh;
-  const uint32_t cap = ht_cap(counters[0]);
-  const uint32_t thr = counters[2];
+  const uint32_t count = s.counters[0];
+  const uint32_t thr = s.counters[2];
   const uint32_t stride = gridDim.x * blockDim.x;
-  const uint32_t span = (cap + stride - 1) / stride * stride;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < span; i += stride) {
-    bool q = false;
-    if (i < cap && ht_key[i] != EMPTY && ht_hit[i]) q = value_bucket(ht_rev[i]) >= thr;
+  const uint32_t span = (count + stride - 1) / stride * stride;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < span; j += stride) {
+    const bool q = j < count && s.hit[j] && value_bucket(s.rev[j]) >= thr;
     uint32_t total;
     const uint32_t pre = block_excl_scan(q ? 1u : 0u, scratch, &total);
     if (total == 0) continue;
-    if (threadIdx.x == 0) base_sh = atomicAdd(&counters[3], total);
+    if (threadIdx.x == 0) base_sh = atomicAdd(&s.counters[3], total);
     __syncthreads();
     const uint32_t pos = base_sh + pre;
-    if (q && pos < cand_cap) {
-      const int32_t j = ht_q[i];
-      cand[pos] = TopEntry{ht_rev[i], qdate[j], ht_key[i], qprio[j], 0};
-    }
+    if (q && pos < CAND_CAP) s.cand[pos] = TopEntry{s.rev[j], s.qdate[j], s.qkey[j], s.qprio[j], 0};
     __syncthreads();
   }
 }
 
 // Step 4 (one block): rank the candidates; if more than fit the list (heavy
-// ties in the threshold bucket), rank the qualifying table slots directly.
-__global__ __launch_bounds__(256) void q3_topk_final(const TopEntry* __restrict__ cand,
-                                                     uint32_t cand_cap,
-                                                     const uint32_t* __restrict__ counters,
-                                                     const int32_t* __restrict__ ht_key,
-                                                     const int32_t* __restrict__ ht_q,
-                                                     const int64_t* __restrict__ ht_rev,
-                                                     const uint32_t* __restrict__ ht_hit,
-                                                     const int32_t* __restrict__ qdate,
-                                                     const int32_t* __restrict__ qprio, int k,
-                                                     TopEntry* __restrict__ out) {
+// ties in the threshold bucket), rank the qualifying list entries directly.
+__global__ __launch_bounds__(256) void q3_topk_final(int k, Q3Scratch s) {
   __shared__ TopEntry buf[TOPK_BUF];
   __shared__ int nbuf;
   BlockTopK<TOPK_BUF> tk{buf, &nbuf, k};
   tk.init();
-  const uint32_t nc = counters[3];
-  if (nc <= cand_cap) {
+  const uint32_t nc = s.counters[3];
+  if (nc <= CAND_CAP) {
     for (uint32_t base = 0; base < nc; base += blockDim.x) {
       const uint32_t i = base + threadIdx.x;
       TopEntry e;
       const bool valid = i < nc;
-      if (valid) e = cand[i];
+      if (valid) e = s.cand[i];
       tk.push(valid, e);
     }
   } else {
-    const uint32_t cap = ht_cap(counters[0]);
-    const uint32_t thr = counters[2];
-    for (uint32_t base = 0; base < cap; base += blockDim.x) {
-      const uint32_t i = base + threadIdx.x;
+    const uint32_t count = s.counters[0], thr = s.counters[2];
+    for (uint32_t base = 0; base < count; base += blockDim.x) {
+      const uint32_t j = base + threadIdx.x;
       TopEntry e;
       bool valid = false;
-      if (i < cap && ht_key[i] != EMPTY && ht_hit[i] && value_bucket(ht_rev[i]) >= thr) {
-        const int32_t j = ht_q[i];
-        e = TopEntry{ht_rev[i], qdate[j], ht_key[i], qprio[j], 0};
+      if (j < count && s.hit[j] && value_bucket(s.rev[j]) >= thr) {
+        e = TopEntry{s.rev[j], s.qdate[j], s.qkey[j], s.qprio[j], 0};
         valid = true;
       }
-      tk.push(valid, e);
+      if (__syncthreads_or(valid)) tk.push(valid, e);
     }
   }
-  tk.finish(out);
+  tk.finish(s.out);
 }
 
 }  // namespace
@@ -440,31 +403,34 @@ extern "C" int gdb_q3(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* 
   const uint64_t nl = lineitem->nrows;
   const uint32_t maxck = (uint32_t)std::max<int64_t>(0, cck->maxv);
   const uint32_t maxok = (uint32_t)std::max<int64_t>(0, cok->maxv);
-  const uint32_t ncw = maxck / 32 + 1, now_ = maxok / 32 + 1;
-  uint64_t capmax = 1024;
-  while (capmax < 2ull * no) capmax <<= 1;
+  const uint32_t ncw = maxck / 32 + 1;
   const int k = p->top_k;
+  const uint32_t nlist = std::max<uint32_t>(no, 1);
 
-  // Scratch: bitmaps + counters + list (A_BITMAP), table (A_HT), top-k (A_AUX).
   Q3Scratch s{};
   {
     char* b;
-    const size_t bm_bytes = ((size_t)ncw + now_ + 16 + 4096) * 4;
-    const size_t list_bytes = (size_t)std::max<uint32_t>(no, 1) * 12;
-    GDB_TRY(arena(ctx, A_BITMAP, bm_bytes + list_bytes + 256, (void**)&b));
-    s.cbm = (uint32_t*)b;
-    s.obm = s.cbm + ncw;
-    s.counters = s.obm + now_;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+      const size_t o = off;
+      off += (bytes + 255) & ~size_t(255);
+      return o;
+    };
+    const size_t o_cbm = take(ncw * 4ull), o_cnt = take((16 + 4096) * 4), o_qk = take(nlist * 4ull),
+                 o_qd = take(nlist * 4ull), o_qp = take(nlist * 4ull), o_rev = take(nlist * 8ull),
+                 o_hit = take(nlist);
+    GDB_TRY(arena(ctx, A_BITMAP, off, (void**)&b));
+    s.cbm = (uint32_t*)(b + o_cbm);
+    s.counters = (uint32_t*)(b + o_cnt);
     s.hist = s.counters + 16;
-    s.qkey = (int32_t*)(b + ((bm_bytes + 255) & ~size_t(255)));
-    s.qdate = s.qkey + std::max<uint32_t>(no, 1);
-    s.qprio = s.qdate + std::max<uint32_t>(no, 1);
-    char* h;
-    GDB_TRY(arena(ctx, A_HT, (size_t)capmax * 20 + 256, (void**)&h));
-    s.ht_rev = (int64_t*)h;
-    s.ht_key = (int32_t*)(h + capmax * 8);
-    s.ht_q = s.ht_key + capmax;
-    s.ht_hit = (uint32_t*)(s.ht_q + capmax);
+    s.qkey = (int32_t*)(b + o_qk);
+    s.qdate = (int32_t*)(b + o_qd);
+    s.qprio = (int32_t*)(b + o_qp);
+    s.rev = (int64_t*)(b + o_rev);
+    s.hit = (uint8_t*)(b + o_hit);
+    const size_t tag_bytes = ((size_t)maxok + 1 + 255) & ~size_t(255);
+    GDB_TRY(arena(ctx, A_Q3BYTE, tag_bytes, (void**)&s.tag));
+    GDB_TRY(arena(ctx, A_Q3MAP, ((size_t)maxok + 1) * 4, (void**)&s.slot));
     char* a;
     GDB_TRY(arena(ctx, A_AUX, sizeof(TopEntry) * ((size_t)CAND_CAP + k) + 64, (void**)&a));
     s.cand = (TopEntry*)a;
@@ -476,29 +442,36 @@ extern "C" int gdb_q3(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* 
 
   hipStream_t st = ctx->stream;
   const int B = 256;
+  const int fill_blocks = ctx->info.cu_count * 4;
   GDB_HIP(hipEventRecord(ctx->ev[0], st));
-  q3_clear<<<std::min<uint32_t>(2048, (ncw + now_ + 16 + 4096 + B - 1) / B + 1), B, 0, st>>>(
-      s.cbm, ncw, s.obm, now_ + 16 + 4096, s.counters);
+  // Epoch of this run in the tag map; a new or reused-after-wrap map is zeroed.
+  const size_t tag_bytes = ((size_t)maxok + 1 + 255) & ~size_t(255);
+  if (ctx->q3_map != (void*)s.tag || ctx->q3_map_bytes < tag_bytes || ctx->q3_epoch == 255) {
+    const size_t alloc_bytes = ctx->arenas[A_Q3BYTE].bytes;
+    q3_fill_bytes<<<fill_blocks, B, 0, st>>>((uint32_t*)s.tag, (uint32_t)(alloc_bytes / 4));
+    ctx->q3_map = s.tag;
+    ctx->q3_map_bytes = alloc_bytes;
+    ctx->q3_epoch = 0;
+  }
+  const uint8_t epoch = ++ctx->q3_epoch;
+  q3_clear<<<std::min<uint32_t>(2048, (ncw + 16 + 4096 + B - 1) / B + 1), B, 0, st>>>(
+      s.cbm, ncw, s.counters, 16 + 4096);
   if (nc)
     q3_customer<<<(nc + B - 1) / B, B, 0, st>>>((const int32_t*)cck->dev, (const uint8_t*)cseg->dev,
                                                  nc, p->segment, maxck, s.cbm);
   if (no)
     q3_orders<<<std::min<uint32_t>(ctx->info.cu_count * 8, (no + 16 * B - 1) / (16 * B)), B, 0, st>>>(
         (const int32_t*)cok->dev, (const int32_t*)coc->dev, (const int32_t*)cod->dev,
-        (const int32_t*)cop->dev, no, p->date, maxck, maxok, s.cbm, s.obm, s.counters, s.qkey,
-        s.qdate, s.qprio);
-  const int fill_blocks = ctx->info.cu_count * 4;
-  q3_ht_clear<<<fill_blocks, B, 0, st>>>(s.counters, s.ht_key);
-  q3_ht_insert<<<fill_blocks, B, 0, st>>>(s.counters, s.qkey, s.ht_key, s.ht_q, s.ht_rev, s.ht_hit);
+        (const int32_t*)cop->dev, no, p->date, maxck, maxok, epoch, s);
+  q3_acc_clear<<<fill_blocks, B, 0, st>>>(s.counters, s.rev, s.hit);
   GDB_HIP(hipGetLastError());
   const int president =
-      resident_blocks(ctx, (const void*)q3_probe<2>, B, 0, ctx->tune[T_Q3].blocks_per_cu);
+      resident_blocks(ctx, (const void*)q3_probe<4>, B, 0, ctx->tune[T_Q3].blocks_per_cu);
   auto consume = [&](const void* const* d, uint32_t rows) -> int {
     const Q3Fact f{(const int32_t*)d[0], (const int32_t*)d[1], (const int32_t*)d[2],
                    (const int64_t*)d[3]};
     const int pblocks = std::max(1, std::min<int>(president, (int)((rows / 4 + B - 1) / B)));
-    q3_probe<2><<<pblocks, B, 0, st>>>(f, rows, p->date, maxok, s.obm, s.counters, s.ht_key,
-                                       s.ht_rev, s.ht_hit);
+    q3_probe<4><<<pblocks, B, 0, st>>>(f, rows, p->date, maxok, epoch, s);
     GDB_HIP(hipGetLastError());
     return GDB_OK;
   };
@@ -510,13 +483,10 @@ extern "C" int gdb_q3(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* 
     GDB_HIP(hipEventRecord(ctx->ev[1], st));
     GDB_HIP(hipEventRecord(ctx->ev[2], st));
   }
-  q3_topk_hist<<<fill_blocks, B, 0, st>>>(s.counters, s.ht_key, s.ht_rev, s.ht_hit, s.hist,
-                                          s.counters);
-  q3_topk_threshold<<<1, B, 0, st>>>(s.hist, k, s.counters);
-  q3_topk_collect<<<fill_blocks, B, 0, st>>>(s.ht_key, s.ht_q, s.ht_rev, s.ht_hit, s.qdate, s.qprio,
-                                             s.counters, CAND_CAP, s.cand);
-  q3_topk_final<<<1, B, 0, st>>>(s.cand, CAND_CAP, s.counters, s.ht_key, s.ht_q, s.ht_rev,
-                                 s.ht_hit, s.qdate, s.qprio, k, s.out);
+  q3_topk_hist<<<fill_blocks, B, 0, st>>>(s);
+  q3_topk_threshold<<<1, B, 0, st>>>(k, s);
+  q3_topk_collect<<<fill_blocks, B, 0, st>>>(s);
+  q3_topk_final<<<1, B, 0, st>>>(k, s);
   GDB_HIP(hipGetLastError());
   GDB_HIP(hipMemcpyAsync(host, s.out, sizeof(TopEntry) * k, hipMemcpyDeviceToHost, st));
   GDB_HIP(hipMemcpyAsync(host_cnt, s.counters, 8, hipMemcpyDeviceToHost, st));

@@ -263,12 +263,6 @@ __device__ __forceinline__ void q9_row(int32_t pk, int32_t sk, int32_t ok, int32
 
 // Orders LDS traffic within one wave: writes by some lanes become visible to
 // reads by other lanes of the same wave after this.
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 constexpr int RING = 512;  // per-wave queue of surviving row ids
 
 // Fact rows are tested 4 per lane on l_partkey alone; the survivors (about 5%)

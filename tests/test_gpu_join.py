@@ -129,6 +129,32 @@ def test_q13_random_fields_against_oracle(engine):
         assert engine.q13(c, o, w1, w2) == O.q13(cust, od, w1, w2), (w1, w2)
 
 
+def test_q13_words_bytes_after_nul_and_long_patterns(engine):
+    """Comments made of words, with non-NUL bytes left after the terminating
+    NUL (they must be ignored), against patterns of 1 to 31 bytes, overlapping
+    pairs and patterns that run past the field."""
+    rng = np.random.default_rng(9)
+    n = 30000
+    words = [b"special", b"requests", b"spec", b"ial", b"re", b"quests", b"pending", b"s", b"sp"]
+    com = np.zeros((n, 80), np.uint8)
+    for i in range(n):
+        s = b" ".join(words[j] for j in rng.integers(0, len(words), rng.integers(0, 14)))[:80]
+        com[i, :len(s)] = np.frombuffer(s, np.uint8)
+        if i % 3 == 0 and len(s) < 79:  # bytes after the NUL that would match
+            tail = b"specialrequests"[:79 - len(s)]
+            com[i, len(s) + 1:len(s) + 1 + len(tail)] = np.frombuffer(tail, np.uint8)
+    cust = dict(c_custkey=np.arange(1, 501, dtype=np.int32))
+    od = dict(o_custkey=rng.integers(1, 520, n).astype(np.int32), o_comment=com)
+    c, o = engine.table(cust), engine.table(od)
+    for w1, w2 in (("special", "requests"), ("spec", "ial"), ("sp", "sp"), ("s", "s"),
+                   ("special requests", "s"), ("requests special", "pending"),
+                   ("ial re", "ests spe"),
+                   ("special requests special reques", "s"), ("s", "special requests special reques"),
+                   ("quests", "re"), ("pending", "special"), ("spec", "ests"), ("reque", "special"),
+                   ("special", "special"), ("sts s", "requests"), ("requests", "spec")):
+        assert engine.q13(c, o, w1, w2) == O.q13(cust, od, w1, w2), (w1, w2)
+
+
 # -------------------------------------------------------------- join primitive
 def test_join_count_golden(engine, golden, small):
     for c in golden["primitives"]["join"]:

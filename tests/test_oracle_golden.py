@@ -223,3 +223,9 @@ def test_substring_helpers_against_python():
         assert O.contains(field, "spec") == ("spec" in t)
         assert O.like_two(field, "sp", "la") == bool(re.search("sp.*la", t))
         assert O.like_two(field, "aa", "aa") == bool(re.search("aa.*aa", t))
+        # bytes after the terminating NUL are not part of the string
+        if n < 29:
+            junk = field.copy()
+            junk[n + 1:] = np.frombuffer(b"splaaspecla" * 3, np.uint8)[:29 - n]
+            assert O.contains(junk, "spec") == ("spec" in t)
+            assert O.like_two(junk, "sp", "la") == bool(re.search("sp.*la", t))
