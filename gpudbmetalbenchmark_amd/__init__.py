@@ -138,23 +138,25 @@ class Engine:
 
     def autotune(self, pipeline, tables, reps=3):
         """Times the candidate launch shapes of `pipeline` ('q6', 'q1', 'q3',
-        'q9', 'q13') on `tables` (argument order of the pipeline) and keeps the
-        fastest for this engine; returns it as a dict."""
+        'q9', 'q13', 'join') on `tables` (argument order of the pipeline; join:
+        build, probe on their first columns) and keeps the fastest for this
+        engine; returns it as a dict."""
         arr = (C.c_void_p * len(tables))(*[t.handle.value for t in tables])
         out = abi.Tuning()
         abi.check(lib().gdb_autotune(self._ctx, abi.PIPE_IDS[pipeline], arr, len(tables), reps,
                                      C.byref(out)))
         return dict(pipeline=pipeline, block=out.block, blocks_per_cu=out.blocks_per_cu,
-                    unroll=out.unroll, ms=out.ms, candidates=out.candidates)
+                    unroll=out.unroll, radix_bits=out.radix_bits, ms=out.ms,
+                    candidates=out.candidates)
 
     def get_tuning(self, pipeline):
         out = abi.Tuning()
         abi.check(lib().gdb_get_tuning(self._ctx, abi.PIPE_IDS[pipeline], C.byref(out)))
         return dict(pipeline=pipeline, block=out.block, blocks_per_cu=out.blocks_per_cu,
-                    unroll=out.unroll)
+                    unroll=out.unroll, radix_bits=out.radix_bits)
 
-    def set_tuning(self, pipeline, block, blocks_per_cu, unroll):
-        t = abi.Tuning(abi.PIPE_IDS[pipeline], block, blocks_per_cu, unroll, 0.0, 0, 0)
+    def set_tuning(self, pipeline, block, blocks_per_cu, unroll, radix_bits=0):
+        t = abi.Tuning(abi.PIPE_IDS[pipeline], block, blocks_per_cu, unroll, 0.0, 0, radix_bits)
         abi.check(lib().gdb_set_tuning(self._ctx, C.byref(t)))
 
     def bandwidth_probe(self, nbytes=1 << 30, reps=10):

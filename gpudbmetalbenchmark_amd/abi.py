@@ -100,7 +100,7 @@ class Q13Result(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("pipeline", C.c_int32), ("block", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("unroll", C.c_int32), ("ms", C.c_double), ("candidates", C.c_int32),
-                ("pad", C.c_int32)]
+                ("radix_bits", C.c_int32)]
 
 
 class ServeCfg(C.Structure):
@@ -130,7 +130,7 @@ class ServeStats(C.Structure):
 REQ_Q6, REQ_Q1, REQ_Q3 = 0, 1, 2
 SERVE_BATCHED, SERVE_SERIAL = 0, 1
 
-PIPE_IDS = {"q6": 0, "q1": 1, "q3": 4, "q9": 5, "q13": 6}
+PIPE_IDS = {"q6": 0, "q1": 1, "q3": 4, "q9": 5, "q13": 6, "join": 7}
 
 
 def q1_row_dict(r):

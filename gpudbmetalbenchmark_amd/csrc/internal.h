@@ -45,6 +45,7 @@ struct Tuning {
   int block = 256;          // threads per workgroup
   int blocks_per_cu = 0;    // 0: occupancy-derived residency
   int unroll = 2;           // vector groups per thread per loop trip
+  int radix_bits = 0;       // join: partition bits (0 = size-derived, -1 = one global table)
 };
 
 // Device scratch that grows on demand and is reused across calls (partials,
@@ -102,7 +103,7 @@ int resident_blocks(gdb_ctx* ctx, const void* kernel, int block, size_t lds, int
 
 enum ArenaSlot {
   A_PARTIALS = 0, A_BITMAP = 1, A_MAP = 2, A_HT = 3, A_AGG = 4, A_AUX = 5, A_OUT = 6,
-  A_AUX2 = 7, A_STREAM0 = 8, A_STREAM1 = 9, A_Q3BYTE = 10, A_Q3MAP = 11, A_NSLOTS = 16
+  A_AUX2 = 7, A_STREAM0 = 8, A_STREAM1 = 9, A_Q3BYTE = 10, A_Q3MAP = 11, A_JOINPART = 12, A_JOINHIST = 13, A_NSLOTS = 16
 };
 enum TuneSlot { T_Q6 = 0, T_Q1 = 1, T_SELECT = 2, T_SUM = 3, T_Q3 = 4, T_Q9 = 5, T_Q13 = 6, T_JOIN = 7 };
 
@@ -119,6 +120,9 @@ struct RunStats {
 // Runs `fn` once over a resident table (bracketed by ev[1], ev[2]) or, for a
 // host table, chunk by chunk: two device slots per column, copies on the copy
 // stream overlapped with compute on the engine stream, event hand-off both ways.
+// Partition bits the join derives for a build side of nb keys.
+int join_derived_bits(uint64_t nb);
+
 int run_over(gdb_ctx* ctx, const gdb_table* t, const Column* const* cols, int ncols,
              const Consume& fn, RunStats* rs);
 // Rejects host tables where a pipeline needs the table resident.

@@ -1,17 +1,29 @@
 // Equi-join pair count, the `join` primitive (reference hash_join_build /
 // hash_join_probe, kernels/microbenchmarks.metal:224-293; host
-// src/microbenchmarks.cpp:135-251).
+// src/microbenchmarks.cpp:135-251): the number of (build row, probe row) pairs
+// with equal keys. Keys may repeat on both sides.
 //
-// Build: open-addressing table (key, multiplicity) with capacity nextpow2(2*nb),
-// a locality-preserving fold hash, linear probing, CAS insert. Probe: dwordx4 loads of 4 probe
-// keys per lane, each matched key adds its multiplicity; per-block partial
-// counts are summed by the finaliser. Keys may repeat on both sides.
+// Three strategies, chosen by the build side's size or by the tuning entry:
+//   broadcast  (build <= LDS_KEYS keys) every workgroup builds the whole
+//              build side as an LDS hash table and probes its slice of the
+//              probe keys
+//   radix      both sides are partitioned by the top B bits of a 32-bit key
+//              hash (two passes of B/2 and B bits when B > 7, so every
+//              pass writes long runs per partition: count, exclusive scan,
+//              scatter), sized so a build partition fits an LDS table; one
+//              workgroup per partition then builds its table in LDS and
+//              probes the partition's probe keys against it. A partition
+//              larger than the table is built and probed in table-sized
+//              pieces. No global atomics anywhere.
+//   global     the reference's shape: one open-addressing table in HBM built
+//              with global CAS (kept for comparison by the autotuner)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 
 #include "internal.h"
 #include "kcommon.h"
+#include "scan_util.h"
 
 using namespace gdb;
 using namespace gdbk;
@@ -19,11 +31,233 @@ using namespace gdbk;
 namespace {
 
 constexpr int32_t EMPTY = INT32_MIN;
+constexpr int JB = 256;               // threads per workgroup of every join kernel
+constexpr uint32_t LDS_SLOTS = 8192;  // LDS table slots (64 KiB: keys + counts)
+constexpr uint32_t LDS_KEYS = LDS_SLOTS / 2;  // keys per table fill (load <= 1/2)
+constexpr int MAX_BITS = 14;
 
+// MurmurHash3's 32-bit finaliser (public domain): every output bit depends on
+// every key bit, so partitions stay balanced for dense or strided keys. The
+// top bits pick the partition, the low bits the LDS slot.
+__device__ __forceinline__ uint32_t mix32(int32_t key) {
+  uint32_t h = (uint32_t)key;
+  h ^= h >> 16;
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  h *= 0xC2B2AE35u;
+  h ^= h >> 16;
+  return h;
+}
+
+// ---------------------------------------------------------------- partitioning
+// Block b owns rows [b*chunk, min(n, (b+1)*chunk)) (chunk a multiple of 4).
+__device__ __forceinline__ void chunk_of(uint32_t n, uint32_t chunk, uint32_t& lo, uint32_t& hi) {
+  lo = min(n, blockIdx.x * chunk);
+  hi = min(n, lo + chunk);
+}
+
+constexpr int PU = 4;  // dwordx4 groups in flight per lane in the partition passes
+
+// Counts of the block's keys per partition, written k-major: hist[p*G + b].
+__global__ __launch_bounds__(JB) void rj_count(const int32_t* __restrict__ keys, uint32_t n,
+                                               int bits, uint32_t chunk,
+                                               uint32_t* __restrict__ hist) {
+  extern __shared__ uint32_t lh[];
+  const uint32_t P = 1u << bits;
+  for (uint32_t i = threadIdx.x; i < P; i += JB) lh[i] = 0u;
+  __syncthreads();
+  uint32_t lo, hi;
+  chunk_of(n, chunk, lo, hi);
+  const uint32_t shift = 32 - bits;
+  const uint32_t g0 = lo / 4, g1 = hi / 4;
+  uint32_t g = g0 + threadIdx.x;
+  for (; g + (PU - 1) * JB < g1; g += PU * JB) {
+    int4 v[PU];
+#pragma unroll
+    for (int u = 0; u < PU; ++u) v[u] = ld4nt(keys, g + u * JB);
+#pragma unroll
+    for (int u = 0; u < PU; ++u) {
+      atomicAdd(&lh[mix32(v[u].x) >> shift], 1u);
+      atomicAdd(&lh[mix32(v[u].y) >> shift], 1u);
+      atomicAdd(&lh[mix32(v[u].z) >> shift], 1u);
+      atomicAdd(&lh[mix32(v[u].w) >> shift], 1u);
+    }
+  }
+  for (; g < g1; g += JB) {
+    const int4 v = ld4nt(keys, g);
+    atomicAdd(&lh[mix32(v.x) >> shift], 1u);
+    atomicAdd(&lh[mix32(v.y) >> shift], 1u);
+    atomicAdd(&lh[mix32(v.z) >> shift], 1u);
+    atomicAdd(&lh[mix32(v.w) >> shift], 1u);
+  }
+  for (uint32_t r = g1 * 4 + threadIdx.x; r < hi; r += JB) atomicAdd(&lh[mix32(keys[r]) >> shift], 1u);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < P; i += JB) hist[(size_t)i * gridDim.x + blockIdx.x] = lh[i];
+}
+
+__device__ __forceinline__ void scatter_key(int32_t k, uint32_t shift, uint32_t* lo_off,
+                                            int32_t* __restrict__ out) {
+  out[atomicAdd(&lo_off[mix32(k) >> shift], 1u)] = k;
+}
+
+// Moves the block's keys to out[offs[p*G + b] ...] (offs = scanned counts).
+__global__ __launch_bounds__(JB) void rj_scatter(const int32_t* __restrict__ keys, uint32_t n,
+                                                 int bits, uint32_t chunk,
+                                                 const uint32_t* __restrict__ offs,
+                                                 int32_t* __restrict__ out) {
+  extern __shared__ uint32_t lo_off[];
+  const uint32_t P = 1u << bits;
+  for (uint32_t i = threadIdx.x; i < P; i += JB) lo_off[i] = offs[(size_t)i * gridDim.x + blockIdx.x];
+  __syncthreads();
+  uint32_t lo, hi;
+  chunk_of(n, chunk, lo, hi);
+  const uint32_t shift = 32 - bits;
+  const uint32_t g0 = lo / 4, g1 = hi / 4;
+  uint32_t g = g0 + threadIdx.x;
+  for (; g + (PU - 1) * JB < g1; g += PU * JB) {
+    int4 v[PU];
+#pragma unroll
+    for (int u = 0; u < PU; ++u) v[u] = ld4nt(keys, g + u * JB);
+#pragma unroll
+    for (int u = 0; u < PU; ++u) {
+      scatter_key(v[u].x, shift, lo_off, out);
+      scatter_key(v[u].y, shift, lo_off, out);
+      scatter_key(v[u].z, shift, lo_off, out);
+      scatter_key(v[u].w, shift, lo_off, out);
+    }
+  }
+  for (; g < g1; g += JB) {
+    const int4 v = ld4nt(keys, g);
+    scatter_key(v.x, shift, lo_off, out);
+    scatter_key(v.y, shift, lo_off, out);
+    scatter_key(v.z, shift, lo_off, out);
+    scatter_key(v.w, shift, lo_off, out);
+  }
+  for (uint32_t r = g1 * 4 + threadIdx.x; r < hi; r += JB) scatter_key(keys[r], shift, lo_off, out);
+}
+
+// ------------------------------------------------------------------ LDS table
+// A slot is one 64-bit word: count << 32 | key, so a probe step is a single
+// LDS read. The empty slot holds key INT32_MIN with count 0.
+constexpr uint64_t SLOT_EMPTY = (uint64_t)(uint32_t)EMPTY;
+constexpr int JT = 512;  // threads per workgroup of the join kernels
+
+struct LdsTable {
+  uint64_t* slot;
+
+  __device__ void clear() {
+    for (uint32_t i = threadIdx.x; i < LDS_SLOTS; i += blockDim.x) slot[i] = SLOT_EMPTY;
+  }
+  // At most LDS_KEYS distinct keys per fill, so a free slot always exists.
+  __device__ void insert(int32_t k) {
+    const uint64_t mine = (1ull << 32) | (uint32_t)k;
+    uint32_t h = mix32(k) & (LDS_SLOTS - 1);
+    while (true) {
+      const uint64_t prev = atomicCAS((unsigned long long*)&slot[h], (unsigned long long)SLOT_EMPTY,
+                                      (unsigned long long)mine);
+      if (prev == SLOT_EMPTY) return;
+      if ((uint32_t)prev == (uint32_t)k) {
+        atomicAdd((unsigned long long*)&slot[h], 1ull << 32);
+        return;
+      }
+      h = (h + 1) & (LDS_SLOTS - 1);
+    }
+  }
+  __device__ uint32_t lookup_from(int32_t k, uint32_t h, uint64_t first) const {
+    uint64_t v = first;
+    while (true) {
+      if ((uint32_t)v == (uint32_t)k) return (uint32_t)(v >> 32);
+      if (v == SLOT_EMPTY) return 0u;
+      h = (h + 1) & (LDS_SLOTS - 1);
+      v = slot[h];
+    }
+  }
+  // Four lookups with their first probes issued together.
+  __device__ uint32_t lookup4(int4 k) const {
+    const uint32_t h0 = mix32(k.x) & (LDS_SLOTS - 1), h1 = mix32(k.y) & (LDS_SLOTS - 1),
+                   h2 = mix32(k.z) & (LDS_SLOTS - 1), h3 = mix32(k.w) & (LDS_SLOTS - 1);
+    const uint64_t v0 = slot[h0], v1 = slot[h1], v2 = slot[h2], v3 = slot[h3];
+    return lookup_from(k.x, h0, v0) + lookup_from(k.y, h1, v1) + lookup_from(k.z, h2, v2) +
+           lookup_from(k.w, h3, v3);
+  }
+  __device__ uint32_t lookup(int32_t k) const {
+    const uint32_t h = mix32(k) & (LDS_SLOTS - 1);
+    return lookup_from(k, h, slot[h]);
+  }
+};
+
+__device__ __forceinline__ uint64_t probe_range(const LdsTable& t, const int32_t* __restrict__ s,
+                                                uint32_t lo, uint32_t hi, uint32_t tid,
+                                                uint32_t nthr) {
+  uint64_t c = 0;
+  // head up to a multiple of 4, then dwordx4 groups, then the tail
+  const uint32_t a = min(hi, (lo + 3) & ~3u);
+  for (uint32_t r = lo + tid; r < a; r += nthr) c += t.lookup(s[r]);
+  const uint32_t g1 = hi / 4;
+  uint32_t g = a / 4 + tid;
+  for (; g + nthr < g1; g += 2 * nthr) {
+    const int4 v = ld4nt(s, g), w = ld4nt(s, g + nthr);
+    c += t.lookup4(v) + t.lookup4(w);
+  }
+  for (; g < g1; g += nthr) c += t.lookup4(ld4nt(s, g));
+  for (uint32_t r = max(a, g1 * 4) + tid; r < hi; r += nthr) c += t.lookup(s[r]);
+  return c;
+}
+
+// One workgroup per partition p: build side keys [rb, re) and probe side keys
+// [sb, se), from the scanned partition offsets (k-major, G blocks).
+__global__ __launch_bounds__(JT) void rj_join(const int32_t* __restrict__ r,
+                                              const uint32_t* __restrict__ roffs, uint32_t rG,
+                                              const uint32_t* __restrict__ rtotal,
+                                              const int32_t* __restrict__ s,
+                                              const uint32_t* __restrict__ soffs, uint32_t sG,
+                                              const uint32_t* __restrict__ stotal, uint32_t P,
+                                              int64_t* __restrict__ partials) {
+  __shared__ uint64_t tslot[LDS_SLOTS];
+  __shared__ int64_t red[JT / 64];
+  const uint32_t p = blockIdx.x;
+  const uint32_t rb = roffs[(size_t)p * rG], re = p + 1 < P ? roffs[(size_t)(p + 1) * rG] : *rtotal;
+  const uint32_t sb = soffs[(size_t)p * sG], se = p + 1 < P ? soffs[(size_t)(p + 1) * sG] : *stotal;
+  LdsTable t{tslot};
+  uint64_t c = 0;
+  if (re > rb && se > sb) {
+    for (uint32_t fb = rb; fb < re; fb += LDS_KEYS) {  // table-sized pieces of the partition
+      const uint32_t fe = min(re, fb + LDS_KEYS);
+      t.clear();
+      __syncthreads();
+      for (uint32_t i = fb + threadIdx.x; i < fe; i += JT) t.insert(r[i]);
+      __syncthreads();
+      c += probe_range(t, s, sb, se, threadIdx.x, JT);
+      __syncthreads();
+    }
+  }
+  const int64_t bs = block_sum((int64_t)c, red);
+  if (threadIdx.x == 0) partials[p] = bs;
+}
+
+// Every workgroup builds the whole (small) build side and probes a slice.
+__global__ __launch_bounds__(JT) void rj_broadcast(const int32_t* __restrict__ r, uint32_t nr,
+                                                   const int32_t* __restrict__ s, uint32_t ns,
+                                                   int64_t* __restrict__ partials) {
+  __shared__ uint64_t tslot[LDS_SLOTS];
+  __shared__ int64_t red[JT / 64];
+  LdsTable t{tslot};
+  t.clear();
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nr; i += JT) t.insert(r[i]);
+  __syncthreads();
+  // contiguous slice of the probe keys per workgroup, a multiple of 4 long
+  const uint32_t per = ((ns + gridDim.x - 1) / gridDim.x + 3) & ~3u;
+  const uint32_t lo = min(ns, blockIdx.x * per), hi = min(ns, lo + per);
+  const uint64_t c = probe_range(t, s, lo, hi, threadIdx.x, JT);
+  const int64_t bs = block_sum((int64_t)c, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+}
+
+// ---------------------------------------------------------------- global table
 // Keys below the capacity map to themselves and higher bits are folded in, so
-// keys arriving in (near) key order touch neighbouring slots and the CAS
-// inserts and probes coalesce (the reference hashes by key modulo size,
-// kernels/microbenchmarks.metal:240).
+// keys arriving in (near) key order touch neighbouring slots (the reference
+// hashes by key modulo size, kernels/microbenchmarks.metal:240).
 __device__ __forceinline__ uint32_t jhash(int32_t key, uint32_t shift) {
   const uint32_t k = (uint32_t)key;
   const uint32_t lg = 32 - shift;
@@ -90,7 +324,58 @@ __global__ __launch_bounds__(256) void join_probe(const int32_t* __restrict__ pr
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
+// Partition bits for a build side of nb keys: partitions average at most half
+// an LDS table fill.
+int derived_bits(uint32_t nb) {
+  int b = 1;
+  while (b < MAX_BITS && ((uint64_t)nb >> b) > LDS_KEYS / 2) ++b;
+  return b;
+}
+
+// Fewest bits a tuning entry may ask for: every table-sized piece of a
+// partition probes the whole probe partition again, so partitions average at
+// most four pieces.
+int min_bits(uint32_t nb) {
+  int b = 1;
+  while (b < MAX_BITS && ((uint64_t)nb >> b) > 4 * LDS_KEYS) ++b;
+  return b;
+}
+
+struct PartPass {
+  int bits;
+  uint32_t G, chunk;
+};
+
+PartPass pass_shape(const gdb_ctx* ctx, uint32_t n, int bits) {
+  uint32_t G = std::max<uint32_t>(1, std::min<uint32_t>(ctx->info.cu_count * 2, (n + 4095) / 4096));
+  uint32_t chunk = ((n + G - 1) / G + 3) & ~3u;
+  if (chunk == 0) chunk = 4;
+  G = std::max<uint32_t>(1, (n + chunk - 1) / chunk);
+  return PartPass{bits, G, chunk};
+}
+
+// One partition pass over n keys: in -> out, offsets k-major in offs[P*G].
+int partition_pass(gdb_ctx* ctx, const int32_t* in, uint32_t n, const PartPass& ps, int32_t* out,
+                   uint32_t* offs, uint32_t* total, uint32_t* tsum, hipStream_t st) {
+  const uint32_t P = 1u << ps.bits;
+  const size_t lds = (size_t)P * 4;
+  if (lds > 32768) {  // 14 bits: 64 KiB of partition counters
+    GDB_HIP(hipFuncSetAttribute((const void*)rj_count, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    GDB_HIP(hipFuncSetAttribute((const void*)rj_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  }
+  rj_count<<<ps.G, JB, lds, st>>>(in, n, ps.bits, ps.chunk, offs);
+  GDB_HIP(hipGetLastError());
+  GDB_TRY(exclusive_scan_u32(ctx, offs, P * ps.G, offs, total, tsum, st));
+  rj_scatter<<<ps.G, JB, lds, st>>>(in, n, ps.bits, ps.chunk, offs, out);
+  GDB_HIP(hipGetLastError());
+  return GDB_OK;
+}
+
 }  // namespace
+
+namespace gdb {
+int join_derived_bits(uint64_t nb) { return derived_bits((uint32_t)std::min<uint64_t>(nb, UINT32_MAX)); }
+}  // namespace gdb
 
 extern "C" int gdb_join_count(gdb_ctx* ctx, const gdb_table* build, const char* build_col,
                               const gdb_table* probe, const char* probe_col, uint64_t* matches,
@@ -103,38 +388,109 @@ extern "C" int gdb_join_count(gdb_ctx* ctx, const gdb_table* build, const char* 
   GDB_TRY(require_resident(probe, "join probe table"));
   if (build->nrows && cb->minv == INT32_MIN)
     return fail(GDB_E_UNSUPPORTED, "join keys must not be INT32_MIN (the empty-slot marker)");
+  if (build->nrows >= (1ull << 31) || probe->nrows >= (1ull << 31))
+    return fail(GDB_E_UNSUPPORTED, "join sides are limited to 2^31-1 rows");
   GDB_HIP(hipSetDevice(ctx->device));
   const uint32_t nb = (uint32_t)build->nrows, np = (uint32_t)probe->nrows;
-  uint32_t cap = 1024, lg = 10;
-  while (cap < 2ull * nb) {
-    cap <<= 1;
-    ++lg;
-  }
-  const uint32_t shift = 32 - lg;
-  char* h;
-  GDB_TRY(arena(ctx, A_HT, (size_t)cap * 8, (void**)&h));
-  int32_t* keys = (int32_t*)h;
-  uint32_t* mult = (uint32_t*)(h + (size_t)cap * 4);
-  const int B = 256;
-  int blocks = resident_blocks(ctx, (const void*)join_probe, B, 0, ctx->tune[T_JOIN].blocks_per_cu);
-  blocks = std::max(1, std::min<int>(blocks, (int)((np / 4 + B - 1) / B)));
-  int64_t *partials, *dout, *host;
-  GDB_TRY(arena(ctx, A_PARTIALS, sizeof(int64_t) * blocks, (void**)&partials));
+  const int32_t* rk = (const int32_t*)cb->dev;
+  const int32_t* sk = (const int32_t*)cp->dev;
+  const int tuned = ctx->tune[T_JOIN].radix_bits;
+  const bool global = tuned < 0;
+  const bool broadcast = !global && tuned == 0 && nb <= LDS_KEYS;
+  const int bits = global || broadcast ? 0
+                   : tuned > 0 ? std::max(min_bits(nb), std::min(tuned, MAX_BITS))
+                               : derived_bits(nb);
+  hipStream_t st = ctx->stream;
+  int64_t *dout, *host;
   GDB_TRY(arena(ctx, A_OUT, 64, (void**)&dout));
   GDB_TRY(pinned(ctx, 64, (void**)&host));
-  hipStream_t st = ctx->stream;
-  const int fill = ctx->info.cu_count * 4;
-  GDB_HIP(hipEventRecord(ctx->ev[0], st));
-  join_clear<<<fill, B, 0, st>>>(keys, mult, cap);
-  if (nb) join_build<<<fill, B, 0, st>>>((const int32_t*)cb->dev, nb, keys, mult, cap, shift);
-  GDB_HIP(hipEventRecord(ctx->ev[1], st));
-  join_probe<<<blocks, B, 0, st>>>((const int32_t*)cp->dev, np, keys, mult, cap, shift, partials);
+  int64_t* partials;
+  uint32_t nparts = 0;
+
+  if (global) {
+    uint32_t cap = 1024, lg = 10;
+    while (cap < 2ull * nb) {
+      cap <<= 1;
+      ++lg;
+    }
+    const uint32_t shift = 32 - lg;
+    char* h;
+    GDB_TRY(arena(ctx, A_HT, (size_t)cap * 8, (void**)&h));
+    int32_t* keys = (int32_t*)h;
+    uint32_t* mult = (uint32_t*)(h + (size_t)cap * 4);
+    int blocks = resident_blocks(ctx, (const void*)join_probe, 256, 0, ctx->tune[T_JOIN].blocks_per_cu);
+    blocks = std::max(1, std::min<int>(blocks, (int)((np / 4 + 255) / 256)));
+    nparts = (uint32_t)blocks;
+    GDB_TRY(arena(ctx, A_PARTIALS, sizeof(int64_t) * nparts, (void**)&partials));
+    const int fill = ctx->info.cu_count * 4;
+    GDB_HIP(hipEventRecord(ctx->ev[0], st));
+    join_clear<<<fill, 256, 0, st>>>(keys, mult, cap);
+    if (nb) join_build<<<fill, 256, 0, st>>>(rk, nb, keys, mult, cap, shift);
+    GDB_HIP(hipEventRecord(ctx->ev[1], st));
+    join_probe<<<blocks, 256, 0, st>>>(sk, np, keys, mult, cap, shift, partials);
+    GDB_HIP(hipEventRecord(ctx->ev[2], st));
+  } else if (broadcast) {
+    const int per_cu = std::max(1, ctx->info.lds_bytes_per_cu / (int)(LDS_SLOTS * 8 + 64));
+    nparts = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>((uint64_t)per_cu * ctx->info.cu_count, ((uint64_t)np + 8191) / 8192));
+    GDB_TRY(arena(ctx, A_PARTIALS, sizeof(int64_t) * nparts, (void**)&partials));
+    GDB_HIP(hipEventRecord(ctx->ev[0], st));
+    GDB_HIP(hipEventRecord(ctx->ev[1], st));
+    rj_broadcast<<<nparts, JT, 0, st>>>(rk, nb, sk, np, partials);
+    GDB_HIP(hipEventRecord(ctx->ev[2], st));
+  } else {
+    const uint32_t P = 1u << bits;
+    const int b1 = bits > 7 ? bits / 2 : bits;  // first pass (or the only one)
+    const PartPass r1 = pass_shape(ctx, nb, b1), r2 = pass_shape(ctx, nb, bits);
+    const PartPass s1 = pass_shape(ctx, np, b1), s2 = pass_shape(ctx, np, bits);
+    const bool two = b1 != bits;
+    // keys: build pass outputs, probe pass outputs (each region 256-byte aligned)
+    auto padded = [](uint32_t n) { return ((size_t)n + 64 + 63) & ~size_t(63); };
+    const size_t rn = padded(nb), sn = padded(np);
+    int32_t* kbuf;
+    GDB_TRY(arena(ctx, A_JOINPART, (rn + sn) * 4 * (two ? 2 : 1), (void**)&kbuf));
+    int32_t* r_a = kbuf;
+    int32_t* r_b = two ? r_a + rn : r_a;
+    int32_t* s_a = r_b + rn;
+    int32_t* s_b = two ? s_a + sn : s_a;
+    // offsets: final build and probe offsets kept for the join, first-pass
+    // offsets, scan tile sums, totals
+    const size_t o_r2 = (size_t)P * r2.G, o_s2 = (size_t)P * s2.G;
+    const size_t o_1 = std::max<size_t>((size_t)(1u << b1) * r1.G, (size_t)(1u << b1) * s1.G);
+    const size_t ts = std::max<size_t>(scan_tile_sums_needed((uint32_t)std::max(o_r2, o_s2)),
+                                       scan_tile_sums_needed((uint32_t)o_1));
+    uint32_t* obuf;
+    GDB_TRY(arena(ctx, A_JOINHIST, (o_r2 + o_s2 + o_1 + ts + 16) * 4, (void**)&obuf));
+    uint32_t* roffs = obuf;
+    uint32_t* soffs = roffs + o_r2;
+    uint32_t* offs1 = soffs + o_s2;
+    uint32_t* tsum = offs1 + o_1;
+    uint32_t* totals = tsum + ts;  // [0] build, [1] probe, [2] first-pass scratch
+    nparts = P;
+    GDB_TRY(arena(ctx, A_PARTIALS, sizeof(int64_t) * nparts, (void**)&partials));
+    GDB_HIP(hipEventRecord(ctx->ev[0], st));
+    if (two) {
+      GDB_TRY(partition_pass(ctx, rk, nb, r1, r_a, offs1, totals + 2, tsum, st));
+      GDB_TRY(partition_pass(ctx, r_a, nb, r2, r_b, roffs, totals + 0, tsum, st));
+      GDB_TRY(partition_pass(ctx, sk, np, s1, s_a, offs1, totals + 2, tsum, st));
+      GDB_TRY(partition_pass(ctx, s_a, np, s2, s_b, soffs, totals + 1, tsum, st));
+    } else {
+      GDB_TRY(partition_pass(ctx, rk, nb, r2, r_b, roffs, totals + 0, tsum, st));
+      GDB_TRY(partition_pass(ctx, sk, np, s2, s_b, soffs, totals + 1, tsum, st));
+    }
+    GDB_HIP(hipEventRecord(ctx->ev[1], st));
+    rj_join<<<P, JT, 0, st>>>(r_b, roffs, r2.G, totals + 0, s_b, soffs, s2.G, totals + 1, P,
+                              partials);
+    GDB_HIP(hipEventRecord(ctx->ev[2], st));
+  }
   GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[2], st));
-  sum_columns<<<1, 256, 0, st>>>(partials, (uint32_t)blocks, dout);
+  sum_columns<<<1, 256, 0, st>>>(partials, nparts, dout);
   GDB_HIP(hipMemcpyAsync(host, dout, 8, hipMemcpyDeviceToHost, st));
   GDB_HIP(hipEventRecord(ctx->ev[5], st));
-  GDB_TRY(timing_finish(ctx, timing, 4ull * nb + 4ull * np, 4ull * np));
+  // algorithmic bytes: each key read once; the main kernel of the radix path
+  // (the per-partition join) reads both partitioned sides
+  const uint64_t main_bytes = (!global && !broadcast) ? 4ull * nb + 4ull * np : 4ull * np;
+  GDB_TRY(timing_finish(ctx, timing, 4ull * nb + 4ull * np, main_bytes));
   *matches = (uint64_t)host[0];
   return GDB_OK;
 }

@@ -11,26 +11,37 @@ using namespace gdb;
 namespace {
 
 struct Shape {
-  int block, bpc, unroll;
+  int block, bpc, unroll, bits;
 };
 
-std::vector<Shape> candidates(int pipeline) {
+std::vector<Shape> candidates(int pipeline, const gdb_table* const* t, int n) {
   std::vector<Shape> c;
   switch (pipeline) {
     case GDB_PIPE_Q6:
       for (int bpc : {1, 2, 3, 4, 8, 0})
-        for (int u : {1, 2, 4}) c.push_back({256, bpc, u});
+        for (int u : {1, 2, 4}) c.push_back({256, bpc, u, 0});
       break;
     case GDB_PIPE_Q1:
       for (int b : {128, 256})
         for (int bpc : {1, 2, 3, 4, 0})
-          for (int u : {1, 2, 4}) c.push_back({b, bpc, u});
+          for (int u : {1, 2, 4}) c.push_back({b, bpc, u, 0});
       break;
     case GDB_PIPE_Q3:
     case GDB_PIPE_Q9:
     case GDB_PIPE_Q13:
-      for (int bpc : {1, 2, 3, 4, 6, 8, 0}) c.push_back({256, bpc, 2});
+      for (int bpc : {1, 2, 3, 4, 6, 8, 0}) c.push_back({256, bpc, 2, 0});
       break;
+    case GDB_PIPE_JOIN: {
+      // one global table, the derived shape, and radix bits around the derived
+      c.push_back({256, 0, 1, -1});
+      c.push_back({256, 0, 1, 0});
+      if (n >= 1) {
+        const int d = join_derived_bits(t[0]->nrows);
+        for (int b = std::max(1, d - 2); b <= std::min(14, d + 1); ++b)
+          if (b != d) c.push_back({256, 0, 1, b});
+      }
+      break;
+    }
     default:
       break;
   }
@@ -81,6 +92,15 @@ int run_once(gdb_ctx* ctx, int pipeline, const gdb_table* const* t, int n, doubl
       rc = gdb_q13(ctx, t[0], t[1], &p, &r, &tm);
       break;
     }
+    case GDB_PIPE_JOIN: {
+      if (n < 2) break;
+      if (t[0]->cols.empty() || t[1]->cols.empty())
+        return fail(GDB_E_INVALID, "join tables need a key column");
+      uint64_t m = 0;
+      rc = gdb_join_count(ctx, t[0], t[0]->cols[0].name.c_str(), t[1], t[1]->cols[0].name.c_str(),
+                          &m, &tm);
+      break;
+    }
     default:
       return fail(GDB_E_INVALID, "unknown pipeline id");
   }
@@ -96,29 +116,31 @@ extern "C" {
 int gdb_get_tuning(gdb_ctx* ctx, int pipeline, gdb_tuning* out) {
   if (!ctx || !out || pipeline < 0 || pipeline >= 16) return fail(GDB_E_INVALID, "bad argument");
   const Tuning& t = ctx->tune[pipeline];
-  *out = gdb_tuning{pipeline, t.block, t.blocks_per_cu, t.unroll, 0.0, 0, 0};
+  *out = gdb_tuning{pipeline, t.block, t.blocks_per_cu, t.unroll, 0.0, 0, t.radix_bits};
   return GDB_OK;
 }
 
 int gdb_set_tuning(gdb_ctx* ctx, const gdb_tuning* t) {
   if (!ctx || !t || t->pipeline < 0 || t->pipeline >= 16) return fail(GDB_E_INVALID, "bad argument");
   if ((t->block != 64 && t->block != 128 && t->block != 256) || t->blocks_per_cu < 0 ||
-      t->blocks_per_cu > 64 || (t->unroll != 1 && t->unroll != 2 && t->unroll != 4))
-    return fail(GDB_E_INVALID, "tuning out of range (block 64/128/256, unroll 1/2/4)");
-  ctx->tune[t->pipeline] = Tuning{t->block, t->blocks_per_cu, t->unroll};
+      t->blocks_per_cu > 64 || (t->unroll != 1 && t->unroll != 2 && t->unroll != 4) ||
+      t->radix_bits < -1 || t->radix_bits > 14)
+    return fail(GDB_E_INVALID,
+                "tuning out of range (block 64/128/256, unroll 1/2/4, radix bits -1..14)");
+  ctx->tune[t->pipeline] = Tuning{t->block, t->blocks_per_cu, t->unroll, t->radix_bits};
   return GDB_OK;
 }
 
 int gdb_autotune(gdb_ctx* ctx, int pipeline, const gdb_table* const* tables, int ntables, int reps,
                  gdb_tuning* best) {
   if (!ctx || !tables || !best || reps < 1) return fail(GDB_E_INVALID, "bad argument");
-  const std::vector<Shape> cands = candidates(pipeline);
+  const std::vector<Shape> cands = candidates(pipeline, tables, ntables);
   if (cands.empty()) return fail(GDB_E_INVALID, "pipeline has no tunable kernel");
   const Tuning saved = ctx->tune[pipeline];
   double best_ms = 1e30;
-  Shape best_s = {saved.block, saved.blocks_per_cu, saved.unroll};
+  Shape best_s = {saved.block, saved.blocks_per_cu, saved.unroll, saved.radix_bits};
   for (const Shape& s : cands) {
-    ctx->tune[pipeline] = Tuning{s.block, s.bpc, s.unroll};
+    ctx->tune[pipeline] = Tuning{s.block, s.bpc, s.unroll, s.bits};
     double ms = 0;
     int rc = run_once(ctx, pipeline, tables, ntables, &ms);  // warm-up
     if (rc != GDB_OK) {
@@ -137,9 +159,9 @@ int gdb_autotune(gdb_ctx* ctx, int pipeline, const gdb_table* const* tables, int
       best_s = s;
     }
   }
-  ctx->tune[pipeline] = Tuning{best_s.block, best_s.bpc, best_s.unroll};
+  ctx->tune[pipeline] = Tuning{best_s.block, best_s.bpc, best_s.unroll, best_s.bits};
   *best = gdb_tuning{pipeline, best_s.block, best_s.bpc, best_s.unroll, best_ms,
-                     (int32_t)cands.size(), 0};
+                     (int32_t)cands.size(), best_s.bits};
   return GDB_OK;
 }
 

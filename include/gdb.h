@@ -294,8 +294,11 @@ void gdb_loadgen_request(uint64_t seed, uint64_t i, double q3_share, gdb_request
  * the given tables (median of `reps` runs after one warm-up), keeps the fastest
  * for the context and returns it. Tables are passed in the pipeline's argument
  * order (q6/q1: lineitem; q3: customer, orders, lineitem; q9: part, supplier,
- * partsupp, orders, lineitem; q13: customer, orders). */
-enum { GDB_PIPE_Q6 = 0, GDB_PIPE_Q1 = 1, GDB_PIPE_Q3 = 4, GDB_PIPE_Q9 = 5, GDB_PIPE_Q13 = 6 };
+ * partsupp, orders, lineitem; q13: customer, orders; join: build, probe, on
+ * each table's first column). For the join the candidates are strategies
+ * and radix bits (the LDS table size follows from the bits). */
+enum { GDB_PIPE_Q6 = 0, GDB_PIPE_Q1 = 1, GDB_PIPE_Q3 = 4, GDB_PIPE_Q9 = 5, GDB_PIPE_Q13 = 6,
+       GDB_PIPE_JOIN = 7 };
 typedef struct {
   int32_t pipeline;
   int32_t block;
@@ -303,7 +306,9 @@ typedef struct {
   int32_t unroll;
   double ms;           /* median pipeline time of the chosen shape */
   int32_t candidates;  /* shapes timed */
-  int32_t pad;
+  int32_t radix_bits;  /* join only: partition bits 1..14; 0 = derived from the
+                          build size (LDS broadcast when it fits one table);
+                          -1 = one global hash table */
 } gdb_tuning;
 int gdb_autotune(gdb_ctx* ctx, int pipeline, const gdb_table* const* tables, int ntables,
                  int reps, gdb_tuning* best);

@@ -168,3 +168,50 @@ def test_join_count_duplicates(engine):
     p = rng.integers(-6000, 6000, 1_000_003).astype(np.int32)
     tb, tp = engine.table({"k": b}), engine.table({"k": p})
     assert engine.join_count(tb, "k", tp, "k") == O.join_count(b, p)
+
+
+# radix bits: -1 one global table, 0 derived (LDS broadcast for small builds),
+# 1 and 4 single-pass partitions far larger than an LDS table (built and probed
+# in pieces), 9 and 14 two partition passes
+STRATEGIES = (-1, 0, 1, 4, 9, 14)
+
+
+@pytest.fixture()
+def join_strategy(engine):
+    saved = engine.get_tuning("join")
+    yield lambda bits: engine.set_tuning("join", 256, 0, 1, bits)
+    engine.set_tuning("join", saved["block"], saved["blocks_per_cu"], saved["unroll"],
+                      saved["radix_bits"])
+
+
+@pytest.mark.parametrize("bits", STRATEGIES)
+def test_join_strategies_against_oracle(engine, join_strategy, bits):
+    join_strategy(bits)
+    rng = np.random.default_rng(100 + bits)
+    cases = [
+        (rng.integers(-5000, 5000, 30001), rng.integers(-6000, 6000, 1_000_003)),  # duplicates
+        (np.arange(1, 2_000_002, 4), rng.integers(0, 8_000_000, 3_000_001)),  # sparse unique keys
+        (np.full(20_000, 7), np.concatenate([np.full(50_000, 7), np.arange(100_000)])),  # skew
+        (np.array([3]), np.array([3, 3, 4])),  # tiny
+        (np.array([], np.int64), np.arange(10)),  # empty build
+        (np.arange(10), np.array([], np.int64)),  # empty probe
+        (rng.integers(-2**31 + 1, 2**31 - 1, 300_007), rng.integers(-2**31 + 1, 2**31 - 1, 5)),
+    ]
+    for b, p in cases:
+        b, p = b.astype(np.int32), p.astype(np.int32)
+        tb, tp = engine.table({"k": b}), engine.table({"k": p})
+        assert engine.join_count(tb, "k", tp, "k") == O.join_count(b, p), (bits, len(b), len(p))
+
+
+def test_join_full_config_all_strategies(engine, join_strategy, full):
+    js, t = full
+    exp = O.join_count(js["orders"]["o_orderkey"], js["lineitem"]["l_orderkey"])
+    for bits in (-1, 0, 1, 9, 12, 14):  # 1 and 9 are raised to the fewest bits allowed
+        join_strategy(bits)
+        assert engine.join_count(t["orders"], "o_orderkey", t["lineitem"], "l_orderkey") == exp, bits
+    kb = engine.table({"k": js["orders"]["o_orderkey"]})
+    kp = engine.table({"k": js["lineitem"]["l_orderkey"]})
+    got = engine.autotune("join", [kb, kp], reps=1)
+    assert -1 <= got["radix_bits"] <= 14
+    assert engine.get_tuning("join")["radix_bits"] == got["radix_bits"]
+    assert engine.join_count(kb, "k", kp, "k") == exp
