@@ -286,14 +286,14 @@ class Server:
     """A serving loop on an engine (gdb_serve_start ... gdb_serve_stop)."""
 
     def __init__(self, engine, lineitem_flat=None, customer=None, orders=None, lineitem=None,
-                 mode="batched", max_batch=64, slots=2, hbm_budget=0):
+                 mode="batched", max_batch=64, slots=2, hbm_budget=0, graphs=True):
         cfg = abi.ServeCfg(
             lineitem_flat.handle.value if lineitem_flat else None,
             customer.handle.value if customer else None,
             orders.handle.value if orders else None,
             lineitem.handle.value if lineitem else None,
             abi.SERVE_BATCHED if mode == "batched" else abi.SERVE_SERIAL,
-            max_batch, slots, 0, hbm_budget)
+            max_batch, slots, 1 if graphs else 0, hbm_budget)
         self._s = C.c_void_p()
         abi.check(lib().gdb_serve_start(engine._ctx, C.byref(cfg), C.byref(self._s)))
 

@@ -106,7 +106,7 @@ class Tuning(C.Structure):
 class ServeCfg(C.Structure):
     _fields_ = [("lineitem_flat", C.c_void_p), ("customer", C.c_void_p), ("orders", C.c_void_p),
                 ("lineitem", C.c_void_p), ("mode", C.c_int32), ("max_batch", C.c_int32),
-                ("slots", C.c_int32), ("pad", C.c_int32), ("hbm_budget", C.c_uint64)]
+                ("slots", C.c_int32), ("graphs", C.c_int32), ("hbm_budget", C.c_uint64)]
 
 
 class Request(C.Structure):
@@ -124,7 +124,8 @@ class ServeStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
                 ("batches", C.c_uint64), ("qps", C.c_double), ("p50_ms", C.c_double),
                 ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
-                ("mean_batch", C.c_double), ("inflight_peak_bytes", C.c_uint64)]
+                ("mean_batch", C.c_double), ("inflight_peak_bytes", C.c_uint64),
+                ("graph_captures", C.c_uint64), ("graph_replays", C.c_uint64)]
 
 
 REQ_Q6, REQ_Q1, REQ_Q3 = 0, 1, 2

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "gdb.h"
+#include "graph.h"
 #include "internal.h"
 
 namespace gdb {
@@ -63,6 +64,8 @@ struct FlatSlot {
   int blocks = 0;
   size_t max_cells = 0;
   size_t lds_limit = 0;  // dynamic LDS the scan kernel may use on this device
+  bool graphs = false;   // replay the scan + finaliser of resident tables as graphs
+  GraphCache gc;         // one graph per (table, LDS size class)
 };
 
 int flat_slot_init(gdb_ctx* ctx, FlatSlot* s);
@@ -93,6 +96,9 @@ struct Q3BatchDesc {
 struct Q3Slot;  // defined in batch_q3.hip
 int q3_slot_create(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* orders,
                    const gdb_table* lineitem, Q3Slot** out);
+// Capture the batch kernels as one graph per (batch size, grid) and replay it.
+void q3_slot_use_graphs(Q3Slot* s, bool on);
+void q3_slot_graph_counts(const Q3Slot* s, uint64_t* captures, uint64_t* replays);
 void q3_slot_free(gdb_ctx* ctx, Q3Slot* s);
 int q3_batch_launch(gdb_ctx* ctx, Q3Slot* s, const std::vector<gdb_q3_params>& reqs,
                     hipStream_t stream);

@@ -187,6 +187,24 @@ __global__ __launch_bounds__(1024) void flat_batch_scan(FlatCols c, uint32_t n,
 
 }  // namespace
 
+namespace {
+// Finaliser of a graph-replayed batch: cell c = sum of its per-block partials,
+// with the cell count taken from the descriptor (fixed grid, cells strided).
+__global__ __launch_bounds__(256) void flat_cells_sum(const FlatBatchDesc* __restrict__ d,
+                                                      const int64_t* __restrict__ partials,
+                                                      uint32_t nparts, int64_t* __restrict__ out) {
+  __shared__ int64_t red[4];
+  const int ncells = d->q6_cells * 2 + d->q1_cells * 6;
+  for (int c = blockIdx.x; c < ncells; c += gridDim.x) {
+    const int64_t* col = partials + (size_t)c * nparts;
+    int64_t s = 0;
+    for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) s += col[i];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) out[c] = s;
+  }
+}
+}  // namespace
+
 int flat_slot_init(gdb_ctx* ctx, FlatSlot* s) {
   // Use up to FB_LDS_LIMIT of dynamic LDS where the device allows it (160 KB per
   // CU on gfx950), else the default 64 KB per workgroup.
@@ -216,6 +234,7 @@ int flat_slot_init(gdb_ctx* ctx, FlatSlot* s) {
 }
 
 void flat_slot_free(gdb_ctx* ctx, FlatSlot* s) {
+  graph_cache_clear(&s->gc);
   dev_free(ctx, s->d_desc, sizeof(FlatBatchDesc));
   dev_free(ctx, s->partials, s->max_cells * s->blocks * sizeof(int64_t));
   dev_free(ctx, s->totals, s->max_cells * sizeof(int64_t));
@@ -256,14 +275,32 @@ int flat_batch_launch(gdb_ctx* ctx, FlatSlot* s, const FlatPlan& plan, const gdb
     if (stream != ctx->stream) return fail(GDB_E_INVALID, "host tables stream on the engine stream");
     RunStats rs;
     GDB_TRY(run_over(ctx, li, c, 7, consume, &rs));
+  } else if (li->nrows == 0) {
+    GDB_HIP(hipMemsetAsync(s->totals, 0, (size_t)plan.ncells * 8, stream));
+  } else if (s->graphs) {
+    // One graph per table and LDS size class: the scan is launched with the
+    // class's LDS (the plan's layout lives in the descriptor) and the
+    // finaliser reads the cell count from the descriptor.
+    const size_t lds_class = plan.lds_bytes <= 32768   ? 32768
+                             : plan.lds_bytes <= 65536 ? 65536
+                             : plan.lds_bytes <= 98304 ? 98304
+                                                       : s->lds_limit;
+    const uint32_t rows = (uint32_t)li->nrows;
+    const int blocks = std::max(1, std::min<int>(s->blocks, (int)((rows / 4 + 1023) / 1024)));
+    const FlatCols fc{(const int32_t*)c[0]->dev, (const int32_t*)c[1]->dev, (const int32_t*)c[2]->dev,
+                      (const int32_t*)c[3]->dev, (const int64_t*)c[4]->dev, (const uint8_t*)c[5]->dev,
+                      (const uint8_t*)c[6]->dev};
+    const uint64_t key = (reinterpret_cast<uintptr_t>(li) << 8) ^ (lds_class >> 12);
+    GDB_TRY(graph_launch(&s->gc, key, stream, [&](hipStream_t st) -> int {
+      flat_batch_scan<<<blocks, 1024, lds_class, st>>>(fc, rows, s->d_desc, s->partials);
+      flat_cells_sum<<<512, 256, 0, st>>>(s->d_desc, s->partials, (uint32_t)blocks, s->totals);
+      GDB_HIP(hipGetLastError());
+      return GDB_OK;
+    }));
   } else {
     const void* dev[7];
     for (int i = 0; i < 7; ++i) dev[i] = c[i]->dev;
-    if (li->nrows == 0) {
-      GDB_HIP(hipMemsetAsync(s->totals, 0, (size_t)plan.ncells * 8, stream));
-    } else {
-      GDB_TRY(consume(dev, (uint32_t)li->nrows));
-    }
+    GDB_TRY(consume(dev, (uint32_t)li->nrows));
   }
   GDB_HIP(hipMemcpyAsync(s->h_totals, s->totals, (size_t)plan.ncells * 8, hipMemcpyDeviceToHost,
                          stream));

@@ -58,6 +58,8 @@ struct Q3Slot {
   uint32_t* h_counters = nullptr;
   uint64_t bytes = 0;
   std::vector<std::pair<void*, size_t>> allocs;
+  bool graphs = false;  // replay the batch kernels as graphs
+  GraphCache gc;        // one graph per (batch size, groups grid)
 };
 
 namespace {
@@ -392,9 +394,19 @@ int q3_slot_create(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* ord
   return GDB_OK;
 }
 
+void q3_slot_use_graphs(Q3Slot* s, bool on) {
+  if (s) s->graphs = on;
+}
+
+void q3_slot_graph_counts(const Q3Slot* s, uint64_t* captures, uint64_t* replays) {
+  *captures = s ? s->gc.captures : 0;
+  *replays = s ? s->gc.replays : 0;
+}
+
 void q3_slot_free(gdb_ctx* ctx, Q3Slot* s) {
   if (!s) return;
   (void)hipStreamSynchronize(ctx->stream);
+  graph_cache_clear(&s->gc);
   for (auto& a : s->allocs) dev_free(ctx, a.first, a.second);
   if (s->h_desc) (void)hipHostFree(s->h_desc);
   if (s->h_win) (void)hipHostFree(s->h_win);
@@ -439,15 +451,25 @@ int q3_batch_launch(gdb_ctx* ctx, Q3Slot* s, const std::vector<gdb_q3_params>& r
   if (obase > s->groups_cap) return fail(GDB_E_ADMISSION, "Q3 batch exceeds its group arena");
   GDB_HIP(hipMemcpyAsync(s->d_desc, s->h_desc, sizeof(Q3BatchDesc), hipMemcpyHostToDevice, st));
   GDB_HIP(hipMemcpyAsync(s->d_win, s->h_win, n * 16, hipMemcpyHostToDevice, st));
-  q3b_clear<<<64, 256, 0, st>>>(s->counters, s->hist, n);
   const uint32_t gx = std::max<uint32_t>(1, std::min<uint32_t>((maxspan + 255) / 256,
                                                                (uint32_t)ctx->info.cu_count * 4 / n + 1));
-  q3b_groups<<<dim3(gx, n), 256, 0, st>>>(s->d_desc, s->d_win, s->byday, s->okey, s->ocust, s->odate,
-                                          s->oprio, s->cseg, s->ncseg, s->csr_off, s->csr_ship,
-                                          s->csr_rev, s->counters, s->hist, s->groups);
-  q3b_threshold<<<n, 256, 0, st>>>(s->d_desc, s->hist, s->counters);
-  q3b_topk<<<n, 256, 0, st>>>(s->d_desc, s->d_win, s->counters, s->groups, s->out);
-  GDB_HIP(hipGetLastError());
+  // Everything a batch varies in lives in the descriptor and the windows, so
+  // the sequence depends only on (n, gx).
+  auto enqueue = [&](hipStream_t q) -> int {
+    q3b_clear<<<64, 256, 0, q>>>(s->counters, s->hist, n);
+    q3b_groups<<<dim3(gx, n), 256, 0, q>>>(s->d_desc, s->d_win, s->byday, s->okey, s->ocust,
+                                           s->odate, s->oprio, s->cseg, s->ncseg, s->csr_off,
+                                           s->csr_ship, s->csr_rev, s->counters, s->hist, s->groups);
+    q3b_threshold<<<n, 256, 0, q>>>(s->d_desc, s->hist, s->counters);
+    q3b_topk<<<n, 256, 0, q>>>(s->d_desc, s->d_win, s->counters, s->groups, s->out);
+    GDB_HIP(hipGetLastError());
+    return GDB_OK;
+  };
+  if (s->graphs) {
+    GDB_TRY(graph_launch(&s->gc, ((uint64_t)n << 32) | gx, st, enqueue));
+  } else {
+    GDB_TRY(enqueue(st));
+  }
   GDB_HIP(hipMemcpyAsync(s->h_out, s->out, (size_t)n * Q3B_TOPK * sizeof(TopEntry),
                          hipMemcpyDeviceToHost, st));
   GDB_HIP(hipMemcpyAsync(s->h_counters, s->counters, n * 16, hipMemcpyDeviceToHost, st));

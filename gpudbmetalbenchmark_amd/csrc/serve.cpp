@@ -350,10 +350,12 @@ int gdb_serve_start(gdb_ctx* ctx, const gdb_serve_cfg* cfg, gdb_server** out) {
         return bail(fail(GDB_E_HIP, "event creation failed"));
       int rc = flat_slot_init(ctx, &s.flat);
       if (rc != GDB_OK) return bail(rc);
+      s.flat.graphs = cfg->graphs != 0;
     }
     if (cfg->customer && cfg->orders && cfg->lineitem) {
       int rc = q3_slot_create(ctx, cfg->customer, cfg->orders, cfg->lineitem, &S->q3);
       if (rc != GDB_OK) return bail(rc);
+      q3_slot_use_graphs(S->q3, cfg->graphs != 0);
     }
   }
   S->th = std::thread(scheduler, S);
@@ -442,11 +444,26 @@ static void fill_stats(const std::vector<double>& lat_in, uint64_t submitted, ui
   o->qps = (t1 > t0) ? completed / ((t1 - t0) / 1000.0) : 0.0;
 }
 
+static void graph_counts(const gdb_server* S, uint64_t* captures, uint64_t* replays) {
+  *captures = *replays = 0;
+  for (const Slot& s : S->slots) {
+    *captures += s.flat.gc.captures;
+    *replays += s.flat.gc.replays;
+  }
+  if (S->q3) {
+    uint64_t c = 0, r = 0;
+    q3_slot_graph_counts(S->q3, &c, &r);
+    *captures += c;
+    *replays += r;
+  }
+}
+
 int gdb_serve_stats_get(gdb_server* S, gdb_serve_stats* out) {
   if (!S || !out) return fail(GDB_E_INVALID, "null argument");
   std::lock_guard<std::mutex> g(S->mu);
   fill_stats(S->lat, S->submitted, S->completed, S->rejected, S->batches, S->batch_reqs, S->t_first,
              S->t_last, S->peak, out);
+  graph_counts(S, &out->graph_captures, &out->graph_replays);
   return S->fatal;
 }
 
@@ -549,6 +566,10 @@ int gdb_serve_loadgen(gdb_server* S, double rate, double seconds, uint64_t seed,
     t_last = S->t_last;
   }
   fill_stats(lat, n, got, rejected + failed, batches, breqs, t0, t_last, S->peak, stats);
+  {
+    std::lock_guard<std::mutex> g(S->mu);
+    graph_counts(S, &stats->graph_captures, &stats->graph_replays);
+  }
   if (got + failed < ids.size()) return fail(GDB_E_TIMEOUT, "responses missing after 120 s");
   return GDB_OK;
 }

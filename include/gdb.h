@@ -240,7 +240,8 @@ typedef struct {
   int32_t mode;                    /* GDB_SERVE_BATCHED / GDB_SERVE_SERIAL */
   int32_t max_batch;               /* requests per batch, 1..64 */
   int32_t slots;                   /* batches in flight, 1..4 */
-  int32_t pad;
+  int32_t graphs;                  /* 1: batch kernels of resident tables are captured
+                                      once per launch shape and replayed as HIP graphs */
   uint64_t hbm_budget;             /* bytes for in-flight batch scratch; 0 = no limit */
 } gdb_serve_cfg;
 typedef struct {
@@ -267,6 +268,8 @@ typedef struct {
   double p50_ms, p99_ms, mean_ms, max_ms;
   double mean_batch;
   uint64_t inflight_peak_bytes;
+  uint64_t graph_captures;         /* launch shapes captured as graphs */
+  uint64_t graph_replays;          /* batches launched as graph replays */
 } gdb_serve_stats;
 int gdb_serve_start(gdb_ctx* ctx, const gdb_serve_cfg* cfg, gdb_server** out);
 /* Thread-safe; GDB_E_ADMISSION if the request alone exceeds the budget. */

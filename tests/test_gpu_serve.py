@@ -52,14 +52,14 @@ def run_mix(engine, flat, t, reqs, **kw):
     return expected, got, stats
 
 
-@pytest.mark.parametrize("mode", ["batched", "serial"])
-def test_mix_results_equal_alone(engine, tables, mode):
+@pytest.mark.parametrize("mode,graphs", [("batched", True), ("batched", False), ("serial", False)])
+def test_mix_results_equal_alone(engine, tables, mode, graphs):
     flat, t = tables
     reqs = [loadgen_request(7, i) for i in range(160)]
     reqs += [("q6", dict(date_lo=0, date_hi=2526, disc_lo=0, disc_hi=10, qty_lt=51)),
              ("q1", dict(cutoff=-1)), ("q3", dict(segment=4, date=0, top_k=100)),
              ("q3", dict(segment=2, date=1169, top_k=1))]
-    expected, got, stats = run_mix(engine, flat, t, reqs, mode=mode)
+    expected, got, stats = run_mix(engine, flat, t, reqs, mode=mode, graphs=graphs)
     for i, (k, p) in enumerate(reqs):
         assert got[i]["status"] == 0
         assert same(k, got[i]["result"], expected[i]), (i, k, p)
@@ -67,6 +67,29 @@ def test_mix_results_equal_alone(engine, tables, mode):
     if mode == "batched":
         assert max(r["batch_size"] for r in got.values()) > 1
         assert stats["batches"] < len(reqs)
+    assert (stats["graph_replays"] > 0) == graphs
+
+
+def test_graph_replays_interleaved_with_eager_work(engine, tables):
+    """Batches replayed from captured graphs, in waves separated by eager
+    pipeline runs and synchronisations on the engine stream: every replayed
+    response equals the eager result of the same request."""
+    flat, t = tables
+    with Server(engine, flat, t["customer"], t["orders"], t["lineitem"], graphs=True) as srv:
+        for wave in range(4):
+            reqs = [loadgen_request(40 + wave, i) for i in range(48)]
+            expected = [alone(engine, flat, t, k, p) for k, p in reqs]
+            ids = {srv.submit(request(k, **p)): i for i, (k, p) in enumerate(reqs)}
+            got = {}
+            while len(got) < len(ids):
+                for r in srv.poll(256, 1000):
+                    got[ids[r["id"]]] = r
+            for i, (k, p) in enumerate(reqs):
+                assert got[i]["status"] == 0
+                assert same(k, got[i]["result"], expected[i]), (wave, i, k, p)
+            engine.q6(flat, 731, 1096, 5, 7, 24)  # eager work between the waves
+        st = srv.stats()
+    assert st["graph_replays"] > st["graph_captures"] > 0
 
 
 def test_admission_rejects_oversized_requests(engine, tables):
