@@ -131,7 +131,7 @@ def build_workloads(eng, names, scale, O):
             otab = (eng.host_table if stream else eng.table)(
                 {k: js["orders"][k] for k in ("o_custkey", "o_comment")})
             out["q13"] = Workload(
-                "q13", "q13_count",
+                "q13", "q13_match",
                 f"Q13 left-outer count: orders {no} (80-byte comments), customer {nc}"
                 f"{' (orders streamed from pinned host memory)' if stream else ''}",
                 no, lambda tm: eng.q13(dims["customer"], otab, timing=tm),
@@ -152,7 +152,7 @@ def build_workloads(eng, names, scale, O):
                     lambda: O.sum_col(li["l_quantity"]), 4 * nl)
             if "join" in names:
                 out["join"] = Workload(
-                    "join", "join_probe", f"join count orders({no}) x lineitem({nl}) on orderkey",
+                    "join", "rj_join", f"join count orders({no}) x lineitem({nl}) on orderkey",
                     nl, lambda tm: eng.join_count(orders_res, "o_orderkey", fact, "l_orderkey",
                                                   timing=tm),
                     lambda: O.join_count(js["orders"]["o_orderkey"], li["l_orderkey"]),
