@@ -43,23 +43,6 @@ constexpr int NBINS = 1024;
 constexpr int BUCKET_SHIFT = 14;                 // 16384 custkeys per bucket (64 KiB of LDS counters)
 constexpr uint32_t MAX_BUCKETS = 4096;
 
-bool make_pattern(const char* s, PatternW* p) {
-  const size_t n = strnlen(s, 32);
-  if (n == 0 || n > 31) return false;
-  memset(p, 0, sizeof(*p));
-  uint8_t b[32] = {0};
-  memcpy(b, s, n);
-  memcpy(p->w, b, 32);
-  p->len = (int)n;
-  p->nw = (int)(n + 3) / 4;
-  const int rem = (int)n - 4 * (p->nw - 1);
-  p->last_mask = rem == 4 ? 0xFFFFFFFFu : ((1u << (8 * rem)) - 1u);
-  p->c0 = 0x01010101u * b[0];
-  p->c1 = 0x01010101u * (n > 1 ? b[1] : b[0]);
-  p->m0 = n >= 4 ? 0xFFFFFFFFu : p->last_mask;
-  return true;
-}
-
 // Comment matchers: a functor on the 80-byte field held in 20 dwords followed
 // by 8 zero dwords, true if the field is LIKE '%w1%w2%'.
 template <bool FULL0>

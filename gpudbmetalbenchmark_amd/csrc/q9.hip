@@ -3,20 +3,28 @@
 // src/infra.h:733-776, SQL sql/q9.sql).
 //
 //   clear      part bitmap, counters, supplier->nation map (-1), orderkey->year map
-//   part       tiles of 256 names staged in LDS; bit[partkey] if the name contains
-//              the pattern (strmatch.h)                      (the reference's SWAR
-//              'g' scan, kernels/q9.metal:58-83, generalised to any pattern)
+//   part       tiles of 256 names copied contiguously into LDS with dwordx4
+//              loads; bit[partkey] if the name contains the pattern
+//              (strmatch.h; the reference's SWAR 'g' scan, kernels/q9.metal:58-83,
+//              generalised to any pattern)
 //   supplier   nation[suppkey]
 //   partsupp   rows whose part passed -> compact list (partkey, suppkey, cost)
 //   ps_clear / ps_insert
-//              open-addressing table keyed by (partkey, suppkey), capacity
-//              nextpow2(2*count) from the device-side count
-//   orders     year[orderkey] (uint8 offset from 1970)
-//   probe      (dominant) fact rows whose part bit is set: supplier nation,
-//              partsupp cost, order year; amount = extprice*(100-disc) -
-//              cost*qty*100 added into an LDS-private [nation][year] table
-//              (64 x 64 int64 + hit flags), merged by global atomics per block
-// Columns other than l_partkey are loaded only for lanes with a surviving row.
+//              open-addressing table of 16-byte {key, cost} slots keyed by
+//              (partkey, suppkey), capacity nextpow2(2*count) from the
+//              device-side count, so a probe touches one cache line
+//   orders     year[orderkey] as an index into the query's year range, found
+//              from the Jan-1 day numbers of those years (a kernel argument)
+//   probe      (dominant) every fact column is streamed once with nontemporal
+//              dwordx4 loads; rows whose part bit is set are queued per wave in
+//              LDS with their values, and full waves of 64 queued rows then
+//              run the dependent chain (nation map, partsupp table, year map)
+//              and add amount = extprice*(100-disc) - cost*qty*100 into an
+//              LDS table sized to the data's (nation, year) ranges, merged into
+//              the global 64 x 64 table by atomics once per block
+// Streaming all six columns reads each byte once; gathering the columns of
+// the ~5% surviving rows touched over 80% of the 128-byte lines anyway and
+// fetched 1.5x the column bytes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,10 +43,10 @@ namespace {
 
 constexpr int TILE = 256;
 constexpr int NW = 55;          // p_name width
-constexpr int SLOT = 60;        // LDS bytes per name: 56 (zero pad) + 4 lookahead
 constexpr int NNAT = 64, NYEAR = 64, YEAR0 = 1970;
 constexpr int64_t EMPTY64 = -1;
 constexpr uint8_t NO_YEAR = 0xFF;
+constexpr int QCAP = 128;       // per-wave survivor queue: < 64 left over + 64 pushed
 
 __device__ __forceinline__ uint32_t ps_cap(uint32_t count) {
   uint32_t c = 1024;
@@ -58,28 +66,30 @@ __device__ __forceinline__ uint32_t ps_hash(uint64_t k, uint32_t cap) {
   return (x ^ (x >> lg) ^ (sk * 0x9E3779B1u >> (32 - lg))) & (cap - 1);
 }
 
-// Year of a day number (day 0 = 1992-01-01) by the days-from-civil inverse.
-__device__ __forceinline__ int year_of_day(int32_t day) {
-  int64_t z = (int64_t)day + 8035 + 719468;
-  int64_t era = (z >= 0 ? z : z - 146096) / 146097;
-  int64_t doe = z - era * 146097;
-  int64_t yoe = (doe - doe / 1460 + doe / 36524 - doe / 146096) / 365;
-  int64_t y = yoe + era * 400;
-  int64_t doy = doe - (365 * yoe + yoe / 4 - yoe / 100);
-  int64_t mp = (5 * doy + 2) / 153;
-  return (int)(y + (mp >= 10));
+// Jan-1 day numbers of the years an orders table spans: year index of `day`
+// = number of year starts after the first that are <= day.
+struct Years {
+  int32_t jan1[NYEAR];
+  int32_t ny;        // years spanned, 1..NYEAR
+  int32_t first;     // first year - YEAR0
+};
+
+__device__ __forceinline__ uint32_t year_index(int32_t day, const Years& y) {
+  uint32_t k = 0;
+  for (int i = 1; i < y.ny; ++i) k += day >= y.jan1[i] ? 1u : 0u;
+  return k;
 }
 
 struct Q9Scratch {
   uint32_t* pbm;       // part bitmap
+  uint32_t* pfold;     // part bitmap folded to the probe's LDS filter size
   uint32_t* counters;  // [0] partsupp list count
   int32_t* snat;       // suppkey -> nation or -1
-  uint8_t* oyear;      // orderkey -> year - 1970 or 0xFF
+  uint8_t* oyear;      // orderkey -> year index or 0xFF
   int32_t* lpk;        // compact partsupp list
   int32_t* lsk;
   int64_t* lcost;
-  int64_t* ht_key;     // (partkey, suppkey) or EMPTY64
-  int64_t* ht_cost;
+  longlong2* ht;       // {(partkey, suppkey) or EMPTY64, cost}
   int64_t* agg;        // [NNAT*NYEAR] profit
   uint32_t* agg_hit;   // [NNAT*NYEAR]
 };
@@ -100,43 +110,57 @@ __global__ __launch_bounds__(256) void q9_clear(Q9Scratch s, uint32_t npw, uint3
   if (blockIdx.x == 0 && threadIdx.x < 4) s.counters[threadIdx.x] = 0u;
 }
 
+// A tile of 256 names is 14080 contiguous bytes (16-byte aligned, as every
+// tile starts at a multiple of 256 rows): it is copied into LDS with dwordx4
+// loads, then lane t assembles name t (55 bytes at LDS offset 55t) into 14
+// aligned dwords plus two zero pad dwords and searches it in registers.
 __global__ __launch_bounds__(TILE) void q9_part(const uint8_t* __restrict__ name,
                                                 const int32_t* __restrict__ partkey, uint32_t n,
-                                                Pattern pat, uint32_t maxpk,
+                                                PatternW pat, uint32_t maxpk,
                                                 uint32_t* __restrict__ pbm) {
-  __shared__ __align__(16) uint8_t tile[TILE * SLOT + 16];
-  uint8_t* mine = tile + threadIdx.x * SLOT;
+  constexpr int TBYTES = TILE * NW;          // 14080
+  constexpr int TCHUNKS = TBYTES / 16;       // 880
+  __shared__ int4 tile[TCHUNKS + 1];
+  const uint32_t* tw = reinterpret_cast<const uint32_t*>(tile);
+  const uint64_t col_chunks = ((uint64_t)n * NW + 15) / 16;
   const uint32_t ntiles = (n + TILE - 1) / TILE;
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const uint32_t row0 = t * TILE;
-    const uint32_t rows = min((uint32_t)TILE, n - row0);
-    const uint8_t* src = name + (size_t)row0 * NW;
+    const uint64_t c0 = (uint64_t)t * TCHUNKS;
+    const int4* src = reinterpret_cast<const int4*>(name) + c0;
     __syncthreads();
-    // Names are 55 bytes: copy byte-exact with dword loads where aligned.
-    const uint32_t bytes = rows * NW;
-    for (uint32_t b = threadIdx.x * 4; b < bytes; b += TILE * 4) {
-      uint32_t v;
-      if (b + 4 <= bytes && (((uintptr_t)(src + b)) & 3) == 0) {
-        v = *reinterpret_cast<const uint32_t*>(src + b);
-      } else {
-        v = 0;
-        for (int q = 0; q < 4 && b + q < bytes; ++q) v |= (uint32_t)src[b + q] << (8 * q);
-      }
-      for (int q = 0; q < 4 && b + q < bytes; ++q) {
-        const uint32_t r = (b + q) / NW, off = (b + q) % NW;
-        tile[r * SLOT + off] = (uint8_t)(v >> (8 * q));
-      }
+    for (uint32_t c = threadIdx.x; c < TCHUNKS; c += TILE)
+      tile[c] = c0 + c < col_chunks ? src[c] : int4{0, 0, 0, 0};
+    if (threadIdx.x == 0) tile[TCHUNKS] = int4{0, 0, 0, 0};
+    __syncthreads();
+    const uint32_t r = t * TILE + threadIdx.x;
+    if (r >= n) continue;
+    const uint32_t off = threadIdx.x * NW, w0 = off >> 2, sh = off & 3;
+    uint32_t d[16];
+#pragma unroll
+    for (int i = 0; i < 14; ++i) {
+      const uint32_t lo = tw[w0 + i], hi = tw[w0 + i + 1];
+      d[i] = __builtin_amdgcn_alignbyte(hi, lo, sh);
     }
-    for (int q = NW; q < SLOT; ++q) mine[q] = 0;
-    __syncthreads();
-    if (threadIdx.x < rows) {
-      const int len = field_len(mine, 56);
-      if (find_from(mine, len, 0, pat) >= 0) {
-        const uint32_t pk = (uint32_t)partkey[row0 + threadIdx.x];
-        if (pk <= maxpk) atomicOr(&pbm[pk >> 5], 1u << (pk & 31));
-      }
+    d[13] &= 0x00FFFFFFu;  // byte 55 belongs to the next name
+    d[14] = 0u;
+    d[15] = 0u;
+    const int len = reg_len<14>(d);
+    if (reg_find<14>(d, len, 0, pat) >= 0) {
+      const uint32_t pk = (uint32_t)partkey[r];
+      if (pk <= maxpk) atomicOr(&pbm[pk >> 5], 1u << (pk & 31));
     }
   }
+}
+
+// The part bitmap folded onto fwords words: bit k of the fold is the OR of
+// the bits of every key congruent to k modulo 32 * fwords.
+__global__ __launch_bounds__(256) void q9_fold(const uint32_t* __restrict__ pbm, uint32_t npw,
+                                               uint32_t fwords, uint32_t* __restrict__ fold) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= fwords) return;
+  uint32_t v = 0;
+  for (uint32_t i = w; i < npw; i += fwords) v |= pbm[i];
+  fold[w] = v;
 }
 
 __global__ __launch_bounds__(256) void q9_supplier(const int32_t* __restrict__ sk,
@@ -148,13 +172,15 @@ __global__ __launch_bounds__(256) void q9_supplier(const int32_t* __restrict__ s
 
 __global__ __launch_bounds__(256) void q9_orders(const int32_t* __restrict__ ok,
                                                  const int32_t* __restrict__ od, uint32_t n,
-                                                 uint32_t maxok, uint8_t* __restrict__ oyear) {
+                                                 uint32_t maxok, Years years,
+                                                 uint8_t* __restrict__ oyear) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && (uint32_t)ok[i] <= maxok) oyear[ok[i]] = (uint8_t)(year_of_day(od[i]) - YEAR0);
+  if (i < n && (uint32_t)ok[i] <= maxok) oyear[ok[i]] = (uint8_t)year_index(od[i], years);
 }
 
 // Partsupp rows whose part passed -> compact list in row order, one atomic per
-// tile of 16 x 256 rows (thread t owns rows [16t, 16t+16) of the tile).
+// tile of 16 x 256 rows: thread t tests rows 4t..4t+3 of each of the tile's
+// four 1024-row quarters with dwordx4 loads of ps_partkey.
 __global__ __launch_bounds__(256) void q9_partsupp(const int32_t* __restrict__ pk,
                                                    const int32_t* __restrict__ sk,
                                                    const int64_t* __restrict__ cost, uint32_t n,
@@ -168,14 +194,32 @@ __global__ __launch_bounds__(256) void q9_partsupp(const int32_t* __restrict__ p
   const uint32_t tile_rows = 16 * blockDim.x;
   const uint32_t ntiles = (n + tile_rows - 1) / tile_rows;
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const uint32_t r0 = t * tile_rows + threadIdx.x * 16;
-    uint32_t mask = 0;
-    for (int j = 0; j < 16; ++j) {
-      const uint32_t r = r0 + j;
-      if (r < n) {
-        const uint32_t p = (uint32_t)pk[r];
-        if (p <= maxpk && ((pbm[p >> 5] >> (p & 31)) & 1u)) mask |= 1u << j;
+    uint32_t rows[4];
+    int32_t ks[16];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) rows[v] = t * tile_rows + v * 4 * blockDim.x + 4 * threadIdx.x;
+    if (t * tile_rows + tile_rows <= n) {
+      int4 k4[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) k4[v] = ld4(pk, rows[v] / 4);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        ks[4 * v] = k4[v].x; ks[4 * v + 1] = k4[v].y; ks[4 * v + 2] = k4[v].z; ks[4 * v + 3] = k4[v].w;
       }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ks[4 * v + j] = rows[v] + j < n ? pk[rows[v] + j] : -1;
+    }
+    uint32_t words[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) words[j] = pbm[min((uint32_t)ks[j], maxpk) >> 5];
+    uint32_t mask = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const uint32_t p = (uint32_t)ks[j];
+      mask |= (p <= maxpk && ((words[j] >> (p & 31)) & 1u) ? 1u : 0u) << j;
     }
     uint32_t total;
     uint32_t pos = block_excl_scan((uint32_t)__popc(mask), scratch, &total);
@@ -186,8 +230,8 @@ __global__ __launch_bounds__(256) void q9_partsupp(const int32_t* __restrict__ p
     while (mask) {
       const int j = __ffs((int)mask) - 1;
       mask &= mask - 1;
-      const uint32_t r = r0 + j;
-      lpk[pos] = pk[r];
+      const uint32_t r = rows[j >> 2] + (j & 3);
+      lpk[pos] = ks[j];
       lsk[pos] = sk[r];
       lcost[pos] = cost[r];
       ++pos;
@@ -197,18 +241,22 @@ __global__ __launch_bounds__(256) void q9_partsupp(const int32_t* __restrict__ p
 }
 
 __global__ __launch_bounds__(256) void q9_ps_clear(const uint32_t* __restrict__ counters,
-                                                   int64_t* __restrict__ ht_key) {
+                                                   longlong2* __restrict__ ht) {
   const uint32_t cap = ps_cap(counters[0]);
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += stride) ht_key[i] = EMPTY64;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += stride) {
+    longlong2 e;
+    e.x = EMPTY64;
+    e.y = 0;
+    ht[i] = e;
+  }
 }
 
 __global__ __launch_bounds__(256) void q9_ps_insert(const uint32_t* __restrict__ counters,
                                                     const int32_t* __restrict__ lpk,
                                                     const int32_t* __restrict__ lsk,
                                                     const int64_t* __restrict__ lcost,
-                                                    int64_t* __restrict__ ht_key,
-                                                    int64_t* __restrict__ ht_cost) {
+                                                    longlong2* __restrict__ ht) {
   const uint32_t count = counters[0];
   const uint32_t cap = ps_cap(count);
   const uint32_t stride = gridDim.x * blockDim.x;
@@ -217,9 +265,9 @@ __global__ __launch_bounds__(256) void q9_ps_insert(const uint32_t* __restrict__
     uint32_t h = ps_hash(key, cap);
     for (uint32_t probe = 0; probe < cap; ++probe) {
       const unsigned long long prev =
-          atomicCAS((unsigned long long*)&ht_key[h], (unsigned long long)EMPTY64, key);
+          atomicCAS((unsigned long long*)&ht[h].x, (unsigned long long)EMPTY64, key);
       if (prev == (unsigned long long)EMPTY64) {
-        ht_cost[h] = lcost[j];
+        ht[h].y = lcost[j];
         break;
       }
       if (prev == key) break;  // duplicate (partkey, suppkey): first insert keeps the slot
@@ -233,115 +281,185 @@ struct Q9Fact {
   const int64_t* price;
 };
 
-__device__ __forceinline__ void q9_row(int32_t pk, int32_t sk, int32_t ok, int32_t qty,
-                                       int32_t disc, int64_t price, uint32_t cap,
-                                       uint32_t maxsk, uint32_t maxok, const Q9Scratch& s,
-                                       int64_t* lagg, uint32_t* lhit) {
-  if ((uint32_t)sk > maxsk || (uint32_t)ok > maxok) return;
-  const int32_t nat = s.snat[sk];
-  if (nat < 0) return;
-  const uint8_t y = s.oyear[ok];
-  if (y == NO_YEAR) return;
+struct Q9Dims {
+  uint32_t maxpk, maxsk, maxok;
+  int32_t ny;        // years in the LDS table
+  int32_t first;     // global year slot of LDS year 0
+  int32_t nslots;    // nations x ny
+  uint32_t fwords;   // words of the folded part filter in LDS (a power of two)
+  int32_t exact;     // 1: every partkey is below 32 * fwords, the fold is the bitmap
+};
+
+// One queued row: nation, year and the first partsupp slot are loaded
+// together (none depends on another), then the rare further probe steps.
+__device__ __forceinline__ void q9_entry(int32_t pk, int32_t sk, int32_t ok, int32_t qty,
+                                         int64_t amt, uint32_t cap, const Q9Dims& dm,
+                                         const Q9Scratch& s, int64_t* ltab, uint32_t* lhit) {
+  const bool inr = (uint32_t)sk <= dm.maxsk && (uint32_t)ok <= dm.maxok;
   const uint64_t key = ps_key(pk, sk);
   uint32_t h = ps_hash(key, cap);
+  const int32_t nat = s.snat[inr ? (uint32_t)sk : 0u];
+  const uint8_t y = s.oyear[inr ? (uint32_t)ok : 0u];
+  longlong2 e = s.ht[h];
   int64_t cost = -1;
   for (uint32_t probe = 0; probe < cap; ++probe) {
-    const int64_t k = s.ht_key[h];
-    if (k == (int64_t)key) {
-      cost = s.ht_cost[h];
+    if (e.x == (int64_t)key) {
+      cost = e.y;
       break;
     }
-    if (k == EMPTY64) break;
+    if (e.x == EMPTY64) break;
     h = (h + 1) & (cap - 1);
+    e = s.ht[h];
   }
-  if (cost < 0) return;
-  const int64_t amount = price * (int64_t)(100 - disc) - cost * (int64_t)qty * 100;
-  const int slot = nat * NYEAR + y;
-  atomicAdd((unsigned long long*)&lagg[slot], (unsigned long long)amount);
+  if (!inr || nat < 0 || y == NO_YEAR || cost < 0) return;
+  const int slot = nat * dm.ny + y;
+  atomicAdd((unsigned long long*)&ltab[slot], (unsigned long long)(amt - cost * (int64_t)qty * 100));
   lhit[slot] = 1u;
 }
 
-// Orders LDS traffic within one wave: writes by some lanes become visible to
-// reads by other lanes of the same wave after this.
-constexpr int RING = 512;  // per-wave queue of surviving row ids
+struct Q9Queue {
+  int32_t pk[QCAP], sk[QCAP], ok[QCAP], qty[QCAP];
+  int64_t amt[QCAP];
+};
 
-// Fact rows are tested 4 per lane on l_partkey alone; the survivors (about 5%)
-// are queued per wave in LDS and processed 64 at a time, one row per lane, so
-// the dependent chain (nation map, partsupp table, year map, LDS add) runs
-// with full waves instead of once per row slot with most lanes idle.
-__global__ __launch_bounds__(256) void q9_probe(Q9Fact f, uint32_t n, uint32_t maxpk,
-                                                uint32_t maxsk, uint32_t maxok, Q9Scratch s) {
-  __shared__ int64_t lagg[NNAT * NYEAR];
-  __shared__ uint32_t lhit[NNAT * NYEAR];
-  __shared__ uint32_t ring[4][RING];
-  for (int i = threadIdx.x; i < NNAT * NYEAR; i += blockDim.x) {
-    lagg[i] = 0;
+// U groups of 4 rows per lane and step: the loads of all six columns for the
+// U groups are issued first; then, row slot by row slot, lanes whose row's part
+// bit is set push the row's values onto the wave's queue, and whenever 64 are
+// queued one full wave of entries is processed.
+// The part test reads the folded filter in LDS (up to 2^20 bits, 128 KiB):
+// random 4-byte gathers from the 250 KB global bitmap cost a vector-cache line
+// per lane and took as long as streaming all six columns. Only keys whose
+// folded bit is set read the global bitmap (none when the fold is exact).
+constexpr int PROBE_THREADS = 512;
+template <int U>
+__global__ __launch_bounds__(PROBE_THREADS) void q9_probe(Q9Fact f, uint32_t n, Q9Dims dm,
+                                                          Q9Scratch s) {
+  extern __shared__ uint32_t dyn[];  // folded filter, then nslots sums and nslots hit flags
+  __shared__ Q9Queue queues[PROBE_THREADS / 64];
+  uint32_t* lf = dyn;
+  int64_t* ltab = reinterpret_cast<int64_t*>(dyn + dm.fwords);
+  uint32_t* lhit = reinterpret_cast<uint32_t*>(ltab + dm.nslots);
+  for (uint32_t i = threadIdx.x; i < dm.fwords; i += blockDim.x) lf[i] = s.pfold[i];
+  for (int i = threadIdx.x; i < dm.nslots; i += blockDim.x) {
+    ltab[i] = 0;
     lhit[i] = 0u;
   }
   __syncthreads();
+  const uint32_t fmask = dm.fwords * 32u - 1u;
   const uint32_t cap = ps_cap(s.counters[0]);
   const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  uint32_t* q = ring[w];
+  const uint64_t below = (1ull << lane) - 1;
+  Q9Queue& q = queues[w];
   uint32_t head = 0, tail = 0;  // wave-uniform
+  auto push = [&](bool pass, int32_t pk, int32_t sk, int32_t ok, int32_t qty, int64_t amt) {
+    const uint64_t b = __ballot(pass);
+    if (pass) {
+      const uint32_t i = (tail + (uint32_t)__popcll(b & below)) & (QCAP - 1);
+      q.pk[i] = pk;
+      q.sk[i] = sk;
+      q.ok[i] = ok;
+      q.qty[i] = qty;
+      q.amt[i] = amt;
+    }
+    tail += (uint32_t)__popcll(b);
+  };
   auto drain = [&](bool all) {
+    wave_lds_sync();
     while (tail - head >= 64 || (all && tail != head)) {
       const uint32_t avail = min(64u, tail - head);
       if (lane < avail) {
-        const uint32_t r = q[(head + lane) & (RING - 1)];
-        q9_row(f.pk[r], f.sk[r], f.ok[r], f.qty[r], f.disc[r], f.price[r], cap, maxsk, maxok, s,
-               lagg, lhit);
+        const uint32_t i = (head + lane) & (QCAP - 1);
+        q9_entry(q.pk[i], q.sk[i], q.ok[i], q.qty[i], q.amt[i], cap, dm, s, ltab, lhit);
       }
       head += avail;
     }
+    wave_lds_sync();
+  };
+  auto test = [&](int32_t pk) {
+    const uint32_t k = (uint32_t)pk, kf = k & fmask;
+    bool pass = k <= dm.maxpk && ((lf[kf >> 5] >> (kf & 31)) & 1u);
+    if (!dm.exact && pass) pass = (s.pbm[k >> 5] >> (k & 31)) & 1u;
+    return pass;
   };
   const uint32_t nvec = n >> 2;
-  const uint32_t wstride = gridDim.x * blockDim.x;           // vector groups per grid step
-  const uint32_t wbase = blockIdx.x * blockDim.x + w * 64;   // first group of this wave
-  for (uint32_t g0 = wbase; g0 < nvec; g0 += wstride) {
-    const uint32_t g = g0 + lane;
-    uint32_t m = 0;
-    if (g < nvec) {
-      const int4 p4 = ld4nt(f.pk, g);
-      const int32_t ps[4] = {p4.x, p4.y, p4.z, p4.w};
+  const uint32_t nthr = gridDim.x * blockDim.x;
+  // Wave-uniform trip count: every lane of a wave runs the same iterations so
+  // the ballots and queue counters stay in step; lanes past the end load nothing.
+  const uint32_t g0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+  for (uint32_t gb = g0; gb < nvec; gb += U * nthr) {
+    int4 p4[U], s4[U], o4[U], q4[U], d4[U];
+    longlong2 pa[U], pb[U];
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if ((uint32_t)ps[j] <= maxpk && ((s.pbm[(uint32_t)ps[j] >> 5] >> (ps[j] & 31)) & 1u))
-          m |= 1u << j;
+    for (int u = 0; u < U; ++u) {
+      const uint32_t g = gb + u * nthr + lane;
+      if (g < nvec) {
+        p4[u] = ld4nt(f.pk, g);
+        s4[u] = ld4nt(f.sk, g);
+        o4[u] = ld4nt(f.ok, g);
+        q4[u] = ld4nt(f.qty, g);
+        d4[u] = ld4nt(f.disc, g);
+        pa[u] = ld2nt(f.price, 2 * g);
+        pb[u] = ld2nt(f.price, 2 * g + 1);
+      } else {
+        p4[u] = int4{-1, -1, -1, -1};
+        s4[u] = o4[u] = q4[u] = d4[u] = int4{0, 0, 0, 0};
+        pa[u].x = pa[u].y = pb[u].x = pb[u].y = 0;
+      }
     }
-    const uint32_t c = (uint32_t)__popc(m);
-    const uint32_t incl = wave_incl_scan(c);
-    const uint32_t total = __shfl(incl, 63, 64);
-    uint32_t pos = tail + incl - c;
-    while (m) {
-      const int j = __ffs((int)m) - 1;
-      m &= m - 1;
-      q[(pos++) & (RING - 1)] = g * 4 + (uint32_t)j;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int32_t ps[4] = {p4[u].x, p4[u].y, p4[u].z, p4[u].w};
+      const int32_t ss[4] = {s4[u].x, s4[u].y, s4[u].z, s4[u].w};
+      const int32_t os[4] = {o4[u].x, o4[u].y, o4[u].z, o4[u].w};
+      const int32_t qs[4] = {q4[u].x, q4[u].y, q4[u].z, q4[u].w};
+      const int32_t ds[4] = {d4[u].x, d4[u].y, d4[u].z, d4[u].w};
+      const int64_t pr[4] = {pa[u].x, pa[u].y, pb[u].x, pb[u].y};
+      bool pass[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) pass[j] = test(ps[j]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        push(pass[j], ps[j], ss[j], os[j], qs[j], pr[j] * (int64_t)(100 - ds[j]));
+        if (tail - head >= 64) drain(false);
+      }
     }
-    tail += total;
-    wave_lds_sync();
-    drain(false);
-    wave_lds_sync();
   }
   if (blockIdx.x == 0 && w == 0) {  // wave-uniform: the last n % 4 rows
     const uint32_t r = nvec * 4 + lane;
-    bool pass = false;
-    if (lane < (n & 3)) {
-      const uint32_t p = (uint32_t)f.pk[r];
-      pass = p <= maxpk && ((s.pbm[p >> 5] >> (p & 31)) & 1u);
-    }
-    const uint64_t b = __ballot(pass);
-    if (pass) q[(tail + (uint32_t)__popcll(b & ((1ull << lane) - 1))) & (RING - 1)] = r;
-    tail += (uint32_t)__popcll(b);
+    const bool in = lane < (n & 3);
+    const int32_t pk = in ? f.pk[r] : -1;
+    const bool pass = in && test(pk);
+    if (tail - head > QCAP - 64) drain(false);
+    push(pass, pk, pass ? f.sk[r] : 0, pass ? f.ok[r] : 0, pass ? f.qty[r] : 0,
+         pass ? f.price[r] * (int64_t)(100 - f.disc[r]) : 0);
   }
-  wave_lds_sync();
   drain(true);
   __syncthreads();
-  for (int i = threadIdx.x; i < NNAT * NYEAR; i += blockDim.x) {
+  for (int i = threadIdx.x; i < dm.nslots; i += blockDim.x) {
     if (lhit[i]) {
-      atomicAdd((unsigned long long*)&s.agg[i], (unsigned long long)lagg[i]);
-      s.agg_hit[i] = 1u;
+      const int gslot = (i / dm.ny) * NYEAR + dm.first + i % dm.ny;
+      atomicAdd((unsigned long long*)&s.agg[gslot], (unsigned long long)ltab[i]);
+      s.agg_hit[gslot] = 1u;
     }
   }
+}
+
+// Days from 1970-01-01 to y-01-01 in the proleptic Gregorian calendar.
+int64_t days_to_jan1(int64_t y) {
+  const int64_t yp = y - 1;  // March-based year of 1 January
+  const int64_t era = (yp >= 0 ? yp : yp - 399) / 400;
+  const int64_t yoe = yp - era * 400;
+  const int64_t doe = yoe * 365 + yoe / 4 - yoe / 100 + 306;  // 306 = Mar 1 .. Jan 1
+  return era * 146097 + doe - 719468;
+}
+constexpr int64_t DAY0_1992 = 8035;  // 1992-01-01 - 1970-01-01
+
+// Year of a day number (day 0 = 1992-01-01).
+int64_t year_of(int64_t day) {
+  int64_t y = 1992 + day / 366;
+  while (days_to_jan1(y + 1) - DAY0_1992 <= day) ++y;
+  while (days_to_jan1(y) - DAY0_1992 > day) --y;
+  return y;
 }
 
 }  // namespace
@@ -352,14 +470,8 @@ extern "C" int gdb_q9(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supp
                       gdb_timing* timing) {
   if (!ctx || !part || !supplier || !partsupp || !orders || !lineitem || !prm || !out)
     return fail(GDB_E_INVALID, "null argument");
-  Pattern pat;
-  {
-    const size_t n = strnlen(prm->pattern, 32);
-    if (n == 0 || n > 31) return fail(GDB_E_INVALID, "Q9 pattern must be 1..31 bytes");
-    memset(&pat, 0, sizeof(pat));
-    memcpy(pat.b, prm->pattern, n);
-    pat.len = (int)n;
-  }
+  PatternW pat;
+  if (!make_pattern(prm->pattern, &pat)) return fail(GDB_E_INVALID, "Q9 pattern must be 1..31 bytes");
   const Column *ppk, *pnm, *ssk, *snk, *pspk, *pssk, *psc, *ook, *ood, *lpk, *lsk, *lok, *lq,
       *ld, *lp;
   GDB_TRY(column(part, "p_partkey", GDB_I32, &ppk));
@@ -387,11 +499,20 @@ extern "C" int gdb_q9(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supp
     return fail(GDB_E_UNSUPPORTED, "Q9 needs non-negative partkey, suppkey and orderkey values");
   if (supplier->nrows && (snk->minv < 0 || snk->maxv >= NNAT))
     return fail(GDB_E_UNSUPPORTED, "Q9 nation keys must be in 0..63");
+  // The order dates' years, 1970..2033, index the LDS table; their Jan-1 day
+  // numbers let the orders kernel find a date's year with compares only.
+  Years years{};
+  years.ny = 1;
+  years.first = 0;
   if (orders->nrows) {
-    const int64_t d0 = ood->minv, d1 = ood->maxv;
-    // days 1970-01-01 .. 2033-12-31 relative to 1992-01-01
-    if (d0 < -8035 || d1 > 15340) return fail(GDB_E_UNSUPPORTED, "Q9 order years must be in 1970..2033");
+    const int64_t y0 = year_of(ood->minv), y1 = year_of(ood->maxv);
+    if (y0 < YEAR0 || y1 >= YEAR0 + NYEAR)
+      return fail(GDB_E_UNSUPPORTED, "Q9 order years must be in 1970..2033");
+    years.ny = (int32_t)(y1 - y0 + 1);
+    years.first = (int32_t)(y0 - YEAR0);
+    for (int i = 0; i < years.ny; ++i) years.jan1[i] = (int32_t)(days_to_jan1(y0 + i) - DAY0_1992);
   }
+  const int nnat = supplier->nrows ? (int)snk->maxv + 1 : 1;
   GDB_HIP(hipSetDevice(ctx->device));
   const uint32_t np = (uint32_t)part->nrows, ns = (uint32_t)supplier->nrows,
                  nps = (uint32_t)partsupp->nrows, no = (uint32_t)orders->nrows;
@@ -400,6 +521,16 @@ extern "C" int gdb_q9(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supp
   const uint32_t maxsk = (uint32_t)std::max<int64_t>(0, ssk->maxv);
   const uint32_t maxok = (uint32_t)std::max<int64_t>(0, ook->maxv);
   const uint32_t npw = maxpk / 32 + 1, nsupp = maxsk + 1, nord = maxok + 1;
+  // LDS part filter: the bitmap itself when it fits 2^20 bits, else folded;
+  // halved until it fits next to the (nation, year) table and the queues.
+  uint32_t fwords = 32;
+  while (fwords < npw && fwords < (1u << 15)) fwords <<= 1;
+  {
+    const size_t lds_max = (size_t)std::max(ctx->info.lds_bytes_per_cu, 65536) -
+                           sizeof(Q9Queue) * (PROBE_THREADS / 64) - 1024;
+    while (fwords > 32 && (size_t)fwords * 4 + (size_t)nnat * years.ny * 12 > lds_max) fwords >>= 1;
+  }
+  const bool exact = npw <= fwords;
   uint64_t capmax = 1024;
   while (capmax < 2ull * nps) capmax <<= 1;
   Q9Scratch s{};
@@ -411,13 +542,15 @@ extern "C" int gdb_q9(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supp
       off += (bytes + 255) & ~size_t(255);
       return o;
     };
-    const size_t o_pbm = take(npw * 4ull), o_cnt = take(16), o_snat = take(nsupp * 4ull),
+    const size_t o_pbm = take(npw * 4ull), o_fold = take(fwords * 4ull), o_cnt = take(16),
+                 o_snat = take(nsupp * 4ull),
                  o_oy = take((nord + 3) / 4 * 4ull), o_agg = take(NNAT * NYEAR * 8),
                  o_hit = take(NNAT * NYEAR * 4), o_lpk = take(std::max<uint32_t>(nps, 1) * 4ull),
                  o_lsk = take(std::max<uint32_t>(nps, 1) * 4ull),
                  o_lc = take(std::max<uint32_t>(nps, 1) * 8ull);
     GDB_TRY(arena(ctx, A_MAP, off, (void**)&b));
     s.pbm = (uint32_t*)(b + o_pbm);
+    s.pfold = (uint32_t*)(b + o_fold);
     s.counters = (uint32_t*)(b + o_cnt);
     s.snat = (int32_t*)(b + o_snat);
     s.oyear = (uint8_t*)(b + o_oy);
@@ -426,10 +559,7 @@ extern "C" int gdb_q9(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supp
     s.lpk = (int32_t*)(b + o_lpk);
     s.lsk = (int32_t*)(b + o_lsk);
     s.lcost = (int64_t*)(b + o_lc);
-    char* h;
-    GDB_TRY(arena(ctx, A_HT, capmax * 16 + 256, (void**)&h));
-    s.ht_key = (int64_t*)h;
-    s.ht_cost = s.ht_key + capmax;
+    GDB_TRY(arena(ctx, A_HT, capmax * 16 + 256, (void**)&s.ht));
   }
   char* host;
   GDB_TRY(pinned(ctx, NNAT * NYEAR * 12 + 64, (void**)&host));
@@ -439,32 +569,39 @@ extern "C" int gdb_q9(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supp
   GDB_HIP(hipEventRecord(ctx->ev[0], st));
   q9_clear<<<std::min<uint32_t>(8192, (clear_items + B - 1) / B), B, 0, st>>>(s, npw, nsupp, nord);
   if (np) {
-    int pb = std::max(1, std::min<int>(resident_blocks(ctx, (const void*)q9_part, TILE, 0, 0),
-                                       (int)((np + TILE - 1) / TILE)));
+    const int pb = std::max(1, std::min<int>(resident_blocks(ctx, (const void*)q9_part, TILE, 0, 0),
+                                             (int)((np + TILE - 1) / TILE)));
     q9_part<<<pb, TILE, 0, st>>>((const uint8_t*)pnm->dev, (const int32_t*)ppk->dev, np, pat,
                                  maxpk, s.pbm);
   }
+  q9_fold<<<(fwords + B - 1) / B, B, 0, st>>>(s.pbm, npw, fwords, s.pfold);
   if (ns)
     q9_supplier<<<(ns + B - 1) / B, B, 0, st>>>((const int32_t*)ssk->dev, (const int32_t*)snk->dev,
                                                  ns, maxsk, s.snat);
   if (no)
     q9_orders<<<(no + B - 1) / B, B, 0, st>>>((const int32_t*)ook->dev, (const int32_t*)ood->dev, no,
-                                               maxok, s.oyear);
+                                               maxok, years, s.oyear);
   if (nps)
     q9_partsupp<<<std::min<uint32_t>(ctx->info.cu_count * 8, (nps + 16 * B - 1) / (16 * B)), B, 0, st>>>(
         (const int32_t*)pspk->dev, (const int32_t*)pssk->dev, (const int64_t*)psc->dev, nps, maxpk,
         s.pbm, s.counters, s.lpk, s.lsk, s.lcost);
   const int fill_blocks = ctx->info.cu_count * 4;
-  q9_ps_clear<<<fill_blocks, B, 0, st>>>(s.counters, s.ht_key);
-  q9_ps_insert<<<fill_blocks, B, 0, st>>>(s.counters, s.lpk, s.lsk, s.lcost, s.ht_key, s.ht_cost);
+  q9_ps_clear<<<fill_blocks, B, 0, st>>>(s.counters, s.ht);
+  q9_ps_insert<<<fill_blocks, B, 0, st>>>(s.counters, s.lpk, s.lsk, s.lcost, s.ht);
   GDB_HIP(hipGetLastError());
+  const Q9Dims dm{maxpk, maxsk, maxok, years.ny, years.first, nnat * years.ny, fwords,
+                  exact ? 1 : 0};
+  const size_t tab_lds = (size_t)fwords * 4 + (size_t)dm.nslots * 12;
+  GDB_HIP(hipFuncSetAttribute((const void*)q9_probe<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)tab_lds));
+  const int PB = PROBE_THREADS;
   const int president =
-      resident_blocks(ctx, (const void*)q9_probe, B, 0, ctx->tune[T_Q9].blocks_per_cu);
+      resident_blocks(ctx, (const void*)q9_probe<2>, PB, tab_lds, ctx->tune[T_Q9].blocks_per_cu);
   auto consume = [&](const void* const* d, uint32_t rows) -> int {
     const Q9Fact f{(const int32_t*)d[0], (const int32_t*)d[1], (const int32_t*)d[2],
                    (const int32_t*)d[3], (const int32_t*)d[4], (const int64_t*)d[5]};
-    const int pblocks = std::max(1, std::min<int>(president, (int)((rows / 4 + B - 1) / B)));
-    q9_probe<<<pblocks, B, 0, st>>>(f, rows, maxpk, maxsk, maxok, s);
+    const int pblocks = std::max(1, std::min<int>(president, (int)((rows / 4 + PB - 1) / PB)));
+    q9_probe<2><<<pblocks, PB, tab_lds, st>>>(f, rows, dm, s);
     GDB_HIP(hipGetLastError());
     return GDB_OK;
   };

@@ -9,6 +9,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 
 namespace gdbk {
 
@@ -88,6 +89,24 @@ struct PatternW {
   uint32_t c0, c1;    // first / second byte broadcast to 4 lanes
   uint32_t m0;        // valid-byte mask of the first dword (~0 when len >= 4)
 };
+
+// Host: packs a 1..31-byte word for the register-resident matchers.
+inline bool make_pattern(const char* s, PatternW* p) {
+  const size_t n = strnlen(s, 32);
+  if (n == 0 || n > 31) return false;
+  memset(p, 0, sizeof(*p));
+  uint8_t b[32] = {0};
+  memcpy(b, s, n);
+  memcpy(p->w, b, 32);
+  p->len = (int)n;
+  p->nw = (int)(n + 3) / 4;
+  const int rem = (int)n - 4 * (p->nw - 1);
+  p->last_mask = rem == 4 ? 0xFFFFFFFFu : ((1u << (8 * rem)) - 1u);
+  p->c0 = 0x01010101u * b[0];
+  p->c1 = 0x01010101u * (n > 1 ? b[1] : b[0]);
+  p->m0 = n >= 4 ? 0xFFFFFFFFu : p->last_mask;
+  return true;
+}
 
 // Value of d[idx] for a run-time idx (select chain over compile-time indices).
 template <int N>
