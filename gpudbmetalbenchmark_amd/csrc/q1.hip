@@ -4,11 +4,13 @@
 // Layout: one pass over 7 columns (26 B/row) with dwordx4 loads; each thread
 // owns groups of 4 rows. Groups are (returnflag, linestatus) -> bin 0..5 (the
 // reference's bins, kernels/q1.metal:27-32); rows that fail the filter or carry
-// another code go to a dump bin 6.
+// another code are skipped (their lanes sit out the LDS adds).
 //
-// Accumulation is lane-private in LDS: every lane has its own 7x4 table of u64
+// Accumulation is lane-private in LDS: every lane has its own 6x4 table of u64
 // counters, updated with ds_add_u64 at an address picked by the bin, so a row
-// costs 4 LDS adds and no per-bin compare/select chain. The 4 counters per bin:
+// costs 4 LDS adds and no per-bin compare/select chain. 12 KiB per wave lets
+// three 256-thread workgroups share a CU (a seventh, dump bin cost a third of
+// that occupancy and 12% of the scan time). The 4 counters per bin:
 //   0: qty | disc << 21 | 1 << 42   (count, SUM(disc), SUM(qty) packed)
 //   1: price                        (SUM(extprice), cents)
 //   2: price * (100 - disc)         (SUM disc_price, cents*hundredths)
@@ -31,7 +33,7 @@ using namespace gdbk;
 
 namespace {
 
-constexpr int NBIN = 7;        // 6 groups + dump bin
+constexpr int NBIN = 6;        // (returnflag, linestatus) groups
 constexpr int NVAL = 4;
 constexpr int K = 6 * 6 + 1;   // partial columns: 6 bins x 6 values + domain-violation count
 constexpr uint32_t MAX_ROWS_PER_LANE = 1u << 14;
@@ -53,13 +55,14 @@ __device__ __forceinline__ void q1_row(const Q1Row& r, int32_t cutoff,
                                        uint32_t& bad) {
   const uint32_t rc = rf_code(r.rf), lc = ls_code(r.ls);
   const bool ok = (r.ship <= cutoff) & (rc < 3u) & (lc < 2u);
-  const uint32_t bin = ok ? rc * 2u + lc : 6u;
+  const uint32_t bin = rc * 2u + lc;
   const uint32_t q = (uint32_t)r.qty, d = (uint32_t)r.disc, t = (uint32_t)r.tax;
   const uint64_t pr = (uint64_t)r.price;
   bad += ((q > 127u) | (d > 100u) | (t > 100u) | ((pr >> 32) != 0)) ? 1u : 0u;
   const uint64_t packed = (uint64_t)(q | (d << 21)) | (1ull << 42);
   const uint64_t dp = (uint64_t)(uint32_t)pr * (uint64_t)(100u - d);
   const uint64_t dpt = dp * (uint64_t)t;
+  if (!ok) return;
   unsigned long long* b = lane_tab + bin * (NVAL * WAVE);
   __hip_atomic_fetch_add(b + 0 * WAVE, (unsigned long long)packed, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -88,7 +91,7 @@ __device__ __forceinline__ void q1_group(const Q1Cols& c, uint32_t g, int32_t cu
   q1_row({s.w, q.w, d.w, t.w, b.y, rf >> 24, ls >> 24}, cutoff, lane_tab, bad);
 }
 
-// LDS: blockDim/64 waves x NBIN x NVAL x 64 lanes x 8 B (14 KiB per wave).
+// LDS: blockDim/64 waves x NBIN x NVAL x 64 lanes x 8 B (12 KiB per wave).
 template <int U>
 __global__ __launch_bounds__(256) void q1_scan(Q1Cols c, uint32_t n, int32_t cutoff,
                                                int64_t* __restrict__ partials) {

@@ -134,9 +134,10 @@ __global__ __launch_bounds__(256) void q3_orders(const int32_t* __restrict__ oke
         }
       }
     }
+    // Customer bits only for orders that pass the date test.
     uint32_t words[R];
 #pragma unroll
-    for (int j = 0; j < R; ++j) words[j] = s.cbm[min((uint32_t)cs[j], maxck) >> 5];
+    for (int j = 0; j < R; ++j) words[j] = ds[j] < date ? s.cbm[min((uint32_t)cs[j], maxck) >> 5] : 0u;
     uint32_t mask = 0;
 #pragma unroll
     for (int j = 0; j < R; ++j) {
