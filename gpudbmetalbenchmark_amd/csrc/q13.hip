@@ -104,83 +104,157 @@ __global__ __launch_bounds__(TILE) void q13_count(const uint8_t* __restrict__ co
   }
 }
 
-// Pass A: block b owns rows [b*tpb*TILE, (b+1)*tpb*TILE); the custkeys of its
-// counted orders go to region b (keys + b*tpb*TILE) in row order, with a
-// per-bucket histogram written k-major to hist[k*G + b]. A wave's 64 comments
-// (5120 contiguous bytes) are read with coalesced dwordx4 loads into the
-// wave's own LDS area and each lane then reads its 80-byte row from there:
-// lane-strided 80-byte loads straight from HBM touch ~40 cache lines per
-// instruction and ran at a third of the coalesced rate.
+// Necessary condition for LIKE '%a%b%': both words' first four bytes (fewer,
+// masked, for a shorter word) occur somewhere in the field. One alignbyte per
+// byte offset is shared by the two words and each costs one compare, about
+// half the work of the exact matchers; only comments that pass (a few percent)
+// are checked exactly.
+template <bool MASKED>
+struct HeadPair {
+  uint32_t a, b, am, bm;
+  __device__ __forceinline__ bool operator()(const uint32_t* d) const {
+    bool ha = false, hb = false;
+#pragma unroll
+    for (int i = 0; i < CW / 4; ++i) {
+#pragma unroll
+      for (int o = 0; o < 4; ++o) {
+        const uint32_t x = o == 0 ? d[i] : __builtin_amdgcn_alignbyte(d[i + 1], d[i], o);
+        ha = ha | ((MASKED ? (x & am) : x) == a);
+        hb = hb | ((MASKED ? (x & bm) : x) == b);
+      }
+    }
+    return ha & hb;
+  }
+};
+
+// Pass A. Block b owns rows [b*tpb*TILE, (b+1)*tpb*TILE); wave w of block b
+// writes the custkeys of its counted orders to region 4b+w (keys +
+// (4b+w)*tpb*WROWS), with a per-block histogram over custkey buckets written
+// k-major to hist[k*G + b]. A wave's 64 comments (5120 contiguous bytes) are
+// read with coalesced dwordx4 loads into the wave's own LDS area and each lane
+// then reads its 80-byte row from there: lane-strided 80-byte loads straight
+// from HBM touch ~40 cache lines per instruction and ran at a third of the
+// coalesced rate. Every lane runs the cheap head test; rows that pass it are
+// queued per wave and, 64 at a time, reloaded (a few percent of the rows) and
+// run through the exact matcher. No workgroup barrier sits in the loop.
 constexpr int WROWS = 64;                  // rows per wave and tile
 constexpr int WCHUNKS = WROWS * CW / 16;   // 16-byte chunks per wave and tile (320)
+constexpr int MQCAP = 128;                 // per-wave queue of rows for the exact matcher
 
-template <class M>
+template <class M, class H>
 __global__ __launch_bounds__(TILE) void q13_match(const uint8_t* __restrict__ comment,
                                                   const int32_t* __restrict__ custkey, uint32_t n,
-                                                  M match, uint32_t maxck,
+                                                  M match, H heads, uint32_t maxck,
                                                   uint32_t tpb, uint32_t nb,
                                                   uint32_t* __restrict__ keys,
                                                   uint32_t* __restrict__ region_count,
                                                   uint32_t* __restrict__ hist) {
   extern __shared__ uint32_t lhist[];  // nb buckets (dynamic: small for small customer tables)
-  __shared__ uint32_t scratch[17];
   __shared__ int4 stage[TILE / WROWS][WCHUNKS];
+  __shared__ uint32_t mq[TILE / WROWS][MQCAP];
   for (uint32_t i = threadIdx.x; i < nb; i += TILE) lhist[i] = 0u;
   __syncthreads();
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t below = (1ull << lane) - 1;
   int4* ws = stage[wave];
+  uint32_t* q = mq[wave];
   const int32_t* base = reinterpret_cast<const int32_t*>(comment);
   const uint64_t nchunks = (uint64_t)n * (CW / 16);  // 16-byte chunks in the column
   const uint32_t ntiles = (n + TILE - 1) / TILE;
   const uint32_t t0 = blockIdx.x * tpb, t1 = min(ntiles, t0 + tpb);
-  uint32_t* out = keys + (size_t)blockIdx.x * tpb * TILE;
-  uint32_t count = 0;  // block-uniform
-  for (uint32_t t = t0; t < t1; ++t) {
-    const uint32_t row0 = t * TILE + wave * WROWS;  // first row of this wave
-    const uint64_t c0 = (uint64_t)row0 * (CW / 16);
-    const int32_t* wbase = base + c0 * 4;
-    int4 v[WCHUNKS / 64];
-#pragma unroll
-    for (int q = 0; q < WCHUNKS / 64; ++q) {
-      const uint32_t c = q * 64 + lane;
-      v[q] = c0 + c < nchunks ? ld4nt(wbase, c) : int4{0, 0, 0, 0};
-    }
-#pragma unroll
-    for (int q = 0; q < WCHUNKS / 64; ++q) ws[q * 64 + lane] = v[q];
-    wave_lds_sync();
-    const uint32_t r = row0 + lane;
-    uint32_t d[28];
-#pragma unroll
-    for (int q = 0; q < CW / 16; ++q) {
-      const int4 x = ws[lane * (CW / 16) + q];
-      d[4 * q] = (uint32_t)x.x;
-      d[4 * q + 1] = (uint32_t)x.y;
-      d[4 * q + 2] = (uint32_t)x.z;
-      d[4 * q + 3] = (uint32_t)x.w;
-    }
-#pragma unroll
-    for (int q = 20; q < 28; ++q) d[q] = 0u;
-    wave_lds_sync();  // the area is rewritten by the next tile
-    bool q = false;
-    uint32_t ck = 0;
-    if (r < n && !match(d)) {
-      ck = (uint32_t)custkey[r];
-      q = ck <= maxck;
-    }
-    uint32_t total;
-    const uint32_t pos = block_excl_scan(q ? 1u : 0u, scratch, &total);
-    if (q) {
-      out[count + pos] = ck;
+  uint32_t* out = keys + ((size_t)blockIdx.x * (TILE / WROWS) + wave) * tpb * WROWS;
+  uint32_t count = 0;          // wave-uniform
+  uint32_t head = 0, tail = 0;  // wave-uniform queue cursors
+  // Appends the custkey of a counted row (ballot over the wave).
+  auto emit = [&](bool counted, uint32_t ck) {
+    counted = counted && ck <= maxck;
+    const uint64_t m = __ballot(counted);
+    if (counted) {
+      out[count + (uint32_t)__popcll(m & below)] = ck;
       atomicAdd(&lhist[ck >> BUCKET_SHIFT], 1u);
     }
-    count += total;
+    count += (uint32_t)__popcll(m);
+  };
+  auto drain = [&](bool all) {
+    wave_lds_sync();
+    while (tail - head >= 64 || (all && tail != head)) {
+      const uint32_t avail = min(64u, tail - head);
+      bool counted = false;
+      uint32_t r = 0;
+      if (lane < avail) {
+        r = q[(head + lane) & (MQCAP - 1)];
+        const int4* src = reinterpret_cast<const int4*>(comment) + (size_t)r * (CW / 16);
+        uint32_t d[28];
+#pragma unroll
+        for (int c = 0; c < CW / 16; ++c) {
+          const int4 x = src[c];
+          d[4 * c] = (uint32_t)x.x;
+          d[4 * c + 1] = (uint32_t)x.y;
+          d[4 * c + 2] = (uint32_t)x.z;
+          d[4 * c + 3] = (uint32_t)x.w;
+        }
+#pragma unroll
+        for (int c = 20; c < 28; ++c) d[c] = 0u;
+        counted = !match(d);
+      }
+      emit(counted, counted ? (uint32_t)custkey[r] : 0u);
+      head += avail;
+    }
+    wave_lds_sync();
+  };
+  // The next tile's comments and custkeys are loaded while the current one
+  // is tested.
+  int4 v[WCHUNKS / 64];
+  uint32_t ckn = 0;
+  auto load_tile = [&](uint32_t t) {
+    const uint32_t row0 = t * TILE + wave * WROWS;
+    const uint64_t c0 = (uint64_t)row0 * (CW / 16);
+    const int32_t* wbase = base + c0 * 4;
+#pragma unroll
+    for (int c = 0; c < WCHUNKS / 64; ++c) {
+      const uint32_t ci = c * 64 + lane;
+      v[c] = c0 + ci < nchunks ? ld4nt(wbase, ci) : int4{0, 0, 0, 0};
+    }
+    ckn = row0 + lane < n ? (uint32_t)custkey[row0 + lane] : 0u;
+  };
+  if (t0 < t1) load_tile(t0);
+  for (uint32_t t = t0; t < t1; ++t) {
+    const uint32_t row0 = t * TILE + wave * WROWS;  // first row of this wave
+#pragma unroll
+    for (int c = 0; c < WCHUNKS / 64; ++c) ws[c * 64 + lane] = v[c];
+    wave_lds_sync();
+    const uint32_t r = row0 + lane;
+    uint32_t d[24];
+#pragma unroll
+    for (int c = 0; c < CW / 16; ++c) {
+      const int4 x = ws[lane * (CW / 16) + c];
+      d[4 * c] = (uint32_t)x.x;
+      d[4 * c + 1] = (uint32_t)x.y;
+      d[4 * c + 2] = (uint32_t)x.z;
+      d[4 * c + 3] = (uint32_t)x.w;
+    }
+#pragma unroll
+    for (int c = 20; c < 24; ++c) d[c] = 0u;
+    wave_lds_sync();  // the area is rewritten by the next tile
+    const uint32_t ck = ckn;
+    if (t + 1 < t1) load_tile(t + 1);
+    const bool in = r < n;
+    const bool cand = in && heads(d);
+    // rows without both heads cannot match: they are counted now
+    emit(in && !cand, ck);
+    const uint64_t m = __ballot(cand);
+    if (cand) q[(tail + (uint32_t)__popcll(m & below)) & (MQCAP - 1)] = r;
+    tail += (uint32_t)__popcll(m);
+    if (tail - head >= 64) drain(false);
   }
+  drain(true);
+  if (lane == 0) region_count[blockIdx.x * (TILE / WROWS) + wave] = count;
   __syncthreads();
-  if (threadIdx.x == 0) region_count[blockIdx.x] = count;
   for (uint32_t k = threadIdx.x; k < nb; k += TILE) hist[(size_t)k * gridDim.x + blockIdx.x] = lhist[k];
 }
 
-// Pass C: region b's custkeys to their bucket segments (offs = scanned hist).
+// Pass C: block b's four wave regions of custkeys to their bucket segments
+// (offs = scanned hist).
 __global__ __launch_bounds__(TILE) void q13_scatter(const uint32_t* __restrict__ keys, uint32_t tpb,
                                                     const uint32_t* __restrict__ region_count,
                                                     const uint32_t* __restrict__ offs, uint32_t nb,
@@ -188,11 +262,14 @@ __global__ __launch_bounds__(TILE) void q13_scatter(const uint32_t* __restrict__
   __shared__ uint32_t base[MAX_BUCKETS];
   for (uint32_t k = threadIdx.x; k < nb; k += TILE) base[k] = offs[(size_t)k * gridDim.x + blockIdx.x];
   __syncthreads();
-  const uint32_t* in = keys + (size_t)blockIdx.x * tpb * TILE;
-  const uint32_t cnt = region_count[blockIdx.x];
-  for (uint32_t i = threadIdx.x; i < cnt; i += TILE) {
-    const uint32_t ck = in[i];
-    sorted[atomicAdd(&base[ck >> BUCKET_SHIFT], 1u)] = ck;
+  for (uint32_t w = 0; w < TILE / WROWS; ++w) {
+    const uint32_t region = blockIdx.x * (TILE / WROWS) + w;
+    const uint32_t* in = keys + (size_t)region * tpb * WROWS;
+    const uint32_t cnt = region_count[region];
+    for (uint32_t i = threadIdx.x; i < cnt; i += TILE) {
+      const uint32_t ck = in[i];
+      sorted[atomicAdd(&base[ck >> BUCKET_SHIFT], 1u)] = ck;
+    }
   }
 }
 
@@ -302,8 +379,8 @@ extern "C" int gdb_q13(gdb_ctx* ctx, const gdb_table* customer, const gdb_table*
     const uint32_t n = (uint32_t)no;
     const uint32_t ntiles = (n + TILE - 1) / TILE;
     const size_t hist_lds = (size_t)nb * 4;
-    uint32_t G = (uint32_t)std::max(1, resident_blocks(ctx, (const void*)q13_match<FixedMatch<7, 8>>, TILE, hist_lds,
-                                                       ctx->tune[T_Q13].blocks_per_cu));
+    uint32_t G = (uint32_t)std::max(1, resident_blocks(ctx, (const void*)q13_match<FixedMatch<7, 8>, HeadPair<false>>,
+                                                       TILE, hist_lds, ctx->tune[T_Q13].blocks_per_cu));
     G = std::max<uint32_t>(1, std::min<uint32_t>(G, ntiles));
     const uint32_t tpb = std::max<uint32_t>(1, (ntiles + G - 1) / G);
     G = std::max<uint32_t>(1, (ntiles + tpb - 1) / tpb);
@@ -313,17 +390,24 @@ extern "C" int gdb_q13(gdb_ctx* ctx, const gdb_table* customer, const gdb_table*
     GDB_TRY(arena(ctx, A_AUX2, (size_t)G * tpb * TILE * 4 + 256, (void**)&keys));
     GDB_TRY(arena(ctx, A_HT, (size_t)std::max<uint32_t>(n, 1) * 4 + 256, (void**)&sorted));
     const uint32_t ntsum = scan_tile_sums_needed(nhist);
-    GDB_TRY(arena(ctx, A_PARTIALS, ((size_t)G + nhist + ntsum + 64) * 4, (void**)&b));
+    GDB_TRY(arena(ctx, A_PARTIALS, ((size_t)4 * G + nhist + ntsum + 64) * 4, (void**)&b));
     rcount = (uint32_t*)b;
-    bh = rcount + G;
+    bh = rcount + 4 * G;
     tsum = bh + nhist;
     total = tsum + ntsum;
     q13_clear<<<5, 256, 0, st>>>(counts, 0, hist);
     GDB_HIP(hipEventRecord(ctx->ev[1], st));
+    const bool heads_full = w1.len >= 4 && w2.len >= 4;
+    const uint32_t ha = pattern_dword(w1, 0) & w1.m0, hb = pattern_dword(w2, 0) & w2.m0;
     auto launch = [&](auto match) {
-      q13_match<decltype(match)><<<G, TILE, hist_lds, st>>>((const uint8_t*)ocom->dev,
-                                                     (const int32_t*)ock->dev, n, match, maxck,
-                                                     tpb, nb, keys, rcount, bh);
+      if (heads_full)
+        q13_match<decltype(match), HeadPair<false>><<<G, TILE, hist_lds, st>>>(
+            (const uint8_t*)ocom->dev, (const int32_t*)ock->dev, n, match,
+            HeadPair<false>{ha, hb, w1.m0, w2.m0}, maxck, tpb, nb, keys, rcount, bh);
+      else
+        q13_match<decltype(match), HeadPair<true>><<<G, TILE, hist_lds, st>>>(
+            (const uint8_t*)ocom->dev, (const int32_t*)ock->dev, n, match,
+            HeadPair<true>{ha, hb, w1.m0, w2.m0}, maxck, tpb, nb, keys, rcount, bh);
     };
     if (!dispatch_fixed<4, 4>(w1, w2, launch)) {
       if (full0) launch(GenericMatch<true>{w1, w2});
