@@ -50,90 +50,126 @@ __device__ __forceinline__ uint32_t mix32(int32_t key) {
 }
 
 // ---------------------------------------------------------------- partitioning
-// Block b owns rows [b*chunk, min(n, (b+1)*chunk)) (chunk a multiple of 4).
-__device__ __forceinline__ void chunk_of(uint32_t n, uint32_t chunk, uint32_t& lo, uint32_t& hi) {
-  lo = min(n, blockIdx.x * chunk);
-  hi = min(n, lo + chunk);
+// A pass moves keys into partitions by (mix32(key) >> shift) & (PL - 1), PL
+// <= 128 per pass. Mode 0: block b owns rows [b*chunk, (b+1)*chunk) of the
+// input and its counts sit at hist[l*G + b]. Mode 1 (second pass): the input
+// is the first pass's output; block b = fp*K + k owns the k-th of K slices of
+// first-level partition fp, and its counts sit at hist[(fp*PL + l)*K + k], so
+// the scanned array orders the final partitions fp*PL + l, each split by k.
+constexpr int MAX_PL = 128;
+constexpr int PT = 16;               // keys per thread and tile
+constexpr int PTILE = PT * JB;       // 4096 keys per tile
+
+struct PassArgs {
+  const int32_t* in;
+  int32_t* out;
+  uint32_t n;            // mode 0: input rows
+  uint32_t chunk;        // mode 0: rows per block
+  int shift;
+  uint32_t lmask;        // PL - 1
+  int mode;
+  const uint32_t* part_offs;   // mode 1: scanned first-pass counts
+  uint32_t part_G;             // mode 1: first-pass blocks
+  const uint32_t* part_total;  // mode 1: first-pass total
+  uint32_t K;                  // mode 1: slices per first-level partition
+};
+
+__device__ __forceinline__ uint32_t local_id(const PassArgs& a, int32_t k) {
+  return (mix32(k) >> a.shift) & a.lmask;
 }
 
-constexpr int PU = 4;  // dwordx4 groups in flight per lane in the partition passes
+// Row range of this block and where its count of local partition l lives:
+// hist[hbase + l * hstride].
+__device__ __forceinline__ void block_range(const PassArgs& a, uint32_t& lo, uint32_t& hi,
+                                            size_t& hbase, uint32_t& hstride) {
+  if (a.mode == 0) {
+    lo = min(a.n, blockIdx.x * a.chunk);
+    hi = min(a.n, lo + a.chunk);
+    hbase = blockIdx.x;
+    hstride = gridDim.x;
+  } else {
+    const uint32_t P1 = gridDim.x / a.K, fp = blockIdx.x / a.K, k = blockIdx.x % a.K;
+    const uint32_t s = a.part_offs[(size_t)fp * a.part_G];
+    const uint32_t e = fp + 1 < P1 ? a.part_offs[(size_t)(fp + 1) * a.part_G] : *a.part_total;
+    const uint32_t c = (e - s + a.K - 1) / a.K;
+    lo = min(e, s + k * c);
+    hi = min(e, lo + c);
+    hbase = (size_t)fp * (a.lmask + 1) * a.K + k;
+    hstride = a.K;
+  }
+}
 
-// Counts of the block's keys per partition, written k-major: hist[p*G + b].
-__global__ __launch_bounds__(JB) void rj_count(const int32_t* __restrict__ keys, uint32_t n,
-                                               int bits, uint32_t chunk,
-                                               uint32_t* __restrict__ hist) {
-  extern __shared__ uint32_t lh[];
-  const uint32_t P = 1u << bits;
-  for (uint32_t i = threadIdx.x; i < P; i += JB) lh[i] = 0u;
+__global__ __launch_bounds__(JB) void rj_count(PassArgs a, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t cnt[MAX_PL];
+  const uint32_t PL = a.lmask + 1;
+  for (uint32_t i = threadIdx.x; i < PL; i += JB) cnt[i] = 0u;
   __syncthreads();
-  uint32_t lo, hi;
-  chunk_of(n, chunk, lo, hi);
-  const uint32_t shift = 32 - bits;
-  const uint32_t g0 = lo / 4, g1 = hi / 4;
-  uint32_t g = g0 + threadIdx.x;
-  for (; g + (PU - 1) * JB < g1; g += PU * JB) {
-    int4 v[PU];
+  uint32_t lo, hi, hstride;
+  size_t hbase;
+  block_range(a, lo, hi, hbase, hstride);
+  uint32_t i = lo + threadIdx.x;
+  for (; i + 3 * JB < hi; i += 4 * JB) {
+    int32_t k[4];
 #pragma unroll
-    for (int u = 0; u < PU; ++u) v[u] = ld4nt(keys, g + u * JB);
+    for (int u = 0; u < 4; ++u) k[u] = __builtin_nontemporal_load(a.in + i + u * JB);
 #pragma unroll
-    for (int u = 0; u < PU; ++u) {
-      atomicAdd(&lh[mix32(v[u].x) >> shift], 1u);
-      atomicAdd(&lh[mix32(v[u].y) >> shift], 1u);
-      atomicAdd(&lh[mix32(v[u].z) >> shift], 1u);
-      atomicAdd(&lh[mix32(v[u].w) >> shift], 1u);
+    for (int u = 0; u < 4; ++u) atomicAdd(&cnt[local_id(a, k[u])], 1u);
+  }
+  for (; i < hi; i += JB) atomicAdd(&cnt[local_id(a, a.in[i])], 1u);
+  __syncthreads();
+  for (uint32_t l = threadIdx.x; l < PL; l += JB) hist[hbase + (size_t)l * hstride] = cnt[l];
+}
+
+// Moves the block's keys to their partitions (offs = scanned counts). Tiles of
+// 4096 keys are sorted by local partition in LDS first (ranks from LDS
+// atomics, a scan of the <= 128 counts), so each partition's keys of a tile
+// leave as one contiguous run: consecutive lanes store consecutive addresses.
+// Scattering key by key sent every lane of a store to a different partition
+// and ran at a third of the copy rate.
+__global__ __launch_bounds__(JB) void rj_scatter(PassArgs a, const uint32_t* __restrict__ offs) {
+  __shared__ uint32_t off[MAX_PL], cnt[MAX_PL], start[MAX_PL];
+  __shared__ int32_t stage[PTILE];
+  const uint32_t PL = a.lmask + 1;
+  uint32_t lo, hi, hstride;
+  size_t hbase;
+  block_range(a, lo, hi, hbase, hstride);
+  for (uint32_t l = threadIdx.x; l < PL; l += JB) off[l] = offs[hbase + (size_t)l * hstride];
+  for (uint32_t t0 = lo; t0 < hi; t0 += PTILE) {
+    const uint32_t m = min((uint32_t)PTILE, hi - t0);
+    int32_t key[PT];
+#pragma unroll
+    for (int j = 0; j < PT; ++j) {
+      const uint32_t i = threadIdx.x + j * JB;
+      key[j] = i < m ? __builtin_nontemporal_load(a.in + t0 + i) : 0;
     }
-  }
-  for (; g < g1; g += JB) {
-    const int4 v = ld4nt(keys, g);
-    atomicAdd(&lh[mix32(v.x) >> shift], 1u);
-    atomicAdd(&lh[mix32(v.y) >> shift], 1u);
-    atomicAdd(&lh[mix32(v.z) >> shift], 1u);
-    atomicAdd(&lh[mix32(v.w) >> shift], 1u);
-  }
-  for (uint32_t r = g1 * 4 + threadIdx.x; r < hi; r += JB) atomicAdd(&lh[mix32(keys[r]) >> shift], 1u);
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < P; i += JB) hist[(size_t)i * gridDim.x + blockIdx.x] = lh[i];
-}
-
-__device__ __forceinline__ void scatter_key(int32_t k, uint32_t shift, uint32_t* lo_off,
-                                            int32_t* __restrict__ out) {
-  out[atomicAdd(&lo_off[mix32(k) >> shift], 1u)] = k;
-}
-
-// Moves the block's keys to out[offs[p*G + b] ...] (offs = scanned counts).
-__global__ __launch_bounds__(JB) void rj_scatter(const int32_t* __restrict__ keys, uint32_t n,
-                                                 int bits, uint32_t chunk,
-                                                 const uint32_t* __restrict__ offs,
-                                                 int32_t* __restrict__ out) {
-  extern __shared__ uint32_t lo_off[];
-  const uint32_t P = 1u << bits;
-  for (uint32_t i = threadIdx.x; i < P; i += JB) lo_off[i] = offs[(size_t)i * gridDim.x + blockIdx.x];
-  __syncthreads();
-  uint32_t lo, hi;
-  chunk_of(n, chunk, lo, hi);
-  const uint32_t shift = 32 - bits;
-  const uint32_t g0 = lo / 4, g1 = hi / 4;
-  uint32_t g = g0 + threadIdx.x;
-  for (; g + (PU - 1) * JB < g1; g += PU * JB) {
-    int4 v[PU];
+    for (uint32_t l = threadIdx.x; l < PL; l += JB) cnt[l] = 0u;
+    __syncthreads();
+    uint32_t rank[PT];
 #pragma unroll
-    for (int u = 0; u < PU; ++u) v[u] = ld4nt(keys, g + u * JB);
-#pragma unroll
-    for (int u = 0; u < PU; ++u) {
-      scatter_key(v[u].x, shift, lo_off, out);
-      scatter_key(v[u].y, shift, lo_off, out);
-      scatter_key(v[u].z, shift, lo_off, out);
-      scatter_key(v[u].w, shift, lo_off, out);
+    for (int j = 0; j < PT; ++j)
+      rank[j] = threadIdx.x + j * JB < m ? atomicAdd(&cnt[local_id(a, key[j])], 1u) : 0u;
+    __syncthreads();
+    if (threadIdx.x < 64) {  // exclusive scan of the counts: two per lane
+      const uint32_t l0 = 2 * threadIdx.x, l1 = l0 + 1;
+      const uint32_t c0 = l0 < PL ? cnt[l0] : 0u, c1 = l1 < PL ? cnt[l1] : 0u;
+      const uint32_t incl = wave_incl_scan(c0 + c1);
+      if (l0 < PL) start[l0] = incl - c0 - c1;
+      if (l1 < PL) start[l1] = incl - c1;
     }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PT; ++j)
+      if (threadIdx.x + j * JB < m) stage[start[local_id(a, key[j])] + rank[j]] = key[j];
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < m; i += JB) {
+      const int32_t k = stage[i];
+      const uint32_t l = local_id(a, k);
+      a.out[off[l] + (i - start[l])] = k;
+    }
+    __syncthreads();
+    for (uint32_t l = threadIdx.x; l < PL; l += JB) off[l] += cnt[l];
+    __syncthreads();
   }
-  for (; g < g1; g += JB) {
-    const int4 v = ld4nt(keys, g);
-    scatter_key(v.x, shift, lo_off, out);
-    scatter_key(v.y, shift, lo_off, out);
-    scatter_key(v.z, shift, lo_off, out);
-    scatter_key(v.w, shift, lo_off, out);
-  }
-  for (uint32_t r = g1 * 4 + threadIdx.x; r < hi; r += JB) scatter_key(keys[r], shift, lo_off, out);
 }
 
 // ------------------------------------------------------------------ LDS table
@@ -341,32 +377,48 @@ int min_bits(uint32_t nb) {
   return b;
 }
 
-struct PartPass {
-  int bits;
-  uint32_t G, chunk;
-};
-
-PartPass pass_shape(const gdb_ctx* ctx, uint32_t n, int bits) {
-  uint32_t G = std::max<uint32_t>(1, std::min<uint32_t>(ctx->info.cu_count * 2, (n + 4095) / 4096));
-  uint32_t chunk = ((n + G - 1) / G + 3) & ~3u;
-  if (chunk == 0) chunk = 4;
-  G = std::max<uint32_t>(1, (n + chunk - 1) / chunk);
-  return PartPass{bits, G, chunk};
+// First pass (or the only one): contiguous chunks over the input.
+PassArgs first_pass(const gdb_ctx* ctx, const int32_t* in, int32_t* out, uint32_t n, int b1,
+                    uint32_t* G) {
+  uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>(ctx->info.cu_count * 2, (n + PTILE - 1) / PTILE));
+  uint32_t chunk = std::max<uint32_t>(1, (n + g - 1) / g);
+  g = std::max<uint32_t>(1, (n + chunk - 1) / chunk);
+  *G = g;
+  PassArgs a{};
+  a.in = in;
+  a.out = out;
+  a.n = n;
+  a.chunk = chunk;
+  a.shift = 32 - b1;
+  a.lmask = (1u << b1) - 1;
+  a.mode = 0;
+  return a;
 }
 
-// One partition pass over n keys: in -> out, offsets k-major in offs[P*G].
-int partition_pass(gdb_ctx* ctx, const int32_t* in, uint32_t n, const PartPass& ps, int32_t* out,
-                   uint32_t* offs, uint32_t* total, uint32_t* tsum, hipStream_t st) {
-  const uint32_t P = 1u << ps.bits;
-  const size_t lds = (size_t)P * 4;
-  if (lds > 32768) {  // 14 bits: 64 KiB of partition counters
-    GDB_HIP(hipFuncSetAttribute((const void*)rj_count, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    GDB_HIP(hipFuncSetAttribute((const void*)rj_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  }
-  rj_count<<<ps.G, JB, lds, st>>>(in, n, ps.bits, ps.chunk, offs);
+// Second pass over the first pass's output: final partition bits `bits`, of
+// which the first b1 were the first pass's.
+PassArgs second_pass(const int32_t* in, int32_t* out, int b1, int bits, const uint32_t* offs1,
+                     uint32_t G1, const uint32_t* total1, uint32_t K) {
+  PassArgs a{};
+  a.in = in;
+  a.out = out;
+  a.shift = 32 - bits;
+  a.lmask = (1u << (bits - b1)) - 1;
+  a.mode = 1;
+  a.part_offs = offs1;
+  a.part_G = G1;
+  a.part_total = total1;
+  a.K = K;
+  return a;
+}
+
+// count, exclusive scan of the counts (in place), scatter
+int run_pass(gdb_ctx* ctx, const PassArgs& a, uint32_t blocks, uint32_t nhist, uint32_t* hist,
+             uint32_t* total, uint32_t* tsum, hipStream_t st) {
+  rj_count<<<blocks, JB, 0, st>>>(a, hist);
   GDB_HIP(hipGetLastError());
-  GDB_TRY(exclusive_scan_u32(ctx, offs, P * ps.G, offs, total, tsum, st));
-  rj_scatter<<<ps.G, JB, lds, st>>>(in, n, ps.bits, ps.chunk, offs, out);
+  GDB_TRY(exclusive_scan_u32(ctx, hist, nhist, hist, total, tsum, st));
+  rj_scatter<<<blocks, JB, 0, st>>>(a, hist);
   GDB_HIP(hipGetLastError());
   return GDB_OK;
 }
@@ -441,10 +493,10 @@ extern "C" int gdb_join_count(gdb_ctx* ctx, const gdb_table* build, const char* 
   } else {
     const uint32_t P = 1u << bits;
     const int b1 = bits > 7 ? bits / 2 : bits;  // first pass (or the only one)
-    const PartPass r1 = pass_shape(ctx, nb, b1), r2 = pass_shape(ctx, nb, bits);
-    const PartPass s1 = pass_shape(ctx, np, b1), s2 = pass_shape(ctx, np, bits);
     const bool two = b1 != bits;
-    // keys: build pass outputs, probe pass outputs (each region 256-byte aligned)
+    const uint32_t P1 = 1u << b1;
+    // second-pass slices per first-level partition: about two workgroups per CU
+    const uint32_t K = std::max<uint32_t>(1, (uint32_t)ctx->info.cu_count * 2 / P1);
     auto padded = [](uint32_t n) { return ((size_t)n + 64 + 63) & ~size_t(63); };
     const size_t rn = padded(nb), sn = padded(np);
     int32_t* kbuf;
@@ -453,33 +505,39 @@ extern "C" int gdb_join_count(gdb_ctx* ctx, const gdb_table* build, const char* 
     int32_t* r_b = two ? r_a + rn : r_a;
     int32_t* s_a = r_b + rn;
     int32_t* s_b = two ? s_a + sn : s_a;
-    // offsets: final build and probe offsets kept for the join, first-pass
-    // offsets, scan tile sums, totals
-    const size_t o_r2 = (size_t)P * r2.G, o_s2 = (size_t)P * s2.G;
-    const size_t o_1 = std::max<size_t>((size_t)(1u << b1) * r1.G, (size_t)(1u << b1) * s1.G);
-    const size_t ts = std::max<size_t>(scan_tile_sums_needed((uint32_t)std::max(o_r2, o_s2)),
-                                       scan_tile_sums_needed((uint32_t)o_1));
+    uint32_t rG1 = 1, sG1 = 1;
+    PassArgs r1 = first_pass(ctx, rk, r_a, nb, b1, &rG1);
+    PassArgs s1 = first_pass(ctx, sk, s_a, np, b1, &sG1);
+    // final offsets per side: P x K (two passes) or P x G1 (one pass)
+    const uint32_t rG = two ? K : rG1, sG = two ? K : sG1;
+    const size_t o_r = (size_t)P * rG, o_s = (size_t)P * sG;
+    const size_t o_1 = two ? std::max<size_t>((size_t)P1 * rG1, (size_t)P1 * sG1) : 0;
+    const size_t ts = scan_tile_sums_needed((uint32_t)std::max({o_r, o_s, o_1}));
     uint32_t* obuf;
-    GDB_TRY(arena(ctx, A_JOINHIST, (o_r2 + o_s2 + o_1 + ts + 16) * 4, (void**)&obuf));
+    GDB_TRY(arena(ctx, A_JOINHIST, (o_r + o_s + o_1 + ts + 16) * 4, (void**)&obuf));
     uint32_t* roffs = obuf;
-    uint32_t* soffs = roffs + o_r2;
-    uint32_t* offs1 = soffs + o_s2;
+    uint32_t* soffs = roffs + o_r;
+    uint32_t* offs1 = soffs + o_s;
     uint32_t* tsum = offs1 + o_1;
     uint32_t* totals = tsum + ts;  // [0] build, [1] probe, [2] first-pass scratch
     nparts = P;
     GDB_TRY(arena(ctx, A_PARTIALS, sizeof(int64_t) * nparts, (void**)&partials));
     GDB_HIP(hipEventRecord(ctx->ev[0], st));
     if (two) {
-      GDB_TRY(partition_pass(ctx, rk, nb, r1, r_a, offs1, totals + 2, tsum, st));
-      GDB_TRY(partition_pass(ctx, r_a, nb, r2, r_b, roffs, totals + 0, tsum, st));
-      GDB_TRY(partition_pass(ctx, sk, np, s1, s_a, offs1, totals + 2, tsum, st));
-      GDB_TRY(partition_pass(ctx, s_a, np, s2, s_b, soffs, totals + 1, tsum, st));
+      GDB_TRY(run_pass(ctx, r1, rG1, (uint32_t)(P1 * rG1), offs1, totals + 2, tsum, st));
+      const PassArgs r2 = second_pass(r_a, r_b, b1, bits, offs1, rG1, totals + 2, K);
+      GDB_TRY(run_pass(ctx, r2, P1 * K, (uint32_t)o_r, roffs, totals + 0, tsum, st));
+      GDB_TRY(run_pass(ctx, s1, sG1, (uint32_t)(P1 * sG1), offs1, totals + 2, tsum, st));
+      const PassArgs s2 = second_pass(s_a, s_b, b1, bits, offs1, sG1, totals + 2, K);
+      GDB_TRY(run_pass(ctx, s2, P1 * K, (uint32_t)o_s, soffs, totals + 1, tsum, st));
     } else {
-      GDB_TRY(partition_pass(ctx, rk, nb, r2, r_b, roffs, totals + 0, tsum, st));
-      GDB_TRY(partition_pass(ctx, sk, np, s2, s_b, soffs, totals + 1, tsum, st));
+      r1.out = r_b;
+      s1.out = s_b;
+      GDB_TRY(run_pass(ctx, r1, rG1, (uint32_t)o_r, roffs, totals + 0, tsum, st));
+      GDB_TRY(run_pass(ctx, s1, sG1, (uint32_t)o_s, soffs, totals + 1, tsum, st));
     }
     GDB_HIP(hipEventRecord(ctx->ev[1], st));
-    rj_join<<<P, JT, 0, st>>>(r_b, roffs, r2.G, totals + 0, s_b, soffs, s2.G, totals + 1, P,
+    rj_join<<<P, JT, 0, st>>>(r_b, roffs, rG, totals + 0, s_b, soffs, sG, totals + 1, P,
                               partials);
     GDB_HIP(hipEventRecord(ctx->ev[2], st));
   }
