@@ -2,16 +2,21 @@
 // launch sequence from resident join indexes (no reference counterpart; the
 // reference rebuilds its maps for every run, src/q3.cpp:80-142).
 //
-// Indexes, built once per table set and kept in HBM:
-//   cseg   custkey -> segment code (0xFF: no such customer)
-//   byday  order rows sorted by orderdate (counting sort) + per-day offsets
-//   csr    for every order row, its fact rows' (shipdate, extprice*(100-disc))
-//          contiguous (CSR by order row, built through orderkey -> order row)
-//   maxlag the largest shipdate - orderdate over all fact rows
+// Indexes, built once per table set and kept in HBM, all in orderdate order
+// (a counting sort of the order rows by date):
+//   dayoff   per-day offsets into that order
+//   key/prio/date/seg
+//            each position's orderkey, shippriority, orderdate and its
+//            customer's segment code (0xFF: no such customer)
+//   csr      for every position, its order's fact rows' (shipdate,
+//            extprice*(100-disc)) contiguous (CSR by position, built through
+//            orderkey -> position)
+//   maxlag   the largest shipdate - orderdate over all fact rows
 // A request (s, D) can only gain revenue from orders dated in
-// [D - maxlag, D): any earlier order's rows all ship by D. Its groups are the
-// orders of that window in segment s with at least one row shipped after D;
-// revenue sums those rows. Per request the groups are ranked with the same
+// [D - maxlag, D): any earlier order's rows all ship by D. Its window is one
+// contiguous range of positions, read with coalesced loads; its groups are the
+// window's orders in segment s with at least one row shipped after D, whose
+// rows sum to the revenue. Per request the groups are ranked with the same
 // bucket-histogram threshold + LDS top-k as gdb_q3 (revenue desc, orderdate
 // asc, orderkey asc), so every result is identical to gdb_q3's.
 #include <hip/hip_runtime.h>
@@ -34,12 +39,13 @@ struct Q3Slot {
   const int32_t *okey = nullptr, *ocust = nullptr, *odate = nullptr, *oprio = nullptr;
   uint32_t no = 0, nl = 0;
   int32_t dmin = 0, dmax = -1, maxlag = 0;
-  // indexes
-  uint8_t* cseg = nullptr;        // [maxck+1]
-  uint32_t ncseg = 0;
-  uint32_t* byday = nullptr;      // [no] order rows sorted by date
+  // indexes, by position in orderdate order
   uint32_t* dayoff = nullptr;     // [ndays+1]
   std::vector<uint32_t> h_dayoff; // host copy
+  int32_t* skey = nullptr;        // [no]
+  int32_t* sprio = nullptr;       // [no]
+  int32_t* sdate = nullptr;       // [no]
+  uint8_t* sseg = nullptr;        // [no]
   uint32_t* csr_off = nullptr;    // [no+1]
   int32_t* csr_ship = nullptr;    // [nl]
   int64_t* csr_rev = nullptr;     // [nl]
@@ -85,12 +91,24 @@ __global__ void k_cseg(const int32_t* ck, const uint8_t* seg, uint32_t n, uint8_
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     if (ck[i] >= 0) cseg[ck[i]] = seg[i];
 }
-__global__ void k_omap(const int32_t* okey, uint32_t n, int32_t* omap) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    if (okey[i] >= 0) omap[okey[i]] = (int32_t)i;
+// orderkey -> position in date order, and the per-position attributes.
+__global__ void k_positions(const uint32_t* byday, uint32_t n, const int32_t* okey,
+                            const int32_t* ocust, const int32_t* odate, const int32_t* oprio,
+                            const uint8_t* cseg, uint32_t ncseg, int32_t* omap, int32_t* skey,
+                            int32_t* sprio, int32_t* sdate, uint8_t* sseg) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t o = byday[i];
+    const int32_t k = okey[o];
+    if (k >= 0) omap[k] = (int32_t)i;
+    const uint32_t ck = (uint32_t)ocust[o];
+    skey[i] = k;
+    sprio[i] = oprio[o];
+    sdate[i] = odate[o];
+    sseg[i] = ck < ncseg ? cseg[ck] : (uint8_t)0xFF;
+  }
 }
-// Per fact row: its order row (or -1) and the per-order row counts; the
-// largest ship - orderdate lag.
+// Per fact row: its order's position (or -1) and the per-position row
+// counts; the largest ship - orderdate lag.
 __global__ void k_fact_count(const int32_t* lkey, const int32_t* lship, uint32_t nl,
                              const int32_t* omap, uint32_t maxok, const int32_t* odate,
                              int32_t* frow, uint32_t* cnt, int32_t* maxlag) {
@@ -136,16 +154,15 @@ __global__ void q3b_clear(uint32_t* counters, uint32_t* hist, int n) {
     for (uint32_t i = threadIdx.x; i < (uint32_t)n * 4; i += blockDim.x) counters[i] = 0u;
 }
 
-// Groups of request blockIdx.y: orders of its window in its segment with a
-// fact row shipped after its date. Tile-aggregated append to its group range.
+// Groups of request blockIdx.y: orders of its window (a contiguous range of
+// date-ordered positions) in its segment with a fact row shipped after its
+// date. Tile-aggregated append to its group range.
 __global__ __launch_bounds__(256) void q3b_groups(const Q3BatchDesc* __restrict__ desc,
                                                   const uint32_t* __restrict__ win,
-                                                  const uint32_t* __restrict__ byday,
-                                                  const int32_t* __restrict__ okey,
-                                                  const int32_t* __restrict__ ocust,
-                                                  const int32_t* __restrict__ odate,
-                                                  const int32_t* __restrict__ oprio,
-                                                  const uint8_t* __restrict__ cseg, uint32_t ncseg,
+                                                  const int32_t* __restrict__ skey,
+                                                  const int32_t* __restrict__ sdate,
+                                                  const int32_t* __restrict__ sprio,
+                                                  const uint8_t* __restrict__ sseg,
                                                   const uint32_t* __restrict__ csr_off,
                                                   const int32_t* __restrict__ csr_ship,
                                                   const int64_t* __restrict__ csr_rev,
@@ -154,46 +171,65 @@ __global__ __launch_bounds__(256) void q3b_groups(const Q3BatchDesc* __restrict_
                                                   TopEntry* __restrict__ groups) {
   __shared__ uint32_t scratch[17];
   __shared__ uint32_t base_sh;
+  __shared__ uint32_t lh[4096];  // the block's share of the request's revenue histogram
   const int r = blockIdx.y;
   if (r >= desc->n) return;
+  for (int i = threadIdx.x; i < 4096; i += blockDim.x) lh[i] = 0u;
+  __syncthreads();
   const int32_t seg = desc->segment[r], D = desc->date[r];
   const uint32_t begin = win[r * 4], end = win[r * 4 + 1], obase = win[r * 4 + 2];
   const uint32_t span = end - begin;
-  const uint32_t stride = gridDim.x * blockDim.x;
+  constexpr int PER = 4;  // positions per thread and round: one scan per 1024
+  const uint32_t stride = gridDim.x * blockDim.x * PER;
   const uint32_t rounds = (span + stride - 1) / stride;
   for (uint32_t it = 0; it < rounds; ++it) {
-    const uint32_t i = begin + it * stride + blockIdx.x * blockDim.x + threadIdx.x;
-    bool q = false;
-    TopEntry e;
-    if (i < end) {
-      const uint32_t o = byday[i];
-      const uint32_t ck = (uint32_t)ocust[o];
-      if (ck < ncseg && cseg[ck] == (uint8_t)seg && odate[o] < D) {
-        int64_t rev = 0;
-        bool any = false;
-        for (uint32_t p = csr_off[o]; p < csr_off[o + 1]; ++p) {
-          if (csr_ship[p] > D) {
-            rev += csr_rev[p];
-            any = true;
-          }
+    const uint32_t i0 = begin + it * stride + (blockIdx.x * blockDim.x + threadIdx.x);
+    const uint32_t step = gridDim.x * blockDim.x;
+    bool cand[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const uint32_t i = i0 + j * step;
+      cand[j] = i < end && sseg[i] == (uint8_t)seg && sdate[i] < D;
+    }
+    TopEntry e[PER];
+    uint32_t mask = 0;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      if (!cand[j]) continue;
+      const uint32_t i = i0 + j * step;
+      int64_t rev = 0;
+      bool any = false;
+      const uint32_t p1 = csr_off[i + 1];
+      for (uint32_t p = csr_off[i]; p < p1; ++p) {
+        if (csr_ship[p] > D) {
+          rev += csr_rev[p];
+          any = true;
         }
-        if (any) {
-          q = true;
-          e = TopEntry{rev, odate[o], okey[o], oprio[o], 0};
-        }
+      }
+      if (any) {
+        mask |= 1u << j;
+        e[j] = TopEntry{rev, sdate[i], skey[i], sprio[i], 0};
       }
     }
     uint32_t total;
-    const uint32_t pre = block_excl_scan(q ? 1u : 0u, scratch, &total);
+    uint32_t pos = block_excl_scan((uint32_t)__popc(mask), scratch, &total);
     if (total == 0) continue;
     if (threadIdx.x == 0) base_sh = atomicAdd(&counters[r * 4], total);
     __syncthreads();
-    if (q) {
-      groups[obase + base_sh + pre] = e;
-      atomicAdd(&hist[r * 4096 + value_bucket(e.rev)], 1u);
+    pos += base_sh;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      if ((mask >> j) & 1u) {
+        groups[obase + pos++] = e[j];
+        atomicAdd(&lh[value_bucket(e[j].rev)], 1u);
+      }
     }
     __syncthreads();
   }
+  // one global add per non-empty bucket: per-group device atomics on a few
+  // hot buckets serialised and dominated large batches
+  for (int i = threadIdx.x; i < 4096; i += blockDim.x)
+    if (lh[i]) atomicAdd(&hist[r * 4096 + i], lh[i]);
 }
 
 __global__ __launch_bounds__(256) void q3b_threshold(const Q3BatchDesc* __restrict__ desc,
@@ -227,11 +263,48 @@ __global__ __launch_bounds__(256) void q3b_threshold(const Q3BatchDesc* __restri
   }
 }
 
-// One block per request: candidates (bucket >= threshold) into LDS top-k.
+// Groups of request blockIdx.y at or above its threshold bucket -> its
+// candidate list (tile-aggregated append; entries past Q3B_CAND are counted
+// but not stored).
+__global__ __launch_bounds__(256) void q3b_collect(const Q3BatchDesc* __restrict__ desc,
+                                                   const uint32_t* __restrict__ win,
+                                                   uint32_t* __restrict__ counters,
+                                                   const TopEntry* __restrict__ groups,
+                                                   TopEntry* __restrict__ cand) {
+  __shared__ uint32_t scratch[17];
+  __shared__ uint32_t base_sh;
+  const int r = blockIdx.y;
+  if (r >= desc->n) return;
+  const uint32_t ng = counters[r * 4], thr = counters[r * 4 + 1];
+  const TopEntry* g = groups + win[r * 4 + 2];
+  TopEntry* c = cand + (size_t)r * Q3B_CAND;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t span = (ng + stride - 1) / stride * stride;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < span; i += stride) {
+    TopEntry e;
+    bool q = false;
+    if (i < ng) {
+      e = g[i];
+      q = value_bucket(e.rev) >= thr;
+    }
+    uint32_t total;
+    const uint32_t pre = block_excl_scan(q ? 1u : 0u, scratch, &total);
+    if (total == 0) continue;
+    if (threadIdx.x == 0) base_sh = atomicAdd(&counters[r * 4 + 2], total);
+    __syncthreads();
+    if (q && base_sh + pre < Q3B_CAND) c[base_sh + pre] = e;
+    __syncthreads();
+  }
+}
+
+// One block per request: its candidates into the LDS top-k; if they
+// overflowed the list (heavy ties in the threshold bucket), all its groups at
+// or above the threshold.
 __global__ __launch_bounds__(256) void q3b_topk(const Q3BatchDesc* __restrict__ desc,
                                                 const uint32_t* __restrict__ win,
                                                 const uint32_t* __restrict__ counters,
                                                 const TopEntry* __restrict__ groups,
+                                                const TopEntry* __restrict__ cand,
                                                 TopEntry* __restrict__ out) {
   __shared__ TopEntry buf[2048];
   __shared__ int nbuf;
@@ -239,15 +312,17 @@ __global__ __launch_bounds__(256) void q3b_topk(const Q3BatchDesc* __restrict__ 
   if (r >= desc->n) return;
   BlockTopK<2048> tk{buf, &nbuf, desc->k[r]};
   tk.init();
-  const uint32_t ng = counters[r * 4], thr = counters[r * 4 + 1];
-  const TopEntry* g = groups + win[r * 4 + 2];
+  const uint32_t nc = counters[r * 4 + 2], thr = counters[r * 4 + 1];
+  const bool fits = nc <= Q3B_CAND;
+  const uint32_t ng = fits ? nc : counters[r * 4];
+  const TopEntry* g = fits ? cand + (size_t)r * Q3B_CAND : groups + win[r * 4 + 2];
   for (uint32_t base = 0; base < ng; base += blockDim.x) {
     const uint32_t i = base + threadIdx.x;
     TopEntry e;
     bool valid = false;
     if (i < ng) {
       e = g[i];
-      valid = value_bucket(e.rev) >= thr;
+      valid = fits || value_bucket(e.rev) >= thr;
     }
     tk.push(valid, e);
   }
@@ -291,15 +366,18 @@ int q3_slot_create(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* ord
   const uint32_t maxck = (uint32_t)std::max<int64_t>(0, cck->maxv);
   const uint32_t maxok = (uint32_t)std::max<int64_t>(0, cok->maxv);
   const uint32_t ndays = (uint32_t)std::max(0, s->dmax - s->dmin + 1);
-  s->ncseg = maxck + 1;
+  const uint32_t ncseg = maxck + 1;
   hipStream_t st = ctx->stream;
   const int G = ctx->info.cu_count * 8, B = 256;
   int32_t *omap = nullptr, *frow = nullptr, *d_lag = nullptr;
-  uint32_t *cnt = nullptr, *tsum = nullptr, *total = nullptr;
+  uint32_t *cnt = nullptr, *tsum = nullptr, *total = nullptr, *byday = nullptr;
+  uint8_t* segmap = nullptr;
   int rc;
-  if ((rc = alloc(ctx, s, (void**)&s->cseg, s->ncseg)) ||
-      (rc = alloc(ctx, s, (void**)&s->byday, (size_t)std::max(1u, s->no) * 4)) ||
-      (rc = alloc(ctx, s, (void**)&s->dayoff, (size_t)(ndays + 1) * 4 + 64)) ||
+  if ((rc = alloc(ctx, s, (void**)&s->dayoff, (size_t)(ndays + 1) * 4 + 64)) ||
+      (rc = alloc(ctx, s, (void**)&s->skey, (size_t)std::max(1u, s->no) * 4)) ||
+      (rc = alloc(ctx, s, (void**)&s->sprio, (size_t)std::max(1u, s->no) * 4)) ||
+      (rc = alloc(ctx, s, (void**)&s->sdate, (size_t)std::max(1u, s->no) * 4)) ||
+      (rc = alloc(ctx, s, (void**)&s->sseg, (size_t)std::max(1u, s->no))) ||
       (rc = alloc(ctx, s, (void**)&s->csr_off, (size_t)(s->no + 1) * 4 + 64)) ||
       (rc = alloc(ctx, s, (void**)&s->csr_ship, (size_t)std::max(1u, s->nl) * 4)) ||
       (rc = alloc(ctx, s, (void**)&s->csr_rev, (size_t)std::max(1u, s->nl) * 8)))
@@ -321,23 +399,37 @@ int q3_slot_create(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* ord
       (rc = talloc((void**)&frow, (size_t)std::max(1u, s->nl) * 4)) ||
       (rc = talloc((void**)&cnt, (size_t)nscan * 4 + 64)) ||
       (rc = talloc((void**)&tsum, (size_t)scan_tile_sums_needed(nscan) * 4 + 64)) ||
-      (rc = talloc((void**)&total, 64)) || (rc = talloc((void**)&d_lag, 64))) {
+      (rc = talloc((void**)&total, 64)) || (rc = talloc((void**)&d_lag, 64)) ||
+      (rc = talloc((void**)&byday, (size_t)std::max(1u, s->no) * 4)) ||
+      (rc = talloc((void**)&segmap, ncseg))) {
     tfree();
     return bail(rc);
   }
-  // customer segments, order map
-  k_fill_u8<<<G, B, 0, st>>>(s->cseg, s->ncseg, 0xFF);
+  // customer segments
+  k_fill_u8<<<G, B, 0, st>>>(segmap, ncseg, 0xFF);
   if (customer->nrows)
     k_cseg<<<G, B, 0, st>>>((const int32_t*)cck->dev, (const uint8_t*)cseg->dev,
-                            (uint32_t)customer->nrows, s->cseg);
+                            (uint32_t)customer->nrows, segmap);
+  // order rows by day (counting sort), then orderkey -> position and the
+  // per-position attributes
+  k_fill_i32<<<G, B, 0, st>>>((int32_t*)cnt, ndays + 1, 0);
+  if (s->no) k_day_count<<<G, B, 0, st>>>(s->odate, s->no, s->dmin, cnt);
+  if ((rc = exclusive_scan_u32(ctx, cnt, ndays + 1, s->dayoff, total, tsum, st))) {
+    tfree();
+    return bail(rc);
+  }
+  (void)hipMemcpyAsync(cnt, s->dayoff, (size_t)(ndays + 1) * 4, hipMemcpyDeviceToDevice, st);
+  if (s->no) k_day_scatter<<<G, B, 0, st>>>(s->odate, s->no, s->dmin, cnt, byday);
   k_fill_i32<<<G, B, 0, st>>>(omap, maxok + 1, -1);
-  if (s->no) k_omap<<<G, B, 0, st>>>(s->okey, s->no, omap);
-  // CSR of fact rows by order row
+  if (s->no)
+    k_positions<<<G, B, 0, st>>>(byday, s->no, s->okey, s->ocust, s->odate, s->oprio, segmap,
+                                 ncseg, omap, s->skey, s->sprio, s->sdate, s->sseg);
+  // CSR of fact rows by position
   k_fill_i32<<<G, B, 0, st>>>((int32_t*)cnt, s->no + 1, 0);
   k_fill_i32<<<1, 64, 0, st>>>(d_lag, 1, 0);
   if (s->nl)
     k_fact_count<<<G, B, 0, st>>>((const int32_t*)clk->dev, (const int32_t*)cls->dev, s->nl, omap,
-                                  maxok, s->odate, frow, cnt, d_lag);
+                                  maxok, s->sdate, frow, cnt, d_lag);
   if ((rc = exclusive_scan_u32(ctx, cnt, s->no + 1, s->csr_off, total, tsum, st))) {
     tfree();
     return bail(rc);
@@ -346,15 +438,6 @@ int q3_slot_create(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* ord
   if (s->nl)
     k_fact_scatter<<<G, B, 0, st>>>(frow, (const int32_t*)cls->dev, (const int32_t*)cld->dev,
                                     (const int64_t*)clp->dev, s->nl, cnt, s->csr_ship, s->csr_rev);
-  // orders by day
-  k_fill_i32<<<G, B, 0, st>>>((int32_t*)cnt, ndays + 1, 0);
-  if (s->no) k_day_count<<<G, B, 0, st>>>(s->odate, s->no, s->dmin, cnt);
-  if ((rc = exclusive_scan_u32(ctx, cnt, ndays + 1, s->dayoff, total, tsum, st))) {
-    tfree();
-    return bail(rc);
-  }
-  (void)hipMemcpyAsync(cnt, s->dayoff, (size_t)(ndays + 1) * 4, hipMemcpyDeviceToDevice, st);
-  if (s->no) k_day_scatter<<<G, B, 0, st>>>(s->odate, s->no, s->dmin, cnt, s->byday);
   if (hipGetLastError() != hipSuccess) {
     tfree();
     return bail(fail(GDB_E_HIP, "Q3 index build launch failed"));
@@ -381,6 +464,7 @@ int q3_slot_create(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* ord
       (rc = alloc(ctx, s, (void**)&s->counters, Q3B_MAX * 16)) ||
       (rc = alloc(ctx, s, (void**)&s->hist, (size_t)Q3B_MAX * 4096 * 4)) ||
       (rc = alloc(ctx, s, (void**)&s->groups, s->groups_cap * sizeof(TopEntry))) ||
+      (rc = alloc(ctx, s, (void**)&s->cand, (size_t)Q3B_MAX * Q3B_CAND * sizeof(TopEntry))) ||
       (rc = alloc(ctx, s, (void**)&s->out, (size_t)Q3B_MAX * Q3B_TOPK * sizeof(TopEntry))))
     return bail(rc);
   if (hipHostMalloc((void**)&s->h_desc, sizeof(Q3BatchDesc), 0) != hipSuccess ||
@@ -457,11 +541,12 @@ int q3_batch_launch(gdb_ctx* ctx, Q3Slot* s, const std::vector<gdb_q3_params>& r
   // the sequence depends only on (n, gx).
   auto enqueue = [&](hipStream_t q) -> int {
     q3b_clear<<<64, 256, 0, q>>>(s->counters, s->hist, n);
-    q3b_groups<<<dim3(gx, n), 256, 0, q>>>(s->d_desc, s->d_win, s->byday, s->okey, s->ocust,
-                                           s->odate, s->oprio, s->cseg, s->ncseg, s->csr_off,
-                                           s->csr_ship, s->csr_rev, s->counters, s->hist, s->groups);
+    q3b_groups<<<dim3(gx, n), 256, 0, q>>>(s->d_desc, s->d_win, s->skey, s->sdate, s->sprio,
+                                           s->sseg, s->csr_off, s->csr_ship, s->csr_rev,
+                                           s->counters, s->hist, s->groups);
     q3b_threshold<<<n, 256, 0, q>>>(s->d_desc, s->hist, s->counters);
-    q3b_topk<<<n, 256, 0, q>>>(s->d_desc, s->d_win, s->counters, s->groups, s->out);
+    q3b_collect<<<dim3(gx, n), 256, 0, q>>>(s->d_desc, s->d_win, s->counters, s->groups, s->cand);
+    q3b_topk<<<n, 256, 0, q>>>(s->d_desc, s->d_win, s->counters, s->groups, s->cand, s->out);
     GDB_HIP(hipGetLastError());
     return GDB_OK;
   };
