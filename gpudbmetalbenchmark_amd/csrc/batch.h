@@ -32,6 +32,7 @@ struct FlatBatchDesc {
   int32_t n_date, n_disc, n_qty;
   int32_t has_q6, has_q1;
   int32_t q6_cells, q1_cells;
+  int32_t copies;         // copies of the cell table in LDS (set at launch)
   uint8_t date_lut[FB_LUT_DAYS];
   uint8_t disc_lut[FB_LUT_SMALL];
   uint8_t qty_lut[FB_LUT_SMALL];
@@ -81,9 +82,9 @@ void flat_batch_results(const FlatPlan& plan, const int64_t* totals,
 
 // ---------------------------------------------------------------------------
 // Q3 batch: up to 64 (segment, date) requests over one customer/orders/fact
-// set. Per order a 64-bit mask of the requests it qualifies for; per fact row
-// the mask of requests whose date is below its shipdate; revenue is added per
-// (orderkey, request) into one hash table; per request a block top-k.
+// set, answered from date-ordered join indexes built once per table set: per
+// request its window of orders, its groups, a revenue histogram, candidates
+// above the k-th bucket and a block top-k.
 constexpr int Q3B_MAX = 64;
 
 struct Q3BatchDesc {

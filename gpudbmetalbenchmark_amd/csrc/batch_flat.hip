@@ -130,13 +130,17 @@ __device__ __forceinline__ void fb_row(const FbState& s, int32_t ship, int32_t q
   }
 }
 
+// The cell table is replicated `copies` times in LDS (up to one copy per 16
+// lanes): with a batch of a few requests the table has a few dozen cells, and
+// 1024 threads adding into one copy serialised on the same addresses.
 __global__ __launch_bounds__(1024) void flat_batch_scan(FlatCols c, uint32_t n,
                                                         const FlatBatchDesc* __restrict__ desc,
                                                         int64_t* __restrict__ partials) {
   extern __shared__ unsigned long long sm[];
   const int ncells = desc->q6_cells * 2 + desc->q1_cells * 6;
-  uint8_t* luts = reinterpret_cast<uint8_t*>(sm + ncells);
-  for (int i = threadIdx.x; i < ncells; i += blockDim.x) sm[i] = 0ull;
+  const int copies = desc->copies;
+  uint8_t* luts = reinterpret_cast<uint8_t*>(sm + (size_t)copies * ncells);
+  for (int i = threadIdx.x; i < copies * ncells; i += blockDim.x) sm[i] = 0ull;
   for (int i = threadIdx.x; i < FB_LUT_DAYS / 4; i += blockDim.x)
     reinterpret_cast<uint32_t*>(luts)[i] = reinterpret_cast<const uint32_t*>(desc->date_lut)[i];
   for (int i = threadIdx.x; i < FB_LUT_SMALL / 4; i += blockDim.x) {
@@ -147,7 +151,7 @@ __global__ __launch_bounds__(1024) void flat_batch_scan(FlatCols c, uint32_t n,
   }
   __syncthreads();
   FbState s;
-  s.tab = sm;
+  s.tab = sm + (size_t)((threadIdx.x >> 4) % copies) * ncells;
   s.dlut = luts;
   s.clut = luts + FB_LUT_DAYS;
   s.qlut = luts + FB_LUT_DAYS + FB_LUT_SMALL;
@@ -181,8 +185,18 @@ __global__ __launch_bounds__(1024) void flat_batch_scan(FlatCols c, uint32_t n,
            s.has_q1 ? c.rf[r] : 0u, s.has_q1 ? c.ls[r] : 0u);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < ncells; i += blockDim.x)
-    partials[(size_t)i * gridDim.x + blockIdx.x] = (int64_t)sm[i];
+  for (int i = threadIdx.x; i < ncells; i += blockDim.x) {
+    uint64_t v = 0;
+    for (int k = 0; k < copies; ++k) v += sm[(size_t)k * ncells + i];
+    partials[(size_t)i * gridDim.x + blockIdx.x] = (int64_t)v;
+  }
+}
+
+// Copies of an ncells table that fit `lds` bytes next to the lookup tables.
+int table_copies(int ncells, size_t lds) {
+  const size_t luts = FB_LUT_DAYS + 2 * FB_LUT_SMALL;
+  if (ncells <= 0 || lds <= luts) return 1;
+  return (int)std::max<size_t>(1, std::min<size_t>(64, (lds - luts) / ((size_t)ncells * 8)));
 }
 
 }  // namespace
@@ -254,7 +268,14 @@ int flat_batch_launch(gdb_ctx* ctx, FlatSlot* s, const FlatPlan& plan, const gdb
   GDB_TRY(column(li, "l_returnflag", GDB_U8, &c[5]));
   GDB_TRY(column(li, "l_linestatus", GDB_U8, &c[6]));
   if ((size_t)plan.ncells > s->max_cells) return fail(GDB_E_UNSUPPORTED, "batch plan too large");
+  // LDS of the launch: the plan's LDS size class (two workgroups per CU up to
+  // half the limit); the spare room holds copies of the cell table.
+  const size_t lds_class = plan.lds_bytes <= 32768   ? 32768
+                           : plan.lds_bytes <= 65536 ? 65536
+                           : plan.lds_bytes <= 98304 ? 98304
+                                                     : s->lds_limit;
   *s->h_desc = plan.desc;
+  s->h_desc->copies = table_copies(plan.ncells, std::min(lds_class, s->lds_limit));
   GDB_HIP(hipMemcpyAsync(s->d_desc, s->h_desc, sizeof(FlatBatchDesc), hipMemcpyHostToDevice, stream));
   int launches = 0;
   auto consume = [&](const void* const* d, uint32_t rows) -> int {
@@ -262,7 +283,8 @@ int flat_batch_launch(gdb_ctx* ctx, FlatSlot* s, const FlatPlan& plan, const gdb
                       (const int32_t*)d[3], (const int64_t*)d[4], (const uint8_t*)d[5],
                       (const uint8_t*)d[6]};
     const int blocks = std::max(1, std::min<int>(s->blocks, (int)((rows / 4 + 1023) / 1024)));
-    flat_batch_scan<<<blocks, 1024, plan.lds_bytes, stream>>>(fc, rows, s->d_desc, s->partials);
+    flat_batch_scan<<<blocks, 1024, std::min(lds_class, s->lds_limit), stream>>>(fc, rows, s->d_desc,
+                                                                              s->partials);
     GDB_HIP(hipGetLastError());
     if (launches++ == 0)
       sum_columns<0><<<plan.ncells, 256, 0, stream>>>(s->partials, (uint32_t)blocks, s->totals);
@@ -281,10 +303,6 @@ int flat_batch_launch(gdb_ctx* ctx, FlatSlot* s, const FlatPlan& plan, const gdb
     // One graph per table and LDS size class: the scan is launched with the
     // class's LDS (the plan's layout lives in the descriptor) and the
     // finaliser reads the cell count from the descriptor.
-    const size_t lds_class = plan.lds_bytes <= 32768   ? 32768
-                             : plan.lds_bytes <= 65536 ? 65536
-                             : plan.lds_bytes <= 98304 ? 98304
-                                                       : s->lds_limit;
     const uint32_t rows = (uint32_t)li->nrows;
     const int blocks = std::max(1, std::min<int>(s->blocks, (int)((rows / 4 + 1023) / 1024)));
     const FlatCols fc{(const int32_t*)c[0]->dev, (const int32_t*)c[1]->dev, (const int32_t*)c[2]->dev,
