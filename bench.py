@@ -7,7 +7,13 @@ whole-pipeline time (every kernel, the finaliser and the result readback),
 with the inputs resident in HBM when the timed region starts.
 
   python bench.py [--steps K] [--warmup W]
-                  [--pipeline q1|q6|q3|q9|q13|select|sum|join|all] [--scale sf10|sf1|sf100]
+                  [--pipeline q1|q6|q3|q9|q13|select|sum|join|all|serve] [--scale sf10|sf1|sf100]
+                  [--no-tune] [--no-cpu] [--probe]
+
+Before the warm-up steps, the engine's autotuner (gdb_autotune) times its
+candidate launch shapes for the pipeline on the same tables and keeps the
+fastest; the chosen shape is reported in config.tuning (--no-tune keeps the
+defaults; streamed sf100 runs are PCIe-bound and are not tuned).
 
 --scale sf100 streams the fact table (orders for Q13) from pinned host memory in
 chunks (the 600M-row variant); its `value` is then host-to-result throughput.
@@ -56,9 +62,10 @@ class Workload:
     """setup() uploads tables; step(timing) runs the pipeline once; oracle()
     runs the CPU restatement once on the same data."""
 
-    def __init__(self, name, kernel, workload, rows, step, oracle, oracle_bytes):
+    def __init__(self, name, kernel, workload, rows, step, oracle, oracle_bytes, tune=None):
         self.name, self.kernel, self.workload, self.rows = name, kernel, workload, rows
         self.step, self.oracle, self.oracle_bytes = step, oracle, oracle_bytes
+        self.tune = tune  # runs the autotuner on these tables, keeps the best shape
 
 
 def build_workloads(eng, names, scale, O):
@@ -81,7 +88,8 @@ def build_workloads(eng, names, scale, O):
                     f"Q1 grouped aggregation, {rows}-row synthetic fact table"
                     f"{' streamed from pinned host memory' if stream else ''}, 6 groups x 8 aggregates",
                     rows, lambda tm, t=t: eng.q1(t, 2435, timing=tm),
-                    lambda li=li: O.q1(li, 2435, nthreads=th), 26 * rows)
+                    lambda li=li: O.q1(li, 2435, nthreads=th), 26 * rows,
+                    tune=lambda t=t: eng.autotune("q1", [t]))
             else:
                 t = (eng.host_table if stream else eng.table)(
                     {k: li[k] for k in ("l_shipdate", "l_discount", "l_quantity",
@@ -93,7 +101,8 @@ def build_workloads(eng, names, scale, O):
                     f"Q6 filter-scan-sum, {rows}-row synthetic fact table{note}"
                     f"{' streamed from pinned host memory' if stream else ''}",
                     rows, lambda tm, t=t: eng.q6(t, timing=tm),
-                    lambda li=li: O.q6(li, nthreads=th), 20 * rows)
+                    lambda li=li: O.q6(li, nthreads=th), 20 * rows,
+                    tune=lambda t=t: eng.autotune("q6", [t]))
     join_needed = [n for n in names if n in ("q3", "q9", "q13", "select", "sum", "join")]
     if join_needed:
         nc, no, npart, ns = JOIN_SCALE[scale]
@@ -115,7 +124,8 @@ def build_workloads(eng, names, scale, O):
                 f"{' (fact streamed from pinned host memory)' if stream else ''}",
                 nl, lambda tm: eng.q3(dims["customer"], orders_res, fact, 1, 1169, 10, timing=tm),
                 lambda: O.q3(js["customer"], js["orders"], js["lineitem"], 1, 1169, 10, nthreads=th),
-                5 * nc + 16 * no + 20 * nl)
+                5 * nc + 16 * no + 20 * nl,
+                tune=lambda: eng.autotune("q3", [dims["customer"], orders_res, fact]))
         if "q9" in names:
             out["q9"] = Workload(
                 "q9", "q9_probe",
@@ -126,7 +136,9 @@ def build_workloads(eng, names, scale, O):
                                       fact, "green", timing=tm),
                 lambda: O.q9(js["part"], js["supplier"], js["partsupp"], js["orders"],
                              js["lineitem"], "green", nthreads=th),
-                28 * nl + 59 * npart + 16 * 4 * npart + 8 * ns + 8 * no)
+                28 * nl + 59 * npart + 16 * 4 * npart + 8 * ns + 8 * no,
+                tune=lambda: eng.autotune("q9", [dims["part"], dims["supplier"], dims["partsupp"],
+                                                 orders_res, fact]))
         if "q13" in names:
             otab = (eng.host_table if stream else eng.table)(
                 {k: js["orders"][k] for k in ("o_custkey", "o_comment")})
@@ -135,7 +147,8 @@ def build_workloads(eng, names, scale, O):
                 f"Q13 left-outer count: orders {no} (80-byte comments), customer {nc}"
                 f"{' (orders streamed from pinned host memory)' if stream else ''}",
                 no, lambda tm: eng.q13(dims["customer"], otab, timing=tm),
-                lambda: O.q13(js["customer"], js["orders"], nthreads=th), 84 * no + 4 * nc)
+                lambda: O.q13(js["customer"], js["orders"], nthreads=th), 84 * no + 4 * nc,
+                tune=lambda: eng.autotune("q13", [dims["customer"], otab]))
         if not stream:
             li = js["lineitem"]
             if "select" in names:
@@ -156,13 +169,18 @@ def build_workloads(eng, names, scale, O):
                     nl, lambda tm: eng.join_count(orders_res, "o_orderkey", fact, "l_orderkey",
                                                   timing=tm),
                     lambda: O.join_count(js["orders"]["o_orderkey"], li["l_orderkey"]),
-                    4 * no + 4 * nl)
+                    4 * no + 4 * nl,
+                    tune=lambda: eng.autotune("join", [orders_res, fact]))
     return out
 
 
 def run_one(w, args, info, O):
     import numpy as np
     from gpudbmetalbenchmark_amd import Timing
+    tuning = None
+    if args.tune and w.tune is not None:
+        t = w.tune()
+        tuning = {k: t[k] for k in ("block", "blocks_per_cu", "unroll", "radix_bits", "candidates")}
     tm = Timing()
     for _ in range(args.warmup):
         w.step(tm)
@@ -187,7 +205,8 @@ def run_one(w, args, info, O):
                    "rows": w.rows, "bytes_per_step": bytes_step,
                    "device": (info.name or info.arch).decode(errors="replace"),
                    "cu_count": info.cu_count,
-                   "pct_hbm_peak": round(100 * value / HBM_PEAK_GBPS, 2)},
+                   "pct_hbm_peak": round(100 * value / HBM_PEAK_GBPS, 2),
+                   "tuning": tuning},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
                      "traffic": load_traffic(w.kernel), "kernel": w.kernel,
@@ -306,6 +325,8 @@ def main():
     ap.add_argument("--scale", default="sf10", choices=["sf1", "sf10", "sf100"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", dest="tune", action="store_false",
+                    help="keep the default launch shapes instead of autotuning first")
     ap.add_argument("--probe", action="store_true", help="also measure plain read/copy GB/s")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     args = ap.parse_args()
@@ -327,6 +348,7 @@ def main():
              if args.pipeline == "all" else args.pipeline.split(","))
     if args.scale == "sf100":
         names = [n for n in names if n in ("q1", "q6", "q3", "q9", "q13")]
+        args.tune = False
     eng = Engine(0)
     info = eng.device_info()
     args.hbm_probe = None

@@ -40,10 +40,6 @@ constexpr int TOPK_BUF = 2048;
 constexpr uint32_t CAND_CAP = 65536;  // candidate list entries
 constexpr int OTILE_PER_THREAD = 16;
 
-__device__ __forceinline__ bool bit_test(const uint32_t* __restrict__ bm, uint32_t k) {
-  return (bm[k >> 5] >> (k & 31)) & 1u;
-}
-
 struct Q3Scratch {
   uint32_t* cbm;      // customer bitmap
   uint32_t* counters; // [0] qualifying orders, [1] groups, [2] threshold, [3] candidates

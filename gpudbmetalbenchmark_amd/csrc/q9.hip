@@ -592,16 +592,20 @@ extern "C" int gdb_q9(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supp
   const Q9Dims dm{maxpk, maxsk, maxok, years.ny, years.first, nnat * years.ny, fwords,
                   exact ? 1 : 0};
   const size_t tab_lds = (size_t)fwords * 4 + (size_t)dm.nslots * 12;
-  GDB_HIP(hipFuncSetAttribute((const void*)q9_probe<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)tab_lds));
+  // 4-row groups in flight per lane: the tuning entry's unroll (1, 2 or 4)
+  const int U = ctx->tune[T_Q9].unroll;
+  const void* kfn = U >= 4 ? (const void*)q9_probe<4> : U == 1 ? (const void*)q9_probe<1>
+                                                                : (const void*)q9_probe<2>;
+  GDB_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)tab_lds));
   const int PB = PROBE_THREADS;
-  const int president =
-      resident_blocks(ctx, (const void*)q9_probe<2>, PB, tab_lds, ctx->tune[T_Q9].blocks_per_cu);
+  const int president = resident_blocks(ctx, kfn, PB, tab_lds, ctx->tune[T_Q9].blocks_per_cu);
   auto consume = [&](const void* const* d, uint32_t rows) -> int {
     const Q9Fact f{(const int32_t*)d[0], (const int32_t*)d[1], (const int32_t*)d[2],
                    (const int32_t*)d[3], (const int32_t*)d[4], (const int64_t*)d[5]};
     const int pblocks = std::max(1, std::min<int>(president, (int)((rows / 4 + PB - 1) / PB)));
-    q9_probe<2><<<pblocks, PB, tab_lds, st>>>(f, rows, dm, s);
+    if (U >= 4) q9_probe<4><<<pblocks, PB, tab_lds, st>>>(f, rows, dm, s);
+    else if (U == 1) q9_probe<1><<<pblocks, PB, tab_lds, st>>>(f, rows, dm, s);
+    else q9_probe<2><<<pblocks, PB, tab_lds, st>>>(f, rows, dm, s);
     GDB_HIP(hipGetLastError());
     return GDB_OK;
   };

@@ -223,8 +223,12 @@ int gdb_open(int device, uint64_t hbm_budget_bytes, gdb_ctx** out) {
   // Launch-shape defaults (measured on MI355X: nontemporal dwordx4 streams run
   // best with 4 groups in flight per lane at 2 workgroups of 256 per CU).
   ctx->tune[T_Q6] = Tuning{256, 2, 4};
-  // Q1's lane-private LDS tables (48 KiB per workgroup) fit three per CU.
-  ctx->tune[T_Q1] = Tuning{256, 3, 2};
+  // Q1's lane-private LDS tables take 12 KiB per wave; the autotuner's pick
+  // on 60M rows is four 128-thread workgroups per CU (233 us, against 243 us
+  // for three of 256).
+  ctx->tune[T_Q1] = Tuning{128, 4, 2};
+  // Q9's probe: one 4-row group in flight per lane measured fastest.
+  ctx->tune[T_Q9] = Tuning{256, 0, 1};
   // The Q13 matcher is instruction-issue bound: more than 4 workgroups per CU
   // only add contention (335 us at 4, 358 us at 7 for 15M comments).
   ctx->tune[T_Q13] = Tuning{256, 4, 1};

@@ -27,9 +27,13 @@ std::vector<Shape> candidates(int pipeline, const gdb_table* const* t, int n) {
           for (int u : {1, 2, 4}) c.push_back({b, bpc, u, 0});
       break;
     case GDB_PIPE_Q3:
-    case GDB_PIPE_Q9:
     case GDB_PIPE_Q13:
       for (int bpc : {1, 2, 3, 4, 6, 8, 0}) c.push_back({256, bpc, 2, 0});
+      break;
+    case GDB_PIPE_Q9:
+      // the probe's LDS filter fixes one 512-thread workgroup per CU; what
+      // varies is the number of 4-row groups in flight per lane
+      for (int u : {1, 2, 4}) c.push_back({256, 0, u, 0});
       break;
     case GDB_PIPE_JOIN: {
       // one global table, the derived shape, and radix bits around the derived
