@@ -103,6 +103,13 @@ class Tuning(C.Structure):
                 ("radix_bits", C.c_int32)]
 
 
+class TblField(C.Structure):
+    _fields_ = [("field", C.c_int), ("kind", C.c_int), ("width", C.c_int), ("out", C.c_void_p)]
+
+
+TBL_INT32, TBL_DATE, TBL_CENTS, TBL_HUNDREDTHS, TBL_CHAR, TBL_TEXT = 1, 2, 3, 4, 5, 6
+
+
 class ServeCfg(C.Structure):
     _fields_ = [("lineitem_flat", C.c_void_p), ("customer", C.c_void_p), ("orders", C.c_void_p),
                 ("lineitem", C.c_void_p), ("mode", C.c_int32), ("max_batch", C.c_int32),
@@ -208,6 +215,8 @@ PROTOTYPES = {
     "gdb_nation_name": (C.c_char_p, [C.c_int]),
     "gdb_segment_name": (C.c_char_p, [C.c_int]),
     "gdb_year_of_day": (C.c_int32, [C.c_int32]),
+    "gdb_tbl_count_rows": (C.c_int, [C.c_char_p, C.c_int, U64P]),
+    "gdb_tbl_parse": (C.c_int, [C.c_char_p, C.POINTER(TblField), C.c_int, C.c_uint64, C.c_int]),
 }
 
 _lib = None

@@ -4,7 +4,11 @@
 //
 // Scales: sf1 = 6M fact rows, sf10 = 60M, sf100 = 600M streamed from pinned host
 // memory (the reference's runQ*BenchmarkSF100 entry points). Tables come from
-// the seeded generator instead of dbgen .tbl files. Output keeps the reference's
+// dbgen .tbl files when data/SF-1/, data/SF-10/ or data/SF-100/ holds them (the
+// reference's dataset paths, src/main.cpp:56-70) or `--data DIR` names a
+// directory that does, parsed by gdb_tbl_parse; otherwise from the seeded
+// generator. With .tbl data the queries take the TPC-H parameters (Q1 cutoff
+// 1998-09-02, as src/q1.cpp:54). Output keeps the reference's
 // contract parsed by scripts/benchmark_gpu.sh:54-66: a `Qn | N rows` header and
 // the four printTimingSummary lines (src/infra.h:511-516); "CPU Parsing (.tbl)"
 // reports the generation + upload time. Each pipeline runs 2 warm-ups and 10
@@ -34,6 +38,7 @@ struct Scale {
 };
 
 gdb_ctx* g_ctx = nullptr;
+std::string g_data;  // directory of .tbl files, empty: generated tables
 
 void die(int rc, const char* what) {
   fprintf(stderr, "%s failed: %s (%s)\n", what, gdb_strerror(rc), gdb_last_error());
@@ -71,6 +76,32 @@ gdb_table* make_table(const std::vector<gdb_col_desc>& cols, uint64_t n, bool ho
   if (host) CHECK(gdb_table_create_host(g_ctx, cols.data(), (int)cols.size(), n, &t));
   else CHECK(gdb_table_create(g_ctx, cols.data(), (int)cols.size(), n, &t));
   return t;
+}
+
+bool file_exists(const std::string& p) {
+  FILE* f = fopen(p.c_str(), "rb");
+  if (f) fclose(f);
+  return f != nullptr;
+}
+
+// One .tbl file's requested fields into host arrays (pinned when streamed).
+struct TblCol {
+  int field, kind, width;
+  void** out;
+  size_t elem;
+};
+uint64_t load_tbl(const std::string& file, const std::vector<TblCol>& cols, bool pin) {
+  const std::string path = g_data + "/" + file;
+  uint64_t n = 0;
+  CHECK(gdb_tbl_count_rows(path.c_str(), 0, &n));
+  std::vector<gdb_tbl_field> f;
+  for (const TblCol& c : cols) {
+    const uint64_t bytes = n * c.elem * (c.kind == GDB_TBL_TEXT ? c.width : 1);
+    *c.out = alloc<uint8_t>(bytes, pin);
+    f.push_back(gdb_tbl_field{c.field, c.kind, c.width, *c.out});
+  }
+  CHECK(gdb_tbl_parse(path.c_str(), f.data(), (int)f.size(), n, 0));
+  return n;
 }
 
 void print_timing(const char* q, uint64_t rows, double parse_ms, double gpu_ms, double post_ms,
@@ -114,8 +145,19 @@ struct Flat {
 };
 Flat make_flat(const Scale& s) {
   Flat f{};
-  f.n = s.flat_rows;
   auto t0 = Clock::now();
+  if (!g_data.empty()) {
+    f.n = load_tbl("lineitem.tbl",
+                   {{10, GDB_TBL_DATE, 0, (void**)&f.ship, 4}, {4, GDB_TBL_INT32, 0, (void**)&f.qty, 4},
+                    {6, GDB_TBL_HUNDREDTHS, 0, (void**)&f.disc, 4},
+                    {7, GDB_TBL_HUNDREDTHS, 0, (void**)&f.tax, 4},
+                    {5, GDB_TBL_CENTS, 0, (void**)&f.price, 8}, {8, GDB_TBL_CHAR, 0, (void**)&f.rf, 1},
+                    {9, GDB_TBL_CHAR, 0, (void**)&f.ls, 1}},
+                   s.stream);
+    f.gen_ms = ms_since(t0);
+    return f;
+  }
+  f.n = s.flat_rows;
   f.ship = alloc<int32_t>(f.n, s.stream);
   f.qty = alloc<int32_t>(f.n, s.stream);
   f.disc = alloc<int32_t>(f.n, s.stream);
@@ -155,7 +197,9 @@ void run_q1(const Scale& s, Flat& f) {
        {"l_linestatus", GDB_U8, 0, f.ls}},
       f.n, s.stream);
   const double load = f.gen_ms + ms_since(t0);
-  gdb_q1_params p{2435};
+  // synthetic tables: day 2435 (the configuration's filter); TPC-H data:
+  // DATE '1998-12-01' - INTERVAL '90' DAY = 1998-09-02 = day 2436
+  gdb_q1_params p{g_data.empty() ? 2435 : 2436};
   gdb_q1_result r{};
   gdb_timing tm = timed([&](gdb_timing* x) { CHECK(gdb_q1(g_ctx, t, &p, &r, x)); });
   printf("TPC-H Query 1 Results:\n");
@@ -176,7 +220,7 @@ void run_q1(const Scale& s, Flat& f) {
 
 // ------------------------------------------------------------------ joined set
 struct Joined {
-  uint64_t nc, no, nl, np, ns;
+  uint64_t nc, no, nl, np, ns, nps;
   int32_t *c_custkey, *o_orderkey, *o_custkey, *o_orderdate, *o_prio;
   uint8_t *c_seg, *o_comment;
   int32_t *l_ok, *l_pk, *l_sk, *l_qty, *l_disc, *l_ship;
@@ -186,10 +230,53 @@ struct Joined {
   int64_t* ps_cost;
   double gen_ms;
 };
+// c_mktsegment text (CHAR(10)) -> segment code 0..4 (255: other).
+void segment_codes(const uint8_t* text, uint64_t n, uint8_t* code) {
+  for (uint64_t i = 0; i < n; ++i) {
+    code[i] = 255;
+    for (int c = 0; c < 5; ++c) {
+      const char* name = gdb_segment_name(c);
+      const size_t len = strlen(name);
+      if (memcmp(text + i * 10, name, len) == 0 && (len == 10 || text[i * 10 + len] == 0)) code[i] = (uint8_t)c;
+    }
+  }
+}
+
 Joined make_joined(const Scale& s, bool comments) {
   Joined j{};
-  j.nc = s.customers, j.no = s.orders, j.np = s.parts, j.ns = s.suppliers;
   auto t0 = Clock::now();
+  if (!g_data.empty()) {
+    uint8_t* segtext = nullptr;
+    j.nc = load_tbl("customer.tbl", {{0, GDB_TBL_INT32, 0, (void**)&j.c_custkey, 4},
+                                     {6, GDB_TBL_TEXT, 10, (void**)&segtext, 1}}, false);
+    j.c_seg = alloc<uint8_t>(j.nc, false);
+    segment_codes(segtext, j.nc, j.c_seg);
+    std::vector<TblCol> oc = {{0, GDB_TBL_INT32, 0, (void**)&j.o_orderkey, 4},
+                              {1, GDB_TBL_INT32, 0, (void**)&j.o_custkey, 4},
+                              {4, GDB_TBL_DATE, 0, (void**)&j.o_orderdate, 4},
+                              {7, GDB_TBL_INT32, 0, (void**)&j.o_prio, 4}};
+    if (comments) oc.push_back({8, GDB_TBL_TEXT, 80, (void**)&j.o_comment, 1});
+    j.no = load_tbl("orders.tbl", oc, s.stream);
+    j.nl = load_tbl("lineitem.tbl",
+                    {{0, GDB_TBL_INT32, 0, (void**)&j.l_ok, 4}, {1, GDB_TBL_INT32, 0, (void**)&j.l_pk, 4},
+                     {2, GDB_TBL_INT32, 0, (void**)&j.l_sk, 4}, {4, GDB_TBL_INT32, 0, (void**)&j.l_qty, 4},
+                     {6, GDB_TBL_HUNDREDTHS, 0, (void**)&j.l_disc, 4},
+                     {5, GDB_TBL_CENTS, 0, (void**)&j.l_price, 8},
+                     {10, GDB_TBL_DATE, 0, (void**)&j.l_ship, 4}},
+                    s.stream);
+    j.np = load_tbl("part.tbl", {{0, GDB_TBL_INT32, 0, (void**)&j.p_partkey, 4},
+                                 {1, GDB_TBL_TEXT, 55, (void**)&j.p_name, 1}}, false);
+    const uint64_t nps = load_tbl("partsupp.tbl", {{0, GDB_TBL_INT32, 0, (void**)&j.ps_pk, 4},
+                                                   {1, GDB_TBL_INT32, 0, (void**)&j.ps_sk, 4},
+                                                   {3, GDB_TBL_CENTS, 0, (void**)&j.ps_cost, 8}}, false);
+    j.nps = nps;
+    j.ns = load_tbl("supplier.tbl", {{0, GDB_TBL_INT32, 0, (void**)&j.s_sk, 4},
+                                     {3, GDB_TBL_INT32, 0, (void**)&j.s_nat, 4}}, false);
+    j.gen_ms = ms_since(t0);
+    return j;
+  }
+  j.nc = s.customers, j.no = s.orders, j.np = s.parts, j.ns = s.suppliers;
+  j.nps = j.np * 4;
   const uint64_t seed = 11;
   j.c_custkey = alloc<int32_t>(j.nc, false);
   j.c_seg = alloc<uint8_t>(j.nc, false);
@@ -264,7 +351,7 @@ void run_q9(const Scale& s, Joined& j) {
   gdb_table* su = make_table({{"s_suppkey", GDB_I32, 0, j.s_sk},
                               {"s_nationkey", GDB_I32, 0, j.s_nat}}, j.ns, false);
   gdb_table* ps = make_table({{"ps_partkey", GDB_I32, 0, j.ps_pk}, {"ps_suppkey", GDB_I32, 0, j.ps_sk},
-                              {"ps_supplycost", GDB_I64, 0, j.ps_cost}}, j.np * 4, false);
+                              {"ps_supplycost", GDB_I64, 0, j.ps_cost}}, j.nps, false);
   gdb_table* o = make_table({{"o_orderkey", GDB_I32, 0, j.o_orderkey},
                              {"o_orderdate", GDB_I32, 0, j.o_orderdate}}, j.no, false);
   gdb_table* l = make_table({{"l_partkey", GDB_I32, 0, j.l_pk}, {"l_suppkey", GDB_I32, 0, j.l_sk},
@@ -354,7 +441,9 @@ void run_join(const Scale&, Joined& j) {
 }
 
 void help() {
-  printf("Usage: gdb_bench [sf1|sf10|sf100] [query]\n");
+  printf("Usage: gdb_bench [sf1|sf10|sf100] [query] [--data DIR]\n");
+  printf("  data: dbgen .tbl files from DIR, or from data/SF-1, data/SF-10, data/SF-100 when\n"
+         "        present; otherwise seeded synthetic tables of the scale's sizes\n");
   printf("  scales: sf1 (6M fact rows), sf10 (60M), sf100 (600M, streamed from pinned host memory)\n");
   printf("  queries: all, q1, q3, q6, q9, q13, selection, aggregation, join\n");
 }
@@ -373,7 +462,15 @@ int main(int argc, char** argv) {
       scale = a;
       continue;
     }
+    if (a == "--data" && i + 1 < argc) {
+      g_data = argv[++i];
+      continue;
+    }
     query = a;
+  }
+  if (g_data.empty()) {  // the reference's dataset directories, if present
+    const std::string d = scale == "sf1" ? "data/SF-1" : scale == "sf10" ? "data/SF-10" : "data/SF-100";
+    if (file_exists(d + "/lineitem.tbl")) g_data = d;
   }
   const std::vector<Scale> scales = {{"sf1", 6000000, 150000, 1500000, 200000, 10000, false},
                                      {"sf10", 60000000, 1500000, 15000000, 2000000, 100000, false},
@@ -404,6 +501,7 @@ int main(int argc, char** argv) {
   CHECK(gdb_device_info_get(g_ctx, &info));
   printf("Device: %s (%s), %d CUs, %.0f GB HBM\n", info.name, info.arch, info.cu_count,
          info.hbm_bytes / 1e9);
+  printf("Data: %s\n", g_data.empty() ? "seeded synthetic tables" : (g_data + "/*.tbl").c_str());
   bool need_flat = false, need_join = false, need_comments = false;
   for (const auto& q : run) {
     need_flat |= std::find(flat_q.begin(), flat_q.end(), q) != flat_q.end();

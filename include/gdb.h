@@ -340,6 +340,35 @@ int gdb_gen_partsupp(uint64_t seed, uint64_t nparts, uint64_t nsuppliers, int32_
                      int32_t* suppkey, int64_t* supplycost, int nthreads);
 int gdb_gen_supplier(uint64_t seed, uint64_t n, int32_t* suppkey, int32_t* nationkey,
                      int nthreads);
+/* ---- TPC-H .tbl ingest (replaces MappedFile, countLines and the
+ *      parse*ColumnChunk loaders, reference src/infra.h:59-171, and the SF100
+ *      chunk parsers, src/infra.h:262-361). Host-side; callers own the arrays.
+ * A field is a 0-based '|'-separated column of every line. Decimals become
+ * exact integers in hundredths (money in cents: "21168.23" -> 2116823;
+ * discount and tax as int32: "0.06" -> 6), dates day numbers from
+ * 1992-01-01, CHAR the first byte, TEXT a fixed-width NUL-padded field
+ * (truncated to `width`). Malformed rows fail with GDB_E_INVALID naming the
+ * row and field. */
+enum {
+  GDB_TBL_INT32 = 1,      /* int32_t (integer-valued decimals such as "17.00" accepted) */
+  GDB_TBL_DATE = 2,       /* int32_t day number, YYYY-MM-DD */
+  GDB_TBL_CENTS = 3,      /* int64_t hundredths */
+  GDB_TBL_HUNDREDTHS = 4, /* int32_t hundredths */
+  GDB_TBL_CHAR = 5,       /* uint8_t first byte */
+  GDB_TBL_TEXT = 6        /* width bytes per row */
+};
+typedef struct {
+  int field;  /* 0-based column of the line */
+  int kind;   /* GDB_TBL_* */
+  int width;  /* GDB_TBL_TEXT only */
+  void* out;  /* rows values */
+} gdb_tbl_field;
+/* Rows of a .tbl file (non-empty lines). */
+int gdb_tbl_count_rows(const char* path, int nthreads, uint64_t* rows);
+/* Parses `rows` rows (as counted) into the fields' arrays, in parallel. */
+int gdb_tbl_parse(const char* path, const gdb_tbl_field* fields, int nfields, uint64_t rows,
+                  int nthreads);
+
 /* Name of nation key 0..24 / market segment code 0..4 (TPC-H). */
 const char* gdb_nation_name(int nationkey);
 const char* gdb_segment_name(int code);
