@@ -66,3 +66,12 @@ def test_cli_reads_tbl_directory(tmp_path):
     top = O.q3(js["customer"], js["orders"], li, 1, 1169, 10)
     keys = [int(x) for x in re.findall(r"^\|\s+(\d+) \|", r.stdout, re.M)]
     assert keys == [row["orderkey"] for row in top["rows"]], r.stdout
+
+
+def test_harness_writes_the_reference_csv(tmp_path):
+    from gpudbmetalbenchmark_amd import harness
+    rows = harness.run("sf1", ["q6"], str(tmp_path), timestamp="t0")
+    assert len(rows) == 1 and rows[0].startswith("t0,SF-1,Q6,")
+    csv = (tmp_path / "gpu_results.csv").read_text().splitlines()
+    assert csv[0] == harness.HEADER and csv[1] == rows[0]
+    assert (tmp_path / "gpu_logs" / "t0" / "SF-1_Q6.log").exists()
