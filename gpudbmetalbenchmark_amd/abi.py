@@ -252,3 +252,218 @@ def check(rc):
 
 def tref(t):
     return C.byref(t) if t is not None else None
+
+
+# ---- include/gdb_tpch.h ------------------------------------------------------
+class TpchDb(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("lineitem", "orders", "customer", "part", "partsupp",
+                                          "supplier", "nation")]
+
+
+M3 = C.c_uint64 * 3
+
+
+class Q2Params(C.Structure):
+    _fields_ = [("size", C.c_int32), ("region", C.c_int32), ("top_k", C.c_int32),
+                ("pad", C.c_int32), ("type_mask", M3)]
+
+
+class Q2Row(C.Structure):
+    _fields_ = [("s_acctbal", C.c_int64), ("suppkey", C.c_int32), ("nationkey", C.c_int32),
+                ("partkey", C.c_int32), ("mfgr", C.c_int32)]
+
+
+class Q2Result(C.Structure):
+    _fields_ = [("nrows", C.c_int32), ("pad", C.c_int32), ("matches", C.c_int64),
+                ("rows", Q2Row * 100)]
+
+
+class Q4Params(C.Structure):
+    _fields_ = [("date_lo", C.c_int32), ("date_hi", C.c_int32)]
+
+
+class Q4Result(C.Structure):
+    _fields_ = [("counts", C.c_int64 * 5)]
+
+
+class Q5Params(C.Structure):
+    _fields_ = [("region", C.c_int32), ("date_lo", C.c_int32), ("date_hi", C.c_int32)]
+
+
+class Q5Result(C.Structure):
+    _fields_ = [("revenue", C.c_int64 * 25), ("has", C.c_uint32)]
+
+
+class Q7Params(C.Structure):
+    _fields_ = [("nation1", C.c_int32), ("nation2", C.c_int32), ("date_lo", C.c_int32),
+                ("date_hi", C.c_int32)]
+
+
+class Q7Result(C.Structure):
+    _fields_ = [("revenue", (C.c_int64 * 8) * 2), ("has", C.c_uint32 * 2)]
+
+
+class Q8Params(C.Structure):
+    _fields_ = [("nation", C.c_int32), ("region", C.c_int32), ("type_code", C.c_int32),
+                ("date_lo", C.c_int32), ("date_hi", C.c_int32)]
+
+
+class Q8Result(C.Structure):
+    _fields_ = [("nation_volume", C.c_int64 * 8), ("total_volume", C.c_int64 * 8),
+                ("has", C.c_uint32)]
+
+
+class Q10Params(C.Structure):
+    _fields_ = [("date_lo", C.c_int32), ("date_hi", C.c_int32), ("top_k", C.c_int32),
+                ("returnflag", C.c_char), ("pad", C.c_char * 3)]
+
+
+class Q10Row(C.Structure):
+    _fields_ = [("custkey", C.c_int32), ("nationkey", C.c_int32), ("revenue", C.c_int64),
+                ("acctbal", C.c_int64)]
+
+
+class Q10Result(C.Structure):
+    _fields_ = [("nrows", C.c_int32), ("pad", C.c_int32), ("groups", C.c_int64),
+                ("rows", Q10Row * 100)]
+
+
+class Q11Params(C.Structure):
+    _fields_ = [("nation", C.c_int32), ("pad", C.c_int32), ("frac_num", C.c_int64),
+                ("frac_den", C.c_int64)]
+
+
+class Q11Row(C.Structure):
+    _fields_ = [("partkey", C.c_int32), ("pad", C.c_int32), ("value", C.c_int64)]
+
+
+class Q12Params(C.Structure):
+    _fields_ = [("mode", C.c_int32 * 2), ("date_lo", C.c_int32), ("date_hi", C.c_int32),
+                ("high_mask", C.c_uint32)]
+
+
+class Q12Result(C.Structure):
+    _fields_ = [("high", C.c_int64 * 2), ("low", C.c_int64 * 2)]
+
+
+class Q14Params(C.Structure):
+    _fields_ = [("date_lo", C.c_int32), ("date_hi", C.c_int32), ("type_mask", M3)]
+
+
+class Q14Result(C.Structure):
+    _fields_ = [("promo", C.c_int64), ("total", C.c_int64), ("promo_revenue", C.c_double)]
+
+
+class Q15Params(C.Structure):
+    _fields_ = [("date_lo", C.c_int32), ("date_hi", C.c_int32)]
+
+
+class Q15Result(C.Structure):
+    _fields_ = [("nrows", C.c_int32), ("pad", C.c_int32), ("max_revenue", C.c_int64),
+                ("suppkey", C.c_int32 * 100)]
+
+
+class Q16Params(C.Structure):
+    _fields_ = [("brand", C.c_int32), ("nsizes", C.c_int32), ("sizes", C.c_int32 * 8),
+                ("type_mask", M3), ("word1", C.c_char * 32), ("word2", C.c_char * 32)]
+
+
+class Q16Row(C.Structure):
+    _fields_ = [("brand", C.c_int32), ("type", C.c_int32), ("size", C.c_int32),
+                ("supplier_cnt", C.c_int32)]
+
+
+class Q17Params(C.Structure):
+    _fields_ = [("brand", C.c_int32), ("container", C.c_int32), ("frac_num", C.c_int32),
+                ("frac_den", C.c_int32)]
+
+
+class Q17Result(C.Structure):
+    _fields_ = [("sum_price", C.c_int64), ("avg_yearly", C.c_double)]
+
+
+class Q18Params(C.Structure):
+    _fields_ = [("quantity", C.c_int32), ("top_k", C.c_int32)]
+
+
+class Q18Row(C.Structure):
+    _fields_ = [("custkey", C.c_int32), ("orderkey", C.c_int32), ("orderdate", C.c_int32),
+                ("sum_qty", C.c_int32), ("totalprice", C.c_int64)]
+
+
+class Q18Result(C.Structure):
+    _fields_ = [("nrows", C.c_int32), ("pad", C.c_int32), ("orders_over", C.c_int64),
+                ("rows", Q18Row * 100)]
+
+
+class Q19Block(C.Structure):
+    _fields_ = [("brand", C.c_int32), ("qty_lo", C.c_int32), ("qty_hi", C.c_int32),
+                ("size_lo", C.c_int32), ("size_hi", C.c_int32), ("pad", C.c_int32),
+                ("container_mask", C.c_uint64)]
+
+
+class Q19Params(C.Structure):
+    _fields_ = [("nblocks", C.c_int32), ("mode_mask", C.c_uint32), ("instruct", C.c_int32),
+                ("pad", C.c_int32), ("blocks", Q19Block * 3)]
+
+
+class Q19Result(C.Structure):
+    _fields_ = [("revenue", C.c_int64)]
+
+
+class Q20Params(C.Structure):
+    _fields_ = [("prefix", C.c_char * 32), ("date_lo", C.c_int32), ("date_hi", C.c_int32),
+                ("nation", C.c_int32), ("frac_num", C.c_int32), ("frac_den", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+class Q21Params(C.Structure):
+    _fields_ = [("nation", C.c_int32), ("top_k", C.c_int32), ("status", C.c_char),
+                ("pad", C.c_char * 7)]
+
+
+class Q21Row(C.Structure):
+    _fields_ = [("suppkey", C.c_int32), ("pad", C.c_int32), ("numwait", C.c_int64)]
+
+
+class Q21Result(C.Structure):
+    _fields_ = [("nrows", C.c_int32), ("pad", C.c_int32), ("suppliers", C.c_int64),
+                ("rows", Q21Row * 100)]
+
+
+class Q22Params(C.Structure):
+    _fields_ = [("ncodes", C.c_int32), ("codes", C.c_int32 * 7)]
+
+
+class Q22Row(C.Structure):
+    _fields_ = [("cntrycode", C.c_int32), ("pad", C.c_int32), ("numcust", C.c_int64),
+                ("totacctbal", C.c_int64)]
+
+
+class Q22Result(C.Structure):
+    _fields_ = [("nrows", C.c_int32), ("pad", C.c_int32), ("rows", Q22Row * 7)]
+
+
+_DB = C.POINTER(TpchDb)
+_T = C.POINTER(Timing)
+PROTOTYPES.update({
+    "gdb_q2": (C.c_int, [P, _DB, C.POINTER(Q2Params), C.POINTER(Q2Result), _T]),
+    "gdb_q4": (C.c_int, [P, _DB, C.POINTER(Q4Params), C.POINTER(Q4Result), _T]),
+    "gdb_q5": (C.c_int, [P, _DB, C.POINTER(Q5Params), C.POINTER(Q5Result), _T]),
+    "gdb_q7": (C.c_int, [P, _DB, C.POINTER(Q7Params), C.POINTER(Q7Result), _T]),
+    "gdb_q8": (C.c_int, [P, _DB, C.POINTER(Q8Params), C.POINTER(Q8Result), _T]),
+    "gdb_q10": (C.c_int, [P, _DB, C.POINTER(Q10Params), C.POINTER(Q10Result), _T]),
+    "gdb_q11": (C.c_int, [P, _DB, C.POINTER(Q11Params), C.POINTER(Q11Row), C.c_int64, I64P,
+                          I64P, _T]),
+    "gdb_q12": (C.c_int, [P, _DB, C.POINTER(Q12Params), C.POINTER(Q12Result), _T]),
+    "gdb_q14": (C.c_int, [P, _DB, C.POINTER(Q14Params), C.POINTER(Q14Result), _T]),
+    "gdb_q15": (C.c_int, [P, _DB, C.POINTER(Q15Params), C.POINTER(Q15Result), _T]),
+    "gdb_q16": (C.c_int, [P, _DB, C.POINTER(Q16Params), C.POINTER(Q16Row), C.c_int64, I64P,
+                          _T]),
+    "gdb_q17": (C.c_int, [P, _DB, C.POINTER(Q17Params), C.POINTER(Q17Result), _T]),
+    "gdb_q18": (C.c_int, [P, _DB, C.POINTER(Q18Params), C.POINTER(Q18Result), _T]),
+    "gdb_q19": (C.c_int, [P, _DB, C.POINTER(Q19Params), C.POINTER(Q19Result), _T]),
+    "gdb_q20": (C.c_int, [P, _DB, C.POINTER(Q20Params), I32P, C.c_int64, I64P, _T]),
+    "gdb_q21": (C.c_int, [P, _DB, C.POINTER(Q21Params), C.POINTER(Q21Result), _T]),
+    "gdb_q22": (C.c_int, [P, _DB, C.POINTER(Q22Params), C.POINTER(Q22Result), _T]),
+})

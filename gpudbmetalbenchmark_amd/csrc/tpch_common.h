@@ -1,0 +1,201 @@
+// Shared host and device pieces of the TPC-H pipelines in tpch_*.hip.
+//
+// Layout and strategy (all pipelines):
+//  - Dimension predicates become direct-address maps over dense integer keys:
+//    bitmaps (1 bit per key, built with atomicOr) for yes/no tests, byte maps
+//    (plain stores) for small payloads such as a nation or a year, int32 /
+//    int64 maps for keys and sums. A map covers [0, max key] of the column it
+//    is built from (the range is recorded when the table is uploaded).
+//  - Fact scans stream their columns with nontemporal dwordx4 loads, 4 rows
+//    per lane per vector group, over ceil(n/4) groups: the last group reads the
+//    zeroed column padding and masks rows >= n. The most selective column is
+//    loaded first and the rest only for groups with a surviving row, so
+//    64-byte lines with no survivor are never fetched.
+//  - Small group tables (<= 64 bins) live in LDS per workgroup and are merged
+//    with one global atomic per non-empty bin.
+//  - Top-k uses the LDS block top-k of topk.h in two stages.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+
+#include "gdb_tpch.h"
+#include "internal.h"
+#include "kcommon.h"
+#include "topk.h"
+
+namespace gdbt {
+
+using gdb::Column;
+
+constexpr int BLOCK = 256;
+
+// ---- host ------------------------------------------------------------------
+
+// Column `name` of table `t` (the `what` table of the query), with type check.
+inline int col(const gdb_table* t, const char* what, const char* name, int type,
+               const Column** out) {
+  if (!t) return gdb::fail(GDB_E_INVALID, std::string("missing the ") + what + " table");
+  GDB_TRY(gdb::column(t, name, type, out));
+  return gdb::require_resident(t, what);
+}
+
+// Entries of a direct-address map over key column c: max key + 1. Keys must be
+// non-negative and below 2^31 - 1.
+inline int key_span(const Column* c, uint64_t nrows, uint64_t* span) {
+  if (nrows == 0 || c->maxv < c->minv) {
+    *span = 1;
+    return GDB_OK;
+  }
+  if (c->minv < 0 || c->maxv >= 0x7FFFFFFELL)
+    return gdb::fail(GDB_E_UNSUPPORTED, "key column " + c->name + " outside [0, 2^31-2]");
+  *span = (uint64_t)c->maxv + 1;
+  return GDB_OK;
+}
+
+// Workgroups for a grid-stride scan of `groups` vector groups.
+inline int scan_blocks(gdb_ctx* ctx, const void* kernel, uint64_t groups, size_t lds = 0) {
+  const int res = gdb::resident_blocks(ctx, kernel, BLOCK, lds, 0);
+  const uint64_t need = (groups + BLOCK - 1) / BLOCK;
+  return (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)res, need));
+}
+
+__host__ __device__ inline uint32_t quads(uint64_t n) { return (uint32_t)((n + 3) / 4); }
+
+// ---- device ------------------------------------------------------------------
+
+__device__ __forceinline__ bool bit_test(const uint32_t* __restrict__ bm, uint32_t k) {
+  return (bm[k >> 5] >> (k & 31)) & 1u;
+}
+__device__ __forceinline__ void bit_set(uint32_t* bm, uint32_t k) {
+  atomicOr(&bm[k >> 5], 1u << (k & 31));
+}
+__device__ __forceinline__ void add_i64(int64_t* p, int64_t v) {
+  atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)v);
+}
+__device__ __forceinline__ void lds_add_i64(int64_t* p, int64_t v) {
+  atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)v);
+}
+__device__ __forceinline__ uint8_t byte_of(uint32_t w, int j) { return (uint8_t)(w >> (8 * j)); }
+__device__ __forceinline__ int lane_of(const int4& v, int j) {
+  return j == 0 ? v.x : j == 1 ? v.y : j == 2 ? v.z : v.w;
+}
+__device__ __forceinline__ int64_t lane_of(const longlong2& a, const longlong2& b, int j) {
+  return j == 0 ? a.x : j == 1 ? a.y : j == 2 ? b.x : b.y;
+}
+
+// Day number (1992-01-01 = 0) -> year - 1992, for days in [0, 2922).
+__device__ __forceinline__ int year_index(int32_t day) {
+  // 1992 and 1996 are leap years; cumulative year starts from day 0.
+  const int starts[8] = {0, 366, 731, 1096, 1461, 1827, 2192, 2557};
+  int y = 0;
+#pragma unroll
+  for (int i = 1; i < 8; ++i) y += day >= starts[i];
+  return y;
+}
+
+// Block sum of `v` added to *out (one atomic per workgroup).
+__device__ __forceinline__ void block_add(int64_t v, int64_t* red, int64_t* out) {
+  const int64_t s = gdbk::block_sum(v, red);
+  if (threadIdx.x == 0 && s) add_i64(out, s);
+}
+
+// Two-stage top-k source over a candidate function: stage 1 runs a grid of
+// workgroups over [0, n), each keeping its k best of a contiguous slice, and
+// counts valid candidates; stage 2 runs one workgroup over the winners.
+template <class Src>
+__global__ __launch_bounds__(256) void topk_stage1(Src src, uint64_t n, int k,
+                                                   gdbk::TopEntry* __restrict__ out,
+                                                   unsigned long long* __restrict__ count) {
+  __shared__ gdbk::TopEntry buf[1024];
+  __shared__ int nbuf;
+  __shared__ int64_t red[4];
+  gdbk::BlockTopK<1024> tk{buf, &nbuf, k};
+  tk.init();
+  const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
+  const uint64_t begin = (uint64_t)blockIdx.x * per;
+  const uint64_t end = begin + per < n ? begin + per : n;
+  int64_t cnt = 0;
+  for (uint64_t base = begin; base < end; base += blockDim.x) {
+    const uint64_t i = base + threadIdx.x;
+    gdbk::TopEntry e = gdbk::worst_entry();
+    const bool v = i < end && src(i, e);
+    cnt += v;
+    tk.push(v, e);
+  }
+  tk.finish(out + (size_t)blockIdx.x * k);
+  const int64_t c = gdbk::block_sum(cnt, red);
+  if (threadIdx.x == 0 && c) atomicAdd(count, (unsigned long long)c);
+}
+
+struct WinnerSrc {
+  const gdbk::TopEntry* w;
+  __device__ bool operator()(uint64_t i, gdbk::TopEntry& e) const {
+    e = w[i];
+    return !(e.rev == INT64_MIN && e.key == INT32_MAX);
+  }
+};
+
+// Runs both stages; the k best (worst-padded) land in `final_out`.
+template <class Src>
+int topk_run(gdb_ctx* ctx, Src src, uint64_t n, int k, gdbk::TopEntry* winners, int blocks,
+             gdbk::TopEntry* final_out, unsigned long long* count, unsigned long long* dummy) {
+  hipStream_t s = ctx->stream;
+  topk_stage1<Src><<<blocks, BLOCK, 0, s>>>(src, n, k, winners, count);
+  GDB_HIP(hipGetLastError());
+  topk_stage1<WinnerSrc><<<1, BLOCK, 0, s>>>(WinnerSrc{winners}, (uint64_t)blocks * k, k,
+                                              final_out, dummy);
+  GDB_HIP(hipGetLastError());
+  return GDB_OK;
+}
+
+// Launch helper: grid-stride over n items, at most the resident workgroups.
+template <typename K>
+int blocks_for(gdb_ctx* ctx, K kernel, uint64_t items) {
+  return scan_blocks(ctx, (const void*)kernel, items);
+}
+
+inline int args_ok(gdb_ctx* ctx, const void* db, const void* p, const void* out) {
+  if (!ctx || !db || !p || !out) return gdb::fail(GDB_E_INVALID, "null argument");
+  if (hipSetDevice(ctx->device) != hipSuccess) return gdb::fail(GDB_E_HIP, "hipSetDevice failed");
+  return GDB_OK;
+}
+
+// A set of dictionary codes 0..255 as a bit mask.
+struct Mask4 {
+  uint64_t w[4];
+  __device__ __forceinline__ bool has(uint32_t c) const { return (w[c >> 6] >> (c & 63)) & 1u; }
+};
+inline Mask4 mask4(const uint64_t* m3) { return Mask4{{m3[0], m3[1], m3[2], 0}}; }
+
+}  // namespace gdbt
+
+namespace {
+
+// bits[key[i]] = mask contains code[i] (part type / container filters).
+__global__ __launch_bounds__(256) void code_bits(const int32_t* __restrict__ key,
+                                                 const uint8_t* __restrict__ code, uint32_t n,
+                                                 gdbt::Mask4 mask, uint32_t* __restrict__ bits) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (mask.has(code[i])) gdbt::bit_set(bits, (uint32_t)key[i]);
+}
+
+// nreg[n_nationkey] = n_regionkey (255 for keys without a row); nkeys < 256.
+__global__ void nation_regions(const int32_t* __restrict__ nk, const int32_t* __restrict__ rk,
+                               uint32_t n, uint8_t* __restrict__ nreg) {
+  for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) nreg[i] = 255;
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+    if (nk[i] >= 0 && nk[i] < 256) nreg[nk[i]] = (uint8_t)(rk[i] >= 0 && rk[i] < 255 ? rk[i] : 254);
+}
+
+// map[key[i]] = val[i] < 255 ? val[i] : 255 (a nation per customer / supplier).
+__global__ __launch_bounds__(256) void nation_bytes(const int32_t* __restrict__ key,
+                                                    const int32_t* __restrict__ val, uint32_t n,
+                                                    uint8_t* __restrict__ map) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    map[key[i]] = (uint8_t)(val[i] >= 0 && val[i] < 255 ? val[i] : 255);
+}
+
+}  // namespace

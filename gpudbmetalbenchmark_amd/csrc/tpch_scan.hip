@@ -1,0 +1,790 @@
+// TPC-H pipelines built around one fact scan: Q14, Q19, Q12, Q4, Q15, Q17, Q22.
+//
+// Each replaces the reference's runQNBenchmark host function and its kernels
+// (reference src/qN.cpp, kernels/qN.metal, cited per pipeline). A dimension
+// table becomes a bitmap or byte map over its dense key; the fact table is
+// streamed once with dwordx4 loads, the most selective column first.
+#include "tpch_common.h"
+
+using namespace gdb;
+using namespace gdbk;
+using namespace gdbt;
+
+namespace {
+
+// ---- Q14 (reference kernels/q14.metal:17-95, src/q14.cpp:8-120) ----------------
+
+__global__ __launch_bounds__(256) void q14_scan(const int32_t* __restrict__ ship,
+                                                const int32_t* __restrict__ pk,
+                                                const int32_t* __restrict__ disc,
+                                                const int64_t* __restrict__ price, uint32_t n,
+                                                int32_t lo, int32_t hi,
+                                                const uint32_t* __restrict__ promo,
+                                                int64_t* __restrict__ out) {
+  __shared__ int64_t red[4];
+  int64_t sp = 0, st = 0;
+  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
+    const int4 s = ld4nt(ship, g);
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int v = lane_of(s, j);
+      m |= (uint32_t)((4 * g + j < n) & (v >= lo) & (v < hi)) << j;
+    }
+    if (!m) continue;
+    const int4 p = ld4nt(pk, g), d = ld4nt(disc, g);
+    const longlong2 a = ld2nt(price, 2 * g), b = ld2nt(price, 2 * g + 1);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (!((m >> j) & 1)) continue;
+      const int64_t rev = lane_of(a, b, j) * (int64_t)(100 - lane_of(d, j));
+      st += rev;
+      if (bit_test(promo, (uint32_t)lane_of(p, j))) sp += rev;
+    }
+  }
+  block_add(sp, red, out);
+  block_add(st, red, out + 1);
+}
+
+// ---- Q19 (reference kernels/q19.metal:22-160, src/q19.cpp) ---------------------
+
+struct Q19Part {
+  int32_t nblocks;
+  int32_t brand[3], size_lo[3], size_hi[3];
+  uint64_t cont[3];
+};
+struct Q19Line {
+  int32_t nblocks;
+  uint32_t modes;
+  int32_t instruct;
+  int32_t qlo[3], qhi[3];
+};
+
+// map[partkey] = bit b set when the part satisfies block b's part predicates.
+__global__ __launch_bounds__(256) void q19_part(const int32_t* __restrict__ pk,
+                                                const uint8_t* __restrict__ brand,
+                                                const uint8_t* __restrict__ cont,
+                                                const int32_t* __restrict__ size, uint32_t n,
+                                                Q19Part q, uint8_t* __restrict__ map) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    uint32_t bits = 0;
+    for (int b = 0; b < q.nblocks; ++b)
+      bits |= (uint32_t)((brand[i] == q.brand[b]) & ((q.cont[b] >> (cont[i] & 63)) & 1) &
+                         (cont[i] < 64) & (size[i] >= q.size_lo[b]) & (size[i] <= q.size_hi[b]))
+              << b;
+    map[pk[i]] = (uint8_t)bits;
+  }
+}
+
+__global__ __launch_bounds__(256) void q19_scan(const uint8_t* __restrict__ mode,
+                                                const uint8_t* __restrict__ instr,
+                                                const int32_t* __restrict__ pk,
+                                                const int32_t* __restrict__ qty,
+                                                const int32_t* __restrict__ disc,
+                                                const int64_t* __restrict__ price, uint32_t n,
+                                                Q19Line q, const uint8_t* __restrict__ map,
+                                                int64_t* __restrict__ out) {
+  __shared__ int64_t red[4];
+  int64_t sum = 0;
+  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
+    const uint32_t mw = ld_u8x4nt(mode, g), iw = ld_u8x4nt(instr, g);
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      m |= (uint32_t)((4 * g + j < n) & ((q.modes >> (byte_of(mw, j) & 31)) & 1) &
+                      (byte_of(mw, j) < 32) & (byte_of(iw, j) == q.instruct))
+           << j;
+    if (!m) continue;
+    const int4 p = ld4nt(pk, g), qq = ld4nt(qty, g);
+    uint32_t m2 = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (!((m >> j) & 1)) continue;
+      const uint32_t pb = map[lane_of(p, j)];
+      const int qv = lane_of(qq, j);
+      bool hit = false;
+      for (int b = 0; b < q.nblocks; ++b)
+        hit |= ((pb >> b) & 1) && qv >= q.qlo[b] && qv <= q.qhi[b];
+      m2 |= (uint32_t)hit << j;
+    }
+    if (!m2) continue;
+    const int4 d = ld4nt(disc, g);
+    const longlong2 a = ld2nt(price, 2 * g), b = ld2nt(price, 2 * g + 1);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if ((m2 >> j) & 1) sum += lane_of(a, b, j) * (int64_t)(100 - lane_of(d, j));
+  }
+  block_add(sum, red, out);
+}
+
+// ---- Q12 (reference kernels/q12.metal:22-100, src/q12.cpp) ---------------------
+
+__global__ __launch_bounds__(256) void order_prio_bits(const int32_t* __restrict__ ok,
+                                                       const uint8_t* __restrict__ prio,
+                                                       uint32_t n, uint32_t mask,
+                                                       uint32_t* __restrict__ bits) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (prio[i] < 32 && ((mask >> prio[i]) & 1)) bit_set(bits, (uint32_t)ok[i]);
+}
+
+__global__ __launch_bounds__(256) void q12_scan(const uint8_t* __restrict__ mode,
+                                                const int32_t* __restrict__ receipt,
+                                                const int32_t* __restrict__ commit,
+                                                const int32_t* __restrict__ ship,
+                                                const int32_t* __restrict__ ok, uint32_t n,
+                                                int32_t m0, int32_t m1, int32_t lo, int32_t hi,
+                                                const uint32_t* __restrict__ high,
+                                                int64_t* __restrict__ out) {
+  __shared__ int64_t red[4];
+  int64_t c[4] = {0, 0, 0, 0};  // [mode index][high]
+  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
+    const uint32_t mw = ld_u8x4nt(mode, g);
+    const int4 r = ld4nt(receipt, g);
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int md = byte_of(mw, j), rv = lane_of(r, j);
+      m |= (uint32_t)((4 * g + j < n) & ((md == m0) | (md == m1)) & (rv >= lo) & (rv < hi)) << j;
+    }
+    if (!m) continue;
+    const int4 cm = ld4nt(commit, g), s = ld4nt(ship, g), o = ld4nt(ok, g);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int cv = lane_of(cm, j);
+      if (!((m >> j) & 1) || !(cv < lane_of(r, j)) || !(lane_of(s, j) < cv)) continue;
+      const int mi = byte_of(mw, j) == m0 ? 0 : 1;
+      const int h = bit_test(high, (uint32_t)lane_of(o, j)) ? 1 : 0;
+      c[mi * 2 + h] += 1;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) block_add(c[i], red, out + i);
+}
+
+// ---- Q4 (reference kernels/q4.metal:21-90, src/q4.cpp) -------------------------
+
+// late[orderkey] = 1 when a line of the order has commitdate < receiptdate.
+__global__ __launch_bounds__(256) void q4_late(const int32_t* __restrict__ ok,
+                                               const int32_t* __restrict__ commit,
+                                               const int32_t* __restrict__ receipt, uint32_t n,
+                                               uint8_t* __restrict__ late) {
+  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
+    const int4 c = ld4nt(commit, g), r = ld4nt(receipt, g), o = ld4nt(ok, g);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (4 * g + j < n && lane_of(c, j) < lane_of(r, j)) late[lane_of(o, j)] = 1;
+  }
+}
+
+__global__ __launch_bounds__(256) void q4_count(const int32_t* __restrict__ date,
+                                                const int32_t* __restrict__ ok,
+                                                const uint8_t* __restrict__ prio, uint32_t n,
+                                                int32_t lo, int32_t hi,
+                                                const uint8_t* __restrict__ late,
+                                                int64_t* __restrict__ out) {
+  __shared__ int64_t red[4];
+  int64_t c[5] = {0, 0, 0, 0, 0};
+  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
+    const int4 d = ld4nt(date, g);
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int v = lane_of(d, j);
+      m |= (uint32_t)((4 * g + j < n) & (v >= lo) & (v < hi)) << j;
+    }
+    if (!m) continue;
+    const int4 o = ld4nt(ok, g);
+    const uint32_t pw = ld_u8x4nt(prio, g);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (!((m >> j) & 1) || !late[lane_of(o, j)]) continue;
+      const uint32_t pr = byte_of(pw, j);
+#pragma unroll
+      for (int b = 0; b < 5; ++b) c[b] += pr == (uint32_t)b;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 5; ++i) block_add(c[i], red, out + i);
+}
+
+// ---- Q15 (reference kernels/q15.metal:13-32, src/q15.cpp) ----------------------
+
+__global__ __launch_bounds__(256) void q15_scan(const int32_t* __restrict__ ship,
+                                                const int32_t* __restrict__ sk,
+                                                const int32_t* __restrict__ disc,
+                                                const int64_t* __restrict__ price, uint32_t n,
+                                                int32_t lo, int32_t hi, int64_t* __restrict__ rev,
+                                                uint8_t* __restrict__ seen) {
+  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
+    const int4 s = ld4nt(ship, g);
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int v = lane_of(s, j);
+      m |= (uint32_t)((4 * g + j < n) & (v >= lo) & (v < hi)) << j;
+    }
+    if (!m) continue;
+    const int4 k = ld4nt(sk, g), d = ld4nt(disc, g);
+    const longlong2 a = ld2nt(price, 2 * g), b = ld2nt(price, 2 * g + 1);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (!((m >> j) & 1)) continue;
+      const uint32_t key = (uint32_t)lane_of(k, j);
+      add_i64(&rev[key], lane_of(a, b, j) * (int64_t)(100 - lane_of(d, j)));
+      seen[key] = 1;
+    }
+  }
+}
+
+// Order-preserving map of int64 onto uint64 (for atomicMax).
+__device__ __forceinline__ unsigned long long ord64(int64_t v) {
+  return (unsigned long long)v ^ 0x8000000000000000ull;
+}
+
+__global__ __launch_bounds__(256) void q15_max(const int64_t* __restrict__ rev,
+                                               const uint8_t* __restrict__ seen, uint32_t n,
+                                               unsigned long long* __restrict__ best) {
+  unsigned long long m = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (seen[i]) m = max(m, ord64(rev[i]));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned long long)__shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0 && m) atomicMax(best, m);
+}
+
+__global__ __launch_bounds__(256) void q15_collect(const int64_t* __restrict__ rev,
+                                                   const uint8_t* __restrict__ seen, uint32_t n,
+                                                   const unsigned long long* __restrict__ best,
+                                                   uint32_t* __restrict__ cnt,
+                                                   int32_t* __restrict__ keys, uint32_t cap) {
+  const unsigned long long b = *best;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (seen[i] && ord64(rev[i]) == b) {
+      const uint32_t pos = atomicAdd(cnt, 1u);
+      if (pos < cap) keys[pos] = (int32_t)i;
+    }
+}
+
+// ---- Q17 (reference kernels/q17.metal:11-57, src/q17.cpp) ----------------------
+
+// bits[partkey] = (p_brand, p_container) = (brand, container).
+__global__ __launch_bounds__(256) void q17_part(const int32_t* __restrict__ pk,
+                                                const uint8_t* __restrict__ brand,
+                                                const uint8_t* __restrict__ cont, uint32_t n,
+                                                int32_t b, int32_t c, uint32_t* __restrict__ bits) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (brand[i] == b && cont[i] == c) bit_set(bits, (uint32_t)pk[i]);
+}
+
+// Pass 1: for lines of selected parts, per-part quantity sum and line count,
+// and the line's row id appended to a list (wave-aggregated append).
+__global__ __launch_bounds__(256) void q17_stats(const int32_t* __restrict__ pk,
+                                                 const int32_t* __restrict__ qty, uint32_t n,
+                                                 const uint32_t* __restrict__ sel,
+                                                 int64_t* __restrict__ sumq,
+                                                 uint32_t* __restrict__ cnt,
+                                                 uint32_t* __restrict__ list,
+                                                 uint32_t* __restrict__ list_n) {
+  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
+  const int lane = threadIdx.x & 63;
+  // every lane of a wave runs the same number of trips (wave-level append)
+  const uint32_t wave_base = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
+  for (uint32_t gw = wave_base; gw < nq; gw += stride) {
+    const uint32_t g = gw + lane;
+    uint32_t m = 0;
+    int4 p = make_int4(0, 0, 0, 0);
+    if (g < nq) {
+      p = ld4nt(pk, g);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        m |= (uint32_t)((4 * g + j < n) && bit_test(sel, (uint32_t)lane_of(p, j))) << j;
+    }
+    const uint32_t c = __popc(m);
+    const uint32_t incl = wave_incl_scan(c);
+    const uint32_t total = __shfl(incl, 63, 64);
+    if (total == 0) continue;
+    uint32_t base = 0;
+    if (lane == 63) base = atomicAdd(list_n, total);
+    base = __shfl(base, 63, 64);
+    uint32_t pos = base + incl - c;
+    if (!m) continue;
+    const int4 q = ld4nt(qty, g);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (!((m >> j) & 1)) continue;
+      const uint32_t key = (uint32_t)lane_of(p, j);
+      add_i64(&sumq[key], lane_of(q, j));
+      atomicAdd(&cnt[key], 1u);
+      list[pos++] = 4 * g + j;
+    }
+  }
+}
+
+// Pass 2 over the listed lines only: quantity * count * den < sum * num.
+__global__ __launch_bounds__(256) void q17_sum(const int32_t* __restrict__ pk,
+                                               const int32_t* __restrict__ qty,
+                                               const int64_t* __restrict__ price,
+                                               const uint32_t* __restrict__ list,
+                                               const uint32_t* __restrict__ list_n,
+                                               const int64_t* __restrict__ sumq,
+                                               const uint32_t* __restrict__ cnt, int32_t num,
+                                               int32_t den, int64_t* __restrict__ out) {
+  __shared__ int64_t red[4];
+  const uint32_t m = *list_n;
+  int64_t s = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+    const uint32_t r = list[i];
+    const uint32_t key = (uint32_t)pk[r];
+    if ((int64_t)qty[r] * cnt[key] * den < sumq[key] * num) s += price[r];
+  }
+  block_add(s, red, out);
+}
+
+// ---- Q22 (reference kernels/q22.metal:11-67, src/q22.cpp) ----------------------
+
+struct Q22Codes {
+  int8_t idx[64];  // country code -> position in the code list, -1 if absent
+};
+
+__global__ __launch_bounds__(256) void mark_bytes(const int32_t* __restrict__ key, uint32_t n,
+                                                  uint8_t* __restrict__ map) {
+  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
+    const int4 k = ld4nt(key, g);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (4 * g + j < n) map[lane_of(k, j)] = 1;
+  }
+}
+
+__device__ __forceinline__ int q22_slot(const Q22Codes& c, int32_t nation) {
+  const int code = nation + 10;
+  return (code >= 0 && code < 64) ? c.idx[code] : -1;
+}
+
+__global__ __launch_bounds__(256) void q22_avg(const int32_t* __restrict__ nat,
+                                               const int64_t* __restrict__ bal, uint32_t n,
+                                               Q22Codes c, int64_t* __restrict__ out) {
+  __shared__ int64_t red[4];
+  int64_t s = 0, k = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (q22_slot(c, nat[i]) >= 0 && bal[i] > 0) {
+      s += bal[i];
+      k += 1;
+    }
+  block_add(s, red, out);
+  block_add(k, red, out + 1);
+}
+
+__global__ __launch_bounds__(256) void q22_final(const int32_t* __restrict__ ck,
+                                                 const int32_t* __restrict__ nat,
+                                                 const int64_t* __restrict__ bal, uint32_t n,
+                                                 Q22Codes c, const uint8_t* __restrict__ has,
+                                                 uint32_t has_n, int64_t* __restrict__ out) {
+  __shared__ int64_t bins[14];
+  if (threadIdx.x < 14) bins[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t sum = out[0], cnt = out[1];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int slot = q22_slot(c, nat[i]);
+    if (slot < 0 || cnt == 0) continue;
+    const int64_t b = bal[i];
+    const uint32_t key = (uint32_t)ck[i];
+    if (b * cnt > sum && !(key < has_n && has[key])) {
+      lds_add_i64(&bins[2 * slot], 1);
+      lds_add_i64(&bins[2 * slot + 1], b);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 14 && bins[threadIdx.x]) add_i64(&out[2 + threadIdx.x], bins[threadIdx.x]);
+}
+
+}  // namespace
+
+extern "C" {
+
+int gdb_q14(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q14_params* p, gdb_q14_result* out,
+            gdb_timing* timing) {
+  GDB_TRY(args_ok(ctx, db, p, out));
+  const Column *ls, *lp, *ld, *le, *pp, *pt;
+  GDB_TRY(col(db->lineitem, "lineitem", "l_shipdate", GDB_I32, &ls));
+  GDB_TRY(col(db->lineitem, "lineitem", "l_partkey", GDB_I32, &lp));
+  GDB_TRY(col(db->lineitem, "lineitem", "l_discount", GDB_I32, &ld));
+  GDB_TRY(col(db->lineitem, "lineitem", "l_extendedprice", GDB_I64, &le));
+  GDB_TRY(col(db->part, "part", "p_partkey", GDB_I32, &pp));
+  GDB_TRY(col(db->part, "part", "p_type", GDB_U8, &pt));
+  const uint64_t nl = db->lineitem->nrows, np = db->part->nrows;
+  uint64_t s1, s2;
+  GDB_TRY(key_span(pp, np, &s1));
+  GDB_TRY(key_span(lp, nl, &s2));
+  const uint64_t words = (std::max(s1, s2) + 31) / 32;
+  uint32_t* bits;
+  int64_t *dout, *host;
+  GDB_TRY(arena(ctx, A_BITMAP, words * 4, (void**)&bits));
+  GDB_TRY(arena(ctx, A_OUT, 64, (void**)&dout));
+  GDB_TRY(pinned(ctx, 64, (void**)&host));
+  hipStream_t s = ctx->stream;
+  GDB_HIP(hipEventRecord(ctx->ev[0], s));
+  GDB_HIP(hipMemsetAsync(bits, 0, words * 4, s));
+  GDB_HIP(hipMemsetAsync(dout, 0, 16, s));
+  if (np)
+    code_bits<<<blocks_for(ctx, code_bits, np), BLOCK, 0, s>>>(
+        (const int32_t*)pp->dev, (const uint8_t*)pt->dev, (uint32_t)np, mask4(p->type_mask), bits);
+  GDB_HIP(hipEventRecord(ctx->ev[1], s));
+  if (nl)
+    q14_scan<<<blocks_for(ctx, q14_scan, quads(nl)), BLOCK, 0, s>>>(
+        (const int32_t*)ls->dev, (const int32_t*)lp->dev, (const int32_t*)ld->dev,
+        (const int64_t*)le->dev, (uint32_t)nl, p->date_lo, p->date_hi, bits, dout);
+  GDB_HIP(hipGetLastError());
+  GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  GDB_HIP(hipMemcpyAsync(host, dout, 16, hipMemcpyDeviceToHost, s));
+  GDB_HIP(hipEventRecord(ctx->ev[5], s));
+  // SQL columns: l_shipdate, l_partkey, l_discount (4 B), l_extendedprice (8 B);
+  // p_partkey (4 B), p_type (1 B). The scan must read l_shipdate of every row.
+  GDB_TRY(timing_finish(ctx, timing, 20 * nl + 5 * np, 4 * nl));
+  out->promo = host[0];
+  out->total = host[1];
+  out->promo_revenue = host[1] ? 100.0 * (double)host[0] / (double)host[1] : 0.0;
+  return GDB_OK;
+}
+
+int gdb_q19(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q19_params* p, gdb_q19_result* out,
+            gdb_timing* timing) {
+  GDB_TRY(args_ok(ctx, db, p, out));
+  if (p->nblocks < 0 || p->nblocks > 3) return fail(GDB_E_INVALID, "Q19 takes 0..3 blocks");
+  const Column *lm, *li, *lp, *lq, *ld, *le, *pp, *pb, *pc, *ps;
+  GDB_TRY(col(db->lineitem, "lineitem", "l_shipmode", GDB_U8, &lm));
+  GDB_TRY(col(db->lineitem, "lineitem", "l_shipinstruct", GDB_U8, &li));
+  GDB_TRY(col(db->lineitem, "lineitem", "l_partkey", GDB_I32, &lp));
+  GDB_TRY(col(db->lineitem, "lineitem", "l_quantity", GDB_I32, &lq));
+  GDB_TRY(col(db->lineitem, "lineitem", "l_discount", GDB_I32, &ld));
+  GDB_TRY(col(db->lineitem, "lineitem", "l_extendedprice", GDB_I64, &le));
+  GDB_TRY(col(db->part, "part", "p_partkey", GDB_I32, &pp));
+  GDB_TRY(col(db->part, "part", "p_brand", GDB_U8, &pb));
+  GDB_TRY(col(db->part, "part", "p_container", GDB_U8, &pc));
+  GDB_TRY(col(db->part, "part", "p_size", GDB_I32, &ps));
+  const uint64_t nl = db->lineitem->nrows, np = db->part->nrows;
+  uint64_t s1, s2;
+  GDB_TRY(key_span(pp, np, &s1));
+  GDB_TRY(key_span(lp, nl, &s2));
+  const uint64_t span = std::max(s1, s2);
+  uint8_t* map;
+  int64_t *dout, *host;
+  GDB_TRY(arena(ctx, A_MAP, span, (void**)&map));
+  GDB_TRY(arena(ctx, A_OUT, 64, (void**)&dout));
+  GDB_TRY(pinned(ctx, 64, (void**)&host));
+  Q19Part qp{};
+  Q19Line ql{};
+  qp.nblocks = ql.nblocks = p->nblocks;
+  ql.modes = p->mode_mask;
+  ql.instruct = p->instruct;
+  for (int b = 0; b < p->nblocks; ++b) {
+    qp.brand[b] = p->blocks[b].brand;
+    qp.cont[b] = p->blocks[b].container_mask;
+    qp.size_lo[b] = p->blocks[b].size_lo;
+    qp.size_hi[b] = p->blocks[b].size_hi;
+    ql.qlo[b] = p->blocks[b].qty_lo;
+    ql.qhi[b] = p->blocks[b].qty_hi;
+  }
+  hipStream_t s = ctx->stream;
+  GDB_HIP(hipEventRecord(ctx->ev[0], s));
+  GDB_HIP(hipMemsetAsync(map, 0, span, s));
+  GDB_HIP(hipMemsetAsync(dout, 0, 8, s));
+  if (np)
+    q19_part<<<blocks_for(ctx, q19_part, np), BLOCK, 0, s>>>(
+        (const int32_t*)pp->dev, (const uint8_t*)pb->dev, (const uint8_t*)pc->dev,
+        (const int32_t*)ps->dev, (uint32_t)np, qp, map);
+  GDB_HIP(hipEventRecord(ctx->ev[1], s));
+  if (nl && p->nblocks)
+    q19_scan<<<blocks_for(ctx, q19_scan, quads(nl)), BLOCK, 0, s>>>(
+        (const uint8_t*)lm->dev, (const uint8_t*)li->dev, (const int32_t*)lp->dev,
+        (const int32_t*)lq->dev, (const int32_t*)ld->dev, (const int64_t*)le->dev,
+        (uint32_t)nl, ql, map, dout);
+  GDB_HIP(hipGetLastError());
+  GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  GDB_HIP(hipMemcpyAsync(host, dout, 8, hipMemcpyDeviceToHost, s));
+  GDB_HIP(hipEventRecord(ctx->ev[5], s));
+  // SQL columns: shipmode, shipinstruct (1 B), partkey, quantity, discount (4 B),
+  // extendedprice (8 B); part key, brand, container, size. Mandatory: 2 B per line.
+  GDB_TRY(timing_finish(ctx, timing, 22 * nl + 10 * np, 2 * nl));
+  out->revenue = host[0];
+  return GDB_OK;
+}
+
+int gdb_q12(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q12_params* p, gdb_q12_result* out,
+            gdb_timing* timing) {
+  GDB_TRY(args_ok(ctx, db, p, out));
+  const Column *lm, *lr, *lc, *ls, *lo, *oo, *op;
+  GDB_TRY(col(db->lineitem, "lineitem", "l_shipmode", GDB_U8, &lm));
+  GDB_TRY(col(db->lineitem, "lineitem", "l_receiptdate", GDB_I32, &lr));
+  GDB_TRY(col(db->lineitem, "lineitem", "l_commitdate", GDB_I32, &lc));
+  GDB_TRY(col(db->lineitem, "lineitem", "l_shipdate", GDB_I32, &ls));
+  GDB_TRY(col(db->lineitem, "lineitem", "l_orderkey", GDB_I32, &lo));
+  GDB_TRY(col(db->orders, "orders", "o_orderkey", GDB_I32, &oo));
+  GDB_TRY(col(db->orders, "orders", "o_orderpriority", GDB_U8, &op));
+  const uint64_t nl = db->lineitem->nrows, no = db->orders->nrows;
+  uint64_t s1, s2;
+  GDB_TRY(key_span(oo, no, &s1));
+  GDB_TRY(key_span(lo, nl, &s2));
+  const uint64_t words = (std::max(s1, s2) + 31) / 32;
+  uint32_t* bits;
+  int64_t *dout, *host;
+  GDB_TRY(arena(ctx, A_BITMAP, words * 4, (void**)&bits));
+  GDB_TRY(arena(ctx, A_OUT, 64, (void**)&dout));
+  GDB_TRY(pinned(ctx, 64, (void**)&host));
+  hipStream_t s = ctx->stream;
+  GDB_HIP(hipEventRecord(ctx->ev[0], s));
+  GDB_HIP(hipMemsetAsync(bits, 0, words * 4, s));
+  GDB_HIP(hipMemsetAsync(dout, 0, 32, s));
+  if (no)
+    order_prio_bits<<<blocks_for(ctx, order_prio_bits, no), BLOCK, 0, s>>>(
+        (const int32_t*)oo->dev, (const uint8_t*)op->dev, (uint32_t)no, p->high_mask, bits);
+  GDB_HIP(hipEventRecord(ctx->ev[1], s));
+  if (nl)
+    q12_scan<<<blocks_for(ctx, q12_scan, quads(nl)), BLOCK, 0, s>>>(
+        (const uint8_t*)lm->dev, (const int32_t*)lr->dev, (const int32_t*)lc->dev,
+        (const int32_t*)ls->dev, (const int32_t*)lo->dev, (uint32_t)nl, p->mode[0], p->mode[1],
+        p->date_lo, p->date_hi, bits, dout);
+  GDB_HIP(hipGetLastError());
+  GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  GDB_HIP(hipMemcpyAsync(host, dout, 32, hipMemcpyDeviceToHost, s));
+  GDB_HIP(hipEventRecord(ctx->ev[5], s));
+  // SQL columns: shipmode (1 B), receipt, commit, ship, orderkey (4 B); o_orderkey,
+  // o_orderpriority. Mandatory: shipmode and receiptdate, 5 B per line.
+  GDB_TRY(timing_finish(ctx, timing, 17 * nl + 5 * no, 5 * nl));
+  out->low[0] = host[0];
+  out->high[0] = host[1];
+  out->low[1] = host[2];
+  out->high[1] = host[3];
+  return GDB_OK;
+}
+
+int gdb_q4(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q4_params* p, gdb_q4_result* out,
+           gdb_timing* timing) {
+  GDB_TRY(args_ok(ctx, db, p, out));
+  const Column *lo, *lc, *lr, *oo, *od, *op;
+  GDB_TRY(col(db->lineitem, "lineitem", "l_orderkey", GDB_I32, &lo));
+  GDB_TRY(col(db->lineitem, "lineitem", "l_commitdate", GDB_I32, &lc));
+  GDB_TRY(col(db->lineitem, "lineitem", "l_receiptdate", GDB_I32, &lr));
+  GDB_TRY(col(db->orders, "orders", "o_orderkey", GDB_I32, &oo));
+  GDB_TRY(col(db->orders, "orders", "o_orderdate", GDB_I32, &od));
+  GDB_TRY(col(db->orders, "orders", "o_orderpriority", GDB_U8, &op));
+  const uint64_t nl = db->lineitem->nrows, no = db->orders->nrows;
+  uint64_t s1, s2;
+  GDB_TRY(key_span(oo, no, &s1));
+  GDB_TRY(key_span(lo, nl, &s2));
+  const uint64_t span = std::max(s1, s2);
+  uint8_t* late;
+  int64_t *dout, *host;
+  GDB_TRY(arena(ctx, A_MAP, span, (void**)&late));
+  GDB_TRY(arena(ctx, A_OUT, 64, (void**)&dout));
+  GDB_TRY(pinned(ctx, 64, (void**)&host));
+  hipStream_t s = ctx->stream;
+  GDB_HIP(hipEventRecord(ctx->ev[0], s));
+  GDB_HIP(hipMemsetAsync(late, 0, span, s));
+  GDB_HIP(hipMemsetAsync(dout, 0, 40, s));
+  GDB_HIP(hipEventRecord(ctx->ev[1], s));
+  if (nl)
+    q4_late<<<blocks_for(ctx, q4_late, quads(nl)), BLOCK, 0, s>>>(
+        (const int32_t*)lo->dev, (const int32_t*)lc->dev, (const int32_t*)lr->dev, (uint32_t)nl,
+        late);
+  GDB_HIP(hipGetLastError());
+  GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  if (no)
+    q4_count<<<blocks_for(ctx, q4_count, quads(no)), BLOCK, 0, s>>>(
+        (const int32_t*)od->dev, (const int32_t*)oo->dev, (const uint8_t*)op->dev, (uint32_t)no,
+        p->date_lo, p->date_hi, late, dout);
+  GDB_HIP(hipGetLastError());
+  GDB_HIP(hipMemcpyAsync(host, dout, 40, hipMemcpyDeviceToHost, s));
+  GDB_HIP(hipEventRecord(ctx->ev[5], s));
+  // SQL columns: l_orderkey, l_commitdate, l_receiptdate (12 B per line, all read by
+  // the dominant q4_late scan); o_orderkey, o_orderdate (4 B), o_orderpriority (1 B).
+  GDB_TRY(timing_finish(ctx, timing, 12 * nl + 9 * no, 12 * nl));
+  for (int i = 0; i < 5; ++i) out->counts[i] = host[i];
+  return GDB_OK;
+}
+
+int gdb_q15(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q15_params* p, gdb_q15_result* out,
+            gdb_timing* timing) {
+  GDB_TRY(args_ok(ctx, db, p, out));
+  const Column *ls, *lk, *ld, *le;
+  GDB_TRY(col(db->lineitem, "lineitem", "l_shipdate", GDB_I32, &ls));
+  GDB_TRY(col(db->lineitem, "lineitem", "l_suppkey", GDB_I32, &lk));
+  GDB_TRY(col(db->lineitem, "lineitem", "l_discount", GDB_I32, &ld));
+  GDB_TRY(col(db->lineitem, "lineitem", "l_extendedprice", GDB_I64, &le));
+  const uint64_t nl = db->lineitem->nrows;
+  uint64_t span;
+  GDB_TRY(key_span(lk, nl, &span));
+  int64_t* rev;
+  uint8_t* seen;
+  uint64_t* dout;
+  int64_t* host;
+  GDB_TRY(arena(ctx, A_AGG, span * 8, (void**)&rev));
+  GDB_TRY(arena(ctx, A_MAP, span, (void**)&seen));
+  GDB_TRY(arena(ctx, A_OUT, 64 + 4 * 100, (void**)&dout));
+  GDB_TRY(pinned(ctx, 64 + 4 * 100, (void**)&host));
+  unsigned long long* best = (unsigned long long*)dout;
+  uint32_t* cnt = (uint32_t*)(dout + 1);
+  int32_t* keys = (int32_t*)(dout + 2);
+  hipStream_t s = ctx->stream;
+  GDB_HIP(hipEventRecord(ctx->ev[0], s));
+  GDB_HIP(hipMemsetAsync(rev, 0, span * 8, s));
+  GDB_HIP(hipMemsetAsync(seen, 0, span, s));
+  GDB_HIP(hipMemsetAsync(dout, 0, 16, s));
+  GDB_HIP(hipEventRecord(ctx->ev[1], s));
+  if (nl)
+    q15_scan<<<blocks_for(ctx, q15_scan, quads(nl)), BLOCK, 0, s>>>(
+        (const int32_t*)ls->dev, (const int32_t*)lk->dev, (const int32_t*)ld->dev,
+        (const int64_t*)le->dev, (uint32_t)nl, p->date_lo, p->date_hi, rev, seen);
+  GDB_HIP(hipGetLastError());
+  GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  q15_max<<<blocks_for(ctx, q15_max, span), BLOCK, 0, s>>>(rev, seen, (uint32_t)span, best);
+  q15_collect<<<blocks_for(ctx, q15_collect, span), BLOCK, 0, s>>>(rev, seen, (uint32_t)span,
+                                                                   best, cnt, keys, 100);
+  GDB_HIP(hipGetLastError());
+  GDB_HIP(hipMemcpyAsync(host, dout, 16 + 4 * 100, hipMemcpyDeviceToHost, s));
+  GDB_HIP(hipEventRecord(ctx->ev[5], s));
+  // SQL columns: l_shipdate, l_suppkey, l_discount (4 B), l_extendedprice (8 B).
+  GDB_TRY(timing_finish(ctx, timing, 20 * nl, 4 * nl));
+  const uint64_t b = (uint64_t)host[0];
+  const uint32_t n = (uint32_t)((const uint64_t*)host)[1];
+  out->nrows = (int32_t)std::min<uint32_t>(n, 100);
+  out->max_revenue = b ? (int64_t)(b ^ 0x8000000000000000ull) : 0;
+  const int32_t* hk = (const int32_t*)(host + 2);
+  for (int i = 0; i < out->nrows; ++i) out->suppkey[i] = hk[i];
+  std::sort(out->suppkey, out->suppkey + out->nrows);
+  return GDB_OK;
+}
+
+int gdb_q17(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q17_params* p, gdb_q17_result* out,
+            gdb_timing* timing) {
+  GDB_TRY(args_ok(ctx, db, p, out));
+  if (p->frac_den <= 0 || p->frac_num < 0) return fail(GDB_E_INVALID, "Q17 fraction must be >= 0");
+  const Column *lp, *lq, *le, *pp, *pb, *pc;
+  GDB_TRY(col(db->lineitem, "lineitem", "l_partkey", GDB_I32, &lp));
+  GDB_TRY(col(db->lineitem, "lineitem", "l_quantity", GDB_I32, &lq));
+  GDB_TRY(col(db->lineitem, "lineitem", "l_extendedprice", GDB_I64, &le));
+  GDB_TRY(col(db->part, "part", "p_partkey", GDB_I32, &pp));
+  GDB_TRY(col(db->part, "part", "p_brand", GDB_U8, &pb));
+  GDB_TRY(col(db->part, "part", "p_container", GDB_U8, &pc));
+  const uint64_t nl = db->lineitem->nrows, np = db->part->nrows;
+  uint64_t s1, s2;
+  GDB_TRY(key_span(pp, np, &s1));
+  GDB_TRY(key_span(lp, nl, &s2));
+  const uint64_t span = std::max(s1, s2), words = (span + 31) / 32;
+  uint32_t *bits, *cnt, *list;
+  int64_t *sumq, *dout, *host;
+  GDB_TRY(arena(ctx, A_BITMAP, words * 4, (void**)&bits));
+  GDB_TRY(arena(ctx, A_AGG, span * 8, (void**)&sumq));
+  GDB_TRY(arena(ctx, A_MAP, span * 4, (void**)&cnt));
+  GDB_TRY(arena(ctx, A_AUX, std::max<uint64_t>(nl, 1) * 4, (void**)&list));
+  GDB_TRY(arena(ctx, A_OUT, 64, (void**)&dout));
+  GDB_TRY(pinned(ctx, 64, (void**)&host));
+  uint32_t* list_n = (uint32_t*)(dout + 1);
+  hipStream_t s = ctx->stream;
+  GDB_HIP(hipEventRecord(ctx->ev[0], s));
+  GDB_HIP(hipMemsetAsync(bits, 0, words * 4, s));
+  GDB_HIP(hipMemsetAsync(sumq, 0, span * 8, s));
+  GDB_HIP(hipMemsetAsync(cnt, 0, span * 4, s));
+  GDB_HIP(hipMemsetAsync(dout, 0, 16, s));
+  if (np)
+    q17_part<<<blocks_for(ctx, q17_part, np), BLOCK, 0, s>>>(
+        (const int32_t*)pp->dev, (const uint8_t*)pb->dev, (const uint8_t*)pc->dev, (uint32_t)np,
+        p->brand, p->container, bits);
+  GDB_HIP(hipGetLastError());
+  GDB_HIP(hipEventRecord(ctx->ev[1], s));
+  if (nl)
+    q17_stats<<<blocks_for(ctx, q17_stats, quads(nl)), BLOCK, 0, s>>>(
+        (const int32_t*)lp->dev, (const int32_t*)lq->dev, (uint32_t)nl, bits, sumq, cnt, list,
+        list_n);
+  GDB_HIP(hipGetLastError());
+  GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  if (nl)
+    q17_sum<<<blocks_for(ctx, q17_sum, nl / 64 + 1), BLOCK, 0, s>>>(
+        (const int32_t*)lp->dev, (const int32_t*)lq->dev, (const int64_t*)le->dev, list, list_n,
+        sumq, cnt, p->frac_num, p->frac_den, dout);
+  GDB_HIP(hipGetLastError());
+  GDB_HIP(hipMemcpyAsync(host, dout, 16, hipMemcpyDeviceToHost, s));
+  GDB_HIP(hipEventRecord(ctx->ev[5], s));
+  // SQL columns: l_partkey, l_quantity (4 B), l_extendedprice (8 B); p_partkey,
+  // p_brand, p_container. Mandatory: l_partkey, 4 B per line.
+  GDB_TRY(timing_finish(ctx, timing, 16 * nl + 6 * np, 4 * nl));
+  out->sum_price = host[0];
+  out->avg_yearly = (double)host[0] / 100.0 / 7.0;
+  return GDB_OK;
+}
+
+int gdb_q22(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q22_params* p, gdb_q22_result* out,
+            gdb_timing* timing) {
+  GDB_TRY(args_ok(ctx, db, p, out));
+  if (p->ncodes < 0 || p->ncodes > 7) return fail(GDB_E_INVALID, "Q22 takes 0..7 country codes");
+  const Column *ck, *cn, *cb, *oc;
+  GDB_TRY(col(db->customer, "customer", "c_custkey", GDB_I32, &ck));
+  GDB_TRY(col(db->customer, "customer", "c_nationkey", GDB_I32, &cn));
+  GDB_TRY(col(db->customer, "customer", "c_acctbal", GDB_I64, &cb));
+  GDB_TRY(col(db->orders, "orders", "o_custkey", GDB_I32, &oc));
+  const uint64_t nc = db->customer->nrows, no = db->orders->nrows;
+  uint64_t s1, s2;
+  GDB_TRY(key_span(ck, nc, &s1));
+  GDB_TRY(key_span(oc, no, &s2));
+  const uint64_t span = std::max(s1, s2);
+  Q22Codes codes;
+  for (int i = 0; i < 64; ++i) codes.idx[i] = -1;
+  for (int i = 0; i < p->ncodes; ++i) {
+    const int c = p->codes[i];
+    if (c < 0 || c >= 64) return fail(GDB_E_INVALID, "Q22 country codes must lie in 0..63");
+    if (codes.idx[c] < 0) codes.idx[c] = (int8_t)i;
+  }
+  uint8_t* has;
+  int64_t *dout, *host;
+  GDB_TRY(arena(ctx, A_MAP, span, (void**)&has));
+  GDB_TRY(arena(ctx, A_OUT, 256, (void**)&dout));
+  GDB_TRY(pinned(ctx, 256, (void**)&host));
+  hipStream_t s = ctx->stream;
+  GDB_HIP(hipEventRecord(ctx->ev[0], s));
+  GDB_HIP(hipMemsetAsync(has, 0, span, s));
+  GDB_HIP(hipMemsetAsync(dout, 0, 16 * 8, s));
+  GDB_HIP(hipEventRecord(ctx->ev[1], s));
+  if (no)
+    mark_bytes<<<blocks_for(ctx, mark_bytes, quads(no)), BLOCK, 0, s>>>((const int32_t*)oc->dev,
+                                                                         (uint32_t)no, has);
+  GDB_HIP(hipGetLastError());
+  GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  if (nc) {
+    q22_avg<<<blocks_for(ctx, q22_avg, nc), BLOCK, 0, s>>>(
+        (const int32_t*)cn->dev, (const int64_t*)cb->dev, (uint32_t)nc, codes, dout);
+    q22_final<<<blocks_for(ctx, q22_final, nc), BLOCK, 0, s>>>(
+        (const int32_t*)ck->dev, (const int32_t*)cn->dev, (const int64_t*)cb->dev, (uint32_t)nc,
+        codes, has, (uint32_t)span, dout);
+  }
+  GDB_HIP(hipGetLastError());
+  GDB_HIP(hipMemcpyAsync(host, dout, 16 * 8, hipMemcpyDeviceToHost, s));
+  GDB_HIP(hipEventRecord(ctx->ev[5], s));
+  // SQL columns: c_custkey, c_nationkey (4 B), c_acctbal (8 B); o_custkey (4 B, the
+  // dominant mark_bytes scan).
+  GDB_TRY(timing_finish(ctx, timing, 16 * nc + 4 * no, 4 * no));
+  // rows ordered by country code
+  int order[7];
+  for (int i = 0; i < p->ncodes; ++i) order[i] = i;
+  std::sort(order, order + p->ncodes, [&](int a, int b) { return p->codes[a] < p->codes[b]; });
+  out->nrows = 0;
+  for (int t = 0; t < p->ncodes; ++t) {
+    const int i = order[t];
+    if (codes.idx[p->codes[i]] != i) continue;  // duplicate code
+    if (host[2 + 2 * i] == 0) continue;
+    gdb_q22_row& r = out->rows[out->nrows++];
+    r.cntrycode = p->codes[i];
+    r.pad = 0;
+    r.numcust = host[2 + 2 * i];
+    r.totacctbal = host[3 + 2 * i];
+  }
+  return GDB_OK;
+}
+
+}  // extern "C"
