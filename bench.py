@@ -7,7 +7,8 @@ whole-pipeline time (every kernel, the finaliser and the result readback),
 with the inputs resident in HBM when the timed region starts.
 
   python bench.py [--steps K] [--warmup W]
-                  [--pipeline q1|q6|q3|q9|q13|select|sum|join|all|serve] [--scale sf10|sf1|sf100]
+                  [--pipeline q1|q6|q3|q9|q13|select|sum|join|all|serve|tpch|q2,q4,...]
+                  [--scale sf10|sf1|sf100]
                   [--no-tune] [--no-cpu] [--probe]
 
 Before the warm-up steps, the engine's autotuner (gdb_autotune) times its
@@ -17,6 +18,10 @@ defaults; streamed sf100 runs are PCIe-bound and are not tuned).
 
 --scale sf100 streams the fact table (orders for Q13) from pinned host memory in
 chunks (the 600M-row variant); its `value` is then host-to-result throughput.
+--pipeline tpch runs the other 17 TPC-H pipelines (Q2, Q4, Q5, Q7, Q8, Q10,
+Q11, Q12, Q14-Q22) with the reference SQL's parameters over a seeded
+TPC-H-shaped database (SF10: 60M lineitem rows; --scale sf1 for 6M); their
+CPU baseline is the numpy oracle (oracle/tpch_oracle.py) on one thread.
 Prints one JSON line per pipeline (one for the default run). `roofline` covers
 the pipeline's dominant kernel, timed with hipEvents on the engine stream inside
 every call; `cpu_baseline` times the CPU oracle (oracle/oracle.c, all host
@@ -62,10 +67,46 @@ class Workload:
     """setup() uploads tables; step(timing) runs the pipeline once; oracle()
     runs the CPU restatement once on the same data."""
 
-    def __init__(self, name, kernel, workload, rows, step, oracle, oracle_bytes, tune=None):
+    def __init__(self, name, kernel, workload, rows, step, oracle, oracle_bytes, tune=None,
+                 oracle_desc=None, oracle_threads=None):
         self.name, self.kernel, self.workload, self.rows = name, kernel, workload, rows
         self.step, self.oracle, self.oracle_bytes = step, oracle, oracle_bytes
         self.tune = tune  # runs the autotuner on these tables, keeps the best shape
+        self.oracle_desc = oracle_desc      # None: oracle/oracle.c over all host threads
+        self.oracle_threads = oracle_threads
+
+
+# TPC-H pipelines over the generated TPC-H-shaped database: name -> dominant
+# kernel (the one the engine brackets with its kernel events).
+TPCH_KERNELS = {"q2": "q2_min", "q4": "q4_late", "q5": "q5_scan", "q7": "q7_scan",
+                "q8": "q8_scan", "q10": "q10_scan", "q11": "q11_scan", "q12": "q12_scan",
+                "q14": "q14_scan", "q15": "q15_scan", "q16": "q16_scan", "q17": "q17_stats",
+                "q18": "q18_sums", "q19": "q19_scan", "q20": "q20_scan", "q21": "q21_track",
+                "q22": "mark_bytes"}
+TPCH_SF = {"sf1": 1.0, "sf10": 10.0}
+
+
+def build_tpch_workloads(eng, names, scale):
+    from gpudbmetalbenchmark_amd import tpch
+    from gpudbmetalbenchmark_amd.tpchdb import TpchDb
+    import tpch_oracle as T
+    sf = TPCH_SF[scale]
+    tabs = tpch.generate(sf, 11)
+    db = TpchDb(eng, tabs)
+    nl = len(tabs["lineitem"]["l_orderkey"])
+    brass = tpch.codes_like(tpch.P_TYPE, "%BRASS")
+    medpol = tpch.codes_like(tpch.P_TYPE, "MEDIUM POLISHED%")
+    oracle = {"q2": lambda: T.q2(tabs, 15, brass, 3), "q16": lambda: T.q16(tabs, 45, medpol)}
+    out = {}
+    for n in names:
+        out[n] = Workload(
+            n, TPCH_KERNELS[n],
+            f"TPC-H {n.upper()} with the reference SQL's parameters over a seeded TPC-H-shaped "
+            f"database at SF{sf:g} ({nl} lineitem rows)",
+            nl, lambda tm, f=getattr(db, n): f(timing=tm),
+            oracle.get(n, lambda f=getattr(T, n): f(tabs)), None,
+            oracle_desc="oracle/tpch_oracle.py (numpy, one thread)", oracle_threads=1)
+    return out
 
 
 def build_workloads(eng, names, scale, O):
@@ -192,6 +233,8 @@ def run_one(w, args, info, O):
         h2d = tm.h2d_bytes
     elapsed = time.perf_counter() - t0
     bytes_step = tm.bytes
+    if w.oracle_bytes is None:
+        w.oracle_bytes = bytes_step
     value = bytes_step * args.steps / elapsed / 1e9
     kavg = float(np.mean(kernel_ms))
     achieved = tm.main_kernel_bytes / (kavg / 1000.0) / 1e9 if kavg > 0 else 0.0
@@ -228,7 +271,8 @@ def run_one(w, args, info, O):
 def cpu_baseline(w, seconds):
     """The oracle over the same tables, all host threads (at most 16), repeated
     for about `seconds`; reported in the metric's unit (GB/s)."""
-    threads = cpu_threads()
+    threads = w.oracle_threads or cpu_threads()
+    desc = w.oracle_desc or f"oracle/oracle.c, {threads} threads"
     w.oracle()
     reps, t0 = 0, time.perf_counter()
     while True:
@@ -239,8 +283,7 @@ def cpu_baseline(w, seconds):
             break
     return {"value": round(w.oracle_bytes * reps / el / 1e9, 3), "unit": "GB/s",
             "cores": threads, "kind": "port",
-            "sample": f"the full {w.rows}-row workload, {reps} runs in {el:.1f} s "
-                      f"(oracle/oracle.c, {threads} threads)"}
+            "sample": f"the full {w.rows}-row workload, {reps} runs in {el:.1f} s ({desc})"}
 
 
 def run_serve(args, info):
@@ -345,9 +388,13 @@ def main():
     import pyoracle as O
     from gpudbmetalbenchmark_amd import Engine
     names = (["q1", "q6", "q3", "q9", "q13", "select", "sum", "join"]
-             if args.pipeline == "all" else args.pipeline.split(","))
+             if args.pipeline == "all" else
+             list(TPCH_KERNELS) if args.pipeline == "tpch" else args.pipeline.split(","))
+    tpch_names = [n for n in names if n in TPCH_KERNELS]
+    names = [n for n in names if n not in TPCH_KERNELS]
     if args.scale == "sf100":
         names = [n for n in names if n in ("q1", "q6", "q3", "q9", "q13")]
+        tpch_names = []
         args.tune = False
     eng = Engine(0)
     info = eng.device_info()
@@ -355,9 +402,15 @@ def main():
     if args.probe:
         r, c = eng.bandwidth_probe(1 << 31, 10)
         args.hbm_probe = {"read_gbps": round(r, 1), "copy_gbps": round(c, 1)}
-    wl = build_workloads(eng, names, args.scale, O)
-    for n in names:
-        print(json.dumps(run_one(wl[n], args, info, O)), flush=True)
+    if names:
+        wl = build_workloads(eng, names, args.scale, O)
+        for n in names:
+            print(json.dumps(run_one(wl[n], args, info, O)), flush=True)
+        del wl
+    if tpch_names:
+        wl = build_tpch_workloads(eng, tpch_names, args.scale)
+        for n in tpch_names:
+            print(json.dumps(run_one(wl[n], args, info, O)), flush=True)
     eng.close()
 
 
