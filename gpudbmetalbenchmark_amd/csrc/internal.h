@@ -37,6 +37,7 @@ struct Column {
   void* dev = nullptr;
   size_t bytes = 0;
   int64_t minv = 0, maxv = -1;  // value range of integer columns (empty: max < min)
+  bool sorted = false;          // int32 column uploaded in non-decreasing order
   const void* host = nullptr;   // host tables: caller memory streamed in chunks
 };
 

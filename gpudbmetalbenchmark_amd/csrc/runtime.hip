@@ -137,7 +137,8 @@ int timing_finish(gdb_ctx* ctx, gdb_timing* t, uint64_t bytes, uint64_t main_byt
 
 // Min / max of an integer host column, split over threads for large columns.
 template <typename T>
-static void value_range(const T* v, uint64_t n, int64_t* lo, int64_t* hi) {
+static void value_range(const T* v, uint64_t n, int64_t* lo, int64_t* hi, bool* sorted = nullptr) {
+  if (sorted) *sorted = true;
   if (n == 0) {
     *lo = 0;
     *hi = -1;
@@ -145,16 +146,20 @@ static void value_range(const T* v, uint64_t n, int64_t* lo, int64_t* hi) {
   }
   const int T_ = n > (1u << 22) ? (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency())) : 1;
   std::vector<int64_t> mn(T_, INT64_MAX), mx(T_, INT64_MIN);
+  std::vector<char> up(T_, 1);
   const uint64_t per = (n + T_ - 1) / T_;
   auto body = [&](int t) {
     const uint64_t a = (uint64_t)t * per, b = std::min(n, a + per);
     int64_t l = INT64_MAX, h = INT64_MIN;
+    bool asc = true;
     for (uint64_t i = a; i < b; ++i) {
       l = std::min<int64_t>(l, (int64_t)v[i]);
       h = std::max<int64_t>(h, (int64_t)v[i]);
+      asc &= i == 0 || v[i - 1] <= v[i];
     }
     mn[t] = l;
     mx[t] = h;
+    up[t] = asc;
   };
   std::vector<std::thread> th;
   for (int t = 1; t < T_; ++t) th.emplace_back(body, t);
@@ -162,6 +167,7 @@ static void value_range(const T* v, uint64_t n, int64_t* lo, int64_t* hi) {
   for (auto& x : th) x.join();
   *lo = *std::min_element(mn.begin(), mn.end());
   *hi = *std::max_element(mx.begin(), mx.end());
+  if (sorted) *sorted = std::all_of(up.begin(), up.end(), [](char c) { return c != 0; });
 }
 
 }  // namespace gdb
@@ -330,7 +336,7 @@ int gdb_table_create(gdb_ctx* ctx, const gdb_col_desc* cols, int ncols, uint64_t
       return rc;
     }
     if (d.host_data && d.type == GDB_I32)
-      value_range((const int32_t*)d.host_data, nrows, &c.minv, &c.maxv);
+      value_range((const int32_t*)d.host_data, nrows, &c.minv, &c.maxv, &c.sorted);
     else if (d.host_data && d.type == GDB_I64)
       value_range((const int64_t*)d.host_data, nrows, &c.minv, &c.maxv);
     t->cols.push_back(c);

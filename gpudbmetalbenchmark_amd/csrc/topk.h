@@ -61,17 +61,27 @@ __device__ void bitonic_sort_best_first(TopEntry* buf, int n) {
 // Running top-k state in LDS. `push` must be called by every thread of the
 // block once per round with at most one candidate each (valid = false for none);
 // rounds of blockDim candidates never overflow while k + 2*blockDim <= BUF.
+//
+// With `thr` (an LDS slot) set, every sort that leaves k entries records the k-th
+// best in *thr, and later candidates that do not rank before it are dropped at
+// once: they can no longer reach the top k, and the buffer stops refilling.
 template <int BUF>
 struct BlockTopK {
   TopEntry* buf;
   int* n;
   int k;
+  TopEntry* thr = nullptr;   // running k-th best (valid while *have)
+  int* have = nullptr;
 
   __device__ void init() {
-    if (threadIdx.x == 0) *n = 0;
+    if (threadIdx.x == 0) {
+      *n = 0;
+      if (have) *have = 0;
+    }
     __syncthreads();
   }
   __device__ void push(bool valid, const TopEntry& e) {
+    if (valid && have && *have && !better(e, *thr)) valid = false;
     if (valid) {
       const int pos = atomicAdd(n, 1);
       buf[pos] = e;
@@ -81,7 +91,13 @@ struct BlockTopK {
     __syncthreads();  // every thread has read n before any thread pushes again
     if (cur > BUF - (int)blockDim.x) {
       bitonic_sort_best_first<BUF>(buf, cur);
-      if (threadIdx.x == 0) *n = min(*n, k);
+      if (threadIdx.x == 0) {
+        *n = min(*n, k);
+        if (have && k > 0 && cur >= k) {
+          *thr = buf[k - 1];
+          *have = 1;
+        }
+      }
       __syncthreads();
     }
   }

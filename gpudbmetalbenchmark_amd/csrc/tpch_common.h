@@ -109,9 +109,10 @@ __global__ __launch_bounds__(256) void topk_stage1(Src src, uint64_t n, int k,
                                                    gdbk::TopEntry* __restrict__ out,
                                                    unsigned long long* __restrict__ count) {
   __shared__ gdbk::TopEntry buf[1024];
-  __shared__ int nbuf;
+  __shared__ gdbk::TopEntry thr;
+  __shared__ int nbuf, have;
   __shared__ int64_t red[4];
-  gdbk::BlockTopK<1024> tk{buf, &nbuf, k};
+  gdbk::BlockTopK<1024> tk{buf, &nbuf, k, &thr, &have};
   tk.init();
   const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
   const uint64_t begin = (uint64_t)blockIdx.x * per;

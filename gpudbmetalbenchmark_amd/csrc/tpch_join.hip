@@ -397,14 +397,15 @@ __global__ void q18_gather(const TopEntry* __restrict__ win, int k,
 
 // ---- Q21 (reference kernels/q21.metal:24-119, src/q21.cpp) ---------------------
 
-// bits[orderkey] = (o_orderstatus == status).
-__global__ __launch_bounds__(256) void order_status_bits(const int32_t* __restrict__ ok,
-                                                         const uint8_t* __restrict__ st,
-                                                         uint32_t n, uint32_t status,
-                                                         uint32_t* __restrict__ bits) {
+// fst[orderkey] = (o_orderstatus == status), plain byte stores (no atomics).
+__global__ __launch_bounds__(256) void order_status_bytes(const int32_t* __restrict__ ok,
+                                                          const uint8_t* __restrict__ st,
+                                                          uint32_t n, uint32_t status,
+                                                          uint8_t* __restrict__ fst) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    if (st[i] == status) bit_set(bits, (uint32_t)ok[i]);
+    fst[ok[i]] = st[i] == status;
 }
+
 // Per-order state in one 64-bit word:
 //   bits 32..62 first supplier + 1, bit 63 "a second supplier seen",
 //   bits 0..30 first late supplier + 1, bit 31 "a second late supplier seen".
@@ -436,7 +437,7 @@ __global__ __launch_bounds__(256) void q21_track(const int32_t* __restrict__ ok,
                                                  const int32_t* __restrict__ sk,
                                                  const int32_t* __restrict__ commit,
                                                  const int32_t* __restrict__ receipt, uint32_t n,
-                                                 const uint32_t* __restrict__ fbits,
+                                                 const uint8_t* __restrict__ fst,
                                                  const uint8_t* __restrict__ smap, int32_t nation,
                                                  unsigned long long* __restrict__ state,
                                                  uint32_t* __restrict__ list,
@@ -452,7 +453,7 @@ __global__ __launch_bounds__(256) void q21_track(const int32_t* __restrict__ ok,
       uint32_t m = 0;
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        m |= (uint32_t)((4 * g + j < n) && bit_test(fbits, (uint32_t)lane_of(k, j))) << j;
+        m |= (uint32_t)((4 * g + j < n) && fst[lane_of(k, j)]) << j;
       if (m) {
         const int4 s = ld4nt(sk, g), c = ld4nt(commit, g), r = ld4nt(receipt, g);
         int key = -1;
@@ -503,6 +504,167 @@ __global__ __launch_bounds__(256) void q21_count(const int32_t* __restrict__ ok,
     const uint32_t r = list[i];
     const uint64_t st = state[ok[r]];
     if ((st >> 63) && !((st >> 31) & 1)) atomicAdd(&cnt[sk[r]], 1u);
+  }
+}
+
+// ---- clustered lineitem: one lane resolves each order ---------------------------
+// When l_orderkey is non-decreasing, the rows of an order are contiguous. The lane
+// whose group of 4 rows holds an order's first row owns the order and computes its
+// result alone, so no per-order state is shared. The owner sees a 12-row window in
+// registers: its own 4 rows and the next two lanes' 4 rows each (wave shuffles;
+// TPC-H orders have at most 7 rows). Only rows past the window (the last two lanes
+// of a wave, or longer orders) are read again from memory. Loops run wave-uniform
+// so every lane takes part in the shuffles.
+
+constexpr int WIN = 12;
+
+// w[0..3] = this lane's rows, w[4..11] = the next two lanes' rows.
+__device__ __forceinline__ void row_window(const int4& v, int* w) {
+  w[0] = v.x;
+  w[1] = v.y;
+  w[2] = v.z;
+  w[3] = v.w;
+#pragma unroll
+  for (int d = 1; d < WIN / 4; ++d) {
+    w[4 * d + 0] = __shfl_down(v.x, d, 64);
+    w[4 * d + 1] = __shfl_down(v.y, d, 64);
+    w[4 * d + 2] = __shfl_down(v.z, d, 64);
+    w[4 * d + 3] = __shfl_down(v.w, d, 64);
+  }
+}
+
+// Bit j set when row 4g+j starts an order (rows >= n never do).
+__device__ __forceinline__ uint32_t order_heads(const int32_t* __restrict__ ok, const int4& k,
+                                                uint32_t g, uint32_t n, int lane) {
+  const int up = __shfl_up(k.w, 1, 64);
+  const int prev = lane > 0 ? up : (g > 0 && 4 * g <= n ? ok[4 * g - 1] : 0);
+  uint32_t h = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t r = 4 * g + j;
+    const int pj = j == 0 ? prev : lane_of(k, j - 1);
+    h |= (uint32_t)(r < n && (r == 0 || pj != lane_of(k, j))) << j;
+  }
+  return h;
+}
+
+// Window rows this lane may read from registers (the last lanes see fewer).
+__device__ __forceinline__ int window_rows(int lane) {
+  return lane <= 63 - (WIN / 4 - 1) ? WIN : 4 * (64 - lane);
+}
+
+// Q21: per order of the wanted status, suppliers and late suppliers; the order's
+// late lines count for their supplier when the order has a second supplier and
+// every late line has the same supplier (of `nation`).
+struct Q21Acc {
+  uint32_t first = 0, lfirst = 0, lcount = 0;
+  bool multi = false, lmulti = false;
+  __device__ void add(int sv, int cv, int rv) {
+    const uint32_t s1 = (uint32_t)sv + 1u;
+    if (!first) first = s1;
+    else multi |= first != s1;
+    if (rv > cv) {
+      if (!lfirst) lfirst = s1;
+      else lmulti |= lfirst != s1;
+      ++lcount;
+    }
+  }
+};
+
+__global__ __launch_bounds__(256) void q21_owner(const int32_t* __restrict__ ok,
+                                                 const int32_t* __restrict__ sk,
+                                                 const int32_t* __restrict__ commit,
+                                                 const int32_t* __restrict__ receipt, uint32_t n,
+                                                 const uint8_t* __restrict__ fst,
+                                                 const uint8_t* __restrict__ smap, int32_t nation,
+                                                 uint32_t* __restrict__ cnt) {
+  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
+  const int lane = threadIdx.x & 63;
+  const int avail = window_rows(lane);
+  for (uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; gw < nq; gw += stride) {
+    const uint32_t g = gw + lane;
+    const bool live = g < nq;
+    const int4 z = make_int4(0, 0, 0, 0);
+    const int4 k = live ? ld4nt(ok, g) : z;
+    uint32_t heads = order_heads(ok, k, g, live ? n : 0, lane);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (((heads >> j) & 1) && !fst[lane_of(k, j)]) heads &= ~(1u << j);
+    // every lane takes part in the shuffles; only lanes with owned orders load more
+    const bool need = __any(heads != 0) != 0;
+    if (!need) continue;
+    const int4 s = live ? ld4nt(sk, g) : z, c = live ? ld4nt(commit, g) : z,
+               rc = live ? ld4nt(receipt, g) : z;
+    int wk[WIN], ws[WIN], wc[WIN], wr[WIN];
+    row_window(k, wk);
+    row_window(s, ws);
+    row_window(c, wc);
+    row_window(rc, wr);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (!((heads >> j) & 1)) continue;
+      const int key = wk[j];
+      Q21Acc acc;
+      acc.add(ws[j], wc[j], wr[j]);
+      bool open = true;
+      uint32_t r = 4 * g + j + 1;
+#pragma unroll
+      for (int i = j + 1; i < WIN; ++i) {
+        if (!open) continue;
+        if (i >= avail || 4 * g + i >= n) break;
+        if (wk[i] != key) {
+          open = false;
+          continue;
+        }
+        acc.add(ws[i], wc[i], wr[i]);
+        r = 4 * g + i + 1;
+      }
+      if (open)
+        for (; r < n && ok[r] == key; ++r) acc.add(sk[r], commit[r], receipt[r]);
+      if (acc.multi && acc.lcount && !acc.lmulti && smap[acc.lfirst - 1] == nation)
+        atomicAdd(&cnt[acc.lfirst - 1], acc.lcount);
+    }
+  }
+}
+
+// Q18: the owner writes the order's quantity sum with a plain store.
+__global__ __launch_bounds__(256) void q18_owner(const int32_t* __restrict__ ok,
+                                                 const int32_t* __restrict__ qty, uint32_t n,
+                                                 int32_t* __restrict__ sq) {
+  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
+  const int lane = threadIdx.x & 63;
+  const int avail = window_rows(lane);
+  for (uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; gw < nq; gw += stride) {
+    const uint32_t g = gw + lane;
+    const bool live = g < nq;
+    const int4 z = make_int4(0, 0, 0, 0);
+    const int4 k = live ? ld4nt(ok, g) : z, q = live ? ld4nt(qty, g) : z;
+    const uint32_t heads = order_heads(ok, k, g, live ? n : 0, lane);
+    int wk[WIN], wq[WIN];
+    row_window(k, wk);
+    row_window(q, wq);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (!((heads >> j) & 1)) continue;
+      const int key = wk[j];
+      int sum = wq[j];
+      bool open = true;
+      uint32_t r = 4 * g + j + 1;
+#pragma unroll
+      for (int i = j + 1; i < WIN; ++i) {
+        if (!open) continue;
+        if (i >= avail || 4 * g + i >= n) break;
+        if (wk[i] != key) {
+          open = false;
+          continue;
+        }
+        sum += wq[i];
+        r = 4 * g + i + 1;
+      }
+      if (open)
+        for (; r < n && ok[r] == key; ++r) sum += qty[r];
+      sq[key] = sum;
+    }
   }
 }
 
@@ -910,7 +1072,10 @@ int gdb_q18(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q18_params* p, gdb_q1
   GDB_HIP(hipMemsetAsync(sq, 0, ospan * 4, s));
   GDB_HIP(hipMemsetAsync(dout, 0, out_bytes, s));
   GDB_HIP(hipEventRecord(ctx->ev[1], s));
-  if (nl)
+  if (nl && lo->sorted)  // rows of an order are contiguous: one plain store per order
+    q18_owner<<<blocks_for(ctx, q18_owner, quads(nl)), BLOCK, 0, s>>>(
+        (const int32_t*)lo->dev, (const int32_t*)lq->dev, (uint32_t)nl, sq);
+  else if (nl)
     q18_sums<<<blocks_for(ctx, q18_sums, quads(nl)), BLOCK, 0, s>>>(
         (const int32_t*)lo->dev, (const int32_t*)lq->dev, (uint32_t)nl, sq);
   GDB_HIP(hipGetLastError());
@@ -961,22 +1126,25 @@ int gdb_q21(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q21_params* p, gdb_q2
   uint64_t a, b;
   GDB_TRY(key_span(oo, no, &a));
   GDB_TRY(key_span(lo, nl, &b));
-  const uint64_t ospan = std::max(a, b), words = (ospan + 31) / 32;
+  const uint64_t ospan = std::max(a, b);
   uint64_t sspan;
   GDB_TRY(key_span(sk, ns, &sspan));
   GDB_TRY(key_span(lk, nl, &a));
   sspan = std::max(sspan, a);
   const int k = std::max(p->top_k, 1);
   const int tblocks = 256;
-  unsigned long long* state;
-  uint32_t *fbits, *list, *cnt;
-  uint8_t* smap;
+  const bool clustered = lo->sorted;  // rows of an order are contiguous
+  unsigned long long* state = nullptr;
+  uint32_t *list = nullptr, *cnt;
+  uint8_t *smap, *fst;
   TopEntry* win;
   uint64_t* dout;
   char* host;
-  GDB_TRY(arena(ctx, A_AGG, ospan * 8, (void**)&state));
-  GDB_TRY(arena(ctx, A_BITMAP, words * 4, (void**)&fbits));
-  GDB_TRY(arena(ctx, A_AUX, std::max<uint64_t>(nl, 1) * 4, (void**)&list));
+  if (!clustered) {
+    GDB_TRY(arena(ctx, A_AGG, ospan * 8, (void**)&state));
+    GDB_TRY(arena(ctx, A_AUX, std::max<uint64_t>(nl, 1) * 4, (void**)&list));
+  }
+  GDB_TRY(arena(ctx, A_BITMAP, ospan, (void**)&fst));
   GDB_TRY(arena(ctx, A_MAP, sspan * 5, (void**)&cnt));
   smap = (uint8_t*)(cnt + sspan);
   GDB_TRY(arena(ctx, A_AUX2, sizeof(TopEntry) * (size_t)tblocks * k, (void**)&win));
@@ -987,36 +1155,44 @@ int gdb_q21(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q21_params* p, gdb_q2
   TopEntry* fin = (TopEntry*)(dout + 4);
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_HIP(hipMemsetAsync(state, 0, ospan * 8, s));
-  GDB_HIP(hipMemsetAsync(fbits, 0, words * 4, s));
+  if (!clustered) GDB_HIP(hipMemsetAsync(state, 0, ospan * 8, s));
+  GDB_HIP(hipMemsetAsync(fst, 0, ospan, s));
   GDB_HIP(hipMemsetAsync(cnt, 0, sspan * 4, s));
   GDB_HIP(hipMemsetAsync(smap, NONE, sspan, s));
   GDB_HIP(hipMemsetAsync(dout, 0, out_bytes, s));
   if (no)
-    order_status_bits<<<blocks_for(ctx, order_status_bits, no), BLOCK, 0, s>>>(
+    order_status_bytes<<<blocks_for(ctx, order_status_bytes, no), BLOCK, 0, s>>>(
         (const int32_t*)oo->dev, (const uint8_t*)os->dev, (uint32_t)no,
-        (uint32_t)(uint8_t)p->status, fbits);
+        (uint32_t)(uint8_t)p->status, fst);
   if (ns)
     nation_bytes<<<blocks_for(ctx, nation_bytes, ns), BLOCK, 0, s>>>(
         (const int32_t*)sk->dev, (const int32_t*)sn->dev, (uint32_t)ns, smap);
   GDB_HIP(hipGetLastError());
   GDB_HIP(hipEventRecord(ctx->ev[1], s));
-  if (nl)
-    q21_track<<<blocks_for(ctx, q21_track, quads(nl)), BLOCK, 0, s>>>(
+  if (nl && clustered) {
+    q21_owner<<<blocks_for(ctx, q21_owner, quads(nl)), BLOCK, 0, s>>>(
         (const int32_t*)lo->dev, (const int32_t*)lk->dev, (const int32_t*)lc->dev,
-        (const int32_t*)lr->dev, (uint32_t)nl, fbits, smap, p->nation, state, list, list_n);
-  GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[2], s));
-  if (nl)
-    q21_count<<<blocks_for(ctx, q21_count, nl / 32 + 1), BLOCK, 0, s>>>(
-        (const int32_t*)lo->dev, (const int32_t*)lk->dev, list, list_n, state, cnt);
-  GDB_HIP(hipGetLastError());
+        (const int32_t*)lr->dev, (uint32_t)nl, fst, smap, p->nation, cnt);
+    GDB_HIP(hipGetLastError());
+    GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  } else {
+    if (nl)
+      q21_track<<<blocks_for(ctx, q21_track, quads(nl)), BLOCK, 0, s>>>(
+          (const int32_t*)lo->dev, (const int32_t*)lk->dev, (const int32_t*)lc->dev,
+          (const int32_t*)lr->dev, (uint32_t)nl, fst, smap, p->nation, state, list, list_n);
+    GDB_HIP(hipGetLastError());
+    GDB_HIP(hipEventRecord(ctx->ev[2], s));
+    if (nl)
+      q21_count<<<blocks_for(ctx, q21_count, nl / 32 + 1), BLOCK, 0, s>>>(
+          (const int32_t*)lo->dev, (const int32_t*)lk->dev, list, list_n, state, cnt);
+    GDB_HIP(hipGetLastError());
+  }
   GDB_TRY(topk_run(ctx, CountSrc{cnt}, sspan, k, win, tblocks, fin, (unsigned long long*)dout,
                    (unsigned long long*)(dout + 1)));
   GDB_HIP(hipMemcpyAsync(host, dout, out_bytes, hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
   // SQL columns: lineitem orderkey, suppkey, commitdate, receiptdate (16 B); orders
-  // key + status (5 B); supplier key + nation (8 B). Mandatory in q21_track:
+  // key + status (5 B); supplier key + nation (8 B). Mandatory in the lineitem pass:
   // l_orderkey, 4 B per line.
   GDB_TRY(timing_finish(ctx, timing, 16 * nl + 5 * no + 8 * ns, 4 * nl));
   const uint64_t* h = (const uint64_t*)host;

@@ -315,8 +315,9 @@ __global__ __launch_bounds__(256) void count_bits(const int32_t* __restrict__ pk
   uint32_t c = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     c += bit_test(bits, (uint32_t)pk[i]);
-  c = wave_sum_u32(c);
-  if ((threadIdx.x & 63) == 0 && c) atomicAdd(cnt, (unsigned long long)c);
+  __shared__ int64_t red[4];
+  const int64_t b = block_sum((int64_t)c, red);  // one atomic per workgroup
+  if (threadIdx.x == 0 && b) atomicAdd(cnt, (unsigned long long)b);
 }
 
 __global__ __launch_bounds__(256) void q20_scan(const int32_t* __restrict__ ship,
@@ -689,12 +690,17 @@ int gdb_q16(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q16_params* p, gdb_q1
     out[i] = gdb_q16_row{(int32_t)(g / 8 / 256), (int32_t)((g / 8) % 256),
                          qpart.sizes[g % 8], (int32_t)got[i].y};
   }
-  std::vector<std::string> tn(256);
-  for (int c = 0; c < 256; ++c) tn[c] = type_name(c);
+  static const std::vector<int> trank = [] {  // type code -> position in name order
+    std::vector<int> idx(256), rank(256);
+    for (int c = 0; c < 256; ++c) idx[c] = c;
+    std::sort(idx.begin(), idx.end(), [](int a, int b) { return type_name(a) < type_name(b); });
+    for (int r = 0; r < 256; ++r) rank[idx[r]] = r;
+    return rank;
+  }();
   std::sort(out.begin(), out.end(), [&](const gdb_q16_row& x, const gdb_q16_row& y) {
     if (x.supplier_cnt != y.supplier_cnt) return x.supplier_cnt > y.supplier_cnt;
     if (x.brand != y.brand) return x.brand < y.brand;
-    if (x.type != y.type) return tn[x.type] < tn[y.type];
+    if (x.type != y.type) return trank[x.type & 255] < trank[y.type & 255];
     return x.size < y.size;
   });
   *nrows = (int64_t)n;

@@ -121,12 +121,13 @@ __global__ __launch_bounds__(256) void q19_scan(const uint8_t* __restrict__ mode
 
 // ---- Q12 (reference kernels/q12.metal:22-100, src/q12.cpp) ---------------------
 
-__global__ __launch_bounds__(256) void order_prio_bits(const int32_t* __restrict__ ok,
-                                                       const uint8_t* __restrict__ prio,
-                                                       uint32_t n, uint32_t mask,
-                                                       uint32_t* __restrict__ bits) {
+// high[orderkey] = priority in mask, plain byte stores (no atomics).
+__global__ __launch_bounds__(256) void order_prio_bytes(const int32_t* __restrict__ ok,
+                                                        const uint8_t* __restrict__ prio,
+                                                        uint32_t n, uint32_t mask,
+                                                        uint8_t* __restrict__ high) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    if (prio[i] < 32 && ((mask >> prio[i]) & 1)) bit_set(bits, (uint32_t)ok[i]);
+    high[ok[i]] = prio[i] < 32 && ((mask >> prio[i]) & 1);
 }
 
 __global__ __launch_bounds__(256) void q12_scan(const uint8_t* __restrict__ mode,
@@ -135,7 +136,7 @@ __global__ __launch_bounds__(256) void q12_scan(const uint8_t* __restrict__ mode
                                                 const int32_t* __restrict__ ship,
                                                 const int32_t* __restrict__ ok, uint32_t n,
                                                 int32_t m0, int32_t m1, int32_t lo, int32_t hi,
-                                                const uint32_t* __restrict__ high,
+                                                const uint8_t* __restrict__ high,
                                                 int64_t* __restrict__ out) {
   __shared__ int64_t red[4];
   int64_t c[4] = {0, 0, 0, 0};  // [mode index][high]
@@ -156,7 +157,7 @@ __global__ __launch_bounds__(256) void q12_scan(const uint8_t* __restrict__ mode
       const int cv = lane_of(cm, j);
       if (!((m >> j) & 1) || !(cv < lane_of(r, j)) || !(lane_of(s, j) < cv)) continue;
       const int mi = byte_of(mw, j) == m0 ? 0 : 1;
-      const int h = bit_test(high, (uint32_t)lane_of(o, j)) ? 1 : 0;
+      const int h = high[lane_of(o, j)] ? 1 : 0;
       c[mi * 2 + h] += 1;
     }
   }
@@ -532,25 +533,25 @@ int gdb_q12(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q12_params* p, gdb_q1
   uint64_t s1, s2;
   GDB_TRY(key_span(oo, no, &s1));
   GDB_TRY(key_span(lo, nl, &s2));
-  const uint64_t words = (std::max(s1, s2) + 31) / 32;
-  uint32_t* bits;
+  const uint64_t span = std::max(s1, s2);
+  uint8_t* high;
   int64_t *dout, *host;
-  GDB_TRY(arena(ctx, A_BITMAP, words * 4, (void**)&bits));
+  GDB_TRY(arena(ctx, A_MAP, span, (void**)&high));
   GDB_TRY(arena(ctx, A_OUT, 64, (void**)&dout));
   GDB_TRY(pinned(ctx, 64, (void**)&host));
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_HIP(hipMemsetAsync(bits, 0, words * 4, s));
+  GDB_HIP(hipMemsetAsync(high, 0, span, s));
   GDB_HIP(hipMemsetAsync(dout, 0, 32, s));
   if (no)
-    order_prio_bits<<<blocks_for(ctx, order_prio_bits, no), BLOCK, 0, s>>>(
-        (const int32_t*)oo->dev, (const uint8_t*)op->dev, (uint32_t)no, p->high_mask, bits);
+    order_prio_bytes<<<blocks_for(ctx, order_prio_bytes, no), BLOCK, 0, s>>>(
+        (const int32_t*)oo->dev, (const uint8_t*)op->dev, (uint32_t)no, p->high_mask, high);
   GDB_HIP(hipEventRecord(ctx->ev[1], s));
   if (nl)
     q12_scan<<<blocks_for(ctx, q12_scan, quads(nl)), BLOCK, 0, s>>>(
         (const uint8_t*)lm->dev, (const int32_t*)lr->dev, (const int32_t*)lc->dev,
         (const int32_t*)ls->dev, (const int32_t*)lo->dev, (uint32_t)nl, p->mode[0], p->mode[1],
-        p->date_lo, p->date_hi, bits, dout);
+        p->date_lo, p->date_hi, high, dout);
   GDB_HIP(hipGetLastError());
   GDB_HIP(hipEventRecord(ctx->ev[2], s));
   GDB_HIP(hipMemcpyAsync(host, dout, 32, hipMemcpyDeviceToHost, s));

@@ -24,6 +24,9 @@ def _lib():
     return abi.load()
 
 
+_Q11_DT = np.dtype([("partkey", np.int32), ("pad", np.int32), ("value", np.int64)])
+
+
 def _mask3(codes):
     return abi.M3(*tpch.code_mask(codes, 3))
 
@@ -99,11 +102,13 @@ class TpchDb:
             int(r.groups)
 
     def q11(self, nation="GERMANY", frac_num=1, frac_den=10000, cap=1 << 20, timing=None):
-        rows = (abi.Q11Row * max(cap, 1))()
+        rows = np.empty(max(cap, 1), _Q11_DT)
         n, total = C.c_int64(), C.c_int64()
         p = abi.Q11Params(tpch.N_NAME.index(nation), 0, frac_num, frac_den)
-        self._call("gdb_q11", C.byref(p), rows, cap, C.byref(n), C.byref(total), tref(timing))
-        return [(x.partkey, x.value) for x in rows[:min(n.value, cap)]], total.value
+        self._call("gdb_q11", C.byref(p), rows.ctypes.data_as(C.POINTER(abi.Q11Row)), cap,
+                   C.byref(n), C.byref(total), tref(timing))
+        r = rows[:min(n.value, cap)]
+        return list(zip(r["partkey"].tolist(), r["value"].tolist())), total.value
 
     def q12(self, mode1="MAIL", mode2="SHIP", date_lo=731, date_hi=1096,
             high=("1-URGENT", "2-HIGH"), timing=None):
@@ -132,13 +137,14 @@ class TpchDb:
     def q16(self, brand="Brand#45", type_like="MEDIUM POLISHED%",
             sizes=(49, 14, 23, 45, 19, 3, 36, 9), complaint=("Customer", "Complaints"),
             cap=1 << 20, timing=None):
-        rows = (abi.Q16Row * max(cap, 1))()
+        rows = np.empty((max(cap, 1), 4), np.int32)  # gdb_q16_row: four int32
         n = C.c_int64()
         p = abi.Q16Params(tpch.brand_code(brand), len(sizes), (C.c_int32 * 8)(*sizes),
                           _mask3(tpch.codes_like(tpch.P_TYPE, type_like)),
                           complaint[0].encode(), complaint[1].encode())
-        self._call("gdb_q16", C.byref(p), rows, cap, C.byref(n), tref(timing))
-        return [(x.brand, x.type, x.size, x.supplier_cnt) for x in rows[:min(n.value, cap)]]
+        self._call("gdb_q16", C.byref(p), rows.ctypes.data_as(C.POINTER(abi.Q16Row)), cap,
+                   C.byref(n), tref(timing))
+        return rows[:min(n.value, cap)]  # (brand, type, size, supplier_cnt) per row
 
     def q17(self, brand="Brand#23", container="MED BOX", frac_num=1, frac_den=5, timing=None):
         r = abi.Q17Result()
