@@ -95,6 +95,23 @@ __device__ __forceinline__ int year_index(int32_t day) {
   return y;
 }
 
+// Grid-stride walk over nq vector groups, two groups per trip: the first-stage
+// loads of both groups (`first(g)`, the streamed columns every row needs) are
+// issued before either group is processed (`rest(g, loaded)`, which may load
+// more for surviving rows), so two independent loads are in flight per lane.
+template <class First, class Rest>
+__device__ __forceinline__ void walk2(uint32_t nq, First first, Rest rest) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  for (; g + stride < nq; g += 2 * stride) {
+    const auto a = first(g);
+    const auto b = first(g + stride);
+    rest(g, a);
+    rest(g + stride, b);
+  }
+  if (g < nq) rest(g, first(g));
+}
+
 // Block sum of `v` added to *out (one atomic per workgroup).
 __device__ __forceinline__ void block_add(int64_t v, int64_t* red, int64_t* out) {
   const int64_t s = gdbk::block_sum(v, red);
@@ -151,6 +168,34 @@ int topk_run(gdb_ctx* ctx, Src src, uint64_t n, int k, gdbk::TopEntry* winners, 
   return GDB_OK;
 }
 
+// Folded filter: bit k of a key bitmap is ORed into bit (k & 31) of word
+// (k >> 5) mod FOLD_WORDS (2^18 bits, 32 KiB). Kept in LDS by a scan, a clear
+// folded bit rules a key out without a memory access; a set one is confirmed
+// against the full bitmap. For a selective key set (a few % of parts) the
+// folded bits stay mostly clear.
+constexpr uint32_t FOLD_WORDS = 8192;
+constexpr int FOLD_BLOCK = 512;  // threads of a scan with the 32 KiB LDS filter
+
+__device__ __forceinline__ void fold_set(uint32_t* fold, uint32_t k) {
+  atomicOr(&fold[(k >> 5) & (FOLD_WORDS - 1)], 1u << (k & 31));
+}
+__device__ __forceinline__ bool fold_test(const uint32_t* lfold, uint32_t k) {
+  return (lfold[(k >> 5) & (FOLD_WORDS - 1)] >> (k & 31)) & 1u;
+}
+// Copies the folded filter into LDS; every thread of the block must call it.
+__device__ __forceinline__ void fold_load(uint32_t* lfold, const uint32_t* __restrict__ fold) {
+  for (uint32_t i = threadIdx.x; i < FOLD_WORDS / 4; i += blockDim.x)
+    reinterpret_cast<uint4*>(lfold)[i] = reinterpret_cast<const uint4*>(fold)[i];
+  __syncthreads();
+}
+
+// Workgroups of FOLD_BLOCK threads for a scan with the LDS filter.
+inline int fold_blocks(gdb_ctx* ctx, const void* kernel, uint64_t groups) {
+  const int res = gdb::resident_blocks(ctx, kernel, FOLD_BLOCK, 0, 0);
+  const uint64_t need = (groups + FOLD_BLOCK - 1) / FOLD_BLOCK;
+  return (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)res, need));
+}
+
 // Launch helper: grid-stride over n items, at most the resident workgroups.
 template <typename K>
 int blocks_for(gdb_ctx* ctx, K kernel, uint64_t items) {
@@ -175,11 +220,16 @@ inline Mask4 mask4(const uint64_t* m3) { return Mask4{{m3[0], m3[1], m3[2], 0}};
 namespace {
 
 // bits[key[i]] = mask contains code[i] (part type / container filters).
+// With fold != nullptr the folded filter is filled too.
 __global__ __launch_bounds__(256) void code_bits(const int32_t* __restrict__ key,
                                                  const uint8_t* __restrict__ code, uint32_t n,
-                                                 gdbt::Mask4 mask, uint32_t* __restrict__ bits) {
+                                                 gdbt::Mask4 mask, uint32_t* __restrict__ bits,
+                                                 uint32_t* __restrict__ fold = nullptr) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    if (mask.has(code[i])) gdbt::bit_set(bits, (uint32_t)key[i]);
+    if (mask.has(code[i])) {
+      gdbt::bit_set(bits, (uint32_t)key[i]);
+      if (fold) gdbt::fold_set(fold, (uint32_t)key[i]);
+    }
 }
 
 // nreg[n_nationkey] = n_regionkey (255 for keys without a row); nkeys < 256.

@@ -16,6 +16,10 @@ namespace {
 
 constexpr uint8_t NONE = 255;
 
+struct Ld3 {  // three streamed int32 columns of one vector group
+  int4 a, b, c;
+};
+
 // ---- Q5 (reference kernels/q5.metal:19-125, src/q5.cpp) ------------------------
 
 // cmap[custkey] = nation if the nation's region is `region`, else NONE.
@@ -40,17 +44,16 @@ __global__ __launch_bounds__(256) void orders_cust_byte(const int32_t* __restric
                                                         const uint8_t* __restrict__ cmap,
                                                         uint32_t cspan,
                                                         uint8_t* __restrict__ omap) {
-  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
-    const int4 k = ld4nt(ok, g), c = ld4nt(oc, g), d = ld4nt(od, g);
+  walk2(quads(n), [&](uint32_t g) { return Ld3{ld4nt(ok, g), ld4nt(oc, g), ld4nt(od, g)}; },
+        [&](uint32_t g, const Ld3& l) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       if (4 * g + j >= n) continue;
-      const int dv = lane_of(d, j);
-      const uint32_t cv = (uint32_t)lane_of(c, j);
-      omap[lane_of(k, j)] = (dv >= lo && dv < hi && cv < cspan) ? cmap[cv] : NONE;
+      const int dv = lane_of(l.c, j);
+      const uint32_t cv = (uint32_t)lane_of(l.b, j);
+      omap[lane_of(l.a, j)] = (dv >= lo && dv < hi && cv < cspan) ? cmap[cv] : NONE;
     }
-  }
+  });
 }
 
 __global__ __launch_bounds__(256) void q5_scan(const int32_t* __restrict__ ok,
@@ -65,9 +68,7 @@ __global__ __launch_bounds__(256) void q5_scan(const int32_t* __restrict__ ok,
   if (threadIdx.x < 32) rev[threadIdx.x] = 0;
   if (threadIdx.x == 0) hit = 0;
   __syncthreads();
-  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
-    const int4 k = ld4nt(ok, g);
+  walk2(quads(n), [&](uint32_t g) { return ld4nt(ok, g); }, [&](uint32_t g, const int4& k) {
     uint32_t on[4];
     uint32_t m = 0;
 #pragma unroll
@@ -75,13 +76,13 @@ __global__ __launch_bounds__(256) void q5_scan(const int32_t* __restrict__ ok,
       on[j] = 4 * g + j < n ? omap[lane_of(k, j)] : NONE;
       m |= (uint32_t)(on[j] != NONE) << j;
     }
-    if (!m) continue;
+    if (!m) return;
     const int4 s = ld4nt(sk, g);
     uint32_t m2 = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j)
       m2 |= (uint32_t)(((m >> j) & 1) && smap[lane_of(s, j)] == on[j]) << j;
-    if (!m2) continue;
+    if (!m2) return;
     const int4 d = ld4nt(disc, g);
     const longlong2 a = ld2nt(price, 2 * g), b = ld2nt(price, 2 * g + 1);
 #pragma unroll
@@ -90,7 +91,7 @@ __global__ __launch_bounds__(256) void q5_scan(const int32_t* __restrict__ ok,
         lds_add_i64(&rev[on[j]], lane_of(a, b, j) * (int64_t)(100 - lane_of(d, j)));
         atomicOr(&hit, 1u << on[j]);
       }
-  }
+  });
   __syncthreads();
   if (threadIdx.x < 25 && ((hit >> threadIdx.x) & 1)) add_i64(&out[threadIdx.x], rev[threadIdx.x]);
   if (threadIdx.x == 0 && hit) atomicOr(reinterpret_cast<unsigned int*>(out + 25), hit);
@@ -112,16 +113,15 @@ __global__ __launch_bounds__(256) void orders_side(const int32_t* __restrict__ o
                                                    const int32_t* __restrict__ oc, uint32_t n,
                                                    const uint8_t* __restrict__ cmap,
                                                    uint32_t cspan, uint8_t* __restrict__ omap) {
-  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
-    const int4 k = ld4nt(ok, g), c = ld4nt(oc, g);
+  walk2(quads(n), [&](uint32_t g) { return Ld3{ld4nt(ok, g), ld4nt(oc, g), make_int4(0, 0, 0, 0)}; },
+        [&](uint32_t g, const Ld3& l) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       if (4 * g + j >= n) continue;
-      const uint32_t cv = (uint32_t)lane_of(c, j);
-      omap[lane_of(k, j)] = cv < cspan ? cmap[cv] : NONE;
+      const uint32_t cv = (uint32_t)lane_of(l.b, j);
+      omap[lane_of(l.a, j)] = cv < cspan ? cmap[cv] : NONE;
     }
-  }
+  });
 }
 
 __global__ __launch_bounds__(256) void q7_scan(const int32_t* __restrict__ ship,
@@ -138,16 +138,14 @@ __global__ __launch_bounds__(256) void q7_scan(const int32_t* __restrict__ ship,
   if (threadIdx.x < 16) rev[threadIdx.x] = 0;
   if (threadIdx.x == 0) hit = 0;
   __syncthreads();
-  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
-    const int4 s = ld4nt(ship, g);
+  walk2(quads(n), [&](uint32_t g) { return ld4nt(ship, g); }, [&](uint32_t g, const int4& s) {
     uint32_t m = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int v = lane_of(s, j);
       m |= (uint32_t)((4 * g + j < n) & (v >= lo) & (v <= hi)) << j;
     }
-    if (!m) continue;
+    if (!m) return;
     const int4 k = ld4nt(ok, g);
     uint32_t cs[4];
     uint32_t m2 = 0;
@@ -156,7 +154,7 @@ __global__ __launch_bounds__(256) void q7_scan(const int32_t* __restrict__ ship,
       cs[j] = (m >> j) & 1 ? omap[lane_of(k, j)] : NONE;
       m2 |= (uint32_t)(cs[j] != NONE) << j;
     }
-    if (!m2) continue;
+    if (!m2) return;
     const int4 sp = ld4nt(sk, g);
     uint32_t ss[4];
     uint32_t m3 = 0;
@@ -165,7 +163,7 @@ __global__ __launch_bounds__(256) void q7_scan(const int32_t* __restrict__ ship,
       ss[j] = (m2 >> j) & 1 ? smap[lane_of(sp, j)] : NONE;
       m3 |= (uint32_t)(ss[j] != NONE && (ss[j] != cs[j] || same)) << j;
     }
-    if (!m3) continue;
+    if (!m3) return;
     const int4 d = ld4nt(disc, g);
     const longlong2 a = ld2nt(price, 2 * g), b = ld2nt(price, 2 * g + 1);
 #pragma unroll
@@ -176,7 +174,7 @@ __global__ __launch_bounds__(256) void q7_scan(const int32_t* __restrict__ ship,
         lds_add_i64(&rev[bin], lane_of(a, b, j) * (int64_t)(100 - lane_of(d, j)));
         atomicOr(&hit, 1u << bin);
       }
-  }
+  });
   __syncthreads();
   if (threadIdx.x < 16 && ((hit >> threadIdx.x) & 1)) add_i64(&out[threadIdx.x], rev[threadIdx.x]);
   if (threadIdx.x == 0 && hit) atomicOr(reinterpret_cast<unsigned int*>(out + 16), hit);
@@ -193,46 +191,48 @@ __global__ __launch_bounds__(256) void q8_orders(const int32_t* __restrict__ ok,
                                                  const uint8_t* __restrict__ cnat, uint32_t cspan,
                                                  const uint8_t* __restrict__ nreg, int32_t region,
                                                  uint8_t* __restrict__ omap) {
-  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
-    const int4 k = ld4nt(ok, g), c = ld4nt(oc, g), d = ld4nt(od, g);
+  walk2(quads(n), [&](uint32_t g) { return Ld3{ld4nt(ok, g), ld4nt(oc, g), ld4nt(od, g)}; },
+        [&](uint32_t g, const Ld3& l) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       if (4 * g + j >= n) continue;
-      const int dv = lane_of(d, j);
-      const uint32_t cv = (uint32_t)lane_of(c, j);
+      const int dv = lane_of(l.c, j);
+      const uint32_t cv = (uint32_t)lane_of(l.b, j);
       uint8_t y = NONE;
       if (dv >= lo && dv <= hi && cv < cspan) {
         const uint8_t nat = cnat[cv];
         if (nat != NONE && nreg[nat] == region) y = (uint8_t)year_index(dv);
       }
-      omap[lane_of(k, j)] = y;
+      omap[lane_of(l.a, j)] = y;
     }
-  }
+  });
 }
 
-__global__ __launch_bounds__(256) void q8_scan(const int32_t* __restrict__ pk,
-                                               const int32_t* __restrict__ ok,
-                                               const int32_t* __restrict__ sk,
-                                               const int32_t* __restrict__ disc,
-                                               const int64_t* __restrict__ price, uint32_t n,
-                                               const uint32_t* __restrict__ pbits,
-                                               const uint8_t* __restrict__ omap,
-                                               const uint8_t* __restrict__ smap, int32_t nation,
-                                               int64_t* __restrict__ out) {
+__global__ __launch_bounds__(FOLD_BLOCK) void q8_scan(const int32_t* __restrict__ pk,
+                                                      const int32_t* __restrict__ ok,
+                                                      const int32_t* __restrict__ sk,
+                                                      const int32_t* __restrict__ disc,
+                                                      const int64_t* __restrict__ price,
+                                                      uint32_t n,
+                                                      const uint32_t* __restrict__ pbits,
+                                                      const uint32_t* __restrict__ pfold,
+                                                      const uint8_t* __restrict__ omap,
+                                                      const uint8_t* __restrict__ smap,
+                                                      int32_t nation, int64_t* __restrict__ out) {
+  __shared__ __align__(16) uint32_t lfold[FOLD_WORDS];
   __shared__ int64_t vol[16];  // [0..8) nation volume, [8..16) total volume by year
   __shared__ uint32_t hit;
   if (threadIdx.x < 16) vol[threadIdx.x] = 0;
   if (threadIdx.x == 0) hit = 0;
-  __syncthreads();
-  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
-    const int4 p = ld4nt(pk, g);
+  fold_load(lfold, pfold);
+  walk2(quads(n), [&](uint32_t g) { return ld4nt(pk, g); }, [&](uint32_t g, const int4& p) {
     uint32_t m = 0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-      m |= (uint32_t)((4 * g + j < n) && bit_test(pbits, (uint32_t)lane_of(p, j))) << j;
-    if (!m) continue;
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t key = (uint32_t)lane_of(p, j);
+      m |= (uint32_t)((4 * g + j < n) && fold_test(lfold, key) && bit_test(pbits, key)) << j;
+    }
+    if (!m) return;
     const int4 k = ld4nt(ok, g);
     uint32_t y[4];
     uint32_t m2 = 0;
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(256) void q8_scan(const int32_t* __restrict__ pk,
       y[j] = (m >> j) & 1 ? omap[lane_of(k, j)] : NONE;
       m2 |= (uint32_t)(y[j] < 8) << j;
     }
-    if (!m2) continue;
+    if (!m2) return;
     const int4 sp = ld4nt(sk, g), d = ld4nt(disc, g);
     const longlong2 a = ld2nt(price, 2 * g), b = ld2nt(price, 2 * g + 1);
 #pragma unroll
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(256) void q8_scan(const int32_t* __restrict__ pk,
       if (smap[lane_of(sp, j)] == nation) lds_add_i64(&vol[y[j]], v);
       atomicOr(&hit, 1u << y[j]);
     }
-  }
+  });
   __syncthreads();
   if (threadIdx.x < 16 && vol[threadIdx.x]) add_i64(&out[threadIdx.x], vol[threadIdx.x]);
   if (threadIdx.x == 0 && hit) atomicOr(reinterpret_cast<unsigned int*>(out + 16), hit);
@@ -266,16 +266,15 @@ __global__ __launch_bounds__(256) void q10_orders(const int32_t* __restrict__ ok
                                                   const int32_t* __restrict__ od, uint32_t n,
                                                   int32_t lo, int32_t hi,
                                                   int32_t* __restrict__ ocust) {
-  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
-    const int4 k = ld4nt(ok, g), c = ld4nt(oc, g), d = ld4nt(od, g);
+  walk2(quads(n), [&](uint32_t g) { return Ld3{ld4nt(ok, g), ld4nt(oc, g), ld4nt(od, g)}; },
+        [&](uint32_t g, const Ld3& l) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       if (4 * g + j >= n) continue;
-      const int dv = lane_of(d, j);
-      ocust[lane_of(k, j)] = (dv >= lo && dv < hi) ? lane_of(c, j) : -1;
+      const int dv = lane_of(l.c, j);
+      ocust[lane_of(l.a, j)] = (dv >= lo && dv < hi) ? lane_of(l.b, j) : -1;
     }
-  }
+  });
 }
 
 __global__ __launch_bounds__(256) void q10_scan(const uint8_t* __restrict__ rflag,
@@ -285,13 +284,11 @@ __global__ __launch_bounds__(256) void q10_scan(const uint8_t* __restrict__ rfla
                                                 uint32_t flag, const int32_t* __restrict__ ocust,
                                                 uint32_t cspan, int64_t* __restrict__ crev,
                                                 uint8_t* __restrict__ seen) {
-  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
-    const uint32_t fw = ld_u8x4nt(rflag, g);
+  walk2(quads(n), [&](uint32_t g) { return ld_u8x4nt(rflag, g); }, [&](uint32_t g, uint32_t fw) {
     uint32_t m = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) m |= (uint32_t)((4 * g + j < n) & (byte_of(fw, j) == flag)) << j;
-    if (!m) continue;
+    if (!m) return;
     const int4 k = ld4nt(ok, g);
     int32_t cu[4];
     uint32_t m2 = 0;
@@ -300,7 +297,7 @@ __global__ __launch_bounds__(256) void q10_scan(const uint8_t* __restrict__ rfla
       cu[j] = (m >> j) & 1 ? ocust[lane_of(k, j)] : -1;
       m2 |= (uint32_t)(cu[j] >= 0 && (uint32_t)cu[j] < cspan) << j;
     }
-    if (!m2) continue;
+    if (!m2) return;
     const int4 d = ld4nt(disc, g);
     const longlong2 a = ld2nt(price, 2 * g), b = ld2nt(price, 2 * g + 1);
 #pragma unroll
@@ -309,7 +306,7 @@ __global__ __launch_bounds__(256) void q10_scan(const uint8_t* __restrict__ rfla
         add_i64(&crev[cu[j]], lane_of(a, b, j) * (int64_t)(100 - lane_of(d, j)));
         seen[cu[j]] = 1;
       }
-  }
+  });
 }
 
 struct KeyedRevSrc {  // (revenue desc, key asc) over a per-key revenue array
@@ -880,14 +877,16 @@ int gdb_q8(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q8_params* p, gdb_q8_r
   GDB_TRY(key_span(lp, nl, &a));
   pspan = std::max(pspan, a);
   const uint64_t words = (pspan + 31) / 32;
+  const uint64_t words4 = (words + 3) & ~3ull;  // the folded filter starts 16-byte aligned
   uint8_t *omap, *cmap, *smap, *nreg;
-  uint32_t* pbits;
+  uint32_t *pbits, *pfold;
   int64_t *dout, *host;
   GDB_TRY(arena(ctx, A_MAP, ospan, (void**)&omap));
   GDB_TRY(arena(ctx, A_AUX, cspan + 256, (void**)&cmap));
   nreg = cmap + cspan;
   GDB_TRY(arena(ctx, A_AUX2, sspan, (void**)&smap));
-  GDB_TRY(arena(ctx, A_BITMAP, words * 4, (void**)&pbits));
+  GDB_TRY(arena(ctx, A_BITMAP, (words4 + FOLD_WORDS) * 4, (void**)&pbits));
+  pfold = pbits + words4;
   GDB_TRY(arena(ctx, A_OUT, 256, (void**)&dout));
   GDB_TRY(pinned(ctx, 256, (void**)&host));
   hipStream_t s = ctx->stream;
@@ -895,14 +894,14 @@ int gdb_q8(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q8_params* p, gdb_q8_r
   GDB_HIP(hipMemsetAsync(omap, NONE, ospan, s));
   GDB_HIP(hipMemsetAsync(cmap, NONE, cspan, s));
   GDB_HIP(hipMemsetAsync(smap, NONE, sspan, s));
-  GDB_HIP(hipMemsetAsync(pbits, 0, words * 4, s));
+  GDB_HIP(hipMemsetAsync(pbits, 0, (words4 + FOLD_WORDS) * 4, s));
   GDB_HIP(hipMemsetAsync(dout, 0, 17 * 8, s));
   GDB_TRY(build_nreg(ctx, db, nreg));
   if (np) {
     Mask4 m{{0, 0, 0, 0}};
     if (p->type_code >= 0 && p->type_code < 256) m.w[p->type_code >> 6] = 1ull << (p->type_code & 63);
     code_bits<<<blocks_for(ctx, code_bits, np), BLOCK, 0, s>>>(
-        (const int32_t*)pp->dev, (const uint8_t*)pt->dev, (uint32_t)np, m, pbits);
+        (const int32_t*)pp->dev, (const uint8_t*)pt->dev, (uint32_t)np, m, pbits, pfold);
   }
   if (nc)
     nation_bytes<<<blocks_for(ctx, nation_bytes, nc), BLOCK, 0, s>>>(
@@ -918,9 +917,9 @@ int gdb_q8(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q8_params* p, gdb_q8_r
   GDB_HIP(hipGetLastError());
   GDB_HIP(hipEventRecord(ctx->ev[1], s));
   if (nl)
-    q8_scan<<<blocks_for(ctx, q8_scan, quads(nl)), BLOCK, 0, s>>>(
+    q8_scan<<<fold_blocks(ctx, (const void*)q8_scan, quads(nl)), FOLD_BLOCK, 0, s>>>(
         (const int32_t*)lp->dev, (const int32_t*)lo->dev, (const int32_t*)lk->dev,
-        (const int32_t*)ld->dev, (const int64_t*)le->dev, (uint32_t)nl, pbits, omap, smap,
+        (const int32_t*)ld->dev, (const int64_t*)le->dev, (uint32_t)nl, pbits, pfold, omap, smap,
         p->nation, dout);
   GDB_HIP(hipGetLastError());
   GDB_HIP(hipEventRecord(ctx->ev[2], s));

@@ -143,14 +143,12 @@ __global__ __launch_bounds__(256) void q11_scan(const int32_t* __restrict__ pk,
                                                 int64_t* __restrict__ total) {
   __shared__ int64_t red[4];
   int64_t tot = 0;
-  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
-    const int4 s = ld4nt(sk, g);
+  walk2(quads(n), [&](uint32_t g) { return ld4nt(sk, g); }, [&](uint32_t g, const int4& s) {
     uint32_t m = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j)
       m |= (uint32_t)((4 * g + j < n) && bit_test(sbits, (uint32_t)lane_of(s, j))) << j;
-    if (!m) continue;
+    if (!m) return;
     const int4 p = ld4nt(pk, g), a = ld4nt(avail, g);
     const longlong2 c0 = ld2nt(cost, 2 * g), c1 = ld2nt(cost, 2 * g + 1);
     int key = -1;
@@ -170,7 +168,7 @@ __global__ __launch_bounds__(256) void q11_scan(const int32_t* __restrict__ pk,
       acc += v;
     }
     if (key >= 0) add_i64(&val[key], acc);
-  }
+  });
   block_add(tot, red, total);
 }
 
@@ -253,9 +251,12 @@ __global__ __launch_bounds__(256) void q16_scan(const int32_t* __restrict__ pk,
                                                 const uint8_t* __restrict__ bad,
                                                 unsigned long long* __restrict__ keys,
                                                 uint64_t mask, uint32_t* __restrict__ cnt) {
-  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
-    const int4 p = ld4nt(pk, g), s = ld4nt(sk, g);
+  struct Ld {
+    int4 p, s;
+  };
+  walk2(quads(n), [&](uint32_t g) { return Ld{ld4nt(pk, g), ld4nt(sk, g)}; },
+        [&](uint32_t g, const Ld& ld) {
+    const int4 p = ld.p, s = ld.s;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       if (4 * g + j >= n) continue;
@@ -266,7 +267,7 @@ __global__ __launch_bounds__(256) void q16_scan(const int32_t* __restrict__ pk,
       ht_insert(keys, mask, pair_key((uint32_t)grp, sv), &fresh);
       if (fresh) atomicAdd(&cnt[grp], 1u);
     }
-  }
+  });
 }
 
 __global__ __launch_bounds__(256) void q16_collect(const uint32_t* __restrict__ cnt, uint32_t n,
@@ -329,22 +330,20 @@ __global__ __launch_bounds__(256) void q20_scan(const int32_t* __restrict__ ship
                                                 const unsigned long long* __restrict__ keys,
                                                 uint64_t mask, int64_t* __restrict__ sum,
                                                 uint32_t* __restrict__ lines) {
-  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
-    const int4 d = ld4nt(ship, g);
+  walk2(quads(n), [&](uint32_t g) { return ld4nt(ship, g); }, [&](uint32_t g, const int4& d) {
     uint32_t m = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int v = lane_of(d, j);
       m |= (uint32_t)((4 * g + j < n) & (v >= lo) & (v < hi)) << j;
     }
-    if (!m) continue;
+    if (!m) return;
     const int4 p = ld4nt(pk, g);
     uint32_t m2 = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j)
       m2 |= (uint32_t)(((m >> j) & 1) && bit_test(bits, (uint32_t)lane_of(p, j))) << j;
-    if (!m2) continue;
+    if (!m2) return;
     const int4 s = ld4nt(sk, g), q = ld4nt(qty, g);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -355,7 +354,7 @@ __global__ __launch_bounds__(256) void q20_scan(const int32_t* __restrict__ ship
       add_i64(&sum[slot], lane_of(q, j));
       atomicAdd(&lines[slot], 1u);
     }
-  }
+  });
 }
 
 // good[suppkey] = 1 for pairs with lines and availqty * den > sum * num.

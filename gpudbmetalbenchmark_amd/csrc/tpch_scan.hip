@@ -23,16 +23,14 @@ __global__ __launch_bounds__(256) void q14_scan(const int32_t* __restrict__ ship
                                                 int64_t* __restrict__ out) {
   __shared__ int64_t red[4];
   int64_t sp = 0, st = 0;
-  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
-    const int4 s = ld4nt(ship, g);
+  walk2(quads(n), [&](uint32_t g) { return ld4nt(ship, g); }, [&](uint32_t g, const int4& s) {
     uint32_t m = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int v = lane_of(s, j);
       m |= (uint32_t)((4 * g + j < n) & (v >= lo) & (v < hi)) << j;
     }
-    if (!m) continue;
+    if (!m) return;
     const int4 p = ld4nt(pk, g), d = ld4nt(disc, g);
     const longlong2 a = ld2nt(price, 2 * g), b = ld2nt(price, 2 * g + 1);
 #pragma unroll
@@ -42,7 +40,7 @@ __global__ __launch_bounds__(256) void q14_scan(const int32_t* __restrict__ ship
       st += rev;
       if (bit_test(promo, (uint32_t)lane_of(p, j))) sp += rev;
     }
-  }
+  });
   block_add(sp, red, out);
   block_add(st, red, out + 1);
 }
@@ -87,16 +85,16 @@ __global__ __launch_bounds__(256) void q19_scan(const uint8_t* __restrict__ mode
                                                 int64_t* __restrict__ out) {
   __shared__ int64_t red[4];
   int64_t sum = 0;
-  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
-    const uint32_t mw = ld_u8x4nt(mode, g), iw = ld_u8x4nt(instr, g);
+  walk2(quads(n), [&](uint32_t g) { return make_uint2(ld_u8x4nt(mode, g), ld_u8x4nt(instr, g)); },
+        [&](uint32_t g, const uint2& w) {
+    const uint32_t mw = w.x, iw = w.y;
     uint32_t m = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j)
       m |= (uint32_t)((4 * g + j < n) & ((q.modes >> (byte_of(mw, j) & 31)) & 1) &
                       (byte_of(mw, j) < 32) & (byte_of(iw, j) == q.instruct))
            << j;
-    if (!m) continue;
+    if (!m) return;
     const int4 p = ld4nt(pk, g), qq = ld4nt(qty, g);
     uint32_t m2 = 0;
 #pragma unroll
@@ -109,13 +107,13 @@ __global__ __launch_bounds__(256) void q19_scan(const uint8_t* __restrict__ mode
         hit |= ((pb >> b) & 1) && qv >= q.qlo[b] && qv <= q.qhi[b];
       m2 |= (uint32_t)hit << j;
     }
-    if (!m2) continue;
+    if (!m2) return;
     const int4 d = ld4nt(disc, g);
     const longlong2 a = ld2nt(price, 2 * g), b = ld2nt(price, 2 * g + 1);
 #pragma unroll
     for (int j = 0; j < 4; ++j)
       if ((m2 >> j) & 1) sum += lane_of(a, b, j) * (int64_t)(100 - lane_of(d, j));
-  }
+  });
   block_add(sum, red, out);
 }
 
@@ -140,17 +138,21 @@ __global__ __launch_bounds__(256) void q12_scan(const uint8_t* __restrict__ mode
                                                 int64_t* __restrict__ out) {
   __shared__ int64_t red[4];
   int64_t c[4] = {0, 0, 0, 0};  // [mode index][high]
-  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
-    const uint32_t mw = ld_u8x4nt(mode, g);
-    const int4 r = ld4nt(receipt, g);
+  struct Ld {
+    uint32_t mw;
+    int4 r;
+  };
+  walk2(quads(n), [&](uint32_t g) { return Ld{ld_u8x4nt(mode, g), ld4nt(receipt, g)}; },
+        [&](uint32_t g, const Ld& ld) {
+    const uint32_t mw = ld.mw;
+    const int4 r = ld.r;
     uint32_t m = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int md = byte_of(mw, j), rv = lane_of(r, j);
       m |= (uint32_t)((4 * g + j < n) & ((md == m0) | (md == m1)) & (rv >= lo) & (rv < hi)) << j;
     }
-    if (!m) continue;
+    if (!m) return;
     const int4 cm = ld4nt(commit, g), s = ld4nt(ship, g), o = ld4nt(ok, g);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -160,7 +162,7 @@ __global__ __launch_bounds__(256) void q12_scan(const uint8_t* __restrict__ mode
       const int h = high[lane_of(o, j)] ? 1 : 0;
       c[mi * 2 + h] += 1;
     }
-  }
+  });
 #pragma unroll
   for (int i = 0; i < 4; ++i) block_add(c[i], red, out + i);
 }
@@ -172,13 +174,15 @@ __global__ __launch_bounds__(256) void q4_late(const int32_t* __restrict__ ok,
                                                const int32_t* __restrict__ commit,
                                                const int32_t* __restrict__ receipt, uint32_t n,
                                                uint8_t* __restrict__ late) {
-  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
-    const int4 c = ld4nt(commit, g), r = ld4nt(receipt, g), o = ld4nt(ok, g);
+  struct Ld {
+    int4 c, r, o;
+  };
+  walk2(quads(n), [&](uint32_t g) { return Ld{ld4nt(commit, g), ld4nt(receipt, g), ld4nt(ok, g)}; },
+        [&](uint32_t g, const Ld& ld) {
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-      if (4 * g + j < n && lane_of(c, j) < lane_of(r, j)) late[lane_of(o, j)] = 1;
-  }
+      if (4 * g + j < n && lane_of(ld.c, j) < lane_of(ld.r, j)) late[lane_of(ld.o, j)] = 1;
+  });
 }
 
 __global__ __launch_bounds__(256) void q4_count(const int32_t* __restrict__ date,
@@ -189,16 +193,14 @@ __global__ __launch_bounds__(256) void q4_count(const int32_t* __restrict__ date
                                                 int64_t* __restrict__ out) {
   __shared__ int64_t red[4];
   int64_t c[5] = {0, 0, 0, 0, 0};
-  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
-    const int4 d = ld4nt(date, g);
+  walk2(quads(n), [&](uint32_t g) { return ld4nt(date, g); }, [&](uint32_t g, const int4& d) {
     uint32_t m = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int v = lane_of(d, j);
       m |= (uint32_t)((4 * g + j < n) & (v >= lo) & (v < hi)) << j;
     }
-    if (!m) continue;
+    if (!m) return;
     const int4 o = ld4nt(ok, g);
     const uint32_t pw = ld_u8x4nt(prio, g);
 #pragma unroll
@@ -208,7 +210,7 @@ __global__ __launch_bounds__(256) void q4_count(const int32_t* __restrict__ date
 #pragma unroll
       for (int b = 0; b < 5; ++b) c[b] += pr == (uint32_t)b;
     }
-  }
+  });
 #pragma unroll
   for (int i = 0; i < 5; ++i) block_add(c[i], red, out + i);
 }
@@ -221,16 +223,14 @@ __global__ __launch_bounds__(256) void q15_scan(const int32_t* __restrict__ ship
                                                 const int64_t* __restrict__ price, uint32_t n,
                                                 int32_t lo, int32_t hi, int64_t* __restrict__ rev,
                                                 uint8_t* __restrict__ seen) {
-  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
-    const int4 s = ld4nt(ship, g);
+  walk2(quads(n), [&](uint32_t g) { return ld4nt(ship, g); }, [&](uint32_t g, const int4& s) {
     uint32_t m = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int v = lane_of(s, j);
       m |= (uint32_t)((4 * g + j < n) & (v >= lo) & (v < hi)) << j;
     }
-    if (!m) continue;
+    if (!m) return;
     const int4 k = ld4nt(sk, g), d = ld4nt(disc, g);
     const longlong2 a = ld2nt(price, 2 * g), b = ld2nt(price, 2 * g + 1);
 #pragma unroll
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(256) void q15_scan(const int32_t* __restrict__ ship
       add_i64(&rev[key], lane_of(a, b, j) * (int64_t)(100 - lane_of(d, j)));
       seen[key] = 1;
     }
-  }
+  });
 }
 
 // Order-preserving map of int64 onto uint64 (for atomicMax).
@@ -274,47 +274,44 @@ __global__ __launch_bounds__(256) void q15_collect(const int64_t* __restrict__ r
 
 // ---- Q17 (reference kernels/q17.metal:11-57, src/q17.cpp) ----------------------
 
-// bits[partkey] = (p_brand, p_container) = (brand, container).
+// bits[partkey] (and the folded filter) = (p_brand, p_container) = (brand, container).
 __global__ __launch_bounds__(256) void q17_part(const int32_t* __restrict__ pk,
                                                 const uint8_t* __restrict__ brand,
                                                 const uint8_t* __restrict__ cont, uint32_t n,
-                                                int32_t b, int32_t c, uint32_t* __restrict__ bits) {
+                                                int32_t b, int32_t c, uint32_t* __restrict__ bits,
+                                                uint32_t* __restrict__ fold) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    if (brand[i] == b && cont[i] == c) bit_set(bits, (uint32_t)pk[i]);
+    if (brand[i] == b && cont[i] == c) {
+      bit_set(bits, (uint32_t)pk[i]);
+      fold_set(fold, (uint32_t)pk[i]);
+    }
 }
 
-// Pass 1: for lines of selected parts, per-part quantity sum and line count,
-// and the line's row id appended to a list (wave-aggregated append).
-__global__ __launch_bounds__(256) void q17_stats(const int32_t* __restrict__ pk,
-                                                 const int32_t* __restrict__ qty, uint32_t n,
-                                                 const uint32_t* __restrict__ sel,
-                                                 int64_t* __restrict__ sumq,
-                                                 uint32_t* __restrict__ cnt,
-                                                 uint32_t* __restrict__ list,
-                                                 uint32_t* __restrict__ list_n) {
-  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
-  const int lane = threadIdx.x & 63;
-  // every lane of a wave runs the same number of trips (wave-level append)
-  const uint32_t wave_base = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
-  for (uint32_t gw = wave_base; gw < nq; gw += stride) {
-    const uint32_t g = gw + lane;
-    uint32_t m = 0;
-    int4 p = make_int4(0, 0, 0, 0);
-    if (g < nq) {
-      p = ld4nt(pk, g);
+// Bits of the 4 rows of group g whose part is selected (LDS filter, then bitmap).
+__device__ __forceinline__ uint32_t q17_sel(const int4& p, uint32_t g, uint32_t n,
+                                            const uint32_t* lfold, const uint32_t* bits) {
+  uint32_t m = 0;
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        m |= (uint32_t)((4 * g + j < n) && bit_test(sel, (uint32_t)lane_of(p, j))) << j;
-    }
-    const uint32_t c = __popc(m);
-    const uint32_t incl = wave_incl_scan(c);
-    const uint32_t total = __shfl(incl, 63, 64);
-    if (total == 0) continue;
-    uint32_t base = 0;
-    if (lane == 63) base = atomicAdd(list_n, total);
-    base = __shfl(base, 63, 64);
-    uint32_t pos = base + incl - c;
-    if (!m) continue;
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t key = (uint32_t)lane_of(p, j);
+    m |= (uint32_t)((4 * g + j < n) && fold_test(lfold, key) && bit_test(bits, key)) << j;
+  }
+  return m;
+}
+
+// Pass 1: per selected part, the quantity sum and line count of its lines.
+__global__ __launch_bounds__(FOLD_BLOCK) void q17_stats(const int32_t* __restrict__ pk,
+                                                        const int32_t* __restrict__ qty,
+                                                        uint32_t n,
+                                                        const uint32_t* __restrict__ sel,
+                                                        const uint32_t* __restrict__ fold,
+                                                        int64_t* __restrict__ sumq,
+                                                        uint32_t* __restrict__ cnt) {
+  __shared__ __align__(16) uint32_t lfold[FOLD_WORDS];
+  fold_load(lfold, fold);
+  walk2(quads(n), [&](uint32_t g) { return ld4nt(pk, g); }, [&](uint32_t g, const int4& p) {
+    const uint32_t m = q17_sel(p, g, n, lfold, sel);
+    if (!m) return;
     const int4 q = ld4nt(qty, g);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -322,29 +319,38 @@ __global__ __launch_bounds__(256) void q17_stats(const int32_t* __restrict__ pk,
       const uint32_t key = (uint32_t)lane_of(p, j);
       add_i64(&sumq[key], lane_of(q, j));
       atomicAdd(&cnt[key], 1u);
-      list[pos++] = 4 * g + j;
     }
-  }
+  });
 }
 
-// Pass 2 over the listed lines only: quantity * count * den < sum * num.
-__global__ __launch_bounds__(256) void q17_sum(const int32_t* __restrict__ pk,
-                                               const int32_t* __restrict__ qty,
-                                               const int64_t* __restrict__ price,
-                                               const uint32_t* __restrict__ list,
-                                               const uint32_t* __restrict__ list_n,
-                                               const int64_t* __restrict__ sumq,
-                                               const uint32_t* __restrict__ cnt, int32_t num,
-                                               int32_t den, int64_t* __restrict__ out) {
-  __shared__ int64_t red[4];
-  const uint32_t m = *list_n;
-  int64_t s = 0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
-    const uint32_t r = list[i];
-    const uint32_t key = (uint32_t)pk[r];
-    if ((int64_t)qty[r] * cnt[key] * den < sumq[key] * num) s += price[r];
-  }
-  block_add(s, red, out);
+// Pass 2: SUM(extendedprice) of the selected lines with quantity * count * den <
+// sum * num (quantity below num/den of the part's average).
+__global__ __launch_bounds__(FOLD_BLOCK) void q17_sum(const int32_t* __restrict__ pk,
+                                                      const int32_t* __restrict__ qty,
+                                                      const int64_t* __restrict__ price,
+                                                      uint32_t n, const uint32_t* __restrict__ sel,
+                                                      const uint32_t* __restrict__ fold,
+                                                      const int64_t* __restrict__ sumq,
+                                                      const uint32_t* __restrict__ cnt,
+                                                      int32_t num, int32_t den,
+                                                      int64_t* __restrict__ out) {
+  __shared__ __align__(16) uint32_t lfold[FOLD_WORDS];
+  __shared__ int64_t red[FOLD_BLOCK / 64];
+  fold_load(lfold, fold);
+  int64_t sum = 0;
+  walk2(quads(n), [&](uint32_t g) { return ld4nt(pk, g); }, [&](uint32_t g, const int4& p) {
+    const uint32_t m = q17_sel(p, g, n, lfold, sel);
+    if (!m) return;
+    const int4 q = ld4nt(qty, g);
+    const longlong2 a = ld2nt(price, 2 * g), b = ld2nt(price, 2 * g + 1);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (!((m >> j) & 1)) continue;
+      const uint32_t key = (uint32_t)lane_of(p, j);
+      if ((int64_t)lane_of(q, j) * cnt[key] * den < sumq[key] * num) sum += lane_of(a, b, j);
+    }
+  });
+  block_add(sum, red, out);
 }
 
 // ---- Q22 (reference kernels/q22.metal:11-67, src/q22.cpp) ----------------------
@@ -355,13 +361,11 @@ struct Q22Codes {
 
 __global__ __launch_bounds__(256) void mark_bytes(const int32_t* __restrict__ key, uint32_t n,
                                                   uint8_t* __restrict__ map) {
-  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
-    const int4 k = ld4nt(key, g);
+  walk2(quads(n), [&](uint32_t g) { return ld4nt(key, g); }, [&](uint32_t g, const int4& k) {
 #pragma unroll
     for (int j = 0; j < 4; ++j)
       if (4 * g + j < n) map[lane_of(k, j)] = 1;
-  }
+  });
 }
 
 __device__ __forceinline__ int q22_slot(const Q22Codes& c, int32_t nation) {
@@ -679,42 +683,41 @@ int gdb_q17(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q17_params* p, gdb_q1
   GDB_TRY(key_span(pp, np, &s1));
   GDB_TRY(key_span(lp, nl, &s2));
   const uint64_t span = std::max(s1, s2), words = (span + 31) / 32;
-  uint32_t *bits, *cnt, *list;
+  const uint64_t words4 = (words + 3) & ~3ull;  // the folded filter starts 16-byte aligned
+  uint32_t *bits, *fold, *cnt;
   int64_t *sumq, *dout, *host;
-  GDB_TRY(arena(ctx, A_BITMAP, words * 4, (void**)&bits));
+  GDB_TRY(arena(ctx, A_BITMAP, (words4 + FOLD_WORDS) * 4, (void**)&bits));
+  fold = bits + words4;
   GDB_TRY(arena(ctx, A_AGG, span * 8, (void**)&sumq));
   GDB_TRY(arena(ctx, A_MAP, span * 4, (void**)&cnt));
-  GDB_TRY(arena(ctx, A_AUX, std::max<uint64_t>(nl, 1) * 4, (void**)&list));
   GDB_TRY(arena(ctx, A_OUT, 64, (void**)&dout));
   GDB_TRY(pinned(ctx, 64, (void**)&host));
-  uint32_t* list_n = (uint32_t*)(dout + 1);
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_HIP(hipMemsetAsync(bits, 0, words * 4, s));
+  GDB_HIP(hipMemsetAsync(bits, 0, (words4 + FOLD_WORDS) * 4, s));
   GDB_HIP(hipMemsetAsync(sumq, 0, span * 8, s));
   GDB_HIP(hipMemsetAsync(cnt, 0, span * 4, s));
   GDB_HIP(hipMemsetAsync(dout, 0, 16, s));
   if (np)
     q17_part<<<blocks_for(ctx, q17_part, np), BLOCK, 0, s>>>(
         (const int32_t*)pp->dev, (const uint8_t*)pb->dev, (const uint8_t*)pc->dev, (uint32_t)np,
-        p->brand, p->container, bits);
+        p->brand, p->container, bits, fold);
   GDB_HIP(hipGetLastError());
   GDB_HIP(hipEventRecord(ctx->ev[1], s));
   if (nl)
-    q17_stats<<<blocks_for(ctx, q17_stats, quads(nl)), BLOCK, 0, s>>>(
-        (const int32_t*)lp->dev, (const int32_t*)lq->dev, (uint32_t)nl, bits, sumq, cnt, list,
-        list_n);
+    q17_stats<<<fold_blocks(ctx, (const void*)q17_stats, quads(nl)), FOLD_BLOCK, 0, s>>>(
+        (const int32_t*)lp->dev, (const int32_t*)lq->dev, (uint32_t)nl, bits, fold, sumq, cnt);
   GDB_HIP(hipGetLastError());
   GDB_HIP(hipEventRecord(ctx->ev[2], s));
   if (nl)
-    q17_sum<<<blocks_for(ctx, q17_sum, nl / 64 + 1), BLOCK, 0, s>>>(
-        (const int32_t*)lp->dev, (const int32_t*)lq->dev, (const int64_t*)le->dev, list, list_n,
-        sumq, cnt, p->frac_num, p->frac_den, dout);
+    q17_sum<<<fold_blocks(ctx, (const void*)q17_sum, quads(nl)), FOLD_BLOCK, 0, s>>>(
+        (const int32_t*)lp->dev, (const int32_t*)lq->dev, (const int64_t*)le->dev, (uint32_t)nl,
+        bits, fold, sumq, cnt, p->frac_num, p->frac_den, dout);
   GDB_HIP(hipGetLastError());
   GDB_HIP(hipMemcpyAsync(host, dout, 16, hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
   // SQL columns: l_partkey, l_quantity (4 B), l_extendedprice (8 B); p_partkey,
-  // p_brand, p_container. Mandatory: l_partkey, 4 B per line.
+  // p_brand, p_container. Mandatory per pass: l_partkey, 4 B per line.
   GDB_TRY(timing_finish(ctx, timing, 16 * nl + 6 * np, 4 * nl));
   out->sum_price = host[0];
   out->avg_yearly = (double)host[0] / 100.0 / 7.0;
