@@ -81,7 +81,7 @@ class Workload:
 TPCH_KERNELS = {"q2": "q2_min", "q4": "q4_late", "q5": "q5_scan", "q7": "q7_scan",
                 "q8": "q8_scan", "q10": "q10_scan", "q11": "q11_scan", "q12": "q12_scan",
                 "q14": "q14_scan", "q15": "q15_scan", "q16": "q16_scan", "q17": "q17_stats",
-                "q18": "q18_sums", "q19": "q19_scan", "q20": "q20_scan", "q21": "q21_track",
+                "q18": "q18_owner", "q19": "q19_scan", "q20": "q20_scan", "q21": "q21_owner",
                 "q22": "mark_bytes"}
 TPCH_SF = {"sf1": 1.0, "sf10": 10.0}
 
