@@ -67,7 +67,9 @@ def build_driver():
     src = os.path.join(CSRC, "driver.cpp")
     if not os.path.exists(src):
         return None
-    _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", src, "-o", DRIVER, f"-L{PKG}", "-lgdb",
+    srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC)
+                  if f.startswith("driver") and f.endswith(".cpp"))
+    _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}"] + srcs + ["-o", DRIVER, f"-L{PKG}", "-lgdb",
           "-Wl,-rpath,$ORIGIN", "-lpthread"])
     return DRIVER
 

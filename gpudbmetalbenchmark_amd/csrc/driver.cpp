@@ -24,6 +24,11 @@
 
 #include "gdb.h"
 
+// The other 17 TPC-H pipelines (driver_tpch.cpp): they need a whole TPC-H
+// database of .tbl files.
+const std::vector<std::string>& tpch_queries();
+int run_tpch(gdb_ctx* ctx, const std::string& dir, const std::vector<std::string>& queries);
+
 namespace {
 
 using Clock = std::chrono::steady_clock;
@@ -445,7 +450,8 @@ void help() {
   printf("  data: dbgen .tbl files from DIR, or from data/SF-1, data/SF-10, data/SF-100 when\n"
          "        present; otherwise seeded synthetic tables of the scale's sizes\n");
   printf("  scales: sf1 (6M fact rows), sf10 (60M), sf100 (600M, streamed from pinned host memory)\n");
-  printf("  queries: all, q1, q3, q6, q9, q13, selection, aggregation, join\n");
+  printf("  queries: all, q1, q3, q6, q9, q13, selection, aggregation, join, and with .tbl\n"
+         "           data: q2, q4, q5, q7, q8, q10, q11, q12, q14 .. q22\n");
 }
 
 }  // namespace
@@ -479,12 +485,23 @@ int main(int argc, char** argv) {
   const std::vector<std::string> flat_q = {"q1", "q6"};
   const std::vector<std::string> join_q = {"q3", "q9", "q13", "selection", "aggregation", "join"};
   const std::vector<std::string> all_q = {"selection", "aggregation", "join", "q1", "q3", "q6", "q9", "q13"};
-  std::vector<std::string> run;
+  std::vector<std::string> run, run_tpch_q;
+  const auto& tq = tpch_queries();
   if (query == "all") {
     run = all_q;
     if (s.stream) run = {"q1", "q3", "q6", "q9", "q13"};
+    if (!g_data.empty()) run_tpch_q = tq;
   } else if (std::find(all_q.begin(), all_q.end(), query) != all_q.end()) {
     run = {query};
+  } else if (std::find(tq.begin(), tq.end(), query) != tq.end()) {
+    if (g_data.empty()) {
+      fprintf(stderr,
+              "%s needs a TPC-H database: dbgen .tbl files in data/SF-N or --data DIR "
+              "(python -m gpudbmetalbenchmark_amd.tpch --dump DIR writes a synthetic one)\n",
+              query.c_str());
+      return 1;
+    }
+    run_tpch_q = {query};
   } else {
     fprintf(stderr, "Unknown query: %s\nUse 'help' to see available options.\n", query.c_str());
     return 1;
@@ -523,6 +540,7 @@ int main(int argc, char** argv) {
     else if (q == "join") run_join(s, j);
     printf("\n");
   }
+  if (!run_tpch_q.empty()) run_tpch(g_ctx, g_data, run_tpch_q);
   for (Buf* b : g_bufs) delete b;
   gdb_close(g_ctx);
   return 0;

@@ -154,3 +154,167 @@ def load_columns(directory, mmap=True):
         else:
             out[name] = np.fromfile(p, dtype=dt).reshape(shape)
     return out
+
+
+# ---- whole TPC-H databases (all eight tables, the layout of tpch.generate) ----
+# dbgen's field order per table; None marks a field the engine does not keep.
+DB_FIELDS = {
+    "region": ["r_regionkey", "r_name", None],
+    "nation": ["n_nationkey", "n_name", "n_regionkey", None],
+    "part": ["p_partkey", "p_name", "p_mfgr", "p_brand", "p_type", "p_size", "p_container",
+             "p_retailprice", None],
+    "supplier": ["s_suppkey", "s_name", None, "s_nationkey", "s_phone", "s_acctbal",
+                 "s_comment"],
+    "partsupp": ["ps_partkey", "ps_suppkey", "ps_availqty", "ps_supplycost", None],
+    "customer": ["c_custkey", "c_name", None, "c_nationkey", "c_phone", "c_acctbal",
+                 "c_mktsegment", None],
+    "orders": ["o_orderkey", "o_custkey", "o_orderstatus", "o_totalprice", "o_orderdate",
+               "o_orderpriority", "o_clerk", "o_shippriority", "o_comment"],
+    "lineitem": ["l_orderkey", "l_partkey", "l_suppkey", "l_linenumber", "l_quantity",
+                 "l_extendedprice", "l_discount", "l_tax", "l_returnflag", "l_linestatus",
+                 "l_shipdate", "l_commitdate", "l_receiptdate", "l_shipinstruct", "l_shipmode",
+                 None],
+}
+# engine column -> (parse kind, text width); dictionary columns parse as text first
+_DB_KIND = {
+    "l_orderkey": abi.TBL_INT32, "l_partkey": abi.TBL_INT32, "l_suppkey": abi.TBL_INT32,
+    "l_quantity": abi.TBL_INT32, "l_extendedprice": abi.TBL_CENTS,
+    "l_discount": abi.TBL_HUNDREDTHS, "l_tax": abi.TBL_HUNDREDTHS, "l_returnflag": abi.TBL_CHAR,
+    "l_linestatus": abi.TBL_CHAR, "l_shipdate": abi.TBL_DATE, "l_commitdate": abi.TBL_DATE,
+    "l_receiptdate": abi.TBL_DATE, "l_shipinstruct": (abi.TBL_TEXT, 25),
+    "l_shipmode": (abi.TBL_TEXT, 10),
+    "o_orderkey": abi.TBL_INT32, "o_custkey": abi.TBL_INT32, "o_orderstatus": abi.TBL_CHAR,
+    "o_totalprice": abi.TBL_CENTS, "o_orderdate": abi.TBL_DATE,
+    "o_orderpriority": (abi.TBL_TEXT, 15), "o_shippriority": abi.TBL_INT32,
+    "o_comment": (abi.TBL_TEXT, 80),
+    "c_custkey": abi.TBL_INT32, "c_nationkey": abi.TBL_INT32, "c_acctbal": abi.TBL_CENTS,
+    "c_mktsegment": (abi.TBL_TEXT, 10),
+    "p_partkey": abi.TBL_INT32, "p_name": (abi.TBL_TEXT, 55), "p_mfgr": (abi.TBL_TEXT, 25),
+    "p_brand": (abi.TBL_TEXT, 10), "p_type": (abi.TBL_TEXT, 25), "p_size": abi.TBL_INT32,
+    "p_container": (abi.TBL_TEXT, 10), "p_retailprice": abi.TBL_CENTS,
+    "ps_partkey": abi.TBL_INT32, "ps_suppkey": abi.TBL_INT32, "ps_availqty": abi.TBL_INT32,
+    "ps_supplycost": abi.TBL_CENTS,
+    "s_suppkey": abi.TBL_INT32, "s_nationkey": abi.TBL_INT32, "s_acctbal": abi.TBL_CENTS,
+    "s_comment": (abi.TBL_TEXT, 48),
+    "n_nationkey": abi.TBL_INT32, "n_regionkey": abi.TBL_INT32, "r_regionkey": abi.TBL_INT32,
+}
+
+
+def _dicts():
+    from . import tpch
+    return {"l_shipinstruct": tpch.L_SHIPINSTRUCT, "l_shipmode": tpch.L_SHIPMODE,
+            "o_orderpriority": tpch.O_ORDERPRIORITY, "c_mktsegment": tpch.C_MKTSEGMENT,
+            "p_type": tpch.P_TYPE, "p_container": tpch.P_CONTAINER}
+
+
+def _codes(text, dictionary):
+    """Fixed-width text rows -> dictionary codes (255 for values not in it)."""
+    out = np.full(len(text), 255, np.uint8)
+    w = text.shape[1]
+    for i, s in enumerate(dictionary):
+        pat = np.zeros(w, np.uint8)
+        b = s.encode()[:w]
+        pat[:len(b)] = np.frombuffer(b, np.uint8)
+        out[(text == pat).all(axis=1)] = i
+    return out
+
+
+def _tail_number(text):
+    """'Brand#23' / 'Manufacturer#4' rows -> 23 / 4 (digits after the last '#')."""
+    out = np.zeros(len(text), np.uint8)
+    for r, row in enumerate(text):
+        s = bytes(row).split(b"\0", 1)[0]
+        i = s.rfind(b"#")
+        out[r] = int(s[i + 1:]) if i >= 0 and s[i + 1:].isdigit() else 0
+    return out
+
+
+def load_database(directory, nthreads=0):
+    """The eight .tbl files of a TPC-H database (dbgen output, or dump_database)
+    as the engine's columns, in the layout of tpch.generate."""
+    lib = abi.load()
+    dicts = _dicts()
+    db = {}
+    for table, names in DB_FIELDS.items():
+        path = os.path.join(directory, table + ".tbl")
+        bpath = os.fsencode(path)
+        rows = C.c_uint64()
+        abi.check(lib.gdb_tbl_count_rows(bpath, nthreads, C.byref(rows)))
+        n = rows.value
+        spec = [(i, nm) for i, nm in enumerate(names) if nm in _DB_KIND]
+        fields = (abi.TblField * len(spec))()
+        cols = {}
+        for k, (i, nm) in enumerate(spec):
+            kind = _DB_KIND[nm]
+            kind, width = kind if isinstance(kind, tuple) else (kind, 0)
+            arr = np.empty((n, width) if kind == abi.TBL_TEXT else (n,), _DTYPE[kind])
+            cols[nm] = arr
+            fields[k] = abi.TblField(i, kind, width, arr.ctypes.data if n else None)
+        if n:
+            abi.check(lib.gdb_tbl_parse(bpath, fields, len(spec), n, nthreads))
+        for nm in list(cols):
+            if nm in dicts:
+                cols[nm] = _codes(cols[nm], dicts[nm])
+            elif nm in ("p_brand", "p_mfgr"):
+                cols[nm] = _tail_number(cols[nm])
+        if table == "orders" and "o_shippriority" not in cols:
+            cols["o_shippriority"] = np.zeros(n, np.int32)
+        db[table] = cols
+    return db
+
+
+def dump_database(directory, db):
+    """Writes a tpch.generate database as dbgen-format .tbl files (one per
+    table); text the engine does not keep is synthesised from the keys, as
+    dbgen formats it (Supplier#000000001, Customer#..., Clerk#...)."""
+    from . import tpch
+    os.makedirs(directory, exist_ok=True)
+    dicts = _dicts()
+    for table, names in DB_FIELDS.items():
+        cols = db[table]
+        n = len(next(iter(cols.values())))
+        key = cols[names[0]]
+        with open(os.path.join(directory, table + ".tbl"), "w", encoding="latin-1",
+                  newline="\n") as f:
+            for r in range(n):
+                parts = []
+                for nm in names:
+                    if nm is None:
+                        parts.append("")
+                    elif nm == "r_name":
+                        parts.append(tpch.R_NAME[int(key[r])])
+                    elif nm == "n_name":
+                        parts.append(tpch.N_NAME[int(key[r])])
+                    elif nm == "s_name":
+                        parts.append(f"Supplier#{int(key[r]):09d}")
+                    elif nm == "c_name":
+                        parts.append(f"Customer#{int(key[r]):09d}")
+                    elif nm == "o_clerk":
+                        parts.append(f"Clerk#{1 + int(key[r]) % 1000:09d}")
+                    elif nm in ("s_phone", "c_phone"):
+                        nat = int(cols[nm[0] + "_nationkey"][r])
+                        k = int(key[r])
+                        parts.append(f"{nat + 10}-{100 + k % 900}-{200 + k % 700}-{1000 + k % 9000}")
+                    elif nm == "l_linenumber":
+                        parts.append("1")
+                    elif nm == "p_mfgr":
+                        parts.append(f"Manufacturer#{int(cols[nm][r])}")
+                    elif nm == "p_brand":
+                        parts.append(tpch.brand_name(cols[nm][r]))
+                    elif nm in dicts:
+                        parts.append(dicts[nm][int(cols[nm][r])])
+                    else:
+                        kind = _DB_KIND[nm]
+                        kind = kind[0] if isinstance(kind, tuple) else kind
+                        v = cols[nm][r]
+                        if kind == abi.TBL_DATE:
+                            parts.append(_date_text(v))
+                        elif kind in (abi.TBL_CENTS, abi.TBL_HUNDREDTHS):
+                            parts.append(_money_text(v))
+                        elif kind == abi.TBL_CHAR:
+                            parts.append(chr(int(v)))
+                        elif kind == abi.TBL_TEXT:
+                            parts.append(_bytes_text(v))
+                        else:
+                            parts.append(str(int(v)))
+                f.write("|".join(parts) + "|\n")

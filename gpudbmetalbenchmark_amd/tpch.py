@@ -261,3 +261,17 @@ def generate(sf=0.01, seed=7, comments=True):
 def text(field):
     """A fixed-width NUL-padded byte field as str."""
     return bytes(field).split(b"\0", 1)[0].decode()
+
+
+if __name__ == "__main__":
+    import argparse
+    import os
+    ap = argparse.ArgumentParser(description="Write a seeded TPC-H-shaped database as dbgen "
+                                             "format .tbl files (for gdb_bench --data DIR)")
+    ap.add_argument("--dump", required=True, help="output directory")
+    ap.add_argument("--sf", type=float, default=0.1)
+    ap.add_argument("--seed", type=int, default=7)
+    a = ap.parse_args()
+    from . import tbl
+    tbl.dump_database(a.dump, generate(a.sf, a.seed))
+    print("wrote", sorted(os.listdir(a.dump)))

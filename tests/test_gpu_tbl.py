@@ -75,3 +75,38 @@ def test_harness_writes_the_reference_csv(tmp_path):
     csv = (tmp_path / "gpu_results.csv").read_text().splitlines()
     assert csv[0] == harness.HEADER and csv[1] == rows[0]
     assert (tmp_path / "gpu_logs" / "t0" / "SF-1_Q6.log").exists()
+
+
+def test_cli_runs_the_tpch_pipelines_over_a_tbl_database(tmp_path):
+    """gdb_bench --data DIR all over a whole TPC-H database in dbgen format:
+    every one of the 22 registry names runs, each prints the reference's
+    `Qn | N rows` header, and printed results equal the oracle's."""
+    import os
+    import re
+    import subprocess
+    import tpch_oracle as T
+    from gpudbmetalbenchmark_amd import tbl, tpch
+    db = tpch.generate(0.02, 4)
+    tbl.dump_database(str(tmp_path), db)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "gpudbmetalbenchmark_amd", "gdb_bench")
+    print("case: gdb_bench sf1 all --data <tpch sf0.02>", flush=True)
+    r = subprocess.run([exe, "sf1", "all", "--data", str(tmp_path)], capture_output=True,
+                       text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = r.stdout
+    for q in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22):
+        assert re.search(rf"^Q{q} \| \d+ rows$", out, re.M), f"Q{q} header missing"
+    promo, total = T.q14(db)
+    m = re.search(r"Promo Revenue = ([0-9.]+)%", out)
+    assert m and abs(float(m.group(1)) - round(100.0 * promo / total, 2)) < 0.006
+    m = re.search(r"avg_yearly = ([0-9.]+)", out)
+    assert m and abs(float(m.group(1)) - T.q17(db) / 100.0 / 7.0) < 0.006
+    for code, cnt, bal in T.q22(db):
+        assert f"  {code} numcust {cnt} totacctbal {bal / 100.0:.2f}" in out
+    for prio, cnt in T.q4(db):
+        assert re.search(rf"^  {re.escape(tpch.O_ORDERPRIORITY[prio])}\s+{cnt}$", out, re.M)
+    rows = T.q21(db)
+    assert f"TPC-H Q21: {min(len(rows), 100)} of" in out
+    for sk, n in rows[:5]:
+        assert f"  Supplier#{sk:09d} numwait {n}" in out

@@ -145,3 +145,20 @@ def test_binary_columns_round_trip(tmp_path):
         assert set(back) == set(cols)
         for k in cols:
             assert back[k].dtype == cols[k].dtype and np.array_equal(back[k], cols[k]), k
+
+
+def test_tpch_database_round_trip(tmp_path):
+    """A whole TPC-H-shaped database written in dbgen's field layout (every
+    field, dictionary text, Brand#MN, Manufacturer#M, names, phones) parses
+    back into exactly the same engine columns."""
+    from gpudbmetalbenchmark_amd import tpch
+    db = tpch.generate(0.005, 9)
+    tbl.dump_database(str(tmp_path), db)
+    back = tbl.load_database(str(tmp_path))
+    for t, cols in db.items():
+        assert set(back[t]) == set(cols), t
+        for k, v in cols.items():
+            assert np.array_equal(np.asarray(v), back[t][k]), (t, k)
+    line = open(tmp_path / "part.tbl").readline().split("|")
+    assert line[2].startswith("Manufacturer#") and line[3].startswith("Brand#")
+    assert line[4] in tpch.P_TYPE and line[6] in tpch.P_CONTAINER
