@@ -514,8 +514,9 @@ int gdb_q2(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q2_params* p, gdb_q2_r
   GDB_HIP(hipMemcpyAsync(host, dout, out_bytes, hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
   // SQL columns: part key, size (8 B), type, mfgr (2 B); supplier key, nation (8 B),
-  // acctbal (8 B); partsupp key pair (8 B), supplycost (8 B), read twice.
-  GDB_TRY(timing_finish(ctx, timing, 10 * np + 16 * ns + 32 * nps, 16 * nps));
+  // acctbal (8 B); partsupp key pair (8 B), supplycost (8 B), read twice. Mandatory in
+  // q2_min: ps_partkey, 4 B per row (suppkey and cost only for selected parts).
+  GDB_TRY(timing_finish(ctx, timing, 10 * np + 16 * ns + 32 * nps, 4 * nps));
   const uint64_t* h = (const uint64_t*)host;
   const TopEntry* hf = (const TopEntry*)(h + 2);
   out->matches = (int64_t)h[0];
@@ -683,12 +684,6 @@ int gdb_q16(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q16_params* p, gdb_q1
   // SQL columns: partsupp key pair (8 B); part key, size (8 B), brand, type (2 B);
   // supplier key + comment. Mandatory in q16_scan: the partsupp key pair.
   GDB_TRY(timing_finish(ctx, timing, 8 * nps + 10 * np + (4 + sc->width) * ns, 8 * nps));
-  std::vector<gdb_q16_row> out(n);
-  for (uint64_t i = 0; i < n; ++i) {
-    const uint32_t g = got[i].x;
-    out[i] = gdb_q16_row{(int32_t)(g / 8 / 256), (int32_t)((g / 8) % 256),
-                         qpart.sizes[g % 8], (int32_t)got[i].y};
-  }
   static const std::vector<int> trank = [] {  // type code -> position in name order
     std::vector<int> idx(256), rank(256);
     for (int c = 0; c < 256; ++c) idx[c] = c;
@@ -696,12 +691,40 @@ int gdb_q16(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q16_params* p, gdb_q1
     for (int r = 0; r < 256; ++r) rank[idx[r]] = r;
     return rank;
   }();
-  std::sort(out.begin(), out.end(), [&](const gdb_q16_row& x, const gdb_q16_row& y) {
-    if (x.supplier_cnt != y.supplier_cnt) return x.supplier_cnt > y.supplier_cnt;
-    if (x.brand != y.brand) return x.brand < y.brand;
-    if (x.type != y.type) return trank[x.type & 255] < trank[y.type & 255];
-    return x.size < y.size;
-  });
+  int tcode[256];
+  for (int c = 0; c < 256; ++c) tcode[trank[c]] = c;
+  // size position -> rank among the sizes in numeric order (ORDER BY p_size)
+  int srank[8], spos[8];
+  for (int i = 0; i < p->nsizes; ++i) {
+    srank[i] = 0;
+    for (int j = 0; j < p->nsizes; ++j)
+      srank[i] += p->sizes[j] < p->sizes[i] || (p->sizes[j] == p->sizes[i] && j < i);
+    spos[srank[i]] = i;
+  }
+  // One 64-bit key per group carries its whole ordering: supplier count
+  // descending, then brand, type name and size ascending; sorted by an LSD radix
+  // sort (8 passes of 8 bits) and decoded back into rows.
+  std::vector<uint64_t> key(n), tmp(n);
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint32_t g = got[i].x;
+    const uint32_t brand = g / 8 / 256, type = (g / 8) % 256, pos = g % 8;
+    key[i] = ((uint64_t)(0xFFFFFFFFu - got[i].y) << 32) | (brand << 16) |
+             ((uint32_t)trank[type] << 8) | (uint32_t)srank[pos];
+  }
+  for (int pass = 0; pass < 8; ++pass) {
+    const int sh = pass * 8;
+    uint64_t cnt[257] = {0};
+    for (uint64_t i = 0; i < n; ++i) ++cnt[((key[i] >> sh) & 0xFF) + 1];
+    for (int d = 0; d < 256; ++d) cnt[d + 1] += cnt[d];
+    for (uint64_t i = 0; i < n; ++i) tmp[cnt[(key[i] >> sh) & 0xFF]++] = key[i];
+    key.swap(tmp);
+  }
+  std::vector<gdb_q16_row> out(n);
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint64_t k = key[i];
+    out[i] = gdb_q16_row{(int32_t)((k >> 16) & 0xFFFF), tcode[(k >> 8) & 0xFF],
+                         qpart.sizes[spos[k & 0xFF]], (int32_t)(0xFFFFFFFFu - (uint32_t)(k >> 32))};
+  }
   *nrows = (int64_t)n;
   for (int64_t i = 0; i < std::min<int64_t>(cap, (int64_t)n); ++i) rows[i] = out[i];
   return GDB_OK;

@@ -21,6 +21,7 @@ import argparse
 import collections
 import csv
 import json
+import os
 import re
 import sys
 
@@ -98,12 +99,18 @@ def main(argv=None):
     t.add_argument("--write", required=True)
     t.add_argument("--copy-bytes", type=int, default=1 << 31)
     t.add_argument("--out")
+    t.add_argument("--merge", action="store_true",
+                   help="update the kernels of an existing --out file, keeping the others")
     s = sub.add_parser("stats")
     s.add_argument("--csv", required=True)
     s.add_argument("--top", type=int, default=40)
     a = ap.parse_args(argv)
     if a.cmd == "traffic":
         res = traffic(a.fetch, a.write, a.copy_bytes)
+        if a.merge and a.out and os.path.exists(a.out):
+            with open(a.out) as f:
+                old = json.load(f)
+            res = {**old, **{k: v for k, v in res.items() if not k.startswith("_")}}
         text = json.dumps(res, indent=1, sort_keys=True)
         if a.out:
             with open(a.out, "w") as f:
