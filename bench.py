@@ -206,7 +206,7 @@ def build_workloads(eng, names, scale, O):
                     lambda: O.sum_col(li["l_quantity"]), 4 * nl)
             if "join" in names:
                 out["join"] = Workload(
-                    "join", "rj_join", f"join count orders({no}) x lineitem({nl}) on orderkey",
+                    "join", "jd_probe", f"join count orders({no}) x lineitem({nl}) on orderkey",
                     nl, lambda tm: eng.join_count(orders_res, "o_orderkey", fact, "l_orderkey",
                                                   timing=tm),
                     lambda: O.join_count(js["orders"]["o_orderkey"], li["l_orderkey"]),

@@ -17,6 +17,14 @@
 //              pieces. No global atomics anywhere.
 //   global     the reference's shape: one open-addressing table in HBM built
 //              with global CAS (kept for comparison by the autotuner)
+//   direct     when the build keys' range [min, max] is dense (span <= 8 x the
+//              build rows by default, <= 64 x when asked for): a bitmap over the
+//              range, 1 bit per key
+//              value (7.5 MB for 15M sparse order keys, so it stays in the
+//              Infinity Cache), built with one atomicOr per word per lane and
+//              checked for repeated build keys on the way; with repeats, a u32
+//              count per key value instead. The probe streams its keys with
+//              dwordx4 loads and adds each key's bit (or count).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -360,6 +368,75 @@ __global__ __launch_bounds__(256) void join_probe(const int32_t* __restrict__ pr
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
+// ---------------------------------------------------------------- direct
+constexpr uint64_t DIRECT_MAX_SPAN = 1ull << 31;
+
+// bits[k - lo] for every build key; *dup = 1 when a key value repeats.
+__global__ __launch_bounds__(256) void jd_bits(const int32_t* __restrict__ in, uint32_t n,
+                                               int32_t lo, uint32_t* __restrict__ bits,
+                                               uint32_t* __restrict__ dup) {
+  const uint32_t nq = (n + 3) / 4, stride = gridDim.x * blockDim.x;
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
+    const int4 v = ld4(in, g);
+    const int32_t k[4] = {v.x, v.y, v.z, v.w};
+    uint32_t word = 0, mask = 0, rep = 0;
+    for (int j = 0; j < 4 && 4 * g + j < n; ++j) {  // one atomic per run of one word
+      const uint32_t off = (uint32_t)((int64_t)k[j] - lo);
+      const uint32_t w = off >> 5, m = 1u << (off & 31);
+      if (mask && w != word) {
+        rep |= atomicOr(&bits[word], mask) & mask;
+        mask = 0;
+      }
+      word = w;
+      rep |= mask & m;
+      mask |= m;
+    }
+    if (mask) rep |= atomicOr(&bits[word], mask) & mask;
+    if (rep) *dup = 1;
+  }
+}
+
+// counts[k - lo] += 1 for every build key (the repeated-keys form).
+__global__ __launch_bounds__(256) void jd_counts(const int32_t* __restrict__ in, uint32_t n,
+                                                 int32_t lo, uint32_t* __restrict__ counts) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    atomicAdd(&counts[(uint32_t)((int64_t)in[i] - lo)], 1u);
+}
+
+// Matches of the probe keys: bit (COUNTS = false) or count (COUNTS = true) of
+// each key inside [lo, lo + span). Two vector groups of keys in flight per lane.
+template <bool COUNTS>
+__global__ __launch_bounds__(256) void jd_probe(const int32_t* __restrict__ probe, uint32_t n,
+                                                int32_t lo, uint64_t span,
+                                                const uint32_t* __restrict__ table,
+                                                int64_t* __restrict__ partials) {
+  __shared__ int64_t red[4];
+  const uint32_t nq = (n + 3) / 4, stride = gridDim.x * blockDim.x;
+  uint64_t c = 0;
+  auto one = [&](int32_t key, bool valid) {
+    const uint64_t off = (uint64_t)((int64_t)key - lo);
+    if (!valid || off >= span) return;
+    if (COUNTS) c += table[off];
+    else c += (table[off >> 5] >> (off & 31)) & 1u;
+  };
+  auto quad = [&](uint32_t g, const int4& v) {
+    one(v.x, 4 * g < n);
+    one(v.y, 4 * g + 1 < n);
+    one(v.z, 4 * g + 2 < n);
+    one(v.w, 4 * g + 3 < n);
+  };
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  for (; g + stride < nq; g += 2 * stride) {
+    const int4 a = ld4nt(probe, g), b = ld4nt(probe, g + stride);
+    quad(g, a);
+    quad(g + stride, b);
+  }
+  if (g < nq) quad(g, ld4nt(probe, g));
+  const int64_t s = block_sum((int64_t)c, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
 // Partition bits for a build side of nb keys: partitions average at most half
 // an LDS table fill.
 int derived_bits(uint32_t nb) {
@@ -447,9 +524,14 @@ extern "C" int gdb_join_count(gdb_ctx* ctx, const gdb_table* build, const char* 
   const int32_t* rk = (const int32_t*)cb->dev;
   const int32_t* sk = (const int32_t*)cp->dev;
   const int tuned = ctx->tune[T_JOIN].radix_bits;
-  const bool global = tuned < 0;
-  const bool broadcast = !global && tuned == 0 && nb <= LDS_KEYS;
-  const int bits = global || broadcast ? 0
+  const uint64_t span = nb ? (uint64_t)(cb->maxv - cb->minv) + 1 : 0;
+  // the filter's memory stays within ~256 B per build row (count form) plus 4 MiB
+  const bool direct_ok = nb > 0 && span <= std::min<uint64_t>(DIRECT_MAX_SPAN, 64ull * nb + (1u << 20));
+  // direct: asked for (-2) when the range allows, or derived when it is dense
+  const bool direct = direct_ok && (tuned == -2 || (tuned == 0 && span <= 8ull * nb));
+  const bool global = !direct && tuned == -1;
+  const bool broadcast = !direct && !global && tuned <= 0 && nb <= LDS_KEYS;
+  const int bits = direct || global || broadcast ? 0
                    : tuned > 0 ? std::max(min_bits(nb), std::min(tuned, MAX_BITS))
                                : derived_bits(nb);
   hipStream_t st = ctx->stream;
@@ -459,7 +541,40 @@ extern "C" int gdb_join_count(gdb_ctx* ctx, const gdb_table* build, const char* 
   int64_t* partials;
   uint32_t nparts = 0;
 
-  if (global) {
+  if (direct) {
+    const int32_t lo = (int32_t)cb->minv;
+    const uint64_t words = (span + 31) / 32;
+    uint32_t* table;
+    GDB_TRY(arena(ctx, A_HT, (words + 16) * 4, (void**)&table));
+    uint32_t* dup = table + words;
+    int blocks = resident_blocks(ctx, (const void*)jd_probe<false>, 256, 0,
+                                 ctx->tune[T_JOIN].blocks_per_cu);
+    blocks = std::max(1, std::min<int>(blocks, (int)((np / 4 + 255) / 256)));
+    nparts = (uint32_t)blocks;
+    GDB_TRY(arena(ctx, A_PARTIALS, sizeof(int64_t) * nparts, (void**)&partials));
+    const int fill = ctx->info.cu_count * 4;
+    GDB_HIP(hipEventRecord(ctx->ev[0], st));
+    GDB_HIP(hipMemsetAsync(table, 0, (words + 16) * 4, st));
+    jd_bits<<<fill, 256, 0, st>>>(rk, nb, lo, table, dup);
+    GDB_HIP(hipGetLastError());
+    uint32_t* hdup = (uint32_t*)host;
+    GDB_HIP(hipMemcpyAsync(hdup, dup, 4, hipMemcpyDeviceToHost, st));
+    GDB_HIP(hipStreamSynchronize(st));
+    const bool counts = *hdup != 0;
+    uint32_t* cnt = table;
+    if (counts) {  // repeated build keys: a count per key value
+      GDB_TRY(arena(ctx, A_JOINPART, span * 4, (void**)&cnt));
+      GDB_HIP(hipMemsetAsync(cnt, 0, span * 4, st));
+      jd_counts<<<fill, 256, 0, st>>>(rk, nb, lo, cnt);
+      GDB_HIP(hipGetLastError());
+    }
+    GDB_HIP(hipEventRecord(ctx->ev[1], st));
+    if (counts)
+      jd_probe<true><<<blocks, 256, 0, st>>>(sk, np, lo, span, cnt, partials);
+    else
+      jd_probe<false><<<blocks, 256, 0, st>>>(sk, np, lo, span, table, partials);
+    GDB_HIP(hipEventRecord(ctx->ev[2], st));
+  } else if (global) {
     uint32_t cap = 1024, lg = 10;
     while (cap < 2ull * nb) {
       cap <<= 1;
@@ -547,7 +662,7 @@ extern "C" int gdb_join_count(gdb_ctx* ctx, const gdb_table* build, const char* 
   GDB_HIP(hipEventRecord(ctx->ev[5], st));
   // algorithmic bytes: each key read once; the main kernel of the radix path
   // (the per-partition join) reads both partitioned sides
-  const uint64_t main_bytes = (!global && !broadcast) ? 4ull * nb + 4ull * np : 4ull * np;
+  const uint64_t main_bytes = (!global && !broadcast && !direct) ? 4ull * nb + 4ull * np : 4ull * np;
   GDB_TRY(timing_finish(ctx, timing, 4ull * nb + 4ull * np, main_bytes));
   *matches = (uint64_t)host[0];
   return GDB_OK;

@@ -38,6 +38,7 @@ std::vector<Shape> candidates(int pipeline, const gdb_table* const* t, int n) {
     case GDB_PIPE_JOIN: {
       // one global table, the derived shape, and radix bits around the derived
       c.push_back({256, 0, 1, -1});
+      c.push_back({256, 0, 1, -2});
       c.push_back({256, 0, 1, 0});
       if (n >= 1) {
         const int d = join_derived_bits(t[0]->nrows);
@@ -128,9 +129,9 @@ int gdb_set_tuning(gdb_ctx* ctx, const gdb_tuning* t) {
   if (!ctx || !t || t->pipeline < 0 || t->pipeline >= 16) return fail(GDB_E_INVALID, "bad argument");
   if ((t->block != 64 && t->block != 128 && t->block != 256) || t->blocks_per_cu < 0 ||
       t->blocks_per_cu > 64 || (t->unroll != 1 && t->unroll != 2 && t->unroll != 4) ||
-      t->radix_bits < -1 || t->radix_bits > 14)
+      t->radix_bits < -2 || t->radix_bits > 14)
     return fail(GDB_E_INVALID,
-                "tuning out of range (block 64/128/256, unroll 1/2/4, radix bits -1..14)");
+                "tuning out of range (block 64/128/256, unroll 1/2/4, radix bits -2..14)");
   ctx->tune[t->pipeline] = Tuning{t->block, t->blocks_per_cu, t->unroll, t->radix_bits};
   return GDB_OK;
 }

@@ -310,8 +310,12 @@ typedef struct {
   double ms;           /* median pipeline time of the chosen shape */
   int32_t candidates;  /* shapes timed */
   int32_t radix_bits;  /* join only: partition bits 1..14; 0 = derived from the
-                          build size (LDS broadcast when it fits one table);
-                          -1 = one global hash table */
+                          build side (a direct-address filter over the build
+                          keys' range when it spans <= 8x the build rows, else
+                          LDS broadcast when it fits one table, else radix);
+                          -1 = one global hash table; -2 = the direct-address
+                          filter whenever the range spans <= 64x the build
+                          rows + 2^20 values */
 } gdb_tuning;
 int gdb_autotune(gdb_ctx* ctx, int pipeline, const gdb_table* const* tables, int ntables,
                  int reps, gdb_tuning* best);

@@ -173,7 +173,7 @@ def test_join_count_duplicates(engine):
 # radix bits: -1 one global table, 0 derived (LDS broadcast for small builds),
 # 1 and 4 single-pass partitions far larger than an LDS table (built and probed
 # in pieces), 9 and 14 two partition passes
-STRATEGIES = (-1, 0, 1, 4, 9, 14)
+STRATEGIES = (-2, -1, 0, 1, 4, 9, 14)
 
 
 @pytest.fixture()
@@ -206,12 +206,12 @@ def test_join_strategies_against_oracle(engine, join_strategy, bits):
 def test_join_full_config_all_strategies(engine, join_strategy, full):
     js, t = full
     exp = O.join_count(js["orders"]["o_orderkey"], js["lineitem"]["l_orderkey"])
-    for bits in (-1, 0, 1, 9, 12, 14):  # 1 and 9 are raised to the fewest bits allowed
+    for bits in (-2, -1, 0, 1, 9, 12, 14):  # 1 and 9 are raised to the fewest bits allowed
         join_strategy(bits)
         assert engine.join_count(t["orders"], "o_orderkey", t["lineitem"], "l_orderkey") == exp, bits
     kb = engine.table({"k": js["orders"]["o_orderkey"]})
     kp = engine.table({"k": js["lineitem"]["l_orderkey"]})
     got = engine.autotune("join", [kb, kp], reps=1)
-    assert -1 <= got["radix_bits"] <= 14
+    assert -2 <= got["radix_bits"] <= 14
     assert engine.get_tuning("join")["radix_bits"] == got["radix_bits"]
     assert engine.join_count(kb, "k", kp, "k") == exp
