@@ -18,7 +18,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <initializer_list>
 #include <string>
+#include <vector>
 
 #include "gdb_tpch.h"
 #include "internal.h"
@@ -39,6 +41,33 @@ inline int col(const gdb_table* t, const char* what, const char* name, int type,
   if (!t) return gdb::fail(GDB_E_INVALID, std::string("missing the ") + what + " table");
   GDB_TRY(gdb::column(t, name, type, out));
   return gdb::require_resident(t, what);
+}
+
+// A lineitem column: the table may be host-resident (streamed in chunks).
+inline int scol(const gdb_table* t, const char* what, const char* name, int type,
+                const Column** out) {
+  if (!t) return gdb::fail(GDB_E_INVALID, std::string("missing the ") + what + " table");
+  return gdb::column(t, name, type, out);
+}
+
+// Runs `fn(dev, rows)` over columns of table t: once over the resident columns
+// (bracketed by the dominant-kernel events ev[1], ev[2]) or, for a host table,
+// chunk by chunk through two device slots with copies overlapped (the reference's
+// chunkedStreamLoop). Chunk times add into *rs.
+inline int scan_table(gdb_ctx* ctx, const gdb_table* t, std::initializer_list<const Column*> cols,
+                      const gdb::Consume& fn, gdb::RunStats* rs) {
+  if (t->nrows == 0) {
+    GDB_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+    GDB_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
+    return GDB_OK;
+  }
+  std::vector<const Column*> v(cols);
+  gdb::RunStats one;
+  GDB_TRY(gdb::run_over(ctx, t, v.data(), (int)v.size(), fn, &one));
+  rs->kernel_ms += one.kernel_ms;
+  rs->h2d_bytes += one.h2d_bytes;
+  rs->chunks += one.chunks;
+  return GDB_OK;
 }
 
 // Entries of a direct-address map over key column c: max key + 1. Keys must be

@@ -428,80 +428,68 @@ __device__ __forceinline__ void q21_apply(unsigned long long* st, uint64_t v) {
 }
 
 // Pass 1 over lines of orders with the wanted status: merge each run of one
-// order's rows within a lane's group of 4 into the order's state; queue late
-// lines of the nation's suppliers (wave-aggregated append) for pass 2.
+// order's rows within a lane's group of 4 into the order's state.
 __global__ __launch_bounds__(256) void q21_track(const int32_t* __restrict__ ok,
                                                  const int32_t* __restrict__ sk,
                                                  const int32_t* __restrict__ commit,
                                                  const int32_t* __restrict__ receipt, uint32_t n,
                                                  const uint8_t* __restrict__ fst,
-                                                 const uint8_t* __restrict__ smap, int32_t nation,
-                                                 unsigned long long* __restrict__ state,
-                                                 uint32_t* __restrict__ list,
-                                                 uint32_t* __restrict__ list_n) {
-  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
-  const int lane = threadIdx.x & 63;
-  const uint32_t wave_base = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
-  for (uint32_t gw = wave_base; gw < nq; gw += stride) {
-    const uint32_t g = gw + lane;
-    uint32_t cand = 0;
-    if (g < nq) {
-      const int4 k = ld4nt(ok, g);
-      uint32_t m = 0;
+                                                 unsigned long long* __restrict__ state) {
+  walk2(quads(n), [&](uint32_t g) { return ld4nt(ok, g); }, [&](uint32_t g, const int4& k) {
+    uint32_t m = 0;
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        m |= (uint32_t)((4 * g + j < n) && fst[lane_of(k, j)]) << j;
-      if (m) {
-        const int4 s = ld4nt(sk, g), c = ld4nt(commit, g), r = ld4nt(receipt, g);
-        int key = -1;
-        uint64_t acc = 0;
+    for (int j = 0; j < 4; ++j) m |= (uint32_t)((4 * g + j < n) && fst[lane_of(k, j)]) << j;
+    if (!m) return;
+    const int4 s = ld4nt(sk, g), c = ld4nt(commit, g), r = ld4nt(receipt, g);
+    int key = -1;
+    uint64_t acc = 0;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          if (!((m >> j) & 1)) continue;
-          const int kj = lane_of(k, j);
-          const uint32_t sv = (uint32_t)lane_of(s, j) + 1u;
-          const bool late = lane_of(r, j) > lane_of(c, j);
-          const uint64_t v = ((uint64_t)sv << 32) | (late ? sv : 0u);
-          if (kj != key) {
-            if (key >= 0) q21_apply(&state[key], acc);
-            key = kj;
-            acc = v;
-          } else {
-            acc = q21_merge(acc, v);
-          }
-          if (late && smap[lane_of(s, j)] == nation) cand |= 1u << j;
-        }
+    for (int j = 0; j < 4; ++j) {
+      if (!((m >> j) & 1)) continue;
+      const int kj = lane_of(k, j);
+      const uint32_t sv = (uint32_t)lane_of(s, j) + 1u;
+      const bool late = lane_of(r, j) > lane_of(c, j);
+      const uint64_t v = ((uint64_t)sv << 32) | (late ? sv : 0u);
+      if (kj != key) {
         if (key >= 0) q21_apply(&state[key], acc);
+        key = kj;
+        acc = v;
+      } else {
+        acc = q21_merge(acc, v);
       }
     }
-    const uint32_t c = __popc(cand);
-    const uint32_t incl = wave_incl_scan(c);
-    const uint32_t total = __shfl(incl, 63, 64);
-    if (total == 0) continue;
-    uint32_t base = 0;
-    if (lane == 63) base = atomicAdd(list_n, total);
-    base = __shfl(base, 63, 64);
-    uint32_t pos = base + incl - c;
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-      if ((cand >> j) & 1) list[pos++] = 4 * g + j;
-  }
+    if (key >= 0) q21_apply(&state[key], acc);
+  });
 }
 
-// Pass 2 over the queued late lines: the order has a second supplier and no
-// second late supplier (the line's own supplier is then the only late one).
-__global__ __launch_bounds__(256) void q21_count(const int32_t* __restrict__ ok,
-                                                 const int32_t* __restrict__ sk,
-                                                 const uint32_t* __restrict__ list,
-                                                 const uint32_t* __restrict__ list_n,
-                                                 const unsigned long long* __restrict__ state,
-                                                 uint32_t* __restrict__ cnt) {
-  const uint32_t m = *list_n;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
-    const uint32_t r = list[i];
-    const uint64_t st = state[ok[r]];
-    if ((st >> 63) && !((st >> 31) & 1)) atomicAdd(&cnt[sk[r]], 1u);
-  }
+// Pass 2 over lineitem once every order's state is final: a late line of the
+// nation's supplier on an order of the wanted status counts when the order has
+// a second supplier and no second late supplier (the line's own supplier is
+// then the only late one).
+__global__ __launch_bounds__(256) void q21_count_rows(const int32_t* __restrict__ ok,
+                                                      const int32_t* __restrict__ sk,
+                                                      const int32_t* __restrict__ commit,
+                                                      const int32_t* __restrict__ receipt,
+                                                      uint32_t n, const uint8_t* __restrict__ fst,
+                                                      const uint8_t* __restrict__ smap,
+                                                      int32_t nation,
+                                                      const unsigned long long* __restrict__ state,
+                                                      uint32_t* __restrict__ cnt) {
+  walk2(quads(n), [&](uint32_t g) { return ld4nt(ok, g); }, [&](uint32_t g, const int4& k) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) m |= (uint32_t)((4 * g + j < n) && fst[lane_of(k, j)]) << j;
+    if (!m) return;
+    const int4 s = ld4nt(sk, g), c = ld4nt(commit, g), r = ld4nt(receipt, g);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (!((m >> j) & 1) || !(lane_of(r, j) > lane_of(c, j))) continue;
+      const uint32_t sv = (uint32_t)lane_of(s, j);
+      if (smap[sv] != nation) continue;
+      const uint64_t st = state[lane_of(k, j)];
+      if ((st >> 63) && !((st >> 31) & 1)) atomicAdd(&cnt[sv], 1u);
+    }
+  });
 }
 
 // ---- clustered lineitem: one lane resolves each order ---------------------------
@@ -700,10 +688,10 @@ int gdb_q5(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q5_params* p, gdb_q5_r
            gdb_timing* timing) {
   GDB_TRY(args_ok(ctx, db, p, out));
   const Column *lo, *lk, *ld, *le, *oo, *oc, *od, *ck, *cn, *sk, *sn;
-  GDB_TRY(col(db->lineitem, "lineitem", "l_orderkey", GDB_I32, &lo));
-  GDB_TRY(col(db->lineitem, "lineitem", "l_suppkey", GDB_I32, &lk));
-  GDB_TRY(col(db->lineitem, "lineitem", "l_discount", GDB_I32, &ld));
-  GDB_TRY(col(db->lineitem, "lineitem", "l_extendedprice", GDB_I64, &le));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_orderkey", GDB_I32, &lo));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_suppkey", GDB_I32, &lk));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_discount", GDB_I32, &ld));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_extendedprice", GDB_I64, &le));
   GDB_TRY(col(db->orders, "orders", "o_orderkey", GDB_I32, &oo));
   GDB_TRY(col(db->orders, "orders", "o_custkey", GDB_I32, &oc));
   GDB_TRY(col(db->orders, "orders", "o_orderdate", GDB_I32, &od));
@@ -749,19 +737,20 @@ int gdb_q5(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q5_params* p, gdb_q5_r
         (const int32_t*)oo->dev, (const int32_t*)oc->dev, (const int32_t*)od->dev, (uint32_t)no,
         p->date_lo, p->date_hi, cmap, (uint32_t)cspan, omap);
   GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[1], s));
-  if (nl)
-    q5_scan<<<blocks_for(ctx, q5_scan, quads(nl)), BLOCK, 0, s>>>(
-        (const int32_t*)lo->dev, (const int32_t*)lk->dev, (const int32_t*)ld->dev,
-        (const int64_t*)le->dev, (uint32_t)nl, omap, smap, dout);
-  GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  RunStats rs;
+  GDB_TRY(scan_table(ctx, db->lineitem, {lo, lk, ld, le}, [&](const void* const* d, uint32_t rows) {
+    q5_scan<<<blocks_for(ctx, q5_scan, quads(rows)), BLOCK, 0, s>>>(
+        (const int32_t*)d[0], (const int32_t*)d[1], (const int32_t*)d[2], (const int64_t*)d[3],
+        rows, omap, smap, dout);
+    GDB_HIP(hipGetLastError());
+    return GDB_OK;
+  }, &rs));
   GDB_HIP(hipMemcpyAsync(host, dout, 26 * 8, hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
   // SQL columns: lineitem orderkey, suppkey, discount, extendedprice (20 B); orders
   // orderkey, custkey, orderdate (12 B); customer and supplier key + nation (8 B).
   // Mandatory in the scan: l_orderkey, 4 B per line.
-  GDB_TRY(timing_finish(ctx, timing, 20 * nl + 12 * no + 8 * (nc + ns), 4 * nl));
+  GDB_TRY(timing_finish(ctx, timing, 20 * nl + 12 * no + 8 * (nc + ns), 4 * nl, &rs));
   for (int i = 0; i < 25; ++i) out->revenue[i] = host[i];
   out->has = (uint32_t)host[25];
   return GDB_OK;
@@ -771,11 +760,11 @@ int gdb_q7(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q7_params* p, gdb_q7_r
            gdb_timing* timing) {
   GDB_TRY(args_ok(ctx, db, p, out));
   const Column *ls, *lo, *lk, *ld, *le, *oo, *oc, *ck, *cn, *sk, *sn;
-  GDB_TRY(col(db->lineitem, "lineitem", "l_shipdate", GDB_I32, &ls));
-  GDB_TRY(col(db->lineitem, "lineitem", "l_orderkey", GDB_I32, &lo));
-  GDB_TRY(col(db->lineitem, "lineitem", "l_suppkey", GDB_I32, &lk));
-  GDB_TRY(col(db->lineitem, "lineitem", "l_discount", GDB_I32, &ld));
-  GDB_TRY(col(db->lineitem, "lineitem", "l_extendedprice", GDB_I64, &le));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_shipdate", GDB_I32, &ls));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_orderkey", GDB_I32, &lo));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_suppkey", GDB_I32, &lk));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_discount", GDB_I32, &ld));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_extendedprice", GDB_I64, &le));
   GDB_TRY(col(db->orders, "orders", "o_orderkey", GDB_I32, &oo));
   GDB_TRY(col(db->orders, "orders", "o_custkey", GDB_I32, &oc));
   GDB_TRY(col(db->customer, "customer", "c_custkey", GDB_I32, &ck));
@@ -820,22 +809,25 @@ int gdb_q7(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q7_params* p, gdb_q7_r
         (const int32_t*)oo->dev, (const int32_t*)oc->dev, (uint32_t)no, cmap, (uint32_t)cspan,
         omap);
   GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[1], s));
   // Years index 0..7 (1992..1999): clamp the window so year_index stays in range.
   const int32_t lo_d = std::max(p->date_lo, 0), hi_d = std::min(p->date_hi, 2921);
-  if (nl && lo_d <= hi_d)
-    q7_scan<<<blocks_for(ctx, q7_scan, quads(nl)), BLOCK, 0, s>>>(
-        (const int32_t*)ls->dev, (const int32_t*)lo->dev, (const int32_t*)lk->dev,
-        (const int32_t*)ld->dev, (const int64_t*)le->dev, (uint32_t)nl, lo_d, hi_d,
-        p->nation1 == p->nation2 ? 1 : 0, omap, smap, dout);
-  GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  RunStats rs;
+  GDB_TRY(scan_table(ctx, db->lineitem, {ls, lo, lk, ld, le},
+                     [&](const void* const* d, uint32_t rows) {
+    if (lo_d > hi_d) return GDB_OK;
+    q7_scan<<<blocks_for(ctx, q7_scan, quads(rows)), BLOCK, 0, s>>>(
+        (const int32_t*)d[0], (const int32_t*)d[1], (const int32_t*)d[2], (const int32_t*)d[3],
+        (const int64_t*)d[4], rows, lo_d, hi_d, p->nation1 == p->nation2 ? 1 : 0, omap, smap,
+        dout);
+    GDB_HIP(hipGetLastError());
+    return GDB_OK;
+  }, &rs));
   GDB_HIP(hipMemcpyAsync(host, dout, 17 * 8, hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
   // SQL columns: lineitem shipdate, orderkey, suppkey, discount, extendedprice (24 B);
   // orders key + custkey (8 B); customer and supplier key + nation (8 B). Mandatory
   // in the scan: l_shipdate, 4 B per line.
-  GDB_TRY(timing_finish(ctx, timing, 24 * nl + 8 * (no + nc + ns), 4 * nl));
+  GDB_TRY(timing_finish(ctx, timing, 24 * nl + 8 * (no + nc + ns), 4 * nl, &rs));
   for (int d = 0; d < 2; ++d) {
     for (int y = 0; y < 8; ++y) out->revenue[d][y] = host[d * 8 + y];
     out->has[d] = ((uint32_t)host[16] >> (8 * d)) & 0xFFu;
@@ -847,11 +839,11 @@ int gdb_q8(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q8_params* p, gdb_q8_r
            gdb_timing* timing) {
   GDB_TRY(args_ok(ctx, db, p, out));
   const Column *lp, *lo, *lk, *ld, *le, *oo, *oc, *od, *ck, *cn, *sk, *sn, *pp, *pt;
-  GDB_TRY(col(db->lineitem, "lineitem", "l_partkey", GDB_I32, &lp));
-  GDB_TRY(col(db->lineitem, "lineitem", "l_orderkey", GDB_I32, &lo));
-  GDB_TRY(col(db->lineitem, "lineitem", "l_suppkey", GDB_I32, &lk));
-  GDB_TRY(col(db->lineitem, "lineitem", "l_discount", GDB_I32, &ld));
-  GDB_TRY(col(db->lineitem, "lineitem", "l_extendedprice", GDB_I64, &le));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_partkey", GDB_I32, &lp));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_orderkey", GDB_I32, &lo));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_suppkey", GDB_I32, &lk));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_discount", GDB_I32, &ld));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_extendedprice", GDB_I64, &le));
   GDB_TRY(col(db->orders, "orders", "o_orderkey", GDB_I32, &oo));
   GDB_TRY(col(db->orders, "orders", "o_custkey", GDB_I32, &oc));
   GDB_TRY(col(db->orders, "orders", "o_orderdate", GDB_I32, &od));
@@ -915,20 +907,21 @@ int gdb_q8(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q8_params* p, gdb_q8_r
         (const int32_t*)oo->dev, (const int32_t*)oc->dev, (const int32_t*)od->dev, (uint32_t)no,
         lo_d, hi_d, cmap, (uint32_t)cspan, nreg, p->region, omap);
   GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[1], s));
-  if (nl)
-    q8_scan<<<fold_blocks(ctx, (const void*)q8_scan, quads(nl)), FOLD_BLOCK, 0, s>>>(
-        (const int32_t*)lp->dev, (const int32_t*)lo->dev, (const int32_t*)lk->dev,
-        (const int32_t*)ld->dev, (const int64_t*)le->dev, (uint32_t)nl, pbits, pfold, omap, smap,
-        p->nation, dout);
-  GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  RunStats rs;
+  GDB_TRY(scan_table(ctx, db->lineitem, {lp, lo, lk, ld, le},
+                     [&](const void* const* d, uint32_t rows) {
+    q8_scan<<<fold_blocks(ctx, (const void*)q8_scan, quads(rows)), FOLD_BLOCK, 0, s>>>(
+        (const int32_t*)d[0], (const int32_t*)d[1], (const int32_t*)d[2], (const int32_t*)d[3],
+        (const int64_t*)d[4], rows, pbits, pfold, omap, smap, p->nation, dout);
+    GDB_HIP(hipGetLastError());
+    return GDB_OK;
+  }, &rs));
   GDB_HIP(hipMemcpyAsync(host, dout, 17 * 8, hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
   // SQL columns: lineitem partkey, orderkey, suppkey, discount, extendedprice (24 B);
   // orders key, custkey, date (12 B); customer, supplier key + nation (8 B); part key
   // + type (5 B). Mandatory in the scan: l_partkey, 4 B per line.
-  GDB_TRY(timing_finish(ctx, timing, 24 * nl + 12 * no + 8 * (nc + ns) + 5 * np, 4 * nl));
+  GDB_TRY(timing_finish(ctx, timing, 24 * nl + 12 * no + 8 * (nc + ns) + 5 * np, 4 * nl, &rs));
   for (int y = 0; y < 8; ++y) {
     out->nation_volume[y] = host[y];
     out->total_volume[y] = host[8 + y];
@@ -942,10 +935,10 @@ int gdb_q10(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q10_params* p, gdb_q1
   GDB_TRY(args_ok(ctx, db, p, out));
   GDB_TRY(top_k_arg(p->top_k));
   const Column *lf, *lo, *ld, *le, *oo, *oc, *od, *ck, *cn, *cb;
-  GDB_TRY(col(db->lineitem, "lineitem", "l_returnflag", GDB_U8, &lf));
-  GDB_TRY(col(db->lineitem, "lineitem", "l_orderkey", GDB_I32, &lo));
-  GDB_TRY(col(db->lineitem, "lineitem", "l_discount", GDB_I32, &ld));
-  GDB_TRY(col(db->lineitem, "lineitem", "l_extendedprice", GDB_I64, &le));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_returnflag", GDB_U8, &lf));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_orderkey", GDB_I32, &lo));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_discount", GDB_I32, &ld));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_extendedprice", GDB_I64, &le));
   GDB_TRY(col(db->orders, "orders", "o_orderkey", GDB_I32, &oo));
   GDB_TRY(col(db->orders, "orders", "o_custkey", GDB_I32, &oc));
   GDB_TRY(col(db->orders, "orders", "o_orderdate", GDB_I32, &od));
@@ -993,14 +986,14 @@ int gdb_q10(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q10_params* p, gdb_q1
         (const int32_t*)oo->dev, (const int32_t*)oc->dev, (const int32_t*)od->dev, (uint32_t)no,
         p->date_lo, p->date_hi, ocust);
   GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[1], s));
-  if (nl)
-    q10_scan<<<blocks_for(ctx, q10_scan, quads(nl)), BLOCK, 0, s>>>(
-        (const uint8_t*)lf->dev, (const int32_t*)lo->dev, (const int32_t*)ld->dev,
-        (const int64_t*)le->dev, (uint32_t)nl, (uint32_t)(uint8_t)p->returnflag, ocust,
-        (uint32_t)cspan, crev, seen);
-  GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  RunStats rs;
+  GDB_TRY(scan_table(ctx, db->lineitem, {lf, lo, ld, le}, [&](const void* const* d, uint32_t rows) {
+    q10_scan<<<blocks_for(ctx, q10_scan, quads(rows)), BLOCK, 0, s>>>(
+        (const uint8_t*)d[0], (const int32_t*)d[1], (const int32_t*)d[2], (const int64_t*)d[3],
+        rows, (uint32_t)(uint8_t)p->returnflag, ocust, (uint32_t)cspan, crev, seen);
+    GDB_HIP(hipGetLastError());
+    return GDB_OK;
+  }, &rs));
   if (k > 0) {
     GDB_TRY(topk_run(ctx, KeyedRevSrc{crev, seen}, cspan, k, win, tblocks, fin,
                      (unsigned long long*)dout, (unsigned long long*)(dout + 1)));
@@ -1019,7 +1012,7 @@ int gdb_q10(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q10_params* p, gdb_q1
   // SQL columns: lineitem returnflag (1 B), orderkey, discount (4 B), price (8 B);
   // orders key, custkey, date (12 B); customer key, nation, acctbal (16 B).
   // Mandatory in the scan: l_returnflag, 1 B per line.
-  GDB_TRY(timing_finish(ctx, timing, 17 * nl + 12 * no + 16 * nc, 1 * nl));
+  GDB_TRY(timing_finish(ctx, timing, 17 * nl + 12 * no + 16 * nc, 1 * nl, &rs));
   const uint64_t* h = (const uint64_t*)host;
   const TopEntry* hf = (const TopEntry*)(h + 2);
   const int64_t* hb = (const int64_t*)(hf + k);
@@ -1042,8 +1035,8 @@ int gdb_q18(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q18_params* p, gdb_q1
   GDB_TRY(args_ok(ctx, db, p, out));
   GDB_TRY(top_k_arg(p->top_k));
   const Column *lo, *lq, *oo, *oc, *od, *ot;
-  GDB_TRY(col(db->lineitem, "lineitem", "l_orderkey", GDB_I32, &lo));
-  GDB_TRY(col(db->lineitem, "lineitem", "l_quantity", GDB_I32, &lq));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_orderkey", GDB_I32, &lo));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_quantity", GDB_I32, &lq));
   GDB_TRY(col(db->orders, "orders", "o_orderkey", GDB_I32, &oo));
   GDB_TRY(col(db->orders, "orders", "o_custkey", GDB_I32, &oc));
   GDB_TRY(col(db->orders, "orders", "o_orderdate", GDB_I32, &od));
@@ -1070,15 +1063,18 @@ int gdb_q18(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q18_params* p, gdb_q1
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
   GDB_HIP(hipMemsetAsync(sq, 0, ospan * 4, s));
   GDB_HIP(hipMemsetAsync(dout, 0, out_bytes, s));
-  GDB_HIP(hipEventRecord(ctx->ev[1], s));
-  if (nl && lo->sorted)  // rows of an order are contiguous: one plain store per order
-    q18_owner<<<blocks_for(ctx, q18_owner, quads(nl)), BLOCK, 0, s>>>(
-        (const int32_t*)lo->dev, (const int32_t*)lq->dev, (uint32_t)nl, sq);
-  else if (nl)
-    q18_sums<<<blocks_for(ctx, q18_sums, quads(nl)), BLOCK, 0, s>>>(
-        (const int32_t*)lo->dev, (const int32_t*)lq->dev, (uint32_t)nl, sq);
-  GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  RunStats rs;
+  const bool clustered = lo->sorted && !db->lineitem->host;  // an order's rows are contiguous
+  GDB_TRY(scan_table(ctx, db->lineitem, {lo, lq}, [&](const void* const* d, uint32_t rows) {
+    if (clustered)  // one plain store per order
+      q18_owner<<<blocks_for(ctx, q18_owner, quads(rows)), BLOCK, 0, s>>>(
+          (const int32_t*)d[0], (const int32_t*)d[1], rows, sq);
+    else
+      q18_sums<<<blocks_for(ctx, q18_sums, quads(rows)), BLOCK, 0, s>>>(
+          (const int32_t*)d[0], (const int32_t*)d[1], rows, sq);
+    GDB_HIP(hipGetLastError());
+    return GDB_OK;
+  }, &rs));
   GDB_TRY(topk_run(ctx,
                    Q18Src{(const int32_t*)oo->dev, (const int32_t*)od->dev,
                           (const int64_t*)ot->dev, sq, p->quantity},
@@ -1090,7 +1086,7 @@ int gdb_q18(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q18_params* p, gdb_q1
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
   // SQL columns: l_orderkey, l_quantity (8 B per line, all read by q18_sums); orders
   // key, custkey, date (12 B) and totalprice (8 B).
-  GDB_TRY(timing_finish(ctx, timing, 8 * nl + 20 * no, 8 * nl));
+  GDB_TRY(timing_finish(ctx, timing, 8 * nl + 20 * no, 8 * nl, &rs));
   const uint64_t* h = (const uint64_t*)host;
   const TopEntry* hf = (const TopEntry*)(h + 2);
   const int32_t* hx = (const int32_t*)(hf + k);
@@ -1113,10 +1109,10 @@ int gdb_q21(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q21_params* p, gdb_q2
   GDB_TRY(args_ok(ctx, db, p, out));
   GDB_TRY(top_k_arg(p->top_k));
   const Column *lo, *lk, *lc, *lr, *oo, *os, *sk, *sn;
-  GDB_TRY(col(db->lineitem, "lineitem", "l_orderkey", GDB_I32, &lo));
-  GDB_TRY(col(db->lineitem, "lineitem", "l_suppkey", GDB_I32, &lk));
-  GDB_TRY(col(db->lineitem, "lineitem", "l_commitdate", GDB_I32, &lc));
-  GDB_TRY(col(db->lineitem, "lineitem", "l_receiptdate", GDB_I32, &lr));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_orderkey", GDB_I32, &lo));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_suppkey", GDB_I32, &lk));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_commitdate", GDB_I32, &lc));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_receiptdate", GDB_I32, &lr));
   GDB_TRY(col(db->orders, "orders", "o_orderkey", GDB_I32, &oo));
   GDB_TRY(col(db->orders, "orders", "o_orderstatus", GDB_U8, &os));
   GDB_TRY(col(db->supplier, "supplier", "s_suppkey", GDB_I32, &sk));
@@ -1132,17 +1128,14 @@ int gdb_q21(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q21_params* p, gdb_q2
   sspan = std::max(sspan, a);
   const int k = std::max(p->top_k, 1);
   const int tblocks = 256;
-  const bool clustered = lo->sorted;  // rows of an order are contiguous
+  const bool clustered = lo->sorted && !db->lineitem->host;  // an order's rows are contiguous
   unsigned long long* state = nullptr;
-  uint32_t *list = nullptr, *cnt;
+  uint32_t* cnt;
   uint8_t *smap, *fst;
   TopEntry* win;
   uint64_t* dout;
   char* host;
-  if (!clustered) {
-    GDB_TRY(arena(ctx, A_AGG, ospan * 8, (void**)&state));
-    GDB_TRY(arena(ctx, A_AUX, std::max<uint64_t>(nl, 1) * 4, (void**)&list));
-  }
+  if (!clustered) GDB_TRY(arena(ctx, A_AGG, ospan * 8, (void**)&state));
   GDB_TRY(arena(ctx, A_BITMAP, ospan, (void**)&fst));
   GDB_TRY(arena(ctx, A_MAP, sspan * 5, (void**)&cnt));
   smap = (uint8_t*)(cnt + sspan);
@@ -1150,7 +1143,6 @@ int gdb_q21(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q21_params* p, gdb_q2
   const size_t out_bytes = 32 + (size_t)k * sizeof(TopEntry) + 64;
   GDB_TRY(arena(ctx, A_OUT, out_bytes, (void**)&dout));
   GDB_TRY(pinned(ctx, out_bytes, (void**)&host));
-  uint32_t* list_n = (uint32_t*)(dout + 2);
   TopEntry* fin = (TopEntry*)(dout + 4);
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
@@ -1167,24 +1159,41 @@ int gdb_q21(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q21_params* p, gdb_q2
     nation_bytes<<<blocks_for(ctx, nation_bytes, ns), BLOCK, 0, s>>>(
         (const int32_t*)sk->dev, (const int32_t*)sn->dev, (uint32_t)ns, smap);
   GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[1], s));
-  if (nl && clustered) {
-    q21_owner<<<blocks_for(ctx, q21_owner, quads(nl)), BLOCK, 0, s>>>(
-        (const int32_t*)lo->dev, (const int32_t*)lk->dev, (const int32_t*)lc->dev,
-        (const int32_t*)lr->dev, (uint32_t)nl, fst, smap, p->nation, cnt);
-    GDB_HIP(hipGetLastError());
-    GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  RunStats rs, rs2;
+  if (clustered) {
+    GDB_TRY(scan_table(ctx, db->lineitem, {lo, lk, lc, lr}, [&](const void* const* d, uint32_t rows) {
+      q21_owner<<<blocks_for(ctx, q21_owner, quads(rows)), BLOCK, 0, s>>>(
+          (const int32_t*)d[0], (const int32_t*)d[1], (const int32_t*)d[2], (const int32_t*)d[3],
+          rows, fst, smap, p->nation, cnt);
+      GDB_HIP(hipGetLastError());
+      return GDB_OK;
+    }, &rs));
   } else {
-    if (nl)
-      q21_track<<<blocks_for(ctx, q21_track, quads(nl)), BLOCK, 0, s>>>(
-          (const int32_t*)lo->dev, (const int32_t*)lk->dev, (const int32_t*)lc->dev,
-          (const int32_t*)lr->dev, (uint32_t)nl, fst, smap, p->nation, state, list, list_n);
-    GDB_HIP(hipGetLastError());
-    GDB_HIP(hipEventRecord(ctx->ev[2], s));
-    if (nl)
-      q21_count<<<blocks_for(ctx, q21_count, nl / 32 + 1), BLOCK, 0, s>>>(
-          (const int32_t*)lo->dev, (const int32_t*)lk->dev, list, list_n, state, cnt);
-    GDB_HIP(hipGetLastError());
+    // every order's state over all of lineitem first, then the counting pass
+    // (a host table is streamed twice)
+    GDB_TRY(scan_table(ctx, db->lineitem, {lo, lk, lc, lr}, [&](const void* const* d, uint32_t rows) {
+      q21_track<<<blocks_for(ctx, q21_track, quads(rows)), BLOCK, 0, s>>>(
+          (const int32_t*)d[0], (const int32_t*)d[1], (const int32_t*)d[2], (const int32_t*)d[3],
+          rows, fst, state);
+      GDB_HIP(hipGetLastError());
+      return GDB_OK;
+    }, &rs));
+    auto count = [&](const void* const* d, uint32_t rows) -> int {
+      q21_count_rows<<<blocks_for(ctx, q21_count_rows, quads(rows)), BLOCK, 0, s>>>(
+          (const int32_t*)d[0], (const int32_t*)d[1], (const int32_t*)d[2], (const int32_t*)d[3],
+          rows, fst, smap, p->nation, state, cnt);
+      GDB_HIP(hipGetLastError());
+      return GDB_OK;
+    };
+    if (!db->lineitem->host) {  // resident: launched directly, ev[1..2] keep timing pass 1
+      const void* d[4] = {lo->dev, lk->dev, lc->dev, lr->dev};
+      if (nl) GDB_TRY(count(d, (uint32_t)nl));
+    } else {
+      GDB_TRY(scan_table(ctx, db->lineitem, {lo, lk, lc, lr}, count, &rs2));
+      rs.kernel_ms += rs2.kernel_ms;
+      rs.h2d_bytes += rs2.h2d_bytes;
+      rs.chunks += rs2.chunks;
+    }
   }
   GDB_TRY(topk_run(ctx, CountSrc{cnt}, sspan, k, win, tblocks, fin, (unsigned long long*)dout,
                    (unsigned long long*)(dout + 1)));
@@ -1193,7 +1202,7 @@ int gdb_q21(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q21_params* p, gdb_q2
   // SQL columns: lineitem orderkey, suppkey, commitdate, receiptdate (16 B); orders
   // key + status (5 B); supplier key + nation (8 B). Mandatory in the lineitem pass:
   // l_orderkey, 4 B per line.
-  GDB_TRY(timing_finish(ctx, timing, 16 * nl + 5 * no + 8 * ns, 4 * nl));
+  GDB_TRY(timing_finish(ctx, timing, 16 * nl + 5 * no + 8 * ns, 4 * nl, &rs));
   const uint64_t* h = (const uint64_t*)host;
   const TopEntry* hf = (const TopEntry*)(h + 4);
   out->suppliers = (int64_t)h[0];

@@ -743,10 +743,10 @@ int gdb_q20(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q20_params* p, int32_
   GDB_TRY(col(db->partsupp, "partsupp", "ps_partkey", GDB_I32, &qp));
   GDB_TRY(col(db->partsupp, "partsupp", "ps_suppkey", GDB_I32, &qs));
   GDB_TRY(col(db->partsupp, "partsupp", "ps_availqty", GDB_I32, &qa));
-  GDB_TRY(col(db->lineitem, "lineitem", "l_shipdate", GDB_I32, &ls));
-  GDB_TRY(col(db->lineitem, "lineitem", "l_partkey", GDB_I32, &lp));
-  GDB_TRY(col(db->lineitem, "lineitem", "l_suppkey", GDB_I32, &lk));
-  GDB_TRY(col(db->lineitem, "lineitem", "l_quantity", GDB_I32, &lq));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_shipdate", GDB_I32, &ls));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_partkey", GDB_I32, &lp));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_suppkey", GDB_I32, &lk));
+  GDB_TRY(scol(db->lineitem, "lineitem", "l_quantity", GDB_I32, &lq));
   GDB_TRY(col(db->supplier, "supplier", "s_suppkey", GDB_I32, &sk));
   GDB_TRY(col(db->supplier, "supplier", "s_nationkey", GDB_I32, &sn));
   const uint64_t np = db->part->nrows, nps = db->partsupp->nrows, nl = db->lineitem->nrows,
@@ -803,14 +803,14 @@ int gdb_q20(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q20_params* p, int32_
         (const int32_t*)qp->dev, (const int32_t*)qs->dev, (uint32_t)nps, bits, keys, slots - 1,
         row);
   GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[1], s));
-  if (nl)
-    q20_scan<<<blocks_for(ctx, q20_scan, quads(nl)), BLOCK, 0, s>>>(
-        (const int32_t*)ls->dev, (const int32_t*)lp->dev, (const int32_t*)lk->dev,
-        (const int32_t*)lq->dev, (uint32_t)nl, p->date_lo, p->date_hi, bits, keys, slots - 1,
-        sum, lines);
-  GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  RunStats rs;
+  GDB_TRY(scan_table(ctx, db->lineitem, {ls, lp, lk, lq}, [&](const void* const* d, uint32_t rows) {
+    q20_scan<<<blocks_for(ctx, q20_scan, quads(rows)), BLOCK, 0, s>>>(
+        (const int32_t*)d[0], (const int32_t*)d[1], (const int32_t*)d[2], (const int32_t*)d[3],
+        rows, p->date_lo, p->date_hi, bits, keys, slots - 1, sum, lines);
+    GDB_HIP(hipGetLastError());
+    return GDB_OK;
+  }, &rs));
   q20_pairs<<<blocks_for(ctx, q20_pairs, slots), BLOCK, 0, s>>>(
       keys, slots, row, sum, lines, (const int32_t*)qa->dev, p->frac_num, p->frac_den, good);
   if (ns)
@@ -828,7 +828,7 @@ int gdb_q20(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q20_params* p, int32_
   // pair + availqty (12 B); part key + name; supplier key + nation. Mandatory in
   // q20_scan: l_shipdate, 4 B per line.
   GDB_TRY(timing_finish(ctx, timing, 16 * nl + 12 * nps + (4 + pn->width) * np + 8 * ns,
-                        4 * nl));
+                        4 * nl, &rs));
   std::sort(got.begin(), got.end());
   *nrows = (int64_t)n;
   for (int64_t i = 0; i < std::min<int64_t>(cap, (int64_t)n); ++i) suppkeys[i] = got[i];

@@ -32,12 +32,17 @@ def _mask3(codes):
 
 
 class TpchDb:
-    def __init__(self, engine, tables):
+    """stream_lineitem=True keeps lineitem in host memory (gdb_table_create_host):
+    the pipelines then stream it through the GPU in chunks (the reference's SF100
+    mode), every other table resident."""
+
+    def __init__(self, engine, tables, stream_lineitem=False):
         self.engine = engine
         self.tables = {}
         for name in TABLES:
             if name in tables:
-                self.tables[name] = engine.table(tables[name])
+                make = engine.host_table if (stream_lineitem and name == "lineitem") else engine.table
+                self.tables[name] = make(tables[name])
         self.db = abi.TpchDb(*[self.tables[n].handle.value if n in self.tables else None
                                for n in TABLES])
 
