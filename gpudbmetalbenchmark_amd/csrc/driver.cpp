@@ -27,7 +27,8 @@
 // The other 17 TPC-H pipelines (driver_tpch.cpp): they need a whole TPC-H
 // database of .tbl files.
 const std::vector<std::string>& tpch_queries();
-int run_tpch(gdb_ctx* ctx, const std::string& dir, const std::vector<std::string>& queries);
+int run_tpch(gdb_ctx* ctx, const std::string& dir, const std::vector<std::string>& queries,
+             bool stream);
 
 namespace {
 
@@ -540,7 +541,7 @@ int main(int argc, char** argv) {
     else if (q == "join") run_join(s, j);
     printf("\n");
   }
-  if (!run_tpch_q.empty()) run_tpch(g_ctx, g_data, run_tpch_q);
+  if (!run_tpch_q.empty()) run_tpch(g_ctx, g_data, run_tpch_q, s.stream);
   for (Buf* b : g_bufs) delete b;
   gdb_close(g_ctx);
   return 0;

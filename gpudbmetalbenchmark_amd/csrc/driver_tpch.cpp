@@ -202,10 +202,13 @@ struct Db {
   gdb_table* t[7] = {};  // lineitem, orders, customer, part, partsupp, supplier, nation
   uint64_t rows[7] = {};
   double parse_ms = 0;
+  Loaded streamed;       // a streamed lineitem's host columns (referenced, not copied)
   gdb_tpch_db view() const { return gdb_tpch_db{t[0], t[1], t[2], t[3], t[4], t[5], t[6]}; }
 };
 
-Db load_db(gdb_ctx* ctx, const std::string& dir) {
+// stream: lineitem stays in host memory and the pipelines stream it in chunks
+// (the reference's SF100 mode); the other tables are resident.
+Db load_db(gdb_ctx* ctx, const std::string& dir, bool stream) {
   const auto t0 = Clock::now();
   const std::vector<std::pair<const char*, std::vector<Spec>>> tables = {
       {"lineitem",
@@ -238,8 +241,16 @@ Db load_db(gdb_ctx* ctx, const std::string& dir) {
   Db db;
   for (size_t i = 0; i < tables.size(); ++i) {
     Loaded L = load_table(dir, tables[i].first, tables[i].second);
-    TCHECK(gdb_table_create(ctx, L.cols.data(), (int)L.cols.size(), L.n, &db.t[i]));
     db.rows[i] = L.n;
+    if (stream && i == 0) {
+      db.streamed = std::move(L);
+      for (size_t c = 0; c < db.streamed.cols.size(); ++c)
+        db.streamed.cols[c].name = db.streamed.names[c].c_str();
+      TCHECK(gdb_table_create_host(ctx, db.streamed.cols.data(), (int)db.streamed.cols.size(),
+                                   db.streamed.n, &db.t[i]));
+    } else {
+      TCHECK(gdb_table_create(ctx, L.cols.data(), (int)L.cols.size(), L.n, &db.t[i]));
+    }
   }
   db.parse_ms = ms_since(t0);
   return db;
@@ -279,9 +290,11 @@ const std::vector<std::string>& tpch_queries() {
   return q;
 }
 
-// Runs `queries` over the database in `dir`; returns a process exit code.
-int run_tpch(gdb_ctx* ctx, const std::string& dir, const std::vector<std::string>& queries) {
-  Db db = load_db(ctx, dir);
+// Runs `queries` over the database in `dir` (lineitem streamed from host memory
+// when `stream`); returns a process exit code.
+int run_tpch(gdb_ctx* ctx, const std::string& dir, const std::vector<std::string>& queries,
+             bool stream) {
+  Db db = load_db(ctx, dir, stream);
   const gdb_tpch_db v = db.view();
   const uint64_t L = db.rows[0];
   const std::vector<std::string> types = type_names(), conts = container_names();
