@@ -298,7 +298,7 @@ int flat_batch_launch(gdb_ctx* ctx, FlatSlot* s, const FlatPlan& plan, const gdb
     RunStats rs;
     GDB_TRY(run_over(ctx, li, c, 7, consume, &rs));
   } else if (li->nrows == 0) {
-    GDB_HIP(hipMemsetAsync(s->totals, 0, (size_t)plan.ncells * 8, stream));
+    GDB_TRY(fill(s->totals, 0, (size_t)plan.ncells * 8, stream));
   } else if (s->graphs) {
     // One graph per table and LDS size class: the scan is launched with the
     // class's LDS (the plan's layout lives in the descriptor) and the

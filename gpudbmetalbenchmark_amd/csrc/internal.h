@@ -38,6 +38,7 @@ struct Column {
   size_t bytes = 0;
   int64_t minv = 0, maxv = -1;  // value range of integer columns (empty: max < min)
   bool sorted = false;          // int32 column uploaded in non-decreasing order
+  bool strict = false;          // ... and strictly increasing (unique keys)
   const void* host = nullptr;   // host tables: caller memory streamed in chunks
 };
 
@@ -94,6 +95,8 @@ int dev_alloc(gdb_ctx* ctx, void** p, size_t bytes);
 void dev_free(gdb_ctx* ctx, void* p, size_t bytes);
 // Scratch slot `slot` of at least `bytes` (contents undefined).
 int arena(gdb_ctx* ctx, int slot, size_t bytes, void** out);
+// Sets `bytes` bytes at `p` to `byte` on stream `s` (a fill kernel, no memset node).
+int fill(void* p, int byte, size_t bytes, hipStream_t s);
 // Pinned host staging of at least `bytes`.
 int pinned(gdb_ctx* ctx, size_t bytes, void** out);
 // Column lookup with type check; sets the error message on failure.

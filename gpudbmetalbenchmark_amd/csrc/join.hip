@@ -554,7 +554,7 @@ extern "C" int gdb_join_count(gdb_ctx* ctx, const gdb_table* build, const char* 
     GDB_TRY(arena(ctx, A_PARTIALS, sizeof(int64_t) * nparts, (void**)&partials));
     const int fill = ctx->info.cu_count * 4;
     GDB_HIP(hipEventRecord(ctx->ev[0], st));
-    GDB_HIP(hipMemsetAsync(table, 0, (words + 16) * 4, st));
+    GDB_TRY(gdb::fill(table, 0, (words + 16) * 4, st));
     jd_bits<<<fill, 256, 0, st>>>(rk, nb, lo, table, dup);
     GDB_HIP(hipGetLastError());
     uint32_t* hdup = (uint32_t*)host;
@@ -564,7 +564,7 @@ extern "C" int gdb_join_count(gdb_ctx* ctx, const gdb_table* build, const char* 
     uint32_t* cnt = table;
     if (counts) {  // repeated build keys: a count per key value
       GDB_TRY(arena(ctx, A_JOINPART, span * 4, (void**)&cnt));
-      GDB_HIP(hipMemsetAsync(cnt, 0, span * 4, st));
+      GDB_TRY(gdb::fill(cnt, 0, span * 4, st));
       jd_counts<<<fill, 256, 0, st>>>(rk, nb, lo, cnt);
       GDB_HIP(hipGetLastError());
     }

@@ -208,7 +208,7 @@ extern "C" int gdb_q1(gdb_ctx* ctx, const gdb_table* li, const gdb_q1_params* p,
   RunStats rs;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
   if (n == 0) {
-    GDB_HIP(hipMemsetAsync(dout, 0, sizeof(int64_t) * K, s));
+    GDB_TRY(fill(dout, 0, sizeof(int64_t) * K, s));
     GDB_HIP(hipEventRecord(ctx->ev[1], s));
     GDB_HIP(hipEventRecord(ctx->ev[2], s));
   } else {

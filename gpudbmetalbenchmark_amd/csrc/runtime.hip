@@ -68,6 +68,33 @@ int arena(gdb_ctx* ctx, int slot, size_t bytes, void** out) {
   return GDB_OK;
 }
 
+// Byte fill: 16-byte stores over the aligned body, byte stores for the head and
+// tail. A kernel of its own, so it can also sit in captured graphs.
+__global__ __launch_bounds__(256) void fill_bytes(uint8_t* __restrict__ p, uint32_t word,
+                                                  uint64_t head, uint64_t n16, uint64_t bytes) {
+  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint4* q = (uint4*)(p + head);
+  const uint4 v = make_uint4(word, word, word, word);
+  for (uint64_t i = tid; i < n16; i += stride) q[i] = v;
+  const uint64_t tail0 = head + n16 * 16;
+  if (tid < head) p[tid] = (uint8_t)word;
+  if (tid < bytes - tail0) p[tail0 + tid] = (uint8_t)word;
+}
+
+int fill(void* ptr, int byte, size_t bytes, hipStream_t s) {
+  if (!bytes) return GDB_OK;
+  uint8_t* p = (uint8_t*)ptr;
+  const uint64_t mis = (uintptr_t)p & 15;
+  const uint64_t head = std::min<uint64_t>(bytes, mis ? 16 - mis : 0);
+  const uint64_t n16 = (bytes - head) / 16;
+  const uint32_t word = 0x01010101u * (uint8_t)byte;
+  const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>((n16 + 1023) / 1024, 8192));
+  fill_bytes<<<(unsigned)blocks, 256, 0, s>>>(p, word, head, n16, bytes);
+  GDB_HIP(hipGetLastError());
+  return GDB_OK;
+}
+
 int pinned(gdb_ctx* ctx, size_t bytes, void** out) {
   if (ctx->host_pinned_bytes < bytes) {
     if (ctx->host_pinned) {
@@ -137,8 +164,10 @@ int timing_finish(gdb_ctx* ctx, gdb_timing* t, uint64_t bytes, uint64_t main_byt
 
 // Min / max of an integer host column, split over threads for large columns.
 template <typename T>
-static void value_range(const T* v, uint64_t n, int64_t* lo, int64_t* hi, bool* sorted = nullptr) {
+static void value_range(const T* v, uint64_t n, int64_t* lo, int64_t* hi, bool* sorted = nullptr,
+                        bool* strict = nullptr) {
   if (sorted) *sorted = true;
+  if (strict) *strict = true;
   if (n == 0) {
     *lo = 0;
     *hi = -1;
@@ -146,20 +175,22 @@ static void value_range(const T* v, uint64_t n, int64_t* lo, int64_t* hi, bool* 
   }
   const int T_ = n > (1u << 22) ? (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency())) : 1;
   std::vector<int64_t> mn(T_, INT64_MAX), mx(T_, INT64_MIN);
-  std::vector<char> up(T_, 1);
+  std::vector<char> up(T_, 1), inc(T_, 1);
   const uint64_t per = (n + T_ - 1) / T_;
   auto body = [&](int t) {
     const uint64_t a = (uint64_t)t * per, b = std::min(n, a + per);
     int64_t l = INT64_MAX, h = INT64_MIN;
-    bool asc = true;
+    bool asc = true, str = true;
     for (uint64_t i = a; i < b; ++i) {
       l = std::min<int64_t>(l, (int64_t)v[i]);
       h = std::max<int64_t>(h, (int64_t)v[i]);
       asc &= i == 0 || v[i - 1] <= v[i];
+      str &= i == 0 || v[i - 1] < v[i];
     }
     mn[t] = l;
     mx[t] = h;
     up[t] = asc;
+    inc[t] = str;
   };
   std::vector<std::thread> th;
   for (int t = 1; t < T_; ++t) th.emplace_back(body, t);
@@ -168,6 +199,7 @@ static void value_range(const T* v, uint64_t n, int64_t* lo, int64_t* hi, bool* 
   *lo = *std::min_element(mn.begin(), mn.end());
   *hi = *std::max_element(mx.begin(), mx.end());
   if (sorted) *sorted = std::all_of(up.begin(), up.end(), [](char c) { return c != 0; });
+  if (strict) *strict = std::all_of(inc.begin(), inc.end(), [](char c) { return c != 0; });
 }
 
 }  // namespace gdb
@@ -336,7 +368,7 @@ int gdb_table_create(gdb_ctx* ctx, const gdb_col_desc* cols, int ncols, uint64_t
       return rc;
     }
     if (d.host_data && d.type == GDB_I32)
-      value_range((const int32_t*)d.host_data, nrows, &c.minv, &c.maxv, &c.sorted);
+      value_range((const int32_t*)d.host_data, nrows, &c.minv, &c.maxv, &c.sorted, &c.strict);
     else if (d.host_data && d.type == GDB_I64)
       value_range((const int64_t*)d.host_data, nrows, &c.minv, &c.maxv);
     t->cols.push_back(c);

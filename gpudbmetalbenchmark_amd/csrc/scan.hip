@@ -240,7 +240,7 @@ int gdb_q6(gdb_ctx* ctx, const gdb_table* li, const gdb_q6_params* p, gdb_q6_res
   RunStats rs;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
   if (n == 0) {
-    GDB_HIP(hipMemsetAsync(dout, 0, 16, s));
+    GDB_TRY(fill(dout, 0, 16, s));
     GDB_HIP(hipEventRecord(ctx->ev[1], s));
     GDB_HIP(hipEventRecord(ctx->ev[2], s));
   } else {

@@ -39,18 +39,18 @@ __device__ void bitonic_sort_best_first(TopEntry* buf, int n) {
   while (P < n) P <<= 1;
   for (int i = n + threadIdx.x; i < P; i += blockDim.x) buf[i] = worst_entry();
   __syncthreads();
+  // Each step compares P/2 pairs (i, i + stride); pair p puts a zero bit at
+  // `stride` into p to get i, so every thread has work.
   for (int size = 2; size <= P; size <<= 1) {
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int i = threadIdx.x; i < P; i += blockDim.x) {
-        const int j = i ^ stride;
-        if (j > i) {
-          const bool up = (i & size) == 0;  // this half sorts best-first
-          TopEntry a = buf[i], b = buf[j];
-          const bool swap = up ? better(b, a) : better(a, b);
-          if (swap) {
-            buf[i] = b;
-            buf[j] = a;
-          }
+      for (int p = threadIdx.x; p < P / 2; p += blockDim.x) {
+        const int i = ((p & ~(stride - 1)) << 1) | (p & (stride - 1));
+        const int j = i + stride;
+        const bool up = (i & size) == 0;  // this half sorts best-first
+        const TopEntry a = buf[i], b = buf[j];
+        if (up ? better(b, a) : better(a, b)) {
+          buf[i] = b;
+          buf[j] = a;
         }
       }
       __syncthreads();
@@ -59,8 +59,9 @@ __device__ void bitonic_sort_best_first(TopEntry* buf, int n) {
 }
 
 // Running top-k state in LDS. `push` must be called by every thread of the
-// block once per round with at most one candidate each (valid = false for none);
-// rounds of blockDim candidates never overflow while k + 2*blockDim <= BUF.
+// block (a multiple of 64 threads) once per round with at most one candidate
+// each (valid = false for none); rounds of blockDim candidates never overflow
+// while k + 2*blockDim <= BUF. A wave reserves its slots with one LDS atomic.
 //
 // With `thr` (an LDS slot) set, every sort that leaves k entries records the k-th
 // best in *thr, and later candidates that do not rank before it are dropped at
@@ -82,9 +83,13 @@ struct BlockTopK {
   }
   __device__ void push(bool valid, const TopEntry& e) {
     if (valid && have && *have && !better(e, *thr)) valid = false;
-    if (valid) {
-      const int pos = atomicAdd(n, 1);
-      buf[pos] = e;
+    const unsigned long long m = __ballot(valid);
+    if (m) {
+      const int lane = threadIdx.x & 63, leader = __ffsll((long long)m) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(n, __popcll(m));
+      base = __shfl(base, leader, 64);
+      if (valid) buf[base + __popcll(m & ((1ull << lane) - 1))] = e;
     }
     __syncthreads();
     const int cur = *n;

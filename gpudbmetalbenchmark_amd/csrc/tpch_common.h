@@ -149,9 +149,16 @@ __device__ __forceinline__ void block_add(int64_t v, int64_t* red, int64_t* out)
 
 // Two-stage top-k source over a candidate function: stage 1 runs a grid of
 // workgroups over [0, n), each keeping its k best of a contiguous slice, and
-// counts valid candidates; stage 2 runs one workgroup over the winners.
+// counts valid candidates; stage 2 runs one workgroup over the winners. With
+// `dn`, n is cut to *dn (a count another kernel wrote). A thread evaluates
+// TOPK_U candidates before pushing any, so their loads are in flight together:
+// sources load unconditionally (every index below n is valid) and return the
+// verdict, with no early exit.
+constexpr int TOPK_U = 4;
+
 template <class Src>
-__global__ __launch_bounds__(256) void topk_stage1(Src src, uint64_t n, int k,
+__global__ __launch_bounds__(256) void topk_stage1(Src src, uint64_t n,
+                                                   const uint32_t* __restrict__ dn, int k,
                                                    gdbk::TopEntry* __restrict__ out,
                                                    unsigned long long* __restrict__ count) {
   __shared__ gdbk::TopEntry buf[1024];
@@ -160,39 +167,82 @@ __global__ __launch_bounds__(256) void topk_stage1(Src src, uint64_t n, int k,
   __shared__ int64_t red[4];
   gdbk::BlockTopK<1024> tk{buf, &nbuf, k, &thr, &have};
   tk.init();
+  if (dn) n = n < *dn ? n : *dn;
   const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
   const uint64_t begin = (uint64_t)blockIdx.x * per;
   const uint64_t end = begin + per < n ? begin + per : n;
   int64_t cnt = 0;
-  for (uint64_t base = begin; base < end; base += blockDim.x) {
-    const uint64_t i = base + threadIdx.x;
-    gdbk::TopEntry e = gdbk::worst_entry();
-    const bool v = i < end && src(i, e);
-    cnt += v;
-    tk.push(v, e);
+  for (uint64_t base = begin; base < end; base += TOPK_U * blockDim.x) {
+    gdbk::TopEntry e[TOPK_U];
+    bool v[TOPK_U];
+#pragma unroll
+    for (int u = 0; u < TOPK_U; ++u) {
+      const uint64_t i = base + (uint64_t)u * blockDim.x + threadIdx.x;
+      const bool in = i < end;
+      v[u] = src(in ? i : begin, e[u]) && in;
+    }
+#pragma unroll
+    for (int u = 0; u < TOPK_U; ++u) {
+      cnt += v[u];
+      tk.push(v[u], e[u]);
+    }
   }
   tk.finish(out + (size_t)blockIdx.x * k);
   const int64_t c = gdbk::block_sum(cnt, red);
   if (threadIdx.x == 0 && c) atomicAdd(count, (unsigned long long)c);
 }
 
-struct WinnerSrc {
-  const gdbk::TopEntry* w;
-  __device__ bool operator()(uint64_t i, gdbk::TopEntry& e) const {
-    e = w[i];
-    return !(e.rev == INT64_MIN && e.key == INT32_MAX);
+// Stage 2, one workgroup over `lists` best-first lists of k entries (stage 1's
+// winners, worst-padded): rank 0 of every list, then rank 1, and so on. When no
+// entry of a rank is valid and ranks before the running k-th best, no later rank
+// can (each list only gets worse), so the merge stops there.
+namespace {
+__global__ __launch_bounds__(256) void topk_merge(const gdbk::TopEntry* __restrict__ w, int lists,
+                                                  int k, gdbk::TopEntry* __restrict__ out) {
+  __shared__ gdbk::TopEntry buf[1024];
+  __shared__ gdbk::TopEntry thr;
+  __shared__ int nbuf, have, any;
+  gdbk::BlockTopK<1024> tk{buf, &nbuf, k, &thr, &have};
+  tk.init();
+  for (int r = 0; r < k; ++r) {
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    for (int b0 = 0; b0 < lists; b0 += blockDim.x) {
+      const int b = b0 + threadIdx.x;
+      gdbk::TopEntry e = gdbk::worst_entry();
+      if (b < lists) e = w[(size_t)b * k + r];
+      const bool v = !(e.rev == INT64_MIN && e.key == INT32_MAX);
+      if (v && !(have && !gdbk::better(e, thr))) any = 1;
+      tk.push(v, e);  // ends with a barrier: `any` and thr are settled after it
+    }
+    const int go = any;
+    __syncthreads();  // every thread has read `any` before the next reset
+    if (!go) break;
   }
-};
+  tk.finish(out);
+}
+}  // namespace
 
-// Runs both stages; the k best (worst-padded) land in `final_out`.
+// Stage-1 workgroups for n candidates: about 2048 each, at most TOPK_MAX_BLOCKS
+// (the winners buffer holds that many lists of k). A count known only on the
+// device (`counted`: n is then a bound) gets 256 workgroups.
+constexpr int TOPK_MAX_BLOCKS = 2048;
+inline int topk_blocks(uint64_t n, bool counted = false) {
+  if (counted) return 256;
+  const uint64_t b = (n + 2047) / 2048;
+  return (int)(b < 1 ? 1 : b > (uint64_t)TOPK_MAX_BLOCKS ? TOPK_MAX_BLOCKS : b);
+}
+
+// Runs both stages over topk_blocks(n, dn != nullptr) workgroups (`winners`
+// holds that many lists of k); the k best (worst-padded) land in `final_out`.
 template <class Src>
-int topk_run(gdb_ctx* ctx, Src src, uint64_t n, int k, gdbk::TopEntry* winners, int blocks,
-             gdbk::TopEntry* final_out, unsigned long long* count, unsigned long long* dummy) {
+int topk_run(gdb_ctx* ctx, Src src, uint64_t n, int k, gdbk::TopEntry* winners,
+             gdbk::TopEntry* final_out, unsigned long long* count, const uint32_t* dn = nullptr) {
   hipStream_t s = ctx->stream;
-  topk_stage1<Src><<<blocks, BLOCK, 0, s>>>(src, n, k, winners, count);
+  const int blocks = topk_blocks(n, dn != nullptr);
+  topk_stage1<Src><<<blocks, BLOCK, 0, s>>>(src, n, dn, k, winners, count);
   GDB_HIP(hipGetLastError());
-  topk_stage1<WinnerSrc><<<1, BLOCK, 0, s>>>(WinnerSrc{winners}, (uint64_t)blocks * k, k,
-                                              final_out, dummy);
+  topk_merge<<<1, BLOCK, 0, s>>>(winners, blocks, k, final_out);
   GDB_HIP(hipGetLastError());
   return GDB_OK;
 }

@@ -313,9 +313,8 @@ struct KeyedRevSrc {  // (revenue desc, key asc) over a per-key revenue array
   const int64_t* rev;
   const uint8_t* seen;
   __device__ bool operator()(uint64_t i, TopEntry& e) const {
-    if (!seen[i]) return false;
     e = TopEntry{rev[i], 0, (int32_t)i, 0, 0};
-    return true;
+    return seen[i] != 0;
   }
 };
 
@@ -376,9 +375,8 @@ struct Q18Src {  // orders over the threshold: (totalprice desc, orderdate asc, 
   int32_t thr;
   __device__ bool operator()(uint64_t i, TopEntry& e) const {
     const int32_t key = ok[i];
-    if (sq[key] <= thr) return false;
     e = TopEntry{tp[i], od[i], key, (int32_t)i, 0};
-    return true;
+    return sq[key] > thr;
   }
 };
 
@@ -388,7 +386,7 @@ __global__ void q18_gather(const TopEntry* __restrict__ win, int k,
   for (int i = threadIdx.x; i < k; i += blockDim.x) {
     if (win[i].key == INT32_MAX) continue;
     out[2 * i] = oc[win[i].prio];
-    out[2 * i + 1] = sq[win[i].key];
+    out[2 * i + 1] = sq ? sq[win[i].key] : win[i].pad;
   }
 }
 
@@ -493,69 +491,149 @@ __global__ __launch_bounds__(256) void q21_count_rows(const int32_t* __restrict_
 }
 
 // ---- clustered lineitem: one lane resolves each order ---------------------------
-// When l_orderkey is non-decreasing, the rows of an order are contiguous. The lane
-// whose group of 4 rows holds an order's first row owns the order and computes its
-// result alone, so no per-order state is shared. The owner sees a 12-row window in
-// registers: its own 4 rows and the next two lanes' 4 rows each (wave shuffles;
-// TPC-H orders have at most 7 rows). Only rows past the window (the last two lanes
-// of a wave, or longer orders) are read again from memory. Loops run wave-uniform
-// so every lane takes part in the shuffles.
+// Per-order results over a clustered lineitem (l_orderkey non-decreasing, so
+// an order's rows are contiguous), as a segmented suffix scan across the wave.
+// A lane holds 4 rows; a boundary is a row that starts an order (or lies past n).
+// The order state St is associative: St::merge(a, b) combines the rows of a with
+// the later rows of b, and St{} is empty. Within a lane, r[j] folds rows j.. up
+// to the next boundary and e[j] says that run reaches the lane's end; `lead`
+// folds the lane's rows before its first boundary and `pass` says the lane has
+// none. The suffix scan gives lane L the fold of the leads of lanes L+1.. up to
+// and including the first lane with a boundary. Past the wave's last lane the
+// run continues into quads g0+64 and g0+65, which lanes 0 and 1 load beside their
+// own, and for orders longer than that into rows read from memory. The order
+// starting at row j of lane L is then r[j], merged with that carry when e[j]:
+// the lane holding an order's first row has its whole state and acts alone.
 
-constexpr int WIN = 12;
-
-// w[0..3] = this lane's rows, w[4..11] = the next two lanes' rows.
-__device__ __forceinline__ void row_window(const int4& v, int* w) {
-  w[0] = v.x;
-  w[1] = v.y;
-  w[2] = v.z;
-  w[3] = v.w;
-#pragma unroll
-  for (int d = 1; d < WIN / 4; ++d) {
-    w[4 * d + 0] = __shfl_down(v.x, d, 64);
-    w[4 * d + 1] = __shfl_down(v.y, d, 64);
-    w[4 * d + 2] = __shfl_down(v.z, d, 64);
-    w[4 * d + 3] = __shfl_down(v.w, d, 64);
-  }
+// Lanes 0 and 1 load quads g0+64 and g0+65 (zero past the column's nq quads).
+__device__ __forceinline__ int4 ext_quad(const int32_t* __restrict__ c, uint32_t g0, uint32_t nq,
+                                         int lane) {
+  const uint32_t e = g0 + 64 + (uint32_t)lane;
+  return lane < 2 && e < nq ? ld4nt(c, e) : make_int4(0, 0, 0, 0);
 }
 
-// Bit j set when row 4g+j starts an order (rows >= n never do).
-__device__ __forceinline__ uint32_t order_heads(const int32_t* __restrict__ ok, const int4& k,
-                                                uint32_t g, uint32_t n, int lane) {
-  const int up = __shfl_up(k.w, 1, 64);
-  const int prev = lane > 0 ? up : (g > 0 && 4 * g <= n ? ok[4 * g - 1] : 0);
-  uint32_t h = 0;
+// Lane 0 of a wave reads the row before its quad (the last row of quad g-1).
+__device__ __forceinline__ int prev_row(const int32_t* __restrict__ ok, uint32_t g, bool live,
+                                        int lane) {
+  return lane == 0 && live && g > 0 ? ok[4 * g - 1] : 0;
+}
+
+__device__ __forceinline__ int read_lane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+template <class St>
+struct LaneRuns {
+  St r[4];
+  bool e[4];
+  uint32_t head;  // bit j: row j starts an order
+  St lead;
+  bool pass;
+};
+
+// Runs of the 4 rows at row0.. with keys k and row states row[]; `prev` is the
+// key of row row0-1 (ignored for row 0 of the table).
+template <class St>
+__device__ __forceinline__ void lane_runs(const int4& k, const St* row, int prev, uint32_t row0,
+                                          uint32_t n, LaneRuns<St>& o) {
+  const int key[4] = {k.x, k.y, k.z, k.w};
+  bool b[4], valid[4];
+  o.head = 0;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const uint32_t r = 4 * g + j;
-    const int pj = j == 0 ? prev : lane_of(k, j - 1);
-    h |= (uint32_t)(r < n && (r == 0 || pj != lane_of(k, j))) << j;
+    valid[j] = row0 + j < n;
+    const bool h = valid[j] && (row0 + j == 0 || key[j] != (j ? key[j - 1] : prev));
+    b[j] = !valid[j] || h;
+    o.head |= (uint32_t)h << j;
   }
-  return h;
+  o.r[3] = valid[3] ? row[3] : St{};
+  o.e[3] = valid[3];
+#pragma unroll
+  for (int j = 2; j >= 0; --j) {
+    const St rj = valid[j] ? row[j] : St{};
+    o.r[j] = b[j + 1] ? rj : St::merge(rj, o.r[j + 1]);
+    o.e[j] = !b[j + 1] && o.e[j + 1];
+  }
+  o.lead = b[0] ? St{} : o.r[0];
+  o.pass = !b[0] && o.e[0];
 }
 
-// Window rows this lane may read from registers (the last lanes see fewer).
-__device__ __forceinline__ int window_rows(int lane) {
-  return lane <= 63 - (WIN / 4 - 1) ? WIN : 4 * (64 - lane);
-}
-
-// Q21: per order of the wanted status, suppliers and late suppliers; the order's
-// late lines count for their supplier when the order has a second supplier and
-// every late line has the same supplier (of `nation`).
-struct Q21Acc {
-  uint32_t first = 0, lfirst = 0, lcount = 0;
-  bool multi = false, lmulti = false;
-  __device__ void add(int sv, int cv, int rv) {
-    const uint32_t s1 = (uint32_t)sv + 1u;
-    if (!first) first = s1;
-    else multi |= first != s1;
-    if (rv > cv) {
-      if (!lfirst) lfirst = s1;
-      else lmulti |= lfirst != s1;
-      ++lcount;
+// The carry each lane merges into runs that reach its end. `ext` continues the
+// wave's last run (the leads past lane 63).
+template <class St>
+__device__ __forceinline__ St wave_carry(const LaneRuns<St>& lr, const St& ext, int lane) {
+  St v = lr.lead;
+  bool closed = !lr.pass;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const St v2 = v.shfl_down(d);
+    const int c2 = __shfl_down((int)closed, d, 64);
+    if (lane + d < 64 && !closed) {
+      v = St::merge(v, v2);
+      closed = c2 != 0;
     }
+  }
+  if (!closed) v = St::merge(v, ext);
+  const St nxt = v.shfl_down(1);
+  return lane < 63 ? nxt : ext;
+}
+
+// The fold past lane 63: lanes 0 and 1 run over their extra quads (keys ek, row
+// states xrow), then `more(r, key)` folds rows from r on while they carry `key`
+// (wave-uniform; only for orders longer than the two extra quads).
+template <class St, class More>
+__device__ __forceinline__ St ext_carry(const int4& k, const int4& ek, const St* xrow,
+                                        uint32_t gw, uint32_t nq, uint32_t n, int lane,
+                                        More more) {
+  const int prev = lane == 0 ? read_lane(k.w, 63) : read_lane(ek.w, 0);
+  const uint32_t e = gw + 64 + (uint32_t)lane;
+  LaneRuns<St> x;
+  lane_runs(ek, xrow, prev, 4 * e, lane < 2 && e < nq ? n : 0, x);
+  St ext = x.lead.read(0);
+  if (read_lane((int)x.pass, 0)) {
+    ext = St::merge(ext, x.lead.read(1));
+    if (read_lane((int)x.pass, 1)) ext = St::merge(ext, more(4 * (gw + 66), read_lane(ek.w, 1)));
+  }
+  return ext;
+}
+
+// Q18: quantity sum per order.
+struct QtySt {
+  int v = 0;
+  __device__ static QtySt merge(const QtySt& a, const QtySt& b) { return QtySt{a.v + b.v}; }
+  __device__ QtySt shfl_down(int d) const { return QtySt{__shfl_down(v, d, 64)}; }
+  __device__ QtySt read(int l) const { return QtySt{read_lane(v, l)}; }
+};
+
+// Q21: per order, a supplier (+1) of its lines and of its late lines, the late
+// line count, and whether a second supplier / second late supplier appears.
+struct SuppSt {
+  static constexpr uint32_t COUNT = (1u << 30) - 1, MULTI = 1u << 30, LMULTI = 1u << 31;
+  uint32_t first = 0, lfirst = 0, lc = 0;  // lc: late count | MULTI | LMULTI
+  __device__ static SuppSt row(int s, int c, int r) {
+    const uint32_t s1 = (uint32_t)s + 1u;
+    const bool late = r > c;
+    return SuppSt{s1, late ? s1 : 0u, late ? 1u : 0u};
+  }
+  __device__ static SuppSt merge(const SuppSt& a, const SuppSt& b) {
+    const uint32_t multi = ((a.lc | b.lc) & MULTI) |
+                           (a.first && b.first && a.first != b.first ? MULTI : 0u);
+    const uint32_t lmulti = ((a.lc | b.lc) & LMULTI) |
+                            (a.lfirst && b.lfirst && a.lfirst != b.lfirst ? LMULTI : 0u);
+    return SuppSt{a.first ? a.first : b.first, a.lfirst ? a.lfirst : b.lfirst,
+                  ((a.lc & COUNT) + (b.lc & COUNT)) | multi | lmulti};
+  }
+  __device__ SuppSt shfl_down(int d) const {
+    return SuppSt{(uint32_t)__shfl_down((int)first, d, 64),
+                  (uint32_t)__shfl_down((int)lfirst, d, 64), (uint32_t)__shfl_down((int)lc, d, 64)};
+  }
+  __device__ SuppSt read(int l) const {
+    return SuppSt{(uint32_t)read_lane((int)first, l), (uint32_t)read_lane((int)lfirst, l),
+                  (uint32_t)read_lane((int)lc, l)};
   }
 };
 
+// Q21 over a clustered lineitem: an order of the wanted status counts its late
+// lines for their supplier when it has a second supplier and every late line has
+// the same supplier (of `nation`).
 __global__ __launch_bounds__(256) void q21_owner(const int32_t* __restrict__ ok,
                                                  const int32_t* __restrict__ sk,
                                                  const int32_t* __restrict__ commit,
@@ -565,100 +643,129 @@ __global__ __launch_bounds__(256) void q21_owner(const int32_t* __restrict__ ok,
                                                  uint32_t* __restrict__ cnt) {
   const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
   const int lane = threadIdx.x & 63;
-  const int avail = window_rows(lane);
+  const int4 z = make_int4(0, 0, 0, 0);
   for (uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; gw < nq; gw += stride) {
     const uint32_t g = gw + lane;
     const bool live = g < nq;
-    const int4 z = make_int4(0, 0, 0, 0);
-    const int4 k = live ? ld4nt(ok, g) : z;
-    uint32_t heads = order_heads(ok, k, g, live ? n : 0, lane);
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-      if (((heads >> j) & 1) && !fst[lane_of(k, j)]) heads &= ~(1u << j);
-    // every lane takes part in the shuffles; only lanes with owned orders load more
-    const bool need = __any(heads != 0) != 0;
-    if (!need) continue;
-    const int4 s = live ? ld4nt(sk, g) : z, c = live ? ld4nt(commit, g) : z,
-               rc = live ? ld4nt(receipt, g) : z;
-    int wk[WIN], ws[WIN], wc[WIN], wr[WIN];
-    row_window(k, wk);
-    row_window(s, ws);
-    row_window(c, wc);
-    row_window(rc, wr);
+    const int4 k = live ? ld4nt(ok, g) : z, s = live ? ld4nt(sk, g) : z,
+               c = live ? ld4nt(commit, g) : z, rc = live ? ld4nt(receipt, g) : z;
+    const int4 ek = ext_quad(ok, gw, nq, lane), es = ext_quad(sk, gw, nq, lane),
+               ec = ext_quad(commit, gw, nq, lane), er = ext_quad(receipt, gw, nq, lane);
+    const int p0 = prev_row(ok, g, live, lane);
+    // Order status of every row's key, loaded at once (keys of dead lanes are 0).
+    const uint8_t f[4] = {fst[k.x], fst[k.y], fst[k.z], fst[k.w]};
+    const int up = __shfl_up(k.w, 1, 64);
+    SuppSt row[4], xrow[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      if (!((heads >> j) & 1)) continue;
-      const int key = wk[j];
-      Q21Acc acc;
-      acc.add(ws[j], wc[j], wr[j]);
-      bool open = true;
-      uint32_t r = 4 * g + j + 1;
+      row[j] = SuppSt::row(lane_of(s, j), lane_of(c, j), lane_of(rc, j));
+      xrow[j] = SuppSt::row(lane_of(es, j), lane_of(ec, j), lane_of(er, j));
+    }
+    LaneRuns<SuppSt> lr;
+    lane_runs(k, row, lane > 0 ? up : p0, 4 * g, live ? n : 0, lr);
+    const SuppSt ext = ext_carry<SuppSt>(k, ek, xrow, gw, nq, n, lane, [&](uint32_t r, int key) {
+      SuppSt a;
+      for (; r < n && ok[r] == key; ++r) a = SuppSt::merge(a, SuppSt::row(sk[r], commit[r], receipt[r]));
+      return a;
+    });
+    const SuppSt carry = wave_carry(lr, ext, lane);
+    uint32_t late[4], supp[4];
+    bool want[4];
 #pragma unroll
-      for (int i = j + 1; i < WIN; ++i) {
-        if (!open) continue;
-        if (i >= avail || 4 * g + i >= n) break;
-        if (wk[i] != key) {
-          open = false;
-          continue;
+    for (int j = 0; j < 4; ++j) {
+      const SuppSt t = lr.e[j] ? SuppSt::merge(lr.r[j], carry) : lr.r[j];
+      late[j] = t.lc & SuppSt::COUNT;
+      want[j] = ((lr.head >> j) & 1) && f[j] && (t.lc & SuppSt::MULTI) && late[j] &&
+                !(t.lc & SuppSt::LMULTI);
+      supp[j] = want[j] ? t.lfirst - 1 : 0u;
+    }
+    // Supplier nations of the candidates, loaded at once.
+    const uint8_t sn[4] = {smap[supp[0]], smap[supp[1]], smap[supp[2]], smap[supp[3]]};
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (want[j] && sn[j] == nation) atomicAdd(&cnt[supp[j]], late[j]);
+  }
+}
+
+// Q18 over a clustered lineitem: the quantity sum per order. EMIT = false: a
+// plain store into sq[orderkey]. EMIT = true: orders over `thr` are appended as
+// (key, sum) to `cand` (one atomic per wave and row slot), so nothing keyed by
+// orderkey needs clearing and the top-k runs over the few candidates.
+template <bool EMIT>
+__global__ __launch_bounds__(256) void q18_owner(const int32_t* __restrict__ ok,
+                                                 const int32_t* __restrict__ qty, uint32_t n,
+                                                 int32_t* __restrict__ sq, int32_t thr,
+                                                 int2* __restrict__ cand,
+                                                 uint32_t* __restrict__ ncand) {
+  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
+  const int lane = threadIdx.x & 63;
+  const int4 z = make_int4(0, 0, 0, 0);
+  for (uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; gw < nq; gw += stride) {
+    const uint32_t g = gw + lane;
+    const bool live = g < nq;
+    const int4 k = live ? ld4nt(ok, g) : z, q = live ? ld4nt(qty, g) : z;
+    const int4 ek = ext_quad(ok, gw, nq, lane), eq = ext_quad(qty, gw, nq, lane);
+    const int p0 = prev_row(ok, g, live, lane);
+    const int up = __shfl_up(k.w, 1, 64);
+    const QtySt row[4] = {{q.x}, {q.y}, {q.z}, {q.w}}, xrow[4] = {{eq.x}, {eq.y}, {eq.z}, {eq.w}};
+    LaneRuns<QtySt> lr;
+    lane_runs(k, row, lane > 0 ? up : p0, 4 * g, live ? n : 0, lr);
+    const QtySt ext = ext_carry<QtySt>(k, ek, xrow, gw, nq, n, lane, [&](uint32_t r, int key) {
+      QtySt a;
+      for (; r < n && ok[r] == key; ++r) a.v += qty[r];
+      return a;
+    });
+    const QtySt carry = wave_carry(lr, ext, lane);
+    const int key[4] = {k.x, k.y, k.z, k.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const bool mine = (lr.head >> j) & 1;
+      const int sum = lr.r[j].v + (lr.e[j] ? carry.v : 0);
+      if (EMIT) {  // every lane reaches the ballot
+        const bool emit = mine && sum > thr;
+        const unsigned long long m = __ballot(emit);
+        if (m) {
+          const int leader = __ffsll((long long)m) - 1;
+          uint32_t base = 0;
+          if (lane == leader) base = atomicAdd(ncand, (uint32_t)__popcll(m));
+          base = __shfl(base, leader, 64);
+          if (emit) cand[base + __popcll(m & ((1ull << lane) - 1))] = make_int2(key[j], sum);
         }
-        acc.add(ws[i], wc[i], wr[i]);
-        r = 4 * g + i + 1;
+      } else if (mine) {
+        sq[key[j]] = sum;
       }
-      if (open)
-        for (; r < n && ok[r] == key; ++r) acc.add(sk[r], commit[r], receipt[r]);
-      if (acc.multi && acc.lcount && !acc.lmulti && smap[acc.lfirst - 1] == nation)
-        atomicAdd(&cnt[acc.lfirst - 1], acc.lcount);
     }
   }
 }
 
-// Q18: the owner writes the order's quantity sum with a plain store.
-__global__ __launch_bounds__(256) void q18_owner(const int32_t* __restrict__ ok,
-                                                 const int32_t* __restrict__ qty, uint32_t n,
-                                                 int32_t* __restrict__ sq) {
-  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
-  const int lane = threadIdx.x & 63;
-  const int avail = window_rows(lane);
-  for (uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; gw < nq; gw += stride) {
-    const uint32_t g = gw + lane;
-    const bool live = g < nq;
-    const int4 z = make_int4(0, 0, 0, 0);
-    const int4 k = live ? ld4nt(ok, g) : z, q = live ? ld4nt(qty, g) : z;
-    const uint32_t heads = order_heads(ok, k, g, live ? n : 0, lane);
-    int wk[WIN], wq[WIN];
-    row_window(k, wk);
-    row_window(q, wq);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      if (!((heads >> j) & 1)) continue;
-      const int key = wk[j];
-      int sum = wq[j];
-      bool open = true;
-      uint32_t r = 4 * g + j + 1;
-#pragma unroll
-      for (int i = j + 1; i < WIN; ++i) {
-        if (!open) continue;
-        if (i >= avail || 4 * g + i >= n) break;
-        if (wk[i] != key) {
-          open = false;
-          continue;
-        }
-        sum += wq[i];
-        r = 4 * g + i + 1;
-      }
-      if (open)
-        for (; r < n && ok[r] == key; ++r) sum += qty[r];
-      sq[key] = sum;
+// Candidates (orderkey, sum) looked up in the orders table, whose keys ascend
+// strictly: the first row with o_orderkey >= key by binary search. Keys with no
+// orders row are not candidates (the join drops them).
+struct Q18CandSrc {
+  const int2* cand;
+  const int32_t *ok, *od;
+  const int64_t* tp;
+  uint32_t no;
+  __device__ bool operator()(uint64_t i, TopEntry& e) const {
+    const int2 c = cand[i];
+    uint32_t lo = 0, hi = no;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (ok[mid] < c.x) lo = mid + 1;
+      else hi = mid;
     }
+    const uint32_t row = lo < no ? lo : no - 1;
+    e = TopEntry{tp[row], od[row], c.x, (int32_t)row, c.y};
+    return lo < no && ok[row] == c.x;
   }
-}
+};
 
 struct CountSrc {  // (count desc, key asc)
   const uint32_t* cnt;
   __device__ bool operator()(uint64_t i, TopEntry& e) const {
-    if (!cnt[i]) return false;
-    e = TopEntry{(int64_t)cnt[i], 0, (int32_t)i, 0, 0};
-    return true;
+    const uint32_t c = cnt[i];
+    e = TopEntry{(int64_t)c, 0, (int32_t)i, 0, 0};
+    return c != 0;
   }
 };
 
@@ -721,10 +828,10 @@ int gdb_q5(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q5_params* p, gdb_q5_r
   GDB_TRY(pinned(ctx, 256, (void**)&host));
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_HIP(hipMemsetAsync(omap, NONE, ospan, s));
-  GDB_HIP(hipMemsetAsync(cmap, NONE, cspan, s));
-  GDB_HIP(hipMemsetAsync(smap, NONE, sspan, s));
-  GDB_HIP(hipMemsetAsync(dout, 0, 26 * 8, s));
+  GDB_TRY(fill(omap, NONE, ospan, s));
+  GDB_TRY(fill(cmap, NONE, cspan, s));
+  GDB_TRY(fill(smap, NONE, sspan, s));
+  GDB_TRY(fill(dout, 0, 26 * 8, s));
   GDB_TRY(build_nreg(ctx, db, nreg));
   if (nc)
     cust_nation_in_region<<<blocks_for(ctx, cust_nation_in_region, nc), BLOCK, 0, s>>>(
@@ -792,10 +899,10 @@ int gdb_q7(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q7_params* p, gdb_q7_r
   GDB_TRY(pinned(ctx, 256, (void**)&host));
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_HIP(hipMemsetAsync(omap, NONE, ospan, s));
-  GDB_HIP(hipMemsetAsync(cmap, NONE, cspan, s));
-  GDB_HIP(hipMemsetAsync(smap, NONE, sspan, s));
-  GDB_HIP(hipMemsetAsync(dout, 0, 17 * 8, s));
+  GDB_TRY(fill(omap, NONE, ospan, s));
+  GDB_TRY(fill(cmap, NONE, cspan, s));
+  GDB_TRY(fill(smap, NONE, sspan, s));
+  GDB_TRY(fill(dout, 0, 17 * 8, s));
   if (nc)
     pair_side<<<blocks_for(ctx, pair_side, nc), BLOCK, 0, s>>>(
         (const int32_t*)ck->dev, (const int32_t*)cn->dev, (uint32_t)nc, p->nation1, p->nation2,
@@ -883,11 +990,11 @@ int gdb_q8(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q8_params* p, gdb_q8_r
   GDB_TRY(pinned(ctx, 256, (void**)&host));
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_HIP(hipMemsetAsync(omap, NONE, ospan, s));
-  GDB_HIP(hipMemsetAsync(cmap, NONE, cspan, s));
-  GDB_HIP(hipMemsetAsync(smap, NONE, sspan, s));
-  GDB_HIP(hipMemsetAsync(pbits, 0, (words4 + FOLD_WORDS) * 4, s));
-  GDB_HIP(hipMemsetAsync(dout, 0, 17 * 8, s));
+  GDB_TRY(fill(omap, NONE, ospan, s));
+  GDB_TRY(fill(cmap, NONE, cspan, s));
+  GDB_TRY(fill(smap, NONE, sspan, s));
+  GDB_TRY(fill(pbits, 0, (words4 + FOLD_WORDS) * 4, s));
+  GDB_TRY(fill(dout, 0, 17 * 8, s));
   GDB_TRY(build_nreg(ctx, db, nreg));
   if (np) {
     Mask4 m{{0, 0, 0, 0}};
@@ -954,7 +1061,7 @@ int gdb_q10(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q10_params* p, gdb_q1
   GDB_TRY(key_span(oc, no, &a));
   cspan = std::max(cspan, a);
   const int k = p->top_k;
-  const int tblocks = 256;
+  const int tblocks = topk_blocks(cspan);
   int32_t* ocust;
   int64_t* crev;
   uint8_t *seen, *rank;
@@ -976,11 +1083,11 @@ int gdb_q10(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q10_params* p, gdb_q1
   int32_t* nat = (int32_t*)(bal + k);
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_HIP(hipMemsetAsync(ocust, 0xFF, ospan * 4, s));
-  GDB_HIP(hipMemsetAsync(crev, 0, cspan * 8, s));
-  GDB_HIP(hipMemsetAsync(seen, 0, cspan, s));
-  GDB_HIP(hipMemsetAsync(rank, NONE, cspan, s));
-  GDB_HIP(hipMemsetAsync(dout, 0, out_bytes, s));
+  GDB_TRY(fill(ocust, 0xFF, ospan * 4, s));
+  GDB_TRY(fill(crev, 0, cspan * 8, s));
+  GDB_TRY(fill(seen, 0, cspan, s));
+  GDB_TRY(fill(rank, NONE, cspan, s));
+  GDB_TRY(fill(dout, 0, out_bytes, s));
   if (no)
     q10_orders<<<blocks_for(ctx, q10_orders, quads(no)), BLOCK, 0, s>>>(
         (const int32_t*)oo->dev, (const int32_t*)oc->dev, (const int32_t*)od->dev, (uint32_t)no,
@@ -995,8 +1102,7 @@ int gdb_q10(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q10_params* p, gdb_q1
     return GDB_OK;
   }, &rs));
   if (k > 0) {
-    GDB_TRY(topk_run(ctx, KeyedRevSrc{crev, seen}, cspan, k, win, tblocks, fin,
-                     (unsigned long long*)dout, (unsigned long long*)(dout + 1)));
+    GDB_TRY(topk_run(ctx, KeyedRevSrc{crev, seen}, cspan, k, win, fin, (unsigned long long*)dout));
     mark_ranks<<<1, 128, 0, s>>>(fin, k, rank);
     if (nc)
       gather_customers<<<blocks_for(ctx, gather_customers, nc), BLOCK, 0, s>>>(
@@ -1004,8 +1110,8 @@ int gdb_q10(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q10_params* p, gdb_q1
           (uint32_t)nc, rank, (uint32_t)cspan, bal, nat);
     GDB_HIP(hipGetLastError());
   } else {
-    GDB_TRY(topk_run(ctx, KeyedRevSrc{crev, seen}, cspan, 1, win, tblocks, win + tblocks,
-                     (unsigned long long*)dout, (unsigned long long*)(dout + 1)));
+    GDB_TRY(topk_run(ctx, KeyedRevSrc{crev, seen}, cspan, 1, win, win + tblocks,
+                     (unsigned long long*)dout));
   }
   GDB_HIP(hipMemcpyAsync(host, dout, out_bytes, hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
@@ -1047,39 +1153,56 @@ int gdb_q18(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q18_params* p, gdb_q1
   GDB_TRY(key_span(lo, nl, &b));
   const uint64_t ospan = std::max(a, b);
   const int k = std::max(p->top_k, 1);
-  const int tblocks = 256;
-  int32_t* sq;
+  // Clustered, resident lineitem: owners emit the orders over the threshold
+  // (at most one per distinct l_orderkey), looked up in orders by binary search
+  // when o_orderkey ascends strictly. Otherwise sums go to sq[orderkey] and the
+  // top-k runs over every orders row.
+  const bool clustered = lo->sorted && !db->lineitem->host;  // an order's rows are contiguous
+  const bool emit = clustered && oo->strict && no > 0;
+  const uint64_t cap = std::max<uint64_t>(1, std::min<uint64_t>(nl, ospan));
+  int32_t* sq = nullptr;
+  int2* cand = nullptr;
   TopEntry* win;
   uint64_t* dout;
   char* host;
-  GDB_TRY(arena(ctx, A_MAP, ospan * 4, (void**)&sq));
+  if (emit) GDB_TRY(arena(ctx, A_MAP, cap * sizeof(int2), (void**)&cand));
+  else GDB_TRY(arena(ctx, A_MAP, ospan * 4, (void**)&sq));
+  const int tblocks = emit ? topk_blocks(cap, true) : topk_blocks(no);
   GDB_TRY(arena(ctx, A_AUX2, sizeof(TopEntry) * (size_t)tblocks * k, (void**)&win));
   const size_t out_bytes = 16 + (size_t)k * (sizeof(TopEntry) + 8) + 64;
   GDB_TRY(arena(ctx, A_OUT, out_bytes, (void**)&dout));
   GDB_TRY(pinned(ctx, out_bytes, (void**)&host));
   TopEntry* fin = (TopEntry*)(dout + 2);
   int32_t* extra = (int32_t*)(fin + k);
+  uint32_t* ncand = (uint32_t*)(extra + 2 * k);
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_HIP(hipMemsetAsync(sq, 0, ospan * 4, s));
-  GDB_HIP(hipMemsetAsync(dout, 0, out_bytes, s));
+  if (!emit) GDB_TRY(fill(sq, 0, ospan * 4, s));
+  GDB_TRY(fill(dout, 0, out_bytes, s));
   RunStats rs;
-  const bool clustered = lo->sorted && !db->lineitem->host;  // an order's rows are contiguous
   GDB_TRY(scan_table(ctx, db->lineitem, {lo, lq}, [&](const void* const* d, uint32_t rows) {
-    if (clustered)  // one plain store per order
-      q18_owner<<<blocks_for(ctx, q18_owner, quads(rows)), BLOCK, 0, s>>>(
-          (const int32_t*)d[0], (const int32_t*)d[1], rows, sq);
+    if (emit)
+      q18_owner<true><<<blocks_for(ctx, q18_owner<true>, quads(rows)), BLOCK, 0, s>>>(
+          (const int32_t*)d[0], (const int32_t*)d[1], rows, nullptr, p->quantity, cand, ncand);
+    else if (clustered)  // one plain store per order
+      q18_owner<false><<<blocks_for(ctx, q18_owner<false>, quads(rows)), BLOCK, 0, s>>>(
+          (const int32_t*)d[0], (const int32_t*)d[1], rows, sq, 0, nullptr, nullptr);
     else
       q18_sums<<<blocks_for(ctx, q18_sums, quads(rows)), BLOCK, 0, s>>>(
           (const int32_t*)d[0], (const int32_t*)d[1], rows, sq);
     GDB_HIP(hipGetLastError());
     return GDB_OK;
   }, &rs));
-  GDB_TRY(topk_run(ctx,
-                   Q18Src{(const int32_t*)oo->dev, (const int32_t*)od->dev,
-                          (const int64_t*)ot->dev, sq, p->quantity},
-                   no, k, win, tblocks, fin, (unsigned long long*)dout,
-                   (unsigned long long*)(dout + 1)));
+  if (emit)
+    GDB_TRY(topk_run(ctx,
+                     Q18CandSrc{cand, (const int32_t*)oo->dev, (const int32_t*)od->dev,
+                                (const int64_t*)ot->dev, (uint32_t)no},
+                     cap, k, win, fin, (unsigned long long*)dout, ncand));
+  else
+    GDB_TRY(topk_run(ctx,
+                     Q18Src{(const int32_t*)oo->dev, (const int32_t*)od->dev,
+                            (const int64_t*)ot->dev, sq, p->quantity},
+                     no, k, win, fin, (unsigned long long*)dout));
   q18_gather<<<1, 128, 0, s>>>(fin, k, (const int32_t*)oc->dev, sq, extra);
   GDB_HIP(hipGetLastError());
   GDB_HIP(hipMemcpyAsync(host, dout, out_bytes, hipMemcpyDeviceToHost, s));
@@ -1127,7 +1250,7 @@ int gdb_q21(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q21_params* p, gdb_q2
   GDB_TRY(key_span(lk, nl, &a));
   sspan = std::max(sspan, a);
   const int k = std::max(p->top_k, 1);
-  const int tblocks = 256;
+  const int tblocks = topk_blocks(sspan);
   const bool clustered = lo->sorted && !db->lineitem->host;  // an order's rows are contiguous
   unsigned long long* state = nullptr;
   uint32_t* cnt;
@@ -1146,11 +1269,11 @@ int gdb_q21(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q21_params* p, gdb_q2
   TopEntry* fin = (TopEntry*)(dout + 4);
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  if (!clustered) GDB_HIP(hipMemsetAsync(state, 0, ospan * 8, s));
-  GDB_HIP(hipMemsetAsync(fst, 0, ospan, s));
-  GDB_HIP(hipMemsetAsync(cnt, 0, sspan * 4, s));
-  GDB_HIP(hipMemsetAsync(smap, NONE, sspan, s));
-  GDB_HIP(hipMemsetAsync(dout, 0, out_bytes, s));
+  if (!clustered) GDB_TRY(fill(state, 0, ospan * 8, s));
+  GDB_TRY(fill(fst, 0, ospan, s));
+  GDB_TRY(fill(cnt, 0, sspan * 4, s));
+  GDB_TRY(fill(smap, NONE, sspan, s));
+  GDB_TRY(fill(dout, 0, out_bytes, s));
   if (no)
     order_status_bytes<<<blocks_for(ctx, order_status_bytes, no), BLOCK, 0, s>>>(
         (const int32_t*)oo->dev, (const uint8_t*)os->dev, (uint32_t)no,
@@ -1195,8 +1318,7 @@ int gdb_q21(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q21_params* p, gdb_q2
       rs.chunks += rs2.chunks;
     }
   }
-  GDB_TRY(topk_run(ctx, CountSrc{cnt}, sspan, k, win, tblocks, fin, (unsigned long long*)dout,
-                   (unsigned long long*)(dout + 1)));
+  GDB_TRY(topk_run(ctx, CountSrc{cnt}, sspan, k, win, fin, (unsigned long long*)dout));
   GDB_HIP(hipMemcpyAsync(host, dout, out_bytes, hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
   // SQL columns: lineitem orderkey, suppkey, commitdate, receiptdate (16 B); orders

@@ -111,12 +111,11 @@ struct Q2Src {  // (s_acctbal desc, nation name rank asc, suppkey asc, partkey a
   const uint8_t* rank;  // nation key -> position of its name in name order
   __device__ bool operator()(uint64_t i, TopEntry& e) const {
     const int32_t p = pk[i], s = sk[i];
-    if (!psel[p]) return false;
-    const uint8_t nat = snat[s];
-    if (nat == NONE || ord64(cost[i]) != mn[p]) return false;
+    const uint16_t sel = psel[p];
+    const uint8_t nat = snat[s];  // rank has 256 entries, NONE included
     e = TopEntry{sbal[s], (int32_t)(((uint32_t)rank[nat] << 26) | (uint32_t)s), p,
-                 (int32_t)nat | ((int32_t)(psel[p] & 0xFF) << 8), 0};
-    return true;
+                 (int32_t)nat | ((int32_t)(sel & 0xFF) << 8), 0};
+    return sel && nat != NONE && ord64(cost[i]) == mn[p];
   }
 };
 
@@ -459,7 +458,7 @@ int gdb_q2(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q2_params* p, gdb_q2_r
   sspan = std::max(sspan, a);
   if (sspan > (1ull << 26)) return fail(GDB_E_UNSUPPORTED, "Q2 supports supplier keys below 2^26");
   const int k = std::max(p->top_k, 1);
-  const int tblocks = 256;
+  const int tblocks = topk_blocks(nps);
   uint16_t* psel;
   unsigned long long* mn;
   uint8_t *snat, *nreg, *rank;
@@ -482,11 +481,11 @@ int gdb_q2(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q2_params* p, gdb_q2_r
   nation_rank(hrank);
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_HIP(hipMemsetAsync(psel, 0, pspan * 2, s));
-  GDB_HIP(hipMemsetAsync(mn, 0xFF, pspan * 8, s));
-  GDB_HIP(hipMemsetAsync(snat, NONE, sspan, s));
-  GDB_HIP(hipMemsetAsync(sbal, 0, sspan * 8, s));
-  GDB_HIP(hipMemsetAsync(dout, 0, out_bytes, s));
+  GDB_TRY(fill(psel, 0, pspan * 2, s));
+  GDB_TRY(fill(mn, 0xFF, pspan * 8, s));
+  GDB_TRY(fill(snat, NONE, sspan, s));
+  GDB_TRY(fill(sbal, 0, sspan * 8, s));
+  GDB_TRY(fill(dout, 0, out_bytes, s));
   std::memcpy(host, hrank, 256);
   GDB_HIP(hipMemcpyAsync(rank, host, 256, hipMemcpyHostToDevice, s));
   GDB_TRY(build_nreg_ps(ctx, db, nreg));
@@ -509,8 +508,7 @@ int gdb_q2(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q2_params* p, gdb_q2_r
   GDB_TRY(topk_run(ctx,
                    Q2Src{(const int32_t*)qp->dev, (const int32_t*)qs->dev,
                          (const int64_t*)qc->dev, psel, snat, sbal, mn, rank},
-                   nps, k, win, tblocks, fin, (unsigned long long*)dout,
-                   (unsigned long long*)(dout + 1)));
+                   nps, k, win, fin, (unsigned long long*)dout));
   GDB_HIP(hipMemcpyAsync(host, dout, out_bytes, hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
   // SQL columns: part key, size (8 B), type, mfgr (2 B); supplier key, nation (8 B),
@@ -566,10 +564,10 @@ int gdb_q11(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q11_params* p, gdb_q1
   GDB_TRY(pinned(ctx, 64, (void**)&host));
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_HIP(hipMemsetAsync(val, 0, pspan * 8, s));
-  GDB_HIP(hipMemsetAsync(seen, 0, pspan, s));
-  GDB_HIP(hipMemsetAsync(sbits, 0, words * 4, s));
-  GDB_HIP(hipMemsetAsync(dout, 0, 16, s));
+  GDB_TRY(fill(val, 0, pspan * 8, s));
+  GDB_TRY(fill(seen, 0, pspan, s));
+  GDB_TRY(fill(sbits, 0, words * 4, s));
+  GDB_TRY(fill(dout, 0, 16, s));
   if (ns)
     supp_nation_bits<<<blocks_for(ctx, supp_nation_bits, ns), BLOCK, 0, s>>>(
         (const int32_t*)sk->dev, (const int32_t*)sn->dev, (uint32_t)ns, p->nation, sbits);
@@ -652,11 +650,11 @@ int gdb_q16(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q16_params* p, gdb_q1
   qpart.types = mask4(p->type_mask);
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_HIP(hipMemsetAsync(gid, 0xFF, pspan * 4, s));
-  GDB_HIP(hipMemsetAsync(bad, 0, sspan, s));  // NOT IN keeps suppliers without a row
-  GDB_HIP(hipMemsetAsync(keys, 0xFF, slots * 8, s));
-  GDB_HIP(hipMemsetAsync(cnt, 0, ngroups * 4, s));
-  GDB_HIP(hipMemsetAsync(dout, 0, 16, s));
+  GDB_TRY(fill(gid, 0xFF, pspan * 4, s));
+  GDB_TRY(fill(bad, 0, sspan, s));  // NOT IN keeps suppliers without a row
+  GDB_TRY(fill(keys, 0xFF, slots * 8, s));
+  GDB_TRY(fill(cnt, 0, ngroups * 4, s));
+  GDB_TRY(fill(dout, 0, 16, s));
   if (ns)
     q16_supp<<<blocks_for(ctx, q16_supp, ns), BLOCK, 0, s>>>(
         (const int32_t*)sk->dev, (const uint8_t*)sc->dev, sc->width, (uint32_t)ns, w1, w2, bad);
@@ -773,9 +771,9 @@ int gdb_q20(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q20_params* p, int32_
   GDB_TRY(pinned(ctx, 64, (void**)&host));
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_HIP(hipMemsetAsync(bits, 0, words * 4, s));
-  GDB_HIP(hipMemsetAsync(good, 0, sspan, s));
-  GDB_HIP(hipMemsetAsync(dout, 0, 32, s));
+  GDB_TRY(fill(bits, 0, words * 4, s));
+  GDB_TRY(fill(good, 0, sspan, s));
+  GDB_TRY(fill(dout, 0, 32, s));
   if (np)
     q20_part<<<blocks_for(ctx, q20_part, np), BLOCK, 0, s>>>(
         (const int32_t*)pp->dev, (const uint8_t*)pn->dev, pn->width, (uint32_t)np, pre, bits);
@@ -795,9 +793,9 @@ int gdb_q20(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q20_params* p, int32_
   GDB_TRY(arena(ctx, A_AUX, slots * 8, (void**)&row));
   lines = row + slots;
   GDB_TRY(arena(ctx, A_MAP, std::max<uint64_t>(ns, 1) * 4, (void**)&out_keys));
-  GDB_HIP(hipMemsetAsync(keys, 0xFF, slots * 8, s));
-  GDB_HIP(hipMemsetAsync(sum, 0, slots * 8, s));
-  GDB_HIP(hipMemsetAsync(lines, 0, slots * 4, s));
+  GDB_TRY(fill(keys, 0xFF, slots * 8, s));
+  GDB_TRY(fill(sum, 0, slots * 8, s));
+  GDB_TRY(fill(lines, 0, slots * 4, s));
   if (nps)
     q20_build<<<blocks_for(ctx, q20_build, nps), BLOCK, 0, s>>>(
         (const int32_t*)qp->dev, (const int32_t*)qs->dev, (uint32_t)nps, bits, keys, slots - 1,

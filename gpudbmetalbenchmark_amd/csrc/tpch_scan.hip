@@ -436,8 +436,8 @@ int gdb_q14(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q14_params* p, gdb_q1
   GDB_TRY(pinned(ctx, 64, (void**)&host));
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_HIP(hipMemsetAsync(bits, 0, words * 4, s));
-  GDB_HIP(hipMemsetAsync(dout, 0, 16, s));
+  GDB_TRY(fill(bits, 0, words * 4, s));
+  GDB_TRY(fill(dout, 0, 16, s));
   if (np)
     code_bits<<<blocks_for(ctx, code_bits, np), BLOCK, 0, s>>>(
         (const int32_t*)pp->dev, (const uint8_t*)pt->dev, (uint32_t)np, mask4(p->type_mask), bits);
@@ -500,8 +500,8 @@ int gdb_q19(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q19_params* p, gdb_q1
   }
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_HIP(hipMemsetAsync(map, 0, span, s));
-  GDB_HIP(hipMemsetAsync(dout, 0, 8, s));
+  GDB_TRY(fill(map, 0, span, s));
+  GDB_TRY(fill(dout, 0, 8, s));
   if (np)
     q19_part<<<blocks_for(ctx, q19_part, np), BLOCK, 0, s>>>(
         (const int32_t*)pp->dev, (const uint8_t*)pb->dev, (const uint8_t*)pc->dev,
@@ -548,8 +548,8 @@ int gdb_q12(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q12_params* p, gdb_q1
   GDB_TRY(pinned(ctx, 64, (void**)&host));
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_HIP(hipMemsetAsync(high, 0, span, s));
-  GDB_HIP(hipMemsetAsync(dout, 0, 32, s));
+  GDB_TRY(fill(high, 0, span, s));
+  GDB_TRY(fill(dout, 0, 32, s));
   if (no)
     order_prio_bytes<<<blocks_for(ctx, order_prio_bytes, no), BLOCK, 0, s>>>(
         (const int32_t*)oo->dev, (const uint8_t*)op->dev, (uint32_t)no, p->high_mask, high);
@@ -596,8 +596,8 @@ int gdb_q4(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q4_params* p, gdb_q4_r
   GDB_TRY(pinned(ctx, 64, (void**)&host));
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_HIP(hipMemsetAsync(late, 0, span, s));
-  GDB_HIP(hipMemsetAsync(dout, 0, 40, s));
+  GDB_TRY(fill(late, 0, span, s));
+  GDB_TRY(fill(dout, 0, 40, s));
   RunStats rs;
   GDB_TRY(scan_table(ctx, db->lineitem, {lo, lc, lr}, [&](const void* const* d, uint32_t rows) {
     q4_late<<<blocks_for(ctx, q4_late, quads(rows)), BLOCK, 0, s>>>(
@@ -643,9 +643,9 @@ int gdb_q15(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q15_params* p, gdb_q1
   int32_t* keys = (int32_t*)(dout + 2);
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_HIP(hipMemsetAsync(rev, 0, span * 8, s));
-  GDB_HIP(hipMemsetAsync(seen, 0, span, s));
-  GDB_HIP(hipMemsetAsync(dout, 0, 16, s));
+  GDB_TRY(fill(rev, 0, span * 8, s));
+  GDB_TRY(fill(seen, 0, span, s));
+  GDB_TRY(fill(dout, 0, 16, s));
   RunStats rs;
   GDB_TRY(scan_table(ctx, db->lineitem, {ls, lk, ld, le}, [&](const void* const* d, uint32_t rows) {
     q15_scan<<<blocks_for(ctx, q15_scan, quads(rows)), BLOCK, 0, s>>>(
@@ -699,10 +699,10 @@ int gdb_q17(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q17_params* p, gdb_q1
   GDB_TRY(pinned(ctx, 64, (void**)&host));
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_HIP(hipMemsetAsync(bits, 0, (words4 + FOLD_WORDS) * 4, s));
-  GDB_HIP(hipMemsetAsync(sumq, 0, span * 8, s));
-  GDB_HIP(hipMemsetAsync(cnt, 0, span * 4, s));
-  GDB_HIP(hipMemsetAsync(dout, 0, 16, s));
+  GDB_TRY(fill(bits, 0, (words4 + FOLD_WORDS) * 4, s));
+  GDB_TRY(fill(sumq, 0, span * 8, s));
+  GDB_TRY(fill(cnt, 0, span * 4, s));
+  GDB_TRY(fill(dout, 0, 16, s));
   if (np)
     q17_part<<<blocks_for(ctx, q17_part, np), BLOCK, 0, s>>>(
         (const int32_t*)pp->dev, (const uint8_t*)pb->dev, (const uint8_t*)pc->dev, (uint32_t)np,
@@ -770,8 +770,8 @@ int gdb_q22(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q22_params* p, gdb_q2
   GDB_TRY(pinned(ctx, 256, (void**)&host));
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_HIP(hipMemsetAsync(has, 0, span, s));
-  GDB_HIP(hipMemsetAsync(dout, 0, 16 * 8, s));
+  GDB_TRY(fill(has, 0, span, s));
+  GDB_TRY(fill(dout, 0, 16 * 8, s));
   GDB_HIP(hipEventRecord(ctx->ev[1], s));
   if (no)
     mark_bytes<<<blocks_for(ctx, mark_bytes, quads(no)), BLOCK, 0, s>>>((const int32_t*)oc->dev,

@@ -191,6 +191,53 @@ def test_unclustered_lineitem_takes_the_general_paths(engine):
         d.free()
 
 
+def test_unsorted_orders_table(engine):
+    """Orders rows shuffled (o_orderkey no longer ascending, so Q18 cannot look
+    its candidates up by binary search and sums per orderkey instead), lineitem
+    still clustered: same answers for every query that reads orders."""
+    import numpy as np
+    tabs = tpch.generate(0.02, 8)
+    perm = np.random.default_rng(3).permutation(len(tabs["orders"]["o_orderkey"]))
+    shuffled = dict(tabs, orders={k: v[perm] for k, v in tabs["orders"].items()})
+    d = TpchDb(engine, shuffled)
+    try:
+        assert L(d.q18()) == L(T.q18(tabs))
+        assert L(d.q18(250)) == L(T.q18(tabs, 250))
+        assert L(d.q18(0, 37)) == L(T.q18(tabs, 0, 37))
+        assert L(d.q21()) == L(T.q21(tabs))
+        assert d.q4() == T.q4(tabs)
+        assert d.q5() == T.q5(tabs)
+        rows, groups = d.q10()
+        assert (L(rows), groups) == (L(T.q10(tabs)[0]), T.q10(tabs)[1])
+    finally:
+        d.free()
+
+
+def test_long_orders_run_past_the_owner_window(engine):
+    """Q18 and Q21 over a lineitem whose l_orderkey stays non-decreasing but
+    has runs of 13 to 60 rows (longer than the 12-row window an owner lane holds
+    in registers), placed at every offset within a wave's 256 rows."""
+    import numpy as np
+    tabs = tpch.generate(0.02, 4)
+    ok = tabs["lineitem"]["l_orderkey"].copy()
+    rng = np.random.default_rng(2)
+    b = 0
+    while b < len(ok) - 64:
+        run = int(rng.integers(13, 61))
+        ok[b:b + run] = ok[b]
+        b += run + int(rng.integers(1, 300))
+    assert np.all(np.diff(ok) >= 0)
+    tabs = dict(tabs, lineitem=dict(tabs["lineitem"], l_orderkey=ok))
+    d = TpchDb(engine, tabs)
+    try:
+        assert L(d.q18()) == L(T.q18(tabs))
+        assert L(d.q18(250)) == L(T.q18(tabs, 250))
+        assert L(d.q21()) == L(T.q21(tabs))
+        assert L(d.q21("KENYA", "O", 7)) == L(T.q21(tabs, 14, ord("O"), 7))
+    finally:
+        d.free()
+
+
 def test_streamed_lineitem_gives_the_same_results(engine):
     """Every pipeline with lineitem in host memory, streamed through the GPU in
     64K-row chunks (two passes for Q17 and Q21, the general paths for Q18 and
