@@ -192,13 +192,16 @@ __global__ __launch_bounds__(256) void topk_stage1(Src src, uint64_t n,
   if (threadIdx.x == 0 && c) atomicAdd(count, (unsigned long long)c);
 }
 
-// Stage 2, one workgroup over `lists` best-first lists of k entries (stage 1's
-// winners, worst-padded): rank 0 of every list, then rank 1, and so on. When no
-// entry of a rank is valid and ranks before the running k-th best, no later rank
-// can (each list only gets worse), so the merge stops there.
+// Stage 2, one workgroup over `lists` (at most TOPK_MAX_BLOCKS) best-first lists
+// of k entries (stage 1's winners, worst-padded): rank 0 of every list, then
+// rank 1, and so on; a thread loads its lists' entries of a rank together before
+// pushing them. When no entry of a rank is valid and ranks before the running
+// k-th best, no later rank can (each list only gets worse), so the merge stops.
+constexpr int TOPK_MAX_BLOCKS = 2048;
 namespace {
 __global__ __launch_bounds__(256) void topk_merge(const gdbk::TopEntry* __restrict__ w, int lists,
                                                   int k, gdbk::TopEntry* __restrict__ out) {
+  constexpr int PER = TOPK_MAX_BLOCKS / 256;
   __shared__ gdbk::TopEntry buf[1024];
   __shared__ gdbk::TopEntry thr;
   __shared__ int nbuf, have, any;
@@ -206,14 +209,19 @@ __global__ __launch_bounds__(256) void topk_merge(const gdbk::TopEntry* __restri
   tk.init();
   for (int r = 0; r < k; ++r) {
     if (threadIdx.x == 0) any = 0;
+    gdbk::TopEntry e[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int b = u * (int)blockDim.x + (int)threadIdx.x;
+      e[u] = b < lists ? w[(size_t)b * k + r] : gdbk::worst_entry();
+    }
     __syncthreads();
-    for (int b0 = 0; b0 < lists; b0 += blockDim.x) {
-      const int b = b0 + threadIdx.x;
-      gdbk::TopEntry e = gdbk::worst_entry();
-      if (b < lists) e = w[(size_t)b * k + r];
-      const bool v = !(e.rev == INT64_MIN && e.key == INT32_MAX);
-      if (v && !(have && !gdbk::better(e, thr))) any = 1;
-      tk.push(v, e);  // ends with a barrier: `any` and thr are settled after it
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      if (u * (int)blockDim.x >= lists) break;  // uniform
+      const bool v = !(e[u].rev == INT64_MIN && e[u].key == INT32_MAX);
+      if (v && !(have && !gdbk::better(e[u], thr))) any = 1;
+      tk.push(v, e[u]);  // ends with a barrier: `any` and thr are settled after it
     }
     const int go = any;
     __syncthreads();  // every thread has read `any` before the next reset
@@ -226,7 +234,6 @@ __global__ __launch_bounds__(256) void topk_merge(const gdbk::TopEntry* __restri
 // Stage-1 workgroups for n candidates: about 2048 each, at most TOPK_MAX_BLOCKS
 // (the winners buffer holds that many lists of k). A count known only on the
 // device (`counted`: n is then a bound) gets 256 workgroups.
-constexpr int TOPK_MAX_BLOCKS = 2048;
 inline int topk_blocks(uint64_t n, bool counted = false) {
   if (counted) return 256;
   const uint64_t b = (n + 2047) / 2048;

@@ -261,19 +261,29 @@ __global__ __launch_bounds__(FOLD_BLOCK) void q8_scan(const int32_t* __restrict_
 // ---- Q10 (reference kernels/q10.metal:9-54, src/q10.cpp) -----------------------
 
 // ocust[orderkey] = custkey for orders dated in [lo, hi) (-1 otherwise).
+// Orders in the date window: bit orderkey of `win` set and ocust[orderkey] =
+// custkey (ocust is read only where the bit is set, so it needs no clearing).
 __global__ __launch_bounds__(256) void q10_orders(const int32_t* __restrict__ ok,
                                                   const int32_t* __restrict__ oc,
                                                   const int32_t* __restrict__ od, uint32_t n,
                                                   int32_t lo, int32_t hi,
+                                                  uint32_t* __restrict__ win,
                                                   int32_t* __restrict__ ocust) {
-  walk2(quads(n), [&](uint32_t g) { return Ld3{ld4nt(ok, g), ld4nt(oc, g), ld4nt(od, g)}; },
-        [&](uint32_t g, const Ld3& l) {
+  walk2(quads(n), [&](uint32_t g) { return ld4nt(od, g); }, [&](uint32_t g, const int4& d) {
+    uint32_t m = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      if (4 * g + j >= n) continue;
-      const int dv = lane_of(l.c, j);
-      ocust[lane_of(l.a, j)] = (dv >= lo && dv < hi) ? lane_of(l.b, j) : -1;
+      const int dv = lane_of(d, j);
+      m |= (uint32_t)(4 * g + j < n && dv >= lo && dv < hi) << j;
     }
+    if (!m) return;
+    const int4 k = ld4nt(ok, g), c = ld4nt(oc, g);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if ((m >> j) & 1) {
+        bit_set(win, (uint32_t)lane_of(k, j));
+        ocust[lane_of(k, j)] = lane_of(c, j);
+      }
   });
 }
 
@@ -281,7 +291,8 @@ __global__ __launch_bounds__(256) void q10_scan(const uint8_t* __restrict__ rfla
                                                 const int32_t* __restrict__ ok,
                                                 const int32_t* __restrict__ disc,
                                                 const int64_t* __restrict__ price, uint32_t n,
-                                                uint32_t flag, const int32_t* __restrict__ ocust,
+                                                uint32_t flag, const uint32_t* __restrict__ win,
+                                                const int32_t* __restrict__ ocust,
                                                 uint32_t cspan, int64_t* __restrict__ crev,
                                                 uint8_t* __restrict__ seen) {
   walk2(quads(n), [&](uint32_t g) { return ld_u8x4nt(rflag, g); }, [&](uint32_t g, uint32_t fw) {
@@ -290,11 +301,16 @@ __global__ __launch_bounds__(256) void q10_scan(const uint8_t* __restrict__ rfla
     for (int j = 0; j < 4; ++j) m |= (uint32_t)((4 * g + j < n) & (byte_of(fw, j) == flag)) << j;
     if (!m) return;
     const int4 k = ld4nt(ok, g);
+    uint32_t m1 = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      m1 |= (uint32_t)(((m >> j) & 1) && bit_test(win, (uint32_t)lane_of(k, j))) << j;
+    if (!m1) return;
     int32_t cu[4];
     uint32_t m2 = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      cu[j] = (m >> j) & 1 ? ocust[lane_of(k, j)] : -1;
+      cu[j] = (m1 >> j) & 1 ? ocust[lane_of(k, j)] : -1;
       m2 |= (uint32_t)(cu[j] >= 0 && (uint32_t)cu[j] < cspan) << j;
     }
     if (!m2) return;
@@ -1063,12 +1079,15 @@ int gdb_q10(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q10_params* p, gdb_q1
   const int k = p->top_k;
   const int tblocks = topk_blocks(cspan);
   int32_t* ocust;
+  uint32_t* owin;
+  const uint64_t wwords = ospan / 32 + 1;
   int64_t* crev;
   uint8_t *seen, *rank;
   TopEntry* win;
   uint64_t* dout;
   char* host;
   GDB_TRY(arena(ctx, A_MAP, ospan * 4, (void**)&ocust));
+  GDB_TRY(arena(ctx, A_BITMAP, wwords * 4, (void**)&owin));
   GDB_TRY(arena(ctx, A_AGG, cspan * 8, (void**)&crev));
   GDB_TRY(arena(ctx, A_AUX, cspan * 2, (void**)&seen));
   rank = seen + cspan;
@@ -1083,7 +1102,7 @@ int gdb_q10(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q10_params* p, gdb_q1
   int32_t* nat = (int32_t*)(bal + k);
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_TRY(fill(ocust, 0xFF, ospan * 4, s));
+  GDB_TRY(fill(owin, 0, wwords * 4, s));
   GDB_TRY(fill(crev, 0, cspan * 8, s));
   GDB_TRY(fill(seen, 0, cspan, s));
   GDB_TRY(fill(rank, NONE, cspan, s));
@@ -1091,13 +1110,13 @@ int gdb_q10(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q10_params* p, gdb_q1
   if (no)
     q10_orders<<<blocks_for(ctx, q10_orders, quads(no)), BLOCK, 0, s>>>(
         (const int32_t*)oo->dev, (const int32_t*)oc->dev, (const int32_t*)od->dev, (uint32_t)no,
-        p->date_lo, p->date_hi, ocust);
+        p->date_lo, p->date_hi, owin, ocust);
   GDB_HIP(hipGetLastError());
   RunStats rs;
   GDB_TRY(scan_table(ctx, db->lineitem, {lf, lo, ld, le}, [&](const void* const* d, uint32_t rows) {
     q10_scan<<<blocks_for(ctx, q10_scan, quads(rows)), BLOCK, 0, s>>>(
         (const uint8_t*)d[0], (const int32_t*)d[1], (const int32_t*)d[2], (const int64_t*)d[3],
-        rows, (uint32_t)(uint8_t)p->returnflag, ocust, (uint32_t)cspan, crev, seen);
+        rows, (uint32_t)(uint8_t)p->returnflag, owin, ocust, (uint32_t)cspan, crev, seen);
     GDB_HIP(hipGetLastError());
     return GDB_OK;
   }, &rs));

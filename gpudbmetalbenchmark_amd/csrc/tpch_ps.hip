@@ -269,14 +269,73 @@ __global__ __launch_bounds__(256) void q16_scan(const int32_t* __restrict__ pk,
   });
 }
 
-__global__ __launch_bounds__(256) void q16_collect(const uint32_t* __restrict__ cnt, uint32_t n,
-                                                   uint2* __restrict__ out,
-                                                   unsigned long long* __restrict__ nout) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    if (cnt[i]) {
-      const unsigned long long pos = atomicAdd(nout, 1ull);
-      out[pos] = make_uint2(i, cnt[i]);
-    }
+// The result's group order (brand, type name, size) as an ordinal: brand << 11 |
+// type rank << 3 | size rank. `ord` holds, in device memory, the type code of
+// each type rank, the size position of each size rank and the sizes.
+struct Q16Ord {
+  uint8_t tcode[256];
+  uint8_t spos[8];
+  int32_t sizes[8];
+  int32_t nsizes;
+};
+constexpr uint32_t Q16_ORDINALS = 1u << 19;
+
+__device__ __forceinline__ uint32_t q16_count_at(const uint32_t* __restrict__ cnt, uint32_t i,
+                                                 const Q16Ord* __restrict__ o, int4* row) {
+  const uint32_t sr = i & 7, brand = i >> 11;
+  if ((int)sr >= o->nsizes) return 0;
+  const uint32_t type = o->tcode[(i >> 3) & 255], pos = o->spos[sr];
+  const uint32_t c = cnt[(brand * 256 + type) * 8 + pos];
+  *row = make_int4((int)brand, (int)type, o->sizes[pos], (int)c);
+  return c;
+}
+
+// Groups with suppliers per block of 256 ordinals.
+__global__ __launch_bounds__(256) void q16_block_counts(const uint32_t* __restrict__ cnt,
+                                                        const Q16Ord* __restrict__ o,
+                                                        uint32_t* __restrict__ bcount) {
+  int4 row;
+  const int v = __syncthreads_count(q16_count_at(cnt, blockIdx.x * 256 + threadIdx.x, o, &row) != 0);
+  if (threadIdx.x == 0) bcount[blockIdx.x] = (uint32_t)v;
+}
+
+// Exclusive scan of the 2048 block counts in place (one workgroup of 1024); the
+// total goes to *total.
+__global__ __launch_bounds__(1024) void q16_block_offsets(uint32_t* __restrict__ b,
+                                                          unsigned long long* __restrict__ total) {
+  __shared__ uint32_t sh[1024];
+  const int t = threadIdx.x;
+  const uint32_t a0 = b[2 * t], a1 = b[2 * t + 1];
+  sh[t] = a0 + a1;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    const uint32_t v = t >= d ? sh[t - d] : 0u;
+    __syncthreads();
+    sh[t] += v;
+    __syncthreads();
+  }
+  const uint32_t before = sh[t] - (a0 + a1);
+  b[2 * t] = before;
+  b[2 * t + 1] = before + a0;
+  if (t == 1023) *total = sh[t];
+}
+
+// Rows of the groups with suppliers, in ordinal order (an order-preserving
+// compaction): (brand, type, size, supplier count).
+__global__ __launch_bounds__(256) void q16_emit(const uint32_t* __restrict__ cnt,
+                                                const Q16Ord* __restrict__ o,
+                                                const uint32_t* __restrict__ boff,
+                                                int4* __restrict__ rows) {
+  __shared__ uint32_t wsum[4];
+  int4 row;
+  const bool v = q16_count_at(cnt, blockIdx.x * 256 + threadIdx.x, o, &row) != 0;
+  const unsigned long long m = __ballot(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) wsum[w] = (uint32_t)__popcll(m);
+  __syncthreads();
+  uint32_t base = boff[blockIdx.x];
+  for (int i = 0; i < w; ++i) base += wsum[i];
+  if (v) rows[base + __popcll(m & ((1ull << lane) - 1))] = row;
 }
 
 // ---- Q20 (reference kernels/q20.metal:15-104, src/q20.cpp) ---------------------
@@ -629,20 +688,43 @@ int gdb_q16(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q16_params* p, gdb_q1
   sspan = std::max(sspan, a);
   const uint64_t ngroups = 256ull * 256 * 8;
   const uint64_t slots = pow2_at_least(2 * std::max<uint64_t>(nps, 1));
+  const uint32_t nblk = Q16_ORDINALS / 256;
   int32_t* gid;
   uint8_t* bad;
   unsigned long long* keys;
-  uint32_t* cnt;
-  uint2* grp;
+  uint32_t *cnt, *boff;
+  int4* grp;
   uint64_t* dout;
-  uint64_t* host;
+  Q16Ord* dord;
+  char* host;
   GDB_TRY(arena(ctx, A_MAP, pspan * 4, (void**)&gid));
   GDB_TRY(arena(ctx, A_AUX2, sspan, (void**)&bad));
   GDB_TRY(arena(ctx, A_HT, slots * 8, (void**)&keys));
   GDB_TRY(arena(ctx, A_AGG, ngroups * 4, (void**)&cnt));
-  GDB_TRY(arena(ctx, A_AUX, ngroups * sizeof(uint2), (void**)&grp));
-  GDB_TRY(arena(ctx, A_OUT, 64, (void**)&dout));
-  GDB_TRY(pinned(ctx, 64, (void**)&host));
+  GDB_TRY(arena(ctx, A_AUX, Q16_ORDINALS * sizeof(int4), (void**)&grp));
+  GDB_TRY(arena(ctx, A_OUT, 64 + sizeof(Q16Ord) + nblk * 4, (void**)&dout));
+  dord = (Q16Ord*)(dout + 8);
+  boff = (uint32_t*)(dord + 1);
+  const size_t host_bytes = 64 + sizeof(Q16Ord) + Q16_ORDINALS * sizeof(int4);
+  GDB_TRY(pinned(ctx, host_bytes, (void**)&host));
+  // Group order: type codes by name, size positions by value (ORDER BY p_size).
+  static const std::vector<int> trank = [] {  // type code -> position in name order
+    std::vector<int> idx(256), rank(256);
+    for (int c = 0; c < 256; ++c) idx[c] = c;
+    std::sort(idx.begin(), idx.end(), [](int a, int b) { return type_name(a) < type_name(b); });
+    for (int r = 0; r < 256; ++r) rank[idx[r]] = r;
+    return rank;
+  }();
+  Q16Ord* hord = (Q16Ord*)(host + 64);
+  for (int c = 0; c < 256; ++c) hord->tcode[trank[c]] = (uint8_t)c;
+  for (int i = 0; i < p->nsizes; ++i) {
+    int r = 0;
+    for (int j = 0; j < p->nsizes; ++j)
+      r += p->sizes[j] < p->sizes[i] || (p->sizes[j] == p->sizes[i] && j < i);
+    hord->spos[r] = (uint8_t)i;
+  }
+  for (int i = 0; i < 8; ++i) hord->sizes[i] = i < p->nsizes ? p->sizes[i] : 0;
+  hord->nsizes = p->nsizes;
   Q16Part qpart{};
   qpart.brand = p->brand;
   qpart.nsizes = p->nsizes;
@@ -650,6 +732,7 @@ int gdb_q16(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q16_params* p, gdb_q1
   qpart.types = mask4(p->type_mask);
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
+  GDB_HIP(hipMemcpyAsync(dord, hord, sizeof(Q16Ord), hipMemcpyHostToDevice, s));
   GDB_TRY(fill(gid, 0xFF, pspan * 4, s));
   GDB_TRY(fill(bad, 0, sspan, s));  // NOT IN keeps suppliers without a row
   GDB_TRY(fill(keys, 0xFF, slots * 8, s));
@@ -670,60 +753,39 @@ int gdb_q16(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q16_params* p, gdb_q1
         slots - 1, cnt);
   GDB_HIP(hipGetLastError());
   GDB_HIP(hipEventRecord(ctx->ev[2], s));
-  q16_collect<<<blocks_for(ctx, q16_collect, ngroups), BLOCK, 0, s>>>(
-      cnt, (uint32_t)ngroups, grp, (unsigned long long*)dout);
+  q16_block_counts<<<nblk, 256, 0, s>>>(cnt, dord, boff);
+  q16_block_offsets<<<1, 1024, 0, s>>>(boff, (unsigned long long*)dout);
+  q16_emit<<<nblk, 256, 0, s>>>(cnt, dord, boff, grp);
   GDB_HIP(hipGetLastError());
   GDB_HIP(hipMemcpyAsync(host, dout, 8, hipMemcpyDeviceToHost, s));
   GDB_HIP(hipStreamSynchronize(s));
-  const uint64_t n = host[0];
-  std::vector<uint2> got(n);
-  if (n) GDB_HIP(hipMemcpyAsync(got.data(), grp, n * sizeof(uint2), hipMemcpyDeviceToHost, s));
+  const uint64_t n = *(const uint64_t*)host;
+  int4* got = (int4*)(host + 64 + sizeof(Q16Ord));
+  if (n) GDB_HIP(hipMemcpyAsync(got, grp, n * sizeof(int4), hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
   // SQL columns: partsupp key pair (8 B); part key, size (8 B), brand, type (2 B);
   // supplier key + comment. Mandatory in q16_scan: the partsupp key pair.
   GDB_TRY(timing_finish(ctx, timing, 8 * nps + 10 * np + (4 + sc->width) * ns, 8 * nps));
-  static const std::vector<int> trank = [] {  // type code -> position in name order
-    std::vector<int> idx(256), rank(256);
-    for (int c = 0; c < 256; ++c) idx[c] = c;
-    std::sort(idx.begin(), idx.end(), [](int a, int b) { return type_name(a) < type_name(b); });
-    for (int r = 0; r < 256; ++r) rank[idx[r]] = r;
-    return rank;
-  }();
-  int tcode[256];
-  for (int c = 0; c < 256; ++c) tcode[trank[c]] = c;
-  // size position -> rank among the sizes in numeric order (ORDER BY p_size)
-  int srank[8], spos[8];
-  for (int i = 0; i < p->nsizes; ++i) {
-    srank[i] = 0;
-    for (int j = 0; j < p->nsizes; ++j)
-      srank[i] += p->sizes[j] < p->sizes[i] || (p->sizes[j] == p->sizes[i] && j < i);
-    spos[srank[i]] = i;
-  }
-  // One 64-bit key per group carries its whole ordering: supplier count
-  // descending, then brand, type name and size ascending; sorted by an LSD radix
-  // sort (8 passes of 8 bits) and decoded back into rows.
-  std::vector<uint64_t> key(n), tmp(n);
-  for (uint64_t i = 0; i < n; ++i) {
-    const uint32_t g = got[i].x;
-    const uint32_t brand = g / 8 / 256, type = (g / 8) % 256, pos = g % 8;
-    key[i] = ((uint64_t)(0xFFFFFFFFu - got[i].y) << 32) | (brand << 16) |
-             ((uint32_t)trank[type] << 8) | (uint32_t)srank[pos];
-  }
-  for (int pass = 0; pass < 8; ++pass) {
-    const int sh = pass * 8;
-    uint64_t cnt[257] = {0};
-    for (uint64_t i = 0; i < n; ++i) ++cnt[((key[i] >> sh) & 0xFF) + 1];
-    for (int d = 0; d < 256; ++d) cnt[d + 1] += cnt[d];
-    for (uint64_t i = 0; i < n; ++i) tmp[cnt[(key[i] >> sh) & 0xFF]++] = key[i];
-    key.swap(tmp);
+  // Rows arrive in (brand, type name, size) order; a stable counting sort by
+  // supplier count, descending, completes ORDER BY.
+  uint32_t maxc = 0;
+  for (uint64_t i = 0; i < n; ++i) maxc = std::max(maxc, (uint32_t)got[i].w);
+  *nrows = (int64_t)n;
+  if (maxc <= 4 * n + 1024) {
+    std::vector<uint64_t> at(maxc + 2, 0);
+    for (uint64_t i = 0; i < n; ++i) ++at[maxc - (uint32_t)got[i].w + 1];
+    for (uint32_t c = 0; c <= maxc; ++c) at[c + 1] += at[c];
+    for (uint64_t i = 0; i < n; ++i) {
+      const uint64_t pos = at[maxc - (uint32_t)got[i].w]++;
+      if ((int64_t)pos < cap) rows[pos] = gdb_q16_row{got[i].x, got[i].y, got[i].z, got[i].w};
+    }
+    return GDB_OK;
   }
   std::vector<gdb_q16_row> out(n);
-  for (uint64_t i = 0; i < n; ++i) {
-    const uint64_t k = key[i];
-    out[i] = gdb_q16_row{(int32_t)((k >> 16) & 0xFFFF), tcode[(k >> 8) & 0xFF],
-                         qpart.sizes[spos[k & 0xFF]], (int32_t)(0xFFFFFFFFu - (uint32_t)(k >> 32))};
-  }
-  *nrows = (int64_t)n;
+  for (uint64_t i = 0; i < n; ++i) out[i] = gdb_q16_row{got[i].x, got[i].y, got[i].z, got[i].w};
+  std::stable_sort(out.begin(), out.end(), [](const gdb_q16_row& a, const gdb_q16_row& b) {
+    return a.supplier_cnt > b.supplier_cnt;
+  });
   for (int64_t i = 0; i < std::min<int64_t>(cap, (int64_t)n); ++i) rows[i] = out[i];
   return GDB_OK;
 }

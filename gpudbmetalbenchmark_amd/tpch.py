@@ -29,6 +29,7 @@ the sum of each line's charge rounded half-up to cents, and comments are drawn
 from small vocabularies (with the Q13 and Q16 patterns planted at fixed rates).
 """
 import ctypes as C
+import functools
 import re
 
 import numpy as np
@@ -77,15 +78,27 @@ def brand_name(code):
     return f"Brand#{int(code)}"
 
 
+@functools.lru_cache(maxsize=256)
+def _like_rx(pattern):
+    rx = "".join(".*" if ch == "%" else "." if ch == "_" else re.escape(ch) for ch in pattern)
+    return re.compile(rx, flags=re.S)
+
+
 def like(value, pattern):
     """SQL LIKE (case-sensitive, % and _ wildcards) of one string."""
-    rx = "".join(".*" if ch == "%" else "." if ch == "_" else re.escape(ch) for ch in pattern)
-    return re.fullmatch(rx, value, flags=re.S) is not None
+    return _like_rx(pattern).fullmatch(value) is not None
+
+
+@functools.lru_cache(maxsize=256)
+def _codes_like(dictionary, pattern):
+    rx = _like_rx(pattern)
+    return tuple(i for i, v in enumerate(dictionary) if rx.fullmatch(v))
 
 
 def codes_like(dictionary, pattern):
-    """Codes of the dictionary entries that match the LIKE pattern."""
-    return [i for i, v in enumerate(dictionary) if like(v, pattern)]
+    """Codes of the dictionary entries that match the LIKE pattern (memoised per
+    dictionary and pattern)."""
+    return list(_codes_like(tuple(dictionary), pattern))
 
 
 def code_mask(codes, words=3):

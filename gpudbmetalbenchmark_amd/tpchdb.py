@@ -183,7 +183,7 @@ class TpchDb:
 
     def q20(self, prefix="forest", date_lo=731, date_hi=1096, nation="CANADA", frac_num=1,
             frac_den=2, cap=1 << 20, timing=None):
-        out = np.zeros(max(cap, 1), np.int32)
+        out = np.empty(max(cap, 1), np.int32)
         n = C.c_int64()
         p = abi.Q20Params(prefix.encode(), date_lo, date_hi, tpch.N_NAME.index(nation),
                           frac_num, frac_den, 0)
