@@ -79,7 +79,7 @@ class Workload:
 # TPC-H pipelines over the generated TPC-H-shaped database: name -> dominant
 # kernel (the one the engine brackets with its kernel events).
 TPCH_KERNELS = {"q2": "q2_min", "q4": "q4_late", "q5": "q5_scan", "q7": "q7_scan",
-                "q8": "q8_scan", "q10": "q10_scan", "q11": "q11_scan", "q12": "q12_scan",
+                "q8": "q8_scan", "q10": "q10_probe", "q11": "q11_scan", "q12": "q12_scan",
                 "q14": "q14_scan", "q15": "q15_scan", "q16": "q16_scan", "q17": "q17_stats",
                 "q18": "q18_owner", "q19": "q19_scan", "q20": "q20_scan", "q21": "q21_owner",
                 "q22": "mark_bytes"}

@@ -97,6 +97,24 @@ __host__ __device__ inline uint32_t quads(uint64_t n) { return (uint32_t)((n + 3
 __device__ __forceinline__ bool bit_test(const uint32_t* __restrict__ bm, uint32_t k) {
   return (bm[k >> 5] >> (k & 31)) & 1u;
 }
+// Map entries at the four keys of a quad, loaded together so their latencies
+// overlap (a per-row load under a per-row branch waits once per row). Every key
+// must index the map, including rows that fail earlier tests and padding rows
+// (key 0): maps are sized over the whole key column.
+template <class T>
+__device__ __forceinline__ void gather4(const T* __restrict__ map, const int4& k, uint32_t* v) {
+  v[0] = map[(uint32_t)k.x];
+  v[1] = map[(uint32_t)k.y];
+  v[2] = map[(uint32_t)k.z];
+  v[3] = map[(uint32_t)k.w];
+}
+// Bit j set when bit k.j of the bitmap is set (four loads together).
+__device__ __forceinline__ uint32_t bits4(const uint32_t* __restrict__ bm, const int4& k) {
+  const uint32_t w0 = bm[(uint32_t)k.x >> 5], w1 = bm[(uint32_t)k.y >> 5],
+                 w2 = bm[(uint32_t)k.z >> 5], w3 = bm[(uint32_t)k.w >> 5];
+  return ((w0 >> (k.x & 31)) & 1u) | (((w1 >> (k.y & 31)) & 1u) << 1) |
+         (((w2 >> (k.z & 31)) & 1u) << 2) | (((w3 >> (k.w & 31)) & 1u) << 3);
+}
 __device__ __forceinline__ void bit_set(uint32_t* bm, uint32_t k) {
   atomicOr(&bm[k >> 5], 1u << (k & 31));
 }
@@ -139,6 +157,25 @@ __device__ __forceinline__ void walk2(uint32_t nq, First first, Rest rest) {
     rest(g + stride, b);
   }
   if (g < nq) rest(g, first(g));
+}
+
+// Grid-stride walk over nq vector groups, QB groups per lane per trip (g,
+// g + stride, ...): `body(g, live)` runs each load phase for all QB groups
+// before using any of them, so a lane has QB groups' loads in flight per phase.
+// Groups past nq are not live (their loads must be skipped).
+template <int QB, class Body>
+__device__ __forceinline__ void walk_batch(uint32_t nq, Body body) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t g0 = blockIdx.x * blockDim.x + threadIdx.x; g0 < nq; g0 += QB * stride) {
+    uint32_t g[QB];
+    bool live[QB];
+#pragma unroll
+    for (int b = 0; b < QB; ++b) {
+      g[b] = g0 + (uint32_t)b * stride;
+      live[b] = g[b] < nq;
+    }
+    body(g, live);
+  }
 }
 
 // Block sum of `v` added to *out (one atomic per workgroup).

@@ -70,18 +70,17 @@ __global__ __launch_bounds__(256) void q5_scan(const int32_t* __restrict__ ok,
   __syncthreads();
   walk2(quads(n), [&](uint32_t g) { return ld4nt(ok, g); }, [&](uint32_t g, const int4& k) {
     uint32_t on[4];
+    gather4(omap, k, on);
     uint32_t m = 0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      on[j] = 4 * g + j < n ? omap[lane_of(k, j)] : NONE;
-      m |= (uint32_t)(on[j] != NONE) << j;
-    }
+    for (int j = 0; j < 4; ++j) m |= (uint32_t)(4 * g + j < n && on[j] != NONE) << j;
     if (!m) return;
     const int4 s = ld4nt(sk, g);
+    uint32_t sn[4];
+    gather4(smap, s, sn);
     uint32_t m2 = 0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-      m2 |= (uint32_t)(((m >> j) & 1) && smap[lane_of(s, j)] == on[j]) << j;
+    for (int j = 0; j < 4; ++j) m2 |= (uint32_t)(((m >> j) & 1) && sn[j] == on[j]) << j;
     if (!m2) return;
     const int4 d = ld4nt(disc, g);
     const longlong2 a = ld2nt(price, 2 * g), b = ld2nt(price, 2 * g + 1);
@@ -148,21 +147,18 @@ __global__ __launch_bounds__(256) void q7_scan(const int32_t* __restrict__ ship,
     if (!m) return;
     const int4 k = ld4nt(ok, g);
     uint32_t cs[4];
+    gather4(omap, k, cs);
     uint32_t m2 = 0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      cs[j] = (m >> j) & 1 ? omap[lane_of(k, j)] : NONE;
-      m2 |= (uint32_t)(cs[j] != NONE) << j;
-    }
+    for (int j = 0; j < 4; ++j) m2 |= (uint32_t)(((m >> j) & 1) && cs[j] != NONE) << j;
     if (!m2) return;
     const int4 sp = ld4nt(sk, g);
     uint32_t ss[4];
+    gather4(smap, sp, ss);
     uint32_t m3 = 0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      ss[j] = (m2 >> j) & 1 ? smap[lane_of(sp, j)] : NONE;
-      m3 |= (uint32_t)(ss[j] != NONE && (ss[j] != cs[j] || same)) << j;
-    }
+    for (int j = 0; j < 4; ++j)
+      m3 |= (uint32_t)(((m2 >> j) & 1) && ss[j] != NONE && (ss[j] != cs[j] || same)) << j;
     if (!m3) return;
     const int4 d = ld4nt(disc, g);
     const longlong2 a = ld2nt(price, 2 * g), b = ld2nt(price, 2 * g + 1);
@@ -230,26 +226,28 @@ __global__ __launch_bounds__(FOLD_BLOCK) void q8_scan(const int32_t* __restrict_
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const uint32_t key = (uint32_t)lane_of(p, j);
-      m |= (uint32_t)((4 * g + j < n) && fold_test(lfold, key) && bit_test(pbits, key)) << j;
+      m |= (uint32_t)((4 * g + j < n) && fold_test(lfold, key)) << j;
     }
+    if (!m) return;
+    m &= bits4(pbits, p);
     if (!m) return;
     const int4 k = ld4nt(ok, g);
     uint32_t y[4];
+    gather4(omap, k, y);
     uint32_t m2 = 0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      y[j] = (m >> j) & 1 ? omap[lane_of(k, j)] : NONE;
-      m2 |= (uint32_t)(y[j] < 8) << j;
-    }
+    for (int j = 0; j < 4; ++j) m2 |= (uint32_t)(((m >> j) & 1) && y[j] < 8) << j;
     if (!m2) return;
     const int4 sp = ld4nt(sk, g), d = ld4nt(disc, g);
     const longlong2 a = ld2nt(price, 2 * g), b = ld2nt(price, 2 * g + 1);
+    uint32_t sn[4];
+    gather4(smap, sp, sn);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       if (!((m2 >> j) & 1)) continue;
       const int64_t v = lane_of(a, b, j) * (int64_t)(100 - lane_of(d, j));
       lds_add_i64(&vol[8 + y[j]], v);
-      if (smap[lane_of(sp, j)] == nation) lds_add_i64(&vol[y[j]], v);
+      if (sn[j] == (uint32_t)nation) lds_add_i64(&vol[y[j]], v);
       atomicOr(&hit, 1u << y[j]);
     }
   });
@@ -287,42 +285,58 @@ __global__ __launch_bounds__(256) void q10_orders(const int32_t* __restrict__ ok
   });
 }
 
-__global__ __launch_bounds__(256) void q10_scan(const uint8_t* __restrict__ rflag,
-                                                const int32_t* __restrict__ ok,
-                                                const int32_t* __restrict__ disc,
-                                                const int64_t* __restrict__ price, uint32_t n,
-                                                uint32_t flag, const uint32_t* __restrict__ win,
-                                                const int32_t* __restrict__ ocust,
-                                                uint32_t cspan, int64_t* __restrict__ crev,
-                                                uint8_t* __restrict__ seen) {
-  walk2(quads(n), [&](uint32_t g) { return ld_u8x4nt(rflag, g); }, [&](uint32_t g, uint32_t fw) {
+// Q10 over lineitem: lines with the return flag whose order is in the window
+// (bit set in `win`) add their revenue to their order's customer. Flag and key
+// load together, so the probe's only dependent load is the bitmap word. The rare
+// hits are first listed as chunk-relative rows in the wave's own region of `cap`
+// rows of `hits` (at least the rows it visits; the count stays in a register, so
+// no atomics), then the wave finishes its list: order -> customer, revenue.
+__global__ __launch_bounds__(256) void q10_probe(const uint8_t* __restrict__ rflag,
+                                                 const int32_t* __restrict__ ok,
+                                                 const int32_t* __restrict__ disc,
+                                                 const int64_t* __restrict__ price, uint32_t n,
+                                                 uint32_t flag, const uint32_t* __restrict__ win,
+                                                 const int32_t* __restrict__ ocust, uint32_t cspan,
+                                                 uint32_t cap, uint32_t* __restrict__ hits,
+                                                 int64_t* __restrict__ crev,
+                                                 uint8_t* __restrict__ seen) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nq = quads(n), stride = gridDim.x * blockDim.x;
+  uint32_t* out = hits + (size_t)wave * cap;
+  uint32_t cnt = 0;  // wave-uniform: every trip runs on all lanes
+  auto probe = [&](uint32_t g, uint32_t fw, const int4& k) {
     uint32_t m = 0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) m |= (uint32_t)((4 * g + j < n) & (byte_of(fw, j) == flag)) << j;
-    if (!m) return;
-    const int4 k = ld4nt(ok, g);
-    uint32_t m1 = 0;
-#pragma unroll
     for (int j = 0; j < 4; ++j)
-      m1 |= (uint32_t)(((m >> j) & 1) && bit_test(win, (uint32_t)lane_of(k, j))) << j;
-    if (!m1) return;
-    int32_t cu[4];
-    uint32_t m2 = 0;
+      m |= (uint32_t)((4 * g + j < n) && byte_of(fw, j) == flag &&
+                      bit_test(win, (uint32_t)lane_of(k, j))) << j;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      cu[j] = (m1 >> j) & 1 ? ocust[lane_of(k, j)] : -1;
-      m2 |= (uint32_t)(cu[j] >= 0 && (uint32_t)cu[j] < cspan) << j;
+      const bool h = (m >> j) & 1;
+      const unsigned long long b = __ballot(h);
+      if (h) out[cnt + __popcll(b & ((1ull << lane) - 1))] = 4 * g + j;
+      cnt += (uint32_t)__popcll(b);
     }
-    if (!m2) return;
-    const int4 d = ld4nt(disc, g);
-    const longlong2 a = ld2nt(price, 2 * g), b = ld2nt(price, 2 * g + 1);
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-      if ((m2 >> j) & 1) {
-        add_i64(&crev[cu[j]], lane_of(a, b, j) * (int64_t)(100 - lane_of(d, j)));
-        seen[cu[j]] = 1;
-      }
-  });
+  };
+  const int4 z = make_int4(0, 0, 0, 0);
+  // Two quads per lane in flight: g and g + stride.
+  for (uint32_t gw = wave * 64; gw < nq; gw += 2 * stride) {
+    const uint32_t g0 = gw + lane, g1 = g0 + stride;
+    const bool l0 = g0 < nq, l1 = g1 < nq;
+    const uint32_t f0 = l0 ? ld_u8x4nt(rflag, g0) : 0u, f1 = l1 ? ld_u8x4nt(rflag, g1) : 0u;
+    const int4 k0 = l0 ? ld4nt(ok, g0) : z, k1 = l1 ? ld4nt(ok, g1) : z;
+    probe(g0, f0, k0);
+    probe(g1, f1, k1);
+  }
+  __threadfence_block();  // the list's stores are seen by the wave's other lanes
+  for (uint32_t i = lane; i < cnt; i += 64) {
+    const uint32_t r = out[i];
+    const int32_t cu = ocust[ok[r]];
+    if (cu < 0 || (uint32_t)cu >= cspan) continue;
+    add_i64(&crev[cu], price[r] * (int64_t)(100 - disc[r]));
+    seen[cu] = 1;
+  }
 }
 
 struct KeyedRevSrc {  // (revenue desc, key asc) over a per-key revenue array
@@ -1088,6 +1102,7 @@ int gdb_q10(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q10_params* p, gdb_q1
   char* host;
   GDB_TRY(arena(ctx, A_MAP, ospan * 4, (void**)&ocust));
   GDB_TRY(arena(ctx, A_BITMAP, wwords * 4, (void**)&owin));
+  uint32_t* hits = nullptr;  // at most one per line of a chunk (sized per chunk)
   GDB_TRY(arena(ctx, A_AGG, cspan * 8, (void**)&crev));
   GDB_TRY(arena(ctx, A_AUX, cspan * 2, (void**)&seen));
   rank = seen + cspan;
@@ -1113,10 +1128,16 @@ int gdb_q10(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q10_params* p, gdb_q1
         p->date_lo, p->date_hi, owin, ocust);
   GDB_HIP(hipGetLastError());
   RunStats rs;
-  GDB_TRY(scan_table(ctx, db->lineitem, {lf, lo, ld, le}, [&](const void* const* d, uint32_t rows) {
-    q10_scan<<<blocks_for(ctx, q10_scan, quads(rows)), BLOCK, 0, s>>>(
-        (const uint8_t*)d[0], (const int32_t*)d[1], (const int32_t*)d[2], (const int64_t*)d[3],
-        rows, (uint32_t)(uint8_t)p->returnflag, owin, ocust, (uint32_t)cspan, crev, seen);
+  GDB_TRY(scan_table(ctx, db->lineitem, {lf, lo, ld, le}, [&](const void* const* d,
+                                                              uint32_t rows) -> int {
+    const int grid = blocks_for(ctx, q10_probe, quads(rows));
+    const uint64_t stride = (uint64_t)grid * BLOCK, nwaves = stride / 64;
+    const uint32_t cap = (uint32_t)(256 * ((quads(rows) + stride - 1) / stride));
+    GDB_TRY(arena(ctx, A_HT, std::max<uint64_t>(nwaves * cap, 1) * 4, (void**)&hits));
+    q10_probe<<<grid, BLOCK, 0, s>>>((const uint8_t*)d[0], (const int32_t*)d[1],
+                                     (const int32_t*)d[2], (const int64_t*)d[3], rows,
+                                     (uint32_t)(uint8_t)p->returnflag, owin, ocust,
+                                     (uint32_t)cspan, cap, hits, crev, seen);
     GDB_HIP(hipGetLastError());
     return GDB_OK;
   }, &rs));

@@ -33,12 +33,13 @@ __global__ __launch_bounds__(256) void q14_scan(const int32_t* __restrict__ ship
     if (!m) return;
     const int4 p = ld4nt(pk, g), d = ld4nt(disc, g);
     const longlong2 a = ld2nt(price, 2 * g), b = ld2nt(price, 2 * g + 1);
+    const uint32_t pm = bits4(promo, p);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       if (!((m >> j) & 1)) continue;
       const int64_t rev = lane_of(a, b, j) * (int64_t)(100 - lane_of(d, j));
       st += rev;
-      if (bit_test(promo, (uint32_t)lane_of(p, j))) sp += rev;
+      if ((pm >> j) & 1) sp += rev;
     }
   });
   block_add(sp, red, out);
@@ -138,29 +139,47 @@ __global__ __launch_bounds__(256) void q12_scan(const uint8_t* __restrict__ mode
                                                 int64_t* __restrict__ out) {
   __shared__ int64_t red[4];
   int64_t c[4] = {0, 0, 0, 0};  // [mode index][high]
-  struct Ld {
-    uint32_t mw;
-    int4 r;
-  };
-  walk2(quads(n), [&](uint32_t g) { return Ld{ld_u8x4nt(mode, g), ld4nt(receipt, g)}; },
-        [&](uint32_t g, const Ld& ld) {
-    const uint32_t mw = ld.mw;
-    const int4 r = ld.r;
-    uint32_t m = 0;
+  constexpr int QB = 4;
+  walk_batch<QB>(quads(n), [&](const uint32_t* g, const bool* live) {
+    uint32_t mw[QB], m[QB];
+    int4 r[QB];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int md = byte_of(mw, j), rv = lane_of(r, j);
-      m |= (uint32_t)((4 * g + j < n) & ((md == m0) | (md == m1)) & (rv >= lo) & (rv < hi)) << j;
+    for (int b = 0; b < QB; ++b) {
+      mw[b] = live[b] ? ld_u8x4nt(mode, g[b]) : 0u;
+      r[b] = live[b] ? ld4nt(receipt, g[b]) : make_int4(0, 0, 0, 0);
     }
-    if (!m) return;
-    const int4 cm = ld4nt(commit, g), s = ld4nt(ship, g), o = ld4nt(ok, g);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int cv = lane_of(cm, j);
-      if (!((m >> j) & 1) || !(cv < lane_of(r, j)) || !(lane_of(s, j) < cv)) continue;
-      const int mi = byte_of(mw, j) == m0 ? 0 : 1;
-      const int h = high[lane_of(o, j)] ? 1 : 0;
-      c[mi * 2 + h] += 1;
+    for (int b = 0; b < QB; ++b) {
+      m[b] = 0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int md = byte_of(mw[b], j), rv = lane_of(r[b], j);
+        m[b] |= (uint32_t)(live[b] && (4 * g[b] + j < n) && ((md == m0) | (md == m1)) &&
+                           (rv >= lo) && (rv < hi)) << j;
+      }
+    }
+    int4 cm[QB], s[QB], o[QB];
+#pragma unroll
+    for (int b = 0; b < QB; ++b)
+      if (m[b]) {
+        cm[b] = ld4nt(commit, g[b]);
+        s[b] = ld4nt(ship, g[b]);
+        o[b] = ld4nt(ok, g[b]);
+      }
+    uint32_t hv[QB][4];
+#pragma unroll
+    for (int b = 0; b < QB; ++b)
+      if (m[b]) gather4(high, o[b], hv[b]);
+#pragma unroll
+    for (int b = 0; b < QB; ++b) {
+      if (!m[b]) continue;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int cv = lane_of(cm[b], j);
+        if (!((m[b] >> j) & 1) || !(cv < lane_of(r[b], j)) || !(lane_of(s[b], j) < cv)) continue;
+        const int mi = byte_of(mw[b], j) == m0 ? 0 : 1;
+        c[mi * 2 + (hv[b][j] ? 1 : 0)] += 1;
+      }
     }
   });
 #pragma unroll
@@ -193,7 +212,16 @@ __global__ __launch_bounds__(256) void q4_count(const int32_t* __restrict__ date
                                                 int64_t* __restrict__ out) {
   __shared__ int64_t red[4];
   int64_t c[5] = {0, 0, 0, 0, 0};
-  walk2(quads(n), [&](uint32_t g) { return ld4nt(date, g); }, [&](uint32_t g, const int4& d) {
+  // Date, key and priority are loaded together (one level of dependent loads:
+  // the late flags of the window's orders).
+  struct Ld {
+    int4 d, o;
+    uint32_t pw;
+  };
+  walk2(quads(n), [&](uint32_t g) { return Ld{ld4nt(date, g), ld4nt(ok, g), ld_u8x4nt(prio, g)}; },
+        [&](uint32_t g, const Ld& l) {
+    const int4 d = l.d, o = l.o;
+    const uint32_t pw = l.pw;
     uint32_t m = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -201,11 +229,11 @@ __global__ __launch_bounds__(256) void q4_count(const int32_t* __restrict__ date
       m |= (uint32_t)((4 * g + j < n) & (v >= lo) & (v < hi)) << j;
     }
     if (!m) return;
-    const int4 o = ld4nt(ok, g);
-    const uint32_t pw = ld_u8x4nt(prio, g);
+    uint32_t lt[4];
+    gather4(late, o, lt);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      if (!((m >> j) & 1) || !late[lane_of(o, j)]) continue;
+      if (!((m >> j) & 1) || !lt[j]) continue;
       const uint32_t pr = byte_of(pw, j);
 #pragma unroll
       for (int b = 0; b < 5; ++b) c[b] += pr == (uint32_t)b;
