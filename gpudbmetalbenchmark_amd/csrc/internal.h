@@ -108,7 +108,7 @@ int resident_blocks(gdb_ctx* ctx, const void* kernel, int block, size_t lds, int
 enum ArenaSlot {
   A_PARTIALS = 0, A_BITMAP = 1, A_MAP = 2, A_HT = 3, A_AGG = 4, A_AUX = 5, A_OUT = 6,
   A_AUX2 = 7, A_STREAM0 = 8, A_STREAM1 = 9, A_Q3BYTE = 10, A_Q3MAP = 11, A_JOINPART = 12, A_JOINHIST = 13,
-  A_T0 = 14, A_T1 = 15, A_NSLOTS = 16
+  A_T0 = 14, A_T1 = 15, A_ACC = 16, A_NSLOTS = 17
 };
 enum TuneSlot { T_Q6 = 0, T_Q1 = 1, T_SELECT = 2, T_SUM = 3, T_Q3 = 4, T_Q9 = 5, T_Q13 = 6, T_JOIN = 7 };
 

@@ -178,6 +178,49 @@ __device__ __forceinline__ void walk_batch(uint32_t nq, Body body) {
   }
 }
 
+// Spread accumulators. Atomics from every workgroup on one address serialise
+// (measured on MI355X: 2048 workgroups x 5 int64 counters on one line cost about
+// 120 us). A kernel that accumulates into a small result therefore adds into copy
+// blockIdx.x % ACC_COPIES of it: the copies lie ACC_STRIDE int64 (256 bytes) apart,
+// so at most 32 workgroups share an address. acc_fold then combines the copies into
+// copy 0 (sum, OR or unsigned max per value). A result of nv values needs a buffer
+// of ACC_COPIES * ACC_STRIDE int64 cleared first, and nv <= ACC_STRIDE.
+constexpr int ACC_COPIES = 64, ACC_STRIDE = 32;
+constexpr size_t ACC_BYTES = (size_t)ACC_COPIES * ACC_STRIDE * 8;
+
+template <class T>
+__device__ __forceinline__ T* acc_copy(T* out) {
+  return out + (size_t)(blockIdx.x % ACC_COPIES) * ACC_STRIDE * (8 / sizeof(T));
+}
+
+// Values [lo, hi) of the fold of all copies go to dst[lo..hi) (dst = acc: into
+// copy 0); bit i of or_mask / max_mask selects OR / unsigned max for value i (sum
+// otherwise). One wave per value: lane c reads copy c, then a shuffle reduction.
+static_assert(ACC_COPIES == 64, "acc_fold reads one copy per lane of a wave");
+namespace {
+__global__ __launch_bounds__(64) void acc_fold(int64_t* __restrict__ acc, int lo,
+                                               uint32_t or_mask, uint32_t max_mask,
+                                               int64_t* __restrict__ dst) {
+  const int i = lo + (int)blockIdx.x;
+  const bool is_or = (or_mask >> i) & 1, is_max = (max_mask >> i) & 1;
+  uint64_t v = (uint64_t)acc[(size_t)threadIdx.x * ACC_STRIDE + i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint64_t x = (uint64_t)__shfl_xor((long long)v, off, 64);
+    v = is_or ? (v | x) : is_max ? (x > v ? x : v) : v + x;
+  }
+  if (threadIdx.x == 0) dst[i] = (int64_t)v;
+}
+}  // namespace
+
+inline int acc_finish(gdb_ctx* ctx, int64_t* acc, int lo, int hi, uint32_t or_mask = 0,
+                      uint32_t max_mask = 0, int64_t* dst = nullptr) {
+  if (hi <= lo) return GDB_OK;
+  acc_fold<<<hi - lo, 64, 0, ctx->stream>>>(acc, lo, or_mask, max_mask, dst ? dst : acc);
+  GDB_HIP(hipGetLastError());
+  return GDB_OK;
+}
+
 // Block sum of `v` added to *out (one atomic per workgroup).
 __device__ __forceinline__ void block_add(int64_t v, int64_t* red, int64_t* out) {
   const int64_t s = gdbk::block_sum(v, red);
@@ -204,6 +247,7 @@ __global__ __launch_bounds__(256) void topk_stage1(Src src, uint64_t n,
   __shared__ int64_t red[4];
   gdbk::BlockTopK<1024> tk{buf, &nbuf, k, &thr, &have};
   tk.init();
+  count = acc_copy(count);  // this workgroup's copy of the count
   if (dn) n = n < *dn ? n : *dn;
   const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
   const uint64_t begin = (uint64_t)blockIdx.x * per;
@@ -284,8 +328,12 @@ int topk_run(gdb_ctx* ctx, Src src, uint64_t n, int k, gdbk::TopEntry* winners,
              gdbk::TopEntry* final_out, unsigned long long* count, const uint32_t* dn = nullptr) {
   hipStream_t s = ctx->stream;
   const int blocks = topk_blocks(n, dn != nullptr);
-  topk_stage1<Src><<<blocks, BLOCK, 0, s>>>(src, n, dn, k, winners, count);
+  int64_t* acc;  // per-workgroup copies of the candidate count
+  GDB_TRY(gdb::arena(ctx, gdb::A_ACC, ACC_BYTES, (void**)&acc));
+  GDB_TRY(gdb::fill(acc, 0, ACC_BYTES, s));
+  topk_stage1<Src><<<blocks, BLOCK, 0, s>>>(src, n, dn, k, winners, (unsigned long long*)acc);
   GDB_HIP(hipGetLastError());
+  GDB_TRY(acc_finish(ctx, acc, 0, 1, 0, 0, (int64_t*)count));
   topk_merge<<<1, BLOCK, 0, s>>>(winners, blocks, k, final_out);
   GDB_HIP(hipGetLastError());
   return GDB_OK;

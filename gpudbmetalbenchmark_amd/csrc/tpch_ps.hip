@@ -140,6 +140,7 @@ __global__ __launch_bounds__(256) void q11_scan(const int32_t* __restrict__ pk,
                                                 int64_t* __restrict__ val,
                                                 uint8_t* __restrict__ seen,
                                                 int64_t* __restrict__ total) {
+  total = acc_copy(total);  // this workgroup's copy of the result
   __shared__ int64_t red[4];
   int64_t tot = 0;
   walk2(quads(n), [&](uint32_t g) { return ld4nt(sk, g); }, [&](uint32_t g, const int4& s) {
@@ -371,6 +372,7 @@ __global__ __launch_bounds__(256) void q20_build(const int32_t* __restrict__ pk,
 __global__ __launch_bounds__(256) void count_bits(const int32_t* __restrict__ pk, uint32_t n,
                                                   const uint32_t* __restrict__ bits,
                                                   unsigned long long* __restrict__ cnt) {
+  cnt = acc_copy(cnt);  // this workgroup's copy of the result
   uint32_t c = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     c += bit_test(bits, (uint32_t)pk[i]);
@@ -619,14 +621,14 @@ int gdb_q11(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q11_params* p, gdb_q1
   GDB_TRY(arena(ctx, A_MAP, pspan, (void**)&seen));
   GDB_TRY(arena(ctx, A_BITMAP, words * 4, (void**)&sbits));
   GDB_TRY(arena(ctx, A_AUX, pspan * sizeof(KV), (void**)&kv));
-  GDB_TRY(arena(ctx, A_OUT, 64, (void**)&dout));
+  GDB_TRY(arena(ctx, A_OUT, ACC_BYTES, (void**)&dout));
   GDB_TRY(pinned(ctx, 64, (void**)&host));
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
   GDB_TRY(fill(val, 0, pspan * 8, s));
   GDB_TRY(fill(seen, 0, pspan, s));
   GDB_TRY(fill(sbits, 0, words * 4, s));
-  GDB_TRY(fill(dout, 0, 16, s));
+  GDB_TRY(fill(dout, 0, ACC_BYTES, s));
   if (ns)
     supp_nation_bits<<<blocks_for(ctx, supp_nation_bits, ns), BLOCK, 0, s>>>(
         (const int32_t*)sk->dev, (const int32_t*)sn->dev, (uint32_t)ns, p->nation, sbits);
@@ -638,6 +640,7 @@ int gdb_q11(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q11_params* p, gdb_q1
         (const int64_t*)qc->dev, (uint32_t)nps, sbits, val, seen, dout);
   GDB_HIP(hipGetLastError());
   GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  GDB_TRY(acc_finish(ctx, dout, 0, 1));  // q11_collect reads the total
   q11_collect<<<blocks_for(ctx, q11_collect, pspan), BLOCK, 0, s>>>(
       val, seen, (uint32_t)pspan, dout, p->frac_num, p->frac_den, kv,
       (unsigned long long*)(dout + 1));
@@ -829,13 +832,13 @@ int gdb_q20(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q20_params* p, int32_
   uint64_t* host;
   GDB_TRY(arena(ctx, A_BITMAP, words * 4, (void**)&bits));
   GDB_TRY(arena(ctx, A_AUX2, sspan, (void**)&good));
-  GDB_TRY(arena(ctx, A_OUT, 64, (void**)&dout));
+  GDB_TRY(arena(ctx, A_OUT, ACC_BYTES, (void**)&dout));
   GDB_TRY(pinned(ctx, 64, (void**)&host));
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
   GDB_TRY(fill(bits, 0, words * 4, s));
   GDB_TRY(fill(good, 0, sspan, s));
-  GDB_TRY(fill(dout, 0, 32, s));
+  GDB_TRY(fill(dout, 0, ACC_BYTES, s));
   if (np)
     q20_part<<<blocks_for(ctx, q20_part, np), BLOCK, 0, s>>>(
         (const int32_t*)pp->dev, (const uint8_t*)pn->dev, pn->width, (uint32_t)np, pre, bits);
@@ -843,6 +846,7 @@ int gdb_q20(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q20_params* p, int32_
     count_bits<<<blocks_for(ctx, count_bits, nps), BLOCK, 0, s>>>(
         (const int32_t*)qp->dev, (uint32_t)nps, bits, (unsigned long long*)dout);
   GDB_HIP(hipGetLastError());
+  GDB_TRY(acc_finish(ctx, (int64_t*)dout, 0, 1));
   GDB_HIP(hipMemcpyAsync(host, dout, 8, hipMemcpyDeviceToHost, s));
   GDB_HIP(hipStreamSynchronize(s));
   const uint64_t slots = pow2_at_least(2 * std::max<uint64_t>(host[0], 1));

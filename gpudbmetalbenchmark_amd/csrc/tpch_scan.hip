@@ -21,6 +21,7 @@ __global__ __launch_bounds__(256) void q14_scan(const int32_t* __restrict__ ship
                                                 int32_t lo, int32_t hi,
                                                 const uint32_t* __restrict__ promo,
                                                 int64_t* __restrict__ out) {
+  out = acc_copy(out);  // this workgroup's copy of the result
   __shared__ int64_t red[4];
   int64_t sp = 0, st = 0;
   walk2(quads(n), [&](uint32_t g) { return ld4nt(ship, g); }, [&](uint32_t g, const int4& s) {
@@ -84,6 +85,7 @@ __global__ __launch_bounds__(256) void q19_scan(const uint8_t* __restrict__ mode
                                                 const int64_t* __restrict__ price, uint32_t n,
                                                 Q19Line q, const uint8_t* __restrict__ map,
                                                 int64_t* __restrict__ out) {
+  out = acc_copy(out);  // this workgroup's copy of the result
   __shared__ int64_t red[4];
   int64_t sum = 0;
   walk2(quads(n), [&](uint32_t g) { return make_uint2(ld_u8x4nt(mode, g), ld_u8x4nt(instr, g)); },
@@ -137,6 +139,7 @@ __global__ __launch_bounds__(256) void q12_scan(const uint8_t* __restrict__ mode
                                                 int32_t m0, int32_t m1, int32_t lo, int32_t hi,
                                                 const uint8_t* __restrict__ high,
                                                 int64_t* __restrict__ out) {
+  out = acc_copy(out);  // this workgroup's copy of the result
   __shared__ int64_t red[4];
   int64_t c[4] = {0, 0, 0, 0};  // [mode index][high]
   constexpr int QB = 4;
@@ -210,33 +213,43 @@ __global__ __launch_bounds__(256) void q4_count(const int32_t* __restrict__ date
                                                 int32_t lo, int32_t hi,
                                                 const uint8_t* __restrict__ late,
                                                 int64_t* __restrict__ out) {
+  out = acc_copy(out);  // this workgroup's copy of the result
   __shared__ int64_t red[4];
   int64_t c[5] = {0, 0, 0, 0, 0};
-  // Date, key and priority are loaded together (one level of dependent loads:
-  // the late flags of the window's orders).
-  struct Ld {
-    int4 d, o;
-    uint32_t pw;
-  };
-  walk2(quads(n), [&](uint32_t g) { return Ld{ld4nt(date, g), ld4nt(ok, g), ld_u8x4nt(prio, g)}; },
-        [&](uint32_t g, const Ld& l) {
-    const int4 d = l.d, o = l.o;
-    const uint32_t pw = l.pw;
-    uint32_t m = 0;
+  // Four groups per lane per trip; date, key and priority load together for all
+  // of them, then the late flags of the window's orders (one dependent phase).
+  constexpr int QB = 4;
+  walk_batch<QB>(quads(n), [&](const uint32_t* g, const bool* live) {
+    int4 d[QB], o[QB];
+    uint32_t pw[QB], m[QB], lt[QB][4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int v = lane_of(d, j);
-      m |= (uint32_t)((4 * g + j < n) & (v >= lo) & (v < hi)) << j;
+    for (int b = 0; b < QB; ++b)
+      if (live[b]) {
+        d[b] = ld4nt(date, g[b]);
+        o[b] = ld4nt(ok, g[b]);
+        pw[b] = ld_u8x4nt(prio, g[b]);
+      }
+#pragma unroll
+    for (int b = 0; b < QB; ++b) {
+      m[b] = 0;
+      if (!live[b]) continue;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int v = lane_of(d[b], j);
+        m[b] |= (uint32_t)((4 * g[b] + j < n) & (v >= lo) & (v < hi)) << j;
+      }
+      if (m[b]) gather4(late, o[b], lt[b]);
     }
-    if (!m) return;
-    uint32_t lt[4];
-    gather4(late, o, lt);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      if (!((m >> j) & 1) || !lt[j]) continue;
-      const uint32_t pr = byte_of(pw, j);
+    for (int b = 0; b < QB; ++b) {
+      if (!m[b]) continue;
 #pragma unroll
-      for (int b = 0; b < 5; ++b) c[b] += pr == (uint32_t)b;
+      for (int j = 0; j < 4; ++j) {
+        if (!((m[b] >> j) & 1) || !lt[b][j]) continue;
+        const uint32_t pr = byte_of(pw[b], j);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) c[q] += pr == (uint32_t)q;
+      }
     }
   });
 #pragma unroll
@@ -279,6 +292,7 @@ __device__ __forceinline__ unsigned long long ord64(int64_t v) {
 __global__ __launch_bounds__(256) void q15_max(const int64_t* __restrict__ rev,
                                                const uint8_t* __restrict__ seen, uint32_t n,
                                                unsigned long long* __restrict__ best) {
+  best = acc_copy(best);  // this workgroup's copy (folded by unsigned max)
   unsigned long long m = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     if (seen[i]) m = max(m, ord64(rev[i]));
@@ -362,6 +376,7 @@ __global__ __launch_bounds__(FOLD_BLOCK) void q17_sum(const int32_t* __restrict_
                                                       const uint32_t* __restrict__ cnt,
                                                       int32_t num, int32_t den,
                                                       int64_t* __restrict__ out) {
+  out = acc_copy(out);  // this workgroup's copy of the result
   __shared__ __align__(16) uint32_t lfold[FOLD_WORDS];
   __shared__ int64_t red[FOLD_BLOCK / 64];
   fold_load(lfold, fold);
@@ -389,10 +404,18 @@ struct Q22Codes {
 
 __global__ __launch_bounds__(256) void mark_bytes(const int32_t* __restrict__ key, uint32_t n,
                                                   uint8_t* __restrict__ map) {
-  walk2(quads(n), [&](uint32_t g) { return ld4nt(key, g); }, [&](uint32_t g, const int4& k) {
+  constexpr int QB = 8;  // groups per lane per trip, loads issued for all first
+  walk_batch<QB>(quads(n), [&](const uint32_t* g, const bool* live) {
+    int4 k[QB];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-      if (4 * g + j < n) map[lane_of(k, j)] = 1;
+    for (int b = 0; b < QB; ++b)
+      if (live[b]) k[b] = ld4nt(key, g[b]);
+#pragma unroll
+    for (int b = 0; b < QB; ++b)
+      if (live[b])
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (4 * g[b] + j < n) map[lane_of(k[b], j)] = 1;
   });
 }
 
@@ -404,6 +427,7 @@ __device__ __forceinline__ int q22_slot(const Q22Codes& c, int32_t nation) {
 __global__ __launch_bounds__(256) void q22_avg(const int32_t* __restrict__ nat,
                                                const int64_t* __restrict__ bal, uint32_t n,
                                                Q22Codes c, int64_t* __restrict__ out) {
+  out = acc_copy(out);  // this workgroup's copy of the result
   __shared__ int64_t red[4];
   int64_t s = 0, k = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -423,7 +447,8 @@ __global__ __launch_bounds__(256) void q22_final(const int32_t* __restrict__ ck,
   __shared__ int64_t bins[14];
   if (threadIdx.x < 14) bins[threadIdx.x] = 0;
   __syncthreads();
-  const int64_t sum = out[0], cnt = out[1];
+  const int64_t sum = out[0], cnt = out[1];  // folded by acc_fold before this launch
+  out = acc_copy(out);  // this workgroup's copy of the result
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int slot = q22_slot(c, nat[i]);
     if (slot < 0 || cnt == 0) continue;
@@ -460,12 +485,12 @@ int gdb_q14(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q14_params* p, gdb_q1
   uint32_t* bits;
   int64_t *dout, *host;
   GDB_TRY(arena(ctx, A_BITMAP, words * 4, (void**)&bits));
-  GDB_TRY(arena(ctx, A_OUT, 64, (void**)&dout));
+  GDB_TRY(arena(ctx, A_OUT, ACC_BYTES, (void**)&dout));
   GDB_TRY(pinned(ctx, 64, (void**)&host));
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
   GDB_TRY(fill(bits, 0, words * 4, s));
-  GDB_TRY(fill(dout, 0, 16, s));
+  GDB_TRY(fill(dout, 0, ACC_BYTES, s));
   if (np)
     code_bits<<<blocks_for(ctx, code_bits, np), BLOCK, 0, s>>>(
         (const int32_t*)pp->dev, (const uint8_t*)pt->dev, (uint32_t)np, mask4(p->type_mask), bits);
@@ -477,6 +502,7 @@ int gdb_q14(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q14_params* p, gdb_q1
     GDB_HIP(hipGetLastError());
     return GDB_OK;
   }, &rs));
+  GDB_TRY(acc_finish(ctx, dout, 0, 2));
   GDB_HIP(hipMemcpyAsync(host, dout, 16, hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
   // SQL columns: l_shipdate, l_partkey, l_discount (4 B), l_extendedprice (8 B);
@@ -511,7 +537,7 @@ int gdb_q19(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q19_params* p, gdb_q1
   uint8_t* map;
   int64_t *dout, *host;
   GDB_TRY(arena(ctx, A_MAP, span, (void**)&map));
-  GDB_TRY(arena(ctx, A_OUT, 64, (void**)&dout));
+  GDB_TRY(arena(ctx, A_OUT, ACC_BYTES, (void**)&dout));
   GDB_TRY(pinned(ctx, 64, (void**)&host));
   Q19Part qp{};
   Q19Line ql{};
@@ -529,7 +555,7 @@ int gdb_q19(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q19_params* p, gdb_q1
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
   GDB_TRY(fill(map, 0, span, s));
-  GDB_TRY(fill(dout, 0, 8, s));
+  GDB_TRY(fill(dout, 0, ACC_BYTES, s));
   if (np)
     q19_part<<<blocks_for(ctx, q19_part, np), BLOCK, 0, s>>>(
         (const int32_t*)pp->dev, (const uint8_t*)pb->dev, (const uint8_t*)pc->dev,
@@ -544,6 +570,7 @@ int gdb_q19(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q19_params* p, gdb_q1
     GDB_HIP(hipGetLastError());
     return GDB_OK;
   }, &rs));
+  GDB_TRY(acc_finish(ctx, dout, 0, 1));
   GDB_HIP(hipMemcpyAsync(host, dout, 8, hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
   // SQL columns: shipmode, shipinstruct (1 B), partkey, quantity, discount (4 B),
@@ -572,12 +599,12 @@ int gdb_q12(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q12_params* p, gdb_q1
   uint8_t* high;
   int64_t *dout, *host;
   GDB_TRY(arena(ctx, A_MAP, span, (void**)&high));
-  GDB_TRY(arena(ctx, A_OUT, 64, (void**)&dout));
+  GDB_TRY(arena(ctx, A_OUT, ACC_BYTES, (void**)&dout));
   GDB_TRY(pinned(ctx, 64, (void**)&host));
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
   GDB_TRY(fill(high, 0, span, s));
-  GDB_TRY(fill(dout, 0, 32, s));
+  GDB_TRY(fill(dout, 0, ACC_BYTES, s));
   if (no)
     order_prio_bytes<<<blocks_for(ctx, order_prio_bytes, no), BLOCK, 0, s>>>(
         (const int32_t*)oo->dev, (const uint8_t*)op->dev, (uint32_t)no, p->high_mask, high);
@@ -590,6 +617,7 @@ int gdb_q12(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q12_params* p, gdb_q1
     GDB_HIP(hipGetLastError());
     return GDB_OK;
   }, &rs));
+  GDB_TRY(acc_finish(ctx, dout, 0, 4));
   GDB_HIP(hipMemcpyAsync(host, dout, 32, hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
   // SQL columns: shipmode (1 B), receipt, commit, ship, orderkey (4 B); o_orderkey,
@@ -620,12 +648,12 @@ int gdb_q4(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q4_params* p, gdb_q4_r
   uint8_t* late;
   int64_t *dout, *host;
   GDB_TRY(arena(ctx, A_MAP, span, (void**)&late));
-  GDB_TRY(arena(ctx, A_OUT, 64, (void**)&dout));
+  GDB_TRY(arena(ctx, A_OUT, ACC_BYTES, (void**)&dout));
   GDB_TRY(pinned(ctx, 64, (void**)&host));
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
   GDB_TRY(fill(late, 0, span, s));
-  GDB_TRY(fill(dout, 0, 40, s));
+  GDB_TRY(fill(dout, 0, ACC_BYTES, s));
   RunStats rs;
   GDB_TRY(scan_table(ctx, db->lineitem, {lo, lc, lr}, [&](const void* const* d, uint32_t rows) {
     q4_late<<<blocks_for(ctx, q4_late, quads(rows)), BLOCK, 0, s>>>(
@@ -638,6 +666,7 @@ int gdb_q4(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q4_params* p, gdb_q4_r
         (const int32_t*)od->dev, (const int32_t*)oo->dev, (const uint8_t*)op->dev, (uint32_t)no,
         p->date_lo, p->date_hi, late, dout);
   GDB_HIP(hipGetLastError());
+  GDB_TRY(acc_finish(ctx, dout, 0, 5));
   GDB_HIP(hipMemcpyAsync(host, dout, 40, hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
   // SQL columns: l_orderkey, l_commitdate, l_receiptdate (12 B per line, all read by
@@ -664,16 +693,18 @@ int gdb_q15(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q15_params* p, gdb_q1
   int64_t* host;
   GDB_TRY(arena(ctx, A_AGG, span * 8, (void**)&rev));
   GDB_TRY(arena(ctx, A_MAP, span, (void**)&seen));
-  GDB_TRY(arena(ctx, A_OUT, 64 + 4 * 100, (void**)&dout));
+  GDB_TRY(arena(ctx, A_OUT, 512 + ACC_BYTES, (void**)&dout));
   GDB_TRY(pinned(ctx, 64 + 4 * 100, (void**)&host));
   unsigned long long* best = (unsigned long long*)dout;
   uint32_t* cnt = (uint32_t*)(dout + 1);
   int32_t* keys = (int32_t*)(dout + 2);
+  int64_t* acc = (int64_t*)(dout + 64);  // per-workgroup copies of the maximum
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
   GDB_TRY(fill(rev, 0, span * 8, s));
   GDB_TRY(fill(seen, 0, span, s));
   GDB_TRY(fill(dout, 0, 16, s));
+  GDB_TRY(fill(acc, 0, ACC_BYTES, s));
   RunStats rs;
   GDB_TRY(scan_table(ctx, db->lineitem, {ls, lk, ld, le}, [&](const void* const* d, uint32_t rows) {
     q15_scan<<<blocks_for(ctx, q15_scan, quads(rows)), BLOCK, 0, s>>>(
@@ -682,7 +713,9 @@ int gdb_q15(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q15_params* p, gdb_q1
     GDB_HIP(hipGetLastError());
     return GDB_OK;
   }, &rs));
-  q15_max<<<blocks_for(ctx, q15_max, span), BLOCK, 0, s>>>(rev, seen, (uint32_t)span, best);
+  q15_max<<<blocks_for(ctx, q15_max, span), BLOCK, 0, s>>>(rev, seen, (uint32_t)span,
+                                                           (unsigned long long*)acc);
+  GDB_TRY(acc_finish(ctx, acc, 0, 1, 0, 1u, (int64_t*)best));
   q15_collect<<<blocks_for(ctx, q15_collect, span), BLOCK, 0, s>>>(rev, seen, (uint32_t)span,
                                                                    best, cnt, keys, 100);
   GDB_HIP(hipGetLastError());
@@ -723,14 +756,14 @@ int gdb_q17(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q17_params* p, gdb_q1
   fold = bits + words4;
   GDB_TRY(arena(ctx, A_AGG, span * 8, (void**)&sumq));
   GDB_TRY(arena(ctx, A_MAP, span * 4, (void**)&cnt));
-  GDB_TRY(arena(ctx, A_OUT, 64, (void**)&dout));
+  GDB_TRY(arena(ctx, A_OUT, ACC_BYTES, (void**)&dout));
   GDB_TRY(pinned(ctx, 64, (void**)&host));
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
   GDB_TRY(fill(bits, 0, (words4 + FOLD_WORDS) * 4, s));
   GDB_TRY(fill(sumq, 0, span * 8, s));
   GDB_TRY(fill(cnt, 0, span * 4, s));
-  GDB_TRY(fill(dout, 0, 16, s));
+  GDB_TRY(fill(dout, 0, ACC_BYTES, s));
   if (np)
     q17_part<<<blocks_for(ctx, q17_part, np), BLOCK, 0, s>>>(
         (const int32_t*)pp->dev, (const uint8_t*)pb->dev, (const uint8_t*)pc->dev, (uint32_t)np,
@@ -760,6 +793,7 @@ int gdb_q17(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q17_params* p, gdb_q1
   rs.kernel_ms += rs2.kernel_ms;
   rs.h2d_bytes += rs2.h2d_bytes;
   rs.chunks += rs2.chunks;
+  GDB_TRY(acc_finish(ctx, dout, 0, 2));
   GDB_HIP(hipMemcpyAsync(host, dout, 16, hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
   // SQL columns: l_partkey, l_quantity (4 B), l_extendedprice (8 B); p_partkey,
@@ -794,12 +828,12 @@ int gdb_q22(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q22_params* p, gdb_q2
   uint8_t* has;
   int64_t *dout, *host;
   GDB_TRY(arena(ctx, A_MAP, span, (void**)&has));
-  GDB_TRY(arena(ctx, A_OUT, 256, (void**)&dout));
+  GDB_TRY(arena(ctx, A_OUT, ACC_BYTES, (void**)&dout));
   GDB_TRY(pinned(ctx, 256, (void**)&host));
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
   GDB_TRY(fill(has, 0, span, s));
-  GDB_TRY(fill(dout, 0, 16 * 8, s));
+  GDB_TRY(fill(dout, 0, ACC_BYTES, s));
   GDB_HIP(hipEventRecord(ctx->ev[1], s));
   if (no)
     mark_bytes<<<blocks_for(ctx, mark_bytes, quads(no)), BLOCK, 0, s>>>((const int32_t*)oc->dev,
@@ -809,11 +843,13 @@ int gdb_q22(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q22_params* p, gdb_q2
   if (nc) {
     q22_avg<<<blocks_for(ctx, q22_avg, nc), BLOCK, 0, s>>>(
         (const int32_t*)cn->dev, (const int64_t*)cb->dev, (uint32_t)nc, codes, dout);
+    GDB_TRY(acc_finish(ctx, dout, 0, 2));  // q22_final reads the average's sum and count
     q22_final<<<blocks_for(ctx, q22_final, nc), BLOCK, 0, s>>>(
         (const int32_t*)ck->dev, (const int32_t*)cn->dev, (const int64_t*)cb->dev, (uint32_t)nc,
         codes, has, (uint32_t)span, dout);
   }
   GDB_HIP(hipGetLastError());
+  GDB_TRY(acc_finish(ctx, dout, 2, 16));
   GDB_HIP(hipMemcpyAsync(host, dout, 16 * 8, hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
   // SQL columns: c_custkey, c_nationkey (4 B), c_acctbal (8 B); o_custkey (4 B, the
