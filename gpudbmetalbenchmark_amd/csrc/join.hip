@@ -371,29 +371,60 @@ __global__ __launch_bounds__(256) void join_probe(const int32_t* __restrict__ pr
 // ---------------------------------------------------------------- direct
 constexpr uint64_t DIRECT_MAX_SPAN = 1ull << 31;
 
-// bits[k - lo] for every build key; *dup = 1 when a key value repeats.
+// bits[k - lo] for every build key, with no returning atomics (no lane waits on
+// one). A lane ORs its keys' bits per word; every run but its last is stored at
+// once. The lanes' last runs are then combined across the wave: a doubling OR over
+// neighbouring lanes with the same word (sorted keys give long runs), after which
+// only the first lane of each run stores. Whether a key value repeats is read
+// afterwards from the bitmap's population count (jd_popc).
 __global__ __launch_bounds__(256) void jd_bits(const int32_t* __restrict__ in, uint32_t n,
-                                               int32_t lo, uint32_t* __restrict__ bits,
-                                               uint32_t* __restrict__ dup) {
+                                               int32_t lo, uint32_t* __restrict__ bits) {
   const uint32_t nq = (n + 3) / 4, stride = gridDim.x * blockDim.x;
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += stride) {
-    const int4 v = ld4(in, g);
+  const int lane = threadIdx.x & 63;
+  for (uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; gw < nq; gw += stride) {
+    const uint32_t g = gw + lane;
+    const int4 v = g < nq ? ld4nt(in, g) : make_int4(0, 0, 0, 0);
     const int32_t k[4] = {v.x, v.y, v.z, v.w};
-    uint32_t word = 0, mask = 0, rep = 0;
-    for (int j = 0; j < 4 && 4 * g + j < n; ++j) {  // one atomic per run of one word
+    uint32_t word = 0xFFFFFFFFu, mask = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (4 * g + j >= n) break;
       const uint32_t off = (uint32_t)((int64_t)k[j] - lo);
       const uint32_t w = off >> 5, m = 1u << (off & 31);
       if (mask && w != word) {
-        rep |= atomicOr(&bits[word], mask) & mask;
+        atomicOr(&bits[word], mask);
         mask = 0;
       }
       word = w;
-      rep |= mask & m;
       mask |= m;
     }
-    if (mask) rep |= atomicOr(&bits[word], mask) & mask;
-    if (rep) *dup = 1;
+    const uint32_t mine = word;  // every lane takes part in the shuffles
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t w2 = (uint32_t)__shfl_down((int)mine, d, 64);
+      const uint32_t m2 = (uint32_t)__shfl_down((int)mask, d, 64);
+      if (lane + d < 64 && w2 == mine) mask |= m2;
+    }
+    const uint32_t before = (uint32_t)__shfl_up((int)mine, 1, 64);
+    if (mask && (lane == 0 || before != mine)) atomicOr(&bits[mine], mask);
   }
+}
+
+// Population count of the bitmap: workgroup sums added into JD_POPC_SLOTS
+// counters 256 bytes apart (a few workgroups per counter). The total is below
+// the number of build keys exactly when a key value repeats.
+constexpr int JD_POPC_SLOTS = 64, JD_POPC_STRIDE = 32;  // int64 per slot
+__global__ __launch_bounds__(256) void jd_popc(const uint32_t* __restrict__ bits, uint64_t words,
+                                               int64_t* __restrict__ sums) {
+  __shared__ int64_t red[4];
+  int64_t c = 0;
+  for (uint64_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    c += __popc(bits[i]);
+  const int64_t s = block_sum(c, red);
+  if (threadIdx.x == 0 && s)
+    atomicAdd((unsigned long long*)&sums[(blockIdx.x % JD_POPC_SLOTS) * JD_POPC_STRIDE],
+              (unsigned long long)s);
 }
 
 // counts[k - lo] += 1 for every build key (the repeated-keys form).
@@ -414,25 +445,37 @@ __global__ __launch_bounds__(256) void jd_probe(const int32_t* __restrict__ prob
   __shared__ int64_t red[4];
   const uint32_t nq = (n + 3) / 4, stride = gridDim.x * blockDim.x;
   uint64_t c = 0;
-  auto one = [&](int32_t key, bool valid) {
-    const uint64_t off = (uint64_t)((int64_t)key - lo);
-    if (!valid || off >= span) return;
-    if (COUNTS) c += table[off];
-    else c += (table[off >> 5] >> (off & 31)) & 1u;
-  };
-  auto quad = [&](uint32_t g, const int4& v) {
-    one(v.x, 4 * g < n);
-    one(v.y, 4 * g + 1 < n);
-    one(v.z, 4 * g + 2 < n);
-    one(v.w, 4 * g + 3 < n);
-  };
-  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  for (; g + stride < nq; g += 2 * stride) {
-    const int4 a = ld4nt(probe, g), b = ld4nt(probe, g + stride);
-    quad(g, a);
-    quad(g + stride, b);
+  // QB vector groups per lane per trip: their keys load together, then all 4*QB
+  // table reads are issued (branch-free: a key outside the range reads entry 0
+  // and is masked) before any is summed, so the reads' latencies overlap.
+  constexpr int QB = 4;
+  for (uint32_t g0 = blockIdx.x * blockDim.x + threadIdx.x; g0 < nq; g0 += QB * stride) {
+    int4 v[QB];
+#pragma unroll
+    for (int b = 0; b < QB; ++b) {
+      const uint32_t g = g0 + (uint32_t)b * stride;
+      v[b] = g < nq ? ld4nt(probe, g) : make_int4(0, 0, 0, 0);
+    }
+    uint32_t t[4 * QB];
+    uint32_t ok = 0, sh[4 * QB];
+#pragma unroll
+    for (int b = 0; b < QB; ++b) {
+      const uint32_t g = g0 + (uint32_t)b * stride;
+      const int32_t ks[4] = {v[b].x, v[b].y, v[b].z, v[b].w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint64_t off = (uint64_t)((int64_t)ks[j] - lo);
+        const bool in = g < nq && 4 * g + j < n && off < span;
+        const uint64_t o = in ? off : 0;
+        ok |= (uint32_t)in << (4 * b + j);
+        sh[4 * b + j] = (uint32_t)(o & 31);
+        t[4 * b + j] = COUNTS ? table[o] : table[o >> 5];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4 * QB; ++i)
+      if ((ok >> i) & 1) c += COUNTS ? t[i] : (t[i] >> sh[i]) & 1u;
   }
-  if (g < nq) quad(g, ld4nt(probe, g));
   const int64_t s = block_sum((int64_t)c, red);
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
@@ -537,7 +580,7 @@ extern "C" int gdb_join_count(gdb_ctx* ctx, const gdb_table* build, const char* 
   hipStream_t st = ctx->stream;
   int64_t *dout, *host;
   GDB_TRY(arena(ctx, A_OUT, 64, (void**)&dout));
-  GDB_TRY(pinned(ctx, 64, (void**)&host));
+  GDB_TRY(pinned(ctx, 64 * 8, (void**)&host));  // result, or the bitmap's popcount partials
   int64_t* partials;
   uint32_t nparts = 0;
 
@@ -545,8 +588,10 @@ extern "C" int gdb_join_count(gdb_ctx* ctx, const gdb_table* build, const char* 
     const int32_t lo = (int32_t)cb->minv;
     const uint64_t words = (span + 31) / 32;
     uint32_t* table;
-    GDB_TRY(arena(ctx, A_HT, (words + 16) * 4, (void**)&table));
-    uint32_t* dup = table + words;
+    const uint64_t popc_at = (words + 63) & ~uint64_t(63);  // 256-byte aligned
+    const size_t popc_bytes = (size_t)JD_POPC_SLOTS * JD_POPC_STRIDE * 8;
+    GDB_TRY(arena(ctx, A_HT, popc_at * 4 + popc_bytes, (void**)&table));
+    int64_t* popc = (int64_t*)(table + popc_at);
     int blocks = resident_blocks(ctx, (const void*)jd_probe<false>, 256, 0,
                                  ctx->tune[T_JOIN].blocks_per_cu);
     blocks = std::max(1, std::min<int>(blocks, (int)((np / 4 + 255) / 256)));
@@ -554,13 +599,20 @@ extern "C" int gdb_join_count(gdb_ctx* ctx, const gdb_table* build, const char* 
     GDB_TRY(arena(ctx, A_PARTIALS, sizeof(int64_t) * nparts, (void**)&partials));
     const int fill = ctx->info.cu_count * 4;
     GDB_HIP(hipEventRecord(ctx->ev[0], st));
-    GDB_TRY(gdb::fill(table, 0, (words + 16) * 4, st));
-    jd_bits<<<fill, 256, 0, st>>>(rk, nb, lo, table, dup);
+    GDB_TRY(gdb::fill(table, 0, popc_at * 4 + popc_bytes, st));
+    const uint64_t bq = (nb + 3) / 4;
+    jd_bits<<<(unsigned)std::max<uint64_t>(1, std::min<uint64_t>((bq + 255) / 256, 1u << 16)), 256,
+              0, st>>>(rk, nb, lo, table);
+    jd_popc<<<(unsigned)std::max<uint64_t>(1, std::min<uint64_t>((words + 1023) / 1024, 1024)), 256,
+              0, st>>>(table, words, popc);
     GDB_HIP(hipGetLastError());
-    uint32_t* hdup = (uint32_t*)host;
-    GDB_HIP(hipMemcpyAsync(hdup, dup, 4, hipMemcpyDeviceToHost, st));
+    int64_t* hpop = (int64_t*)host;
+    GDB_HIP(hipMemcpy2DAsync(hpop, 8, popc, JD_POPC_STRIDE * 8, 8, JD_POPC_SLOTS,
+                             hipMemcpyDeviceToHost, st));
     GDB_HIP(hipStreamSynchronize(st));
-    const bool counts = *hdup != 0;
+    int64_t distinct = 0;
+    for (int i = 0; i < JD_POPC_SLOTS; ++i) distinct += hpop[i];
+    const bool counts = (uint64_t)distinct != nb;  // a key value repeats
     uint32_t* cnt = table;
     if (counts) {  // repeated build keys: a count per key value
       GDB_TRY(arena(ctx, A_JOINPART, span * 4, (void**)&cnt));
