@@ -354,12 +354,38 @@ __global__ __launch_bounds__(256) void q20_part(const int32_t* __restrict__ pk,
   }
 }
 
+// The pair table's size is set on the device from the pair count (at least
+// twice the pairs, a power of two), so the host does not wait for the count.
+__global__ void q20_size(const unsigned long long* __restrict__ count,
+                         uint64_t* __restrict__ slots) {
+  const uint64_t want = 2 * (*count > 0 ? *count : 1ull);
+  uint64_t s = 1;
+  while (s < want) s <<= 1;
+  *slots = s;
+}
+
+// Clears the first *slots entries of the pair table.
+__global__ __launch_bounds__(256) void q20_clear(const uint64_t* __restrict__ slots,
+                                                 unsigned long long* __restrict__ keys,
+                                                 int64_t* __restrict__ sum,
+                                                 uint32_t* __restrict__ lines) {
+  const uint64_t ns = *slots;
+  for (uint64_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ns;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    keys[i] = EMPTY;
+    sum[i] = 0;
+    lines[i] = 0u;
+  }
+}
+
 // Inserts the partsupp pairs of selected parts; row[slot] = partsupp row.
 __global__ __launch_bounds__(256) void q20_build(const int32_t* __restrict__ pk,
                                                  const int32_t* __restrict__ sk, uint32_t n,
                                                  const uint32_t* __restrict__ bits,
                                                  unsigned long long* __restrict__ keys,
-                                                 uint64_t mask, uint32_t* __restrict__ row) {
+                                                 const uint64_t* __restrict__ slots,
+                                                 uint32_t* __restrict__ row) {
+  const uint64_t mask = *slots - 1;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int32_t p = pk[i];
     if (!bit_test(bits, (uint32_t)p)) continue;
@@ -388,8 +414,10 @@ __global__ __launch_bounds__(256) void q20_scan(const int32_t* __restrict__ ship
                                                 int32_t lo, int32_t hi,
                                                 const uint32_t* __restrict__ bits,
                                                 const unsigned long long* __restrict__ keys,
-                                                uint64_t mask, int64_t* __restrict__ sum,
+                                                const uint64_t* __restrict__ slots,
+                                                int64_t* __restrict__ sum,
                                                 uint32_t* __restrict__ lines) {
+  const uint64_t mask = *slots - 1;
   walk2(quads(n), [&](uint32_t g) { return ld4nt(ship, g); }, [&](uint32_t g, const int4& d) {
     uint32_t m = 0;
 #pragma unroll
@@ -419,11 +447,13 @@ __global__ __launch_bounds__(256) void q20_scan(const int32_t* __restrict__ ship
 
 // good[suppkey] = 1 for pairs with lines and availqty * den > sum * num.
 __global__ __launch_bounds__(256) void q20_pairs(const unsigned long long* __restrict__ keys,
-                                                 uint64_t slots, const uint32_t* __restrict__ row,
+                                                 const uint64_t* __restrict__ dslots,
+                                                 const uint32_t* __restrict__ row,
                                                  const int64_t* __restrict__ sum,
                                                  const uint32_t* __restrict__ lines,
                                                  const int32_t* __restrict__ avail, int64_t num,
                                                  int64_t den, uint8_t* __restrict__ good) {
+  const uint64_t slots = *dslots;
   for (uint64_t i = blockIdx.x * blockDim.x + threadIdx.x; i < slots;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const unsigned long long k = keys[i];
@@ -829,11 +859,11 @@ int gdb_q20(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q20_params* p, int32_
   uint32_t* bits;
   uint8_t* good;
   uint64_t* dout;
-  uint64_t* host;
+  uint64_t* hall;  // the count, then at most one key per supplier
   GDB_TRY(arena(ctx, A_BITMAP, words * 4, (void**)&bits));
   GDB_TRY(arena(ctx, A_AUX2, sspan, (void**)&good));
   GDB_TRY(arena(ctx, A_OUT, ACC_BYTES, (void**)&dout));
-  GDB_TRY(pinned(ctx, 64, (void**)&host));
+  GDB_TRY(pinned(ctx, 16 + std::max<uint64_t>(ns, 1) * 4, (void**)&hall));
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
   GDB_TRY(fill(bits, 0, words * 4, s));
@@ -847,53 +877,55 @@ int gdb_q20(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q20_params* p, int32_
         (const int32_t*)qp->dev, (uint32_t)nps, bits, (unsigned long long*)dout);
   GDB_HIP(hipGetLastError());
   GDB_TRY(acc_finish(ctx, (int64_t*)dout, 0, 1));
-  GDB_HIP(hipMemcpyAsync(host, dout, 8, hipMemcpyDeviceToHost, s));
-  GDB_HIP(hipStreamSynchronize(s));
-  const uint64_t slots = pow2_at_least(2 * std::max<uint64_t>(host[0], 1));
+  // The pair table is allocated for every partsupp row being a pair; its used
+  // size comes from the count on the device (dout[2]).
+  const uint64_t smax = pow2_at_least(2 * std::max<uint64_t>(nps, 1));
+  uint64_t* dslots = dout + 2;
+  q20_size<<<1, 1, 0, s>>>((const unsigned long long*)dout, dslots);
   unsigned long long* keys;
   uint32_t *row, *lines;
   int64_t* sum;
   int32_t* out_keys;
-  GDB_TRY(arena(ctx, A_HT, slots * 8, (void**)&keys));
-  GDB_TRY(arena(ctx, A_AGG, slots * 8, (void**)&sum));
-  GDB_TRY(arena(ctx, A_AUX, slots * 8, (void**)&row));
-  lines = row + slots;
+  GDB_TRY(arena(ctx, A_HT, smax * 8, (void**)&keys));
+  GDB_TRY(arena(ctx, A_AGG, smax * 8, (void**)&sum));
+  GDB_TRY(arena(ctx, A_AUX, smax * 8, (void**)&row));
+  lines = row + smax;
   GDB_TRY(arena(ctx, A_MAP, std::max<uint64_t>(ns, 1) * 4, (void**)&out_keys));
-  GDB_TRY(fill(keys, 0xFF, slots * 8, s));
-  GDB_TRY(fill(sum, 0, slots * 8, s));
-  GDB_TRY(fill(lines, 0, slots * 4, s));
+  q20_clear<<<blocks_for(ctx, q20_clear, smax), BLOCK, 0, s>>>(dslots, keys, sum, lines);
   if (nps)
     q20_build<<<blocks_for(ctx, q20_build, nps), BLOCK, 0, s>>>(
-        (const int32_t*)qp->dev, (const int32_t*)qs->dev, (uint32_t)nps, bits, keys, slots - 1,
+        (const int32_t*)qp->dev, (const int32_t*)qs->dev, (uint32_t)nps, bits, keys, dslots,
         row);
   GDB_HIP(hipGetLastError());
   RunStats rs;
   GDB_TRY(scan_table(ctx, db->lineitem, {ls, lp, lk, lq}, [&](const void* const* d, uint32_t rows) {
     q20_scan<<<blocks_for(ctx, q20_scan, quads(rows)), BLOCK, 0, s>>>(
         (const int32_t*)d[0], (const int32_t*)d[1], (const int32_t*)d[2], (const int32_t*)d[3],
-        rows, p->date_lo, p->date_hi, bits, keys, slots - 1, sum, lines);
+        rows, p->date_lo, p->date_hi, bits, keys, dslots, sum, lines);
     GDB_HIP(hipGetLastError());
     return GDB_OK;
   }, &rs));
-  q20_pairs<<<blocks_for(ctx, q20_pairs, slots), BLOCK, 0, s>>>(
-      keys, slots, row, sum, lines, (const int32_t*)qa->dev, p->frac_num, p->frac_den, good);
+  q20_pairs<<<blocks_for(ctx, q20_pairs, smax), BLOCK, 0, s>>>(
+      keys, dslots, row, sum, lines, (const int32_t*)qa->dev, p->frac_num, p->frac_den, good);
   if (ns)
     q20_supp<<<blocks_for(ctx, q20_supp, ns), BLOCK, 0, s>>>(
         (const int32_t*)sk->dev, (const int32_t*)sn->dev, (uint32_t)ns, p->nation, good,
         out_keys, (unsigned long long*)(dout + 1));
   GDB_HIP(hipGetLastError());
-  GDB_HIP(hipMemcpyAsync(host, dout, 16, hipMemcpyDeviceToHost, s));
-  GDB_HIP(hipStreamSynchronize(s));
-  const uint64_t n = host[1];
-  std::vector<int32_t> got(n);
-  if (n) GDB_HIP(hipMemcpyAsync(got.data(), out_keys, n * 4, hipMemcpyDeviceToHost, s));
+  // The count and every possible supplier key (at most one per supplier) come back
+  // in one copy into pinned memory: no wait for the count before the keys.
+  GDB_HIP(hipMemcpyAsync(hall, dout, 16, hipMemcpyDeviceToHost, s));
+  if (ns)
+    GDB_HIP(hipMemcpyAsync(hall + 2, out_keys, ns * 4, hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
   // SQL columns: lineitem shipdate, partkey, suppkey, quantity (16 B); partsupp key
   // pair + availqty (12 B); part key + name; supplier key + nation. Mandatory in
   // q20_scan: l_shipdate, 4 B per line.
   GDB_TRY(timing_finish(ctx, timing, 16 * nl + 12 * nps + (4 + pn->width) * np + 8 * ns,
                         4 * nl, &rs));
-  std::sort(got.begin(), got.end());
+  const uint64_t n = hall[1];
+  int32_t* got = (int32_t*)(hall + 2);
+  std::sort(got, got + n);
   *nrows = (int64_t)n;
   for (int64_t i = 0; i < std::min<int64_t>(cap, (int64_t)n); ++i) suppkeys[i] = got[i];
   return GDB_OK;
