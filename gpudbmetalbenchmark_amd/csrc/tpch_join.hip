@@ -624,14 +624,23 @@ template <class St>
 __device__ __forceinline__ St wave_carry(const LaneRuns<St>& lr, const St& ext, int lane) {
   St v = lr.lead;
   bool closed = !lr.pass;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
+  auto step = [&](int d) {
     const St v2 = v.shfl_down(d);
     const int c2 = __shfl_down((int)closed, d, 64);
     if (lane + d < 64 && !closed) {
       v = St::merge(v, v2);
       closed = c2 != 0;
     }
+  };
+  // After distances 1 and 2 a lane's fold covers 4 lanes (16 rows): every run is
+  // closed or reaches the wave's end unless an order spans more, which TPC-H's
+  // 1..7 lines never do. The longer distances run only when a lane needs them
+  // (decided for the whole wave).
+  step(1);
+  step(2);
+  if (__any(!closed && lane + 4 < 64)) {
+#pragma unroll
+    for (int d = 4; d < 64; d <<= 1) step(d);
   }
   if (!closed) v = St::merge(v, ext);
   const St nxt = v.shfl_down(1);
