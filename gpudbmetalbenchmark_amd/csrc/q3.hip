@@ -269,30 +269,23 @@ __global__ __launch_bounds__(256) void q3_probe(Q3Fact f, uint32_t n, int32_t da
 }
 
 // Top-k, step 1: histogram of the groups' revenue buckets (LDS per block,
-// merged by global atomics) and the group count.
+// merged by global atomics). Its total is the group count (step 2 records it).
 __global__ __launch_bounds__(256) void q3_topk_hist(Q3Scratch s) {
   __shared__ uint32_t lh[4096];
-  __shared__ int64_t red[4];
   for (int i = threadIdx.x; i < 4096; i += blockDim.x) lh[i] = 0u;
   __syncthreads();
   const uint32_t count = s.counters[0];
   const uint32_t stride = gridDim.x * blockDim.x;
-  uint32_t hits = 0;
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += stride) {
-    if (s.hit[j]) {
-      atomicAdd(&lh[value_bucket(s.rev[j])], 1u);
-      ++hits;
-    }
-  }
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += stride)
+    if (s.hit[j]) atomicAdd(&lh[value_bucket(s.rev[j])], 1u);
   __syncthreads();
   for (int i = threadIdx.x; i < 4096; i += blockDim.x)
     if (lh[i]) atomicAdd(&s.hist[i], lh[i]);
-  const int64_t bh = block_sum((int64_t)hits, red);
-  if (threadIdx.x == 0 && bh) atomicAdd(&s.counters[1], (uint32_t)bh);
 }
 
 // Step 2 (one block): the highest bucket b such that buckets >= b hold at
-// least k groups (0 when there are fewer than k groups).
+// least k groups (0 when there are fewer than k groups), and the group count
+// (every group with revenue is in exactly one bucket).
 __global__ __launch_bounds__(256) void q3_topk_threshold(int k, Q3Scratch s) {
   __shared__ uint32_t part[256];
   uint32_t v = 0;
@@ -314,6 +307,9 @@ __global__ __launch_bounds__(256) void q3_topk_threshold(int k, Q3Scratch s) {
       }
       acc += part[t];
     }
+    uint32_t groups = 0;
+    for (int t = 0; t < 256; ++t) groups += part[t];
+    s.counters[1] = groups;
     s.counters[2] = b;
     s.counters[3] = 0u;
   }
