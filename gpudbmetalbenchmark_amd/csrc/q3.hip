@@ -476,7 +476,7 @@ extern "C" int gdb_q3(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* 
     GDB_HIP(hipEventRecord(ctx->ev[1], st));
     GDB_HIP(hipEventRecord(ctx->ev[2], st));
   }
-  q3_topk_hist<<<fill_blocks, B, 0, st>>>(s);
+  q3_topk_hist<<<ctx->info.cu_count, B, 0, st>>>(s);  // one per CU: fewer bin merges
   q3_topk_threshold<<<1, B, 0, st>>>(k, s);
   q3_topk_collect<<<fill_blocks, B, 0, st>>>(s);
   q3_topk_final<<<1, B, 0, st>>>(k, s);
