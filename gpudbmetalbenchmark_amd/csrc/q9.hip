@@ -174,8 +174,15 @@ __global__ __launch_bounds__(256) void q9_orders(const int32_t* __restrict__ ok,
                                                  const int32_t* __restrict__ od, uint32_t n,
                                                  uint32_t maxok, Years years,
                                                  uint8_t* __restrict__ oyear) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && (uint32_t)ok[i] <= maxok) oyear[ok[i]] = (uint8_t)year_index(od[i], years);
+  // One vector group of 4 orders per thread (dwordx4 loads of key and date).
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (n + 3) / 4) return;
+  const int4 k = ld4nt(ok, g), d = ld4nt(od, g);
+  const int32_t ks[4] = {k.x, k.y, k.z, k.w}, ds[4] = {d.x, d.y, d.z, d.w};
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    if (4 * g + j < n && (uint32_t)ks[j] <= maxok)
+      oyear[ks[j]] = (uint8_t)year_index(ds[j], years);
 }
 
 // Partsupp rows whose part passed -> compact list in row order, one atomic per
@@ -579,7 +586,7 @@ extern "C" int gdb_q9(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supp
     q9_supplier<<<(ns + B - 1) / B, B, 0, st>>>((const int32_t*)ssk->dev, (const int32_t*)snk->dev,
                                                  ns, maxsk, s.snat);
   if (no)
-    q9_orders<<<(no + B - 1) / B, B, 0, st>>>((const int32_t*)ook->dev, (const int32_t*)ood->dev, no,
+    q9_orders<<<((no + 3) / 4 + B - 1) / B, B, 0, st>>>((const int32_t*)ook->dev, (const int32_t*)ood->dev, no,
                                                maxok, years, s.oyear);
   if (nps)
     q9_partsupp<<<std::min<uint32_t>(ctx->info.cu_count * 8, (nps + 16 * B - 1) / (16 * B)), B, 0, st>>>(
