@@ -227,114 +227,159 @@ __device__ __forceinline__ void block_add(int64_t v, int64_t* red, int64_t* out)
   if (threadIdx.x == 0 && s) add_i64(out, s);
 }
 
-// Two-stage top-k source over a candidate function: stage 1 runs a grid of
-// workgroups over [0, n), each keeping its k best of a contiguous slice, and
-// counts valid candidates; stage 2 runs one workgroup over the winners. With
-// `dn`, n is cut to *dn (a count another kernel wrote). A thread evaluates
-// TOPK_U candidates before pushing any, so their loads are in flight together:
-// sources load unconditionally (every index below n is valid) and return the
-// verdict, with no early exit.
-constexpr int TOPK_U = 4;
+// Top-k sources are functors `bool (uint64_t i, gdbk::TopEntry& e)`: candidate i's
+// entry and whether it is valid. They load unconditionally (every index below n
+// is valid) and return the verdict, with no early exit.
+
+// Threshold top-k: a histogram of the candidates' value buckets (gdbk::value_bucket,
+// monotone in the revenue), the bucket b the k-th best falls in, then only the
+// candidates in buckets >= b are collected and ranked by one workgroup. The
+// histogram's total is the number of valid candidates. If heavy ties in bucket b
+// overflow the candidate list, the ranking workgroup scans every candidate.
+constexpr uint32_t THK_BUCKETS = 4096, THK_CAP = 8192;
+struct ThkState {   // device memory, cleared before thk_hist
+  uint32_t hist[THK_BUCKETS];
+  uint32_t thr, ncand;
+  unsigned long long total;
+};
 
 template <class Src>
-__global__ __launch_bounds__(256) void topk_stage1(Src src, uint64_t n,
-                                                   const uint32_t* __restrict__ dn, int k,
-                                                   gdbk::TopEntry* __restrict__ out,
-                                                   unsigned long long* __restrict__ count) {
-  __shared__ gdbk::TopEntry buf[1024];
-  __shared__ gdbk::TopEntry thr;
-  __shared__ int nbuf, have;
-  __shared__ int64_t red[4];
-  gdbk::BlockTopK<1024> tk{buf, &nbuf, k, &thr, &have};
-  tk.init();
-  count = acc_copy(count);  // this workgroup's copy of the count
+__global__ __launch_bounds__(256) void thk_hist(Src src, uint64_t n, const uint32_t* __restrict__ dn,
+                                                ThkState* __restrict__ st) {
+  __shared__ uint32_t lh[THK_BUCKETS];
   if (dn) n = n < *dn ? n : *dn;
-  const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
-  const uint64_t begin = (uint64_t)blockIdx.x * per;
-  const uint64_t end = begin + per < n ? begin + per : n;
-  int64_t cnt = 0;
-  for (uint64_t base = begin; base < end; base += TOPK_U * blockDim.x) {
-    gdbk::TopEntry e[TOPK_U];
-    bool v[TOPK_U];
-#pragma unroll
-    for (int u = 0; u < TOPK_U; ++u) {
-      const uint64_t i = base + (uint64_t)u * blockDim.x + threadIdx.x;
-      const bool in = i < end;
-      v[u] = src(in ? i : begin, e[u]) && in;
-    }
-#pragma unroll
-    for (int u = 0; u < TOPK_U; ++u) {
-      cnt += v[u];
-      tk.push(v[u], e[u]);
-    }
+  for (uint32_t i = threadIdx.x; i < THK_BUCKETS; i += blockDim.x) lh[i] = 0u;
+  __syncthreads();
+  for (uint64_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    gdbk::TopEntry e;
+    if (src(i, e)) atomicAdd(&lh[gdbk::value_bucket(e.rev)], 1u);
   }
-  tk.finish(out + (size_t)blockIdx.x * k);
-  const int64_t c = gdbk::block_sum(cnt, red);
-  if (threadIdx.x == 0 && c) atomicAdd(count, (unsigned long long)c);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < THK_BUCKETS; i += blockDim.x)
+    if (lh[i]) atomicAdd(&st->hist[i], lh[i]);
 }
 
-// Stage 2, one workgroup over `lists` (at most TOPK_MAX_BLOCKS) best-first lists
-// of k entries (stage 1's winners, worst-padded): rank 0 of every list, then
-// rank 1, and so on; a thread loads its lists' entries of a rank together before
-// pushing them. When no entry of a rank is valid and ranks before the running
-// k-th best, no later rank can (each list only gets worse), so the merge stops.
-constexpr int TOPK_MAX_BLOCKS = 2048;
 namespace {
-__global__ __launch_bounds__(256) void topk_merge(const gdbk::TopEntry* __restrict__ w, int lists,
-                                                  int k, gdbk::TopEntry* __restrict__ out) {
-  constexpr int PER = TOPK_MAX_BLOCKS / 256;
-  __shared__ gdbk::TopEntry buf[1024];
-  __shared__ gdbk::TopEntry thr;
-  __shared__ int nbuf, have, any;
-  gdbk::BlockTopK<1024> tk{buf, &nbuf, k, &thr, &have};
-  tk.init();
-  for (int r = 0; r < k; ++r) {
-    if (threadIdx.x == 0) any = 0;
-    gdbk::TopEntry e[PER];
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int b = u * (int)blockDim.x + (int)threadIdx.x;
-      e[u] = b < lists ? w[(size_t)b * k + r] : gdbk::worst_entry();
+// One workgroup of 256: the highest bucket b with at least k candidates in
+// buckets >= b (0 when there are fewer than k), and the candidate total.
+__global__ __launch_bounds__(256) void thk_threshold(int k, ThkState* __restrict__ st,
+                                                     unsigned long long* __restrict__ count) {
+  __shared__ uint32_t part[256];
+  uint32_t v = 0;
+  for (int i = 0; i < 16; ++i) v += st->hist[threadIdx.x * 16 + i];
+  part[threadIdx.x] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0, b = 0;
+    for (int t = 255; t >= 0; --t) {
+      if (acc + part[t] >= (uint32_t)k) {
+        for (int i = 15; i >= 0; --i) {
+          acc += st->hist[t * 16 + i];
+          if (acc >= (uint32_t)k) {
+            b = (uint32_t)(t * 16 + i);
+            break;
+          }
+        }
+        break;
+      }
+      acc += part[t];
     }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      if (u * (int)blockDim.x >= lists) break;  // uniform
-      const bool v = !(e[u].rev == INT64_MIN && e[u].key == INT32_MAX);
-      if (v && !(have && !gdbk::better(e[u], thr))) any = 1;
-      tk.push(v, e[u]);  // ends with a barrier: `any` and thr are settled after it
-    }
-    const int go = any;
-    __syncthreads();  // every thread has read `any` before the next reset
-    if (!go) break;
+    unsigned long long total = 0;
+    for (int t = 0; t < 256; ++t) total += part[t];
+    st->thr = b;
+    st->ncand = 0u;
+    st->total = total;
+    if (count) *count = total;
   }
-  tk.finish(out);
 }
 }  // namespace
 
-// Stage-1 workgroups for n candidates: about 2048 each, at most TOPK_MAX_BLOCKS
-// (the winners buffer holds that many lists of k). A count known only on the
-// device (`counted`: n is then a bound) gets 256 workgroups.
-inline int topk_blocks(uint64_t n, bool counted = false) {
-  if (counted) return 256;
-  const uint64_t b = (n + 2047) / 2048;
-  return (int)(b < 1 ? 1 : b > (uint64_t)TOPK_MAX_BLOCKS ? TOPK_MAX_BLOCKS : b);
+// Candidates in buckets >= thr -> cand (one atomic per wave and trip).
+template <class Src>
+__global__ __launch_bounds__(256) void thk_collect(Src src, uint64_t n,
+                                                   const uint32_t* __restrict__ dn,
+                                                   ThkState* __restrict__ st,
+                                                   gdbk::TopEntry* __restrict__ cand) {
+  if (dn) n = n < *dn ? n : *dn;
+  const uint32_t thr = st->thr;
+  const int lane = threadIdx.x & 63;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t w0 = (blockIdx.x * blockDim.x + threadIdx.x) & ~63ull; w0 < n; w0 += stride) {
+    const uint64_t i = w0 + lane;
+    gdbk::TopEntry e;
+    const bool q = i < n && src(i, e) && gdbk::value_bucket(e.rev) >= thr;
+    const unsigned long long m = __ballot(q);
+    if (!m) continue;
+    const int leader = __ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&st->ncand, (uint32_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    const uint32_t pos = base + __popcll(m & ((1ull << lane) - 1));
+    if (q && pos < THK_CAP) cand[pos] = e;
+  }
 }
 
-// Runs both stages over topk_blocks(n, dn != nullptr) workgroups (`winners`
-// holds that many lists of k); the k best (worst-padded) land in `final_out`.
+// One workgroup ranks the candidates (or, after an overflow, every candidate in
+// buckets >= thr) into out[0, k).
 template <class Src>
-int topk_run(gdb_ctx* ctx, Src src, uint64_t n, int k, gdbk::TopEntry* winners,
-             gdbk::TopEntry* final_out, unsigned long long* count, const uint32_t* dn = nullptr) {
+__global__ __launch_bounds__(256) void thk_final(Src src, uint64_t n, const uint32_t* __restrict__ dn,
+                                                 int k, const ThkState* __restrict__ st,
+                                                 const gdbk::TopEntry* __restrict__ cand,
+                                                 gdbk::TopEntry* __restrict__ out) {
+  __shared__ gdbk::TopEntry buf[1024];
+  __shared__ gdbk::TopEntry thr;
+  __shared__ int nbuf, have;
+  gdbk::BlockTopK<1024> tk{buf, &nbuf, k, &thr, &have};
+  tk.init();
+  if (dn) n = n < *dn ? n : *dn;
+  const uint32_t nc = st->ncand;
+  if (nc <= THK_CAP) {
+    for (uint32_t b0 = 0; b0 < nc; b0 += blockDim.x) {
+      const uint32_t i = b0 + threadIdx.x;
+      const bool v = i < nc;
+      tk.push(v, v ? cand[i] : gdbk::worst_entry());
+    }
+  } else {
+    const uint32_t tb = st->thr;
+    for (uint64_t b0 = 0; b0 < n; b0 += blockDim.x) {
+      const uint64_t i = b0 + threadIdx.x;
+      gdbk::TopEntry e = gdbk::worst_entry();
+      const bool v = i < n && src(i, e) && gdbk::value_bucket(e.rev) >= tb;
+      tk.push(v, e);
+    }
+  }
+  tk.finish(out);
+}
+
+// The state and the THK_CAP candidate entries, in the arena slot `slot`.
+inline int thk_buffers(gdb_ctx* ctx, int slot, ThkState** state, gdbk::TopEntry** cand) {
+  constexpr size_t head = (sizeof(ThkState) + 255) & ~size_t(255);
+  char* p;
+  GDB_TRY(gdb::arena(ctx, slot, head + sizeof(gdbk::TopEntry) * (size_t)THK_CAP, (void**)&p));
+  *state = (ThkState*)p;
+  *cand = (gdbk::TopEntry*)(p + head);
+  return GDB_OK;
+}
+
+// Runs the four steps; `state` points to a ThkState and `cand` to THK_CAP
+// entries (device memory). *count receives the number of valid candidates. With
+// `dn`, n is cut to *dn (a count another kernel wrote). The histogram takes one
+// workgroup per 4096 candidates, at most four per CU (its LDS merge adds only
+// the buckets a workgroup touched).
+template <class Src>
+int thk_run(gdb_ctx* ctx, Src src, uint64_t n, int k, ThkState* state, gdbk::TopEntry* cand,
+            gdbk::TopEntry* final_out, unsigned long long* count, const uint32_t* dn = nullptr) {
   hipStream_t s = ctx->stream;
-  const int blocks = topk_blocks(n, dn != nullptr);
-  int64_t* acc;  // per-workgroup copies of the candidate count
-  GDB_TRY(gdb::arena(ctx, gdb::A_ACC, ACC_BYTES, (void**)&acc));
-  GDB_TRY(gdb::fill(acc, 0, ACC_BYTES, s));
-  topk_stage1<Src><<<blocks, BLOCK, 0, s>>>(src, n, dn, k, winners, (unsigned long long*)acc);
-  GDB_HIP(hipGetLastError());
-  GDB_TRY(acc_finish(ctx, acc, 0, 1, 0, 0, (int64_t*)count));
-  topk_merge<<<1, BLOCK, 0, s>>>(winners, blocks, k, final_out);
+  GDB_TRY(gdb::fill(state, 0, sizeof(ThkState), s));
+  const int hb = (int)std::max<uint64_t>(
+      1, std::min<uint64_t>(4ull * ctx->info.cu_count, (n + 4095) / 4096));
+  thk_hist<Src><<<hb, BLOCK, 0, s>>>(src, n, dn, state);
+  thk_threshold<<<1, 256, 0, s>>>(std::max(k, 1), state, count);
+  if (k > 0) {
+    thk_collect<Src><<<hb * 4, BLOCK, 0, s>>>(src, n, dn, state, cand);
+    thk_final<Src><<<1, BLOCK, 0, s>>>(src, n, dn, k, state, cand, final_out);
+  }
   GDB_HIP(hipGetLastError());
   return GDB_OK;
 }

@@ -1135,13 +1135,11 @@ int gdb_q10(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q10_params* p, gdb_q1
   GDB_TRY(key_span(oc, no, &a));
   cspan = std::max(cspan, a);
   const int k = p->top_k;
-  const int tblocks = topk_blocks(cspan);
   int32_t* ocust;
   uint32_t* owin;
   const uint64_t wwords = ospan / 32 + 1;
   int64_t* crev;
   uint8_t *seen, *rank;
-  TopEntry* win;
   uint64_t* dout;
   char* host;
   GDB_TRY(arena(ctx, A_MAP, ospan * 4, (void**)&ocust));
@@ -1150,8 +1148,9 @@ int gdb_q10(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q10_params* p, gdb_q1
   GDB_TRY(arena(ctx, A_AGG, cspan * 8, (void**)&crev));
   GDB_TRY(arena(ctx, A_AUX, cspan * 2, (void**)&seen));
   rank = seen + cspan;
-  GDB_TRY(arena(ctx, A_AUX2, sizeof(TopEntry) * (size_t)(tblocks + 1) * std::max(k, 1),
-                (void**)&win));
+  ThkState* thk;  // top-k by revenue histogram
+  TopEntry* cand;
+  GDB_TRY(thk_buffers(ctx, A_AUX2, &thk, &cand));
   // dout: [0] groups, [1] dummy, then k TopEntry, k balances, k nations
   const size_t out_bytes = 16 + (size_t)k * (sizeof(TopEntry) + 12) + 64;
   GDB_TRY(arena(ctx, A_OUT, out_bytes, (void**)&dout));
@@ -1186,7 +1185,8 @@ int gdb_q10(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q10_params* p, gdb_q1
     return GDB_OK;
   }, &rs));
   if (k > 0) {
-    GDB_TRY(topk_run(ctx, KeyedRevSrc{crev, seen}, cspan, k, win, fin, (unsigned long long*)dout));
+    GDB_TRY(thk_run(ctx, KeyedRevSrc{crev, seen}, cspan, k, thk, cand, fin,
+                    (unsigned long long*)dout));
     mark_ranks<<<1, 128, 0, s>>>(fin, k, rank);
     if (nc)
       gather_customers<<<blocks_for(ctx, gather_customers, nc), BLOCK, 0, s>>>(
@@ -1194,8 +1194,8 @@ int gdb_q10(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q10_params* p, gdb_q1
           (uint32_t)nc, rank, (uint32_t)cspan, bal, nat);
     GDB_HIP(hipGetLastError());
   } else {
-    GDB_TRY(topk_run(ctx, KeyedRevSrc{crev, seen}, cspan, 1, win, win + tblocks,
-                     (unsigned long long*)dout));
+    GDB_TRY(thk_run(ctx, KeyedRevSrc{crev, seen}, cspan, 0, thk, cand, nullptr,
+                    (unsigned long long*)dout));
   }
   GDB_HIP(hipMemcpyAsync(host, dout, out_bytes, hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
@@ -1246,13 +1246,13 @@ int gdb_q18(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q18_params* p, gdb_q1
   const uint64_t cap = std::max<uint64_t>(1, std::min<uint64_t>(nl, ospan));
   int32_t* sq = nullptr;
   int2* cand = nullptr;
-  TopEntry* win;
+  ThkState* thk;  // top-k by totalprice histogram
+  TopEntry* tcand;
   uint64_t* dout;
   char* host;
   if (emit) GDB_TRY(arena(ctx, A_MAP, cap * sizeof(int2), (void**)&cand));
   else GDB_TRY(arena(ctx, A_MAP, ospan * 4, (void**)&sq));
-  const int tblocks = emit ? topk_blocks(cap, true) : topk_blocks(no);
-  GDB_TRY(arena(ctx, A_AUX2, sizeof(TopEntry) * (size_t)tblocks * k, (void**)&win));
+  GDB_TRY(thk_buffers(ctx, A_AUX2, &thk, &tcand));
   const size_t out_bytes = 16 + (size_t)k * (sizeof(TopEntry) + 8) + 64;
   GDB_TRY(arena(ctx, A_OUT, out_bytes, (void**)&dout));
   GDB_TRY(pinned(ctx, out_bytes, (void**)&host));
@@ -1278,15 +1278,15 @@ int gdb_q18(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q18_params* p, gdb_q1
     return GDB_OK;
   }, &rs));
   if (emit)
-    GDB_TRY(topk_run(ctx,
-                     Q18CandSrc{cand, (const int32_t*)oo->dev, (const int32_t*)od->dev,
-                                (const int64_t*)ot->dev, (uint32_t)no},
-                     cap, k, win, fin, (unsigned long long*)dout, ncand));
+    GDB_TRY(thk_run(ctx,
+                    Q18CandSrc{cand, (const int32_t*)oo->dev, (const int32_t*)od->dev,
+                               (const int64_t*)ot->dev, (uint32_t)no},
+                    cap, k, thk, tcand, fin, (unsigned long long*)dout, ncand));
   else
-    GDB_TRY(topk_run(ctx,
-                     Q18Src{(const int32_t*)oo->dev, (const int32_t*)od->dev,
-                            (const int64_t*)ot->dev, sq, p->quantity},
-                     no, k, win, fin, (unsigned long long*)dout));
+    GDB_TRY(thk_run(ctx,
+                    Q18Src{(const int32_t*)oo->dev, (const int32_t*)od->dev,
+                           (const int64_t*)ot->dev, sq, p->quantity},
+                    no, k, thk, tcand, fin, (unsigned long long*)dout));
   q18_gather<<<1, 128, 0, s>>>(fin, k, (const int32_t*)oc->dev, sq, extra);
   GDB_HIP(hipGetLastError());
   GDB_HIP(hipMemcpyAsync(host, dout, out_bytes, hipMemcpyDeviceToHost, s));
@@ -1334,19 +1334,19 @@ int gdb_q21(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q21_params* p, gdb_q2
   GDB_TRY(key_span(lk, nl, &a));
   sspan = std::max(sspan, a);
   const int k = std::max(p->top_k, 1);
-  const int tblocks = topk_blocks(sspan);
   const bool clustered = lo->sorted && !db->lineitem->host;  // an order's rows are contiguous
   unsigned long long* state = nullptr;
   uint32_t* cnt;
   uint8_t *smap, *fst;
-  TopEntry* win;
+  ThkState* thk;  // top-k by count histogram
+  TopEntry* tcand;
   uint64_t* dout;
   char* host;
   if (!clustered) GDB_TRY(arena(ctx, A_AGG, ospan * 8, (void**)&state));
   GDB_TRY(arena(ctx, A_BITMAP, ospan, (void**)&fst));
   GDB_TRY(arena(ctx, A_MAP, sspan * 5, (void**)&cnt));
   smap = (uint8_t*)(cnt + sspan);
-  GDB_TRY(arena(ctx, A_AUX2, sizeof(TopEntry) * (size_t)tblocks * k, (void**)&win));
+  GDB_TRY(thk_buffers(ctx, A_AUX2, &thk, &tcand));
   const size_t out_bytes = 32 + (size_t)k * sizeof(TopEntry) + 64;
   GDB_TRY(arena(ctx, A_OUT, out_bytes, (void**)&dout));
   GDB_TRY(pinned(ctx, out_bytes, (void**)&host));
@@ -1402,7 +1402,7 @@ int gdb_q21(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q21_params* p, gdb_q2
       rs.chunks += rs2.chunks;
     }
   }
-  GDB_TRY(topk_run(ctx, CountSrc{cnt}, sspan, k, win, fin, (unsigned long long*)dout));
+  GDB_TRY(thk_run(ctx, CountSrc{cnt}, sspan, k, thk, tcand, fin, (unsigned long long*)dout));
   GDB_HIP(hipMemcpyAsync(host, dout, out_bytes, hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
   // SQL columns: lineitem orderkey, suppkey, commitdate, receiptdate (16 B); orders

@@ -549,12 +549,12 @@ int gdb_q2(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q2_params* p, gdb_q2_r
   sspan = std::max(sspan, a);
   if (sspan > (1ull << 26)) return fail(GDB_E_UNSUPPORTED, "Q2 supports supplier keys below 2^26");
   const int k = std::max(p->top_k, 1);
-  const int tblocks = topk_blocks(nps);
   uint16_t* psel;
   unsigned long long* mn;
   uint8_t *snat, *nreg, *rank;
   int64_t* sbal;
-  TopEntry* win;
+  ThkState* thk;  // top-k by account-balance histogram
+  TopEntry* tcand;
   uint64_t* dout;
   char* host;
   GDB_TRY(arena(ctx, A_MAP, pspan * 2, (void**)&psel));
@@ -563,7 +563,7 @@ int gdb_q2(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q2_params* p, gdb_q2_r
   snat = (uint8_t*)(sbal + sspan);
   nreg = snat + sspan;
   rank = nreg + 256;
-  GDB_TRY(arena(ctx, A_AUX2, sizeof(TopEntry) * (size_t)tblocks * k, (void**)&win));
+  GDB_TRY(thk_buffers(ctx, A_AUX2, &thk, &tcand));
   const size_t out_bytes = 16 + (size_t)k * sizeof(TopEntry) + 64;
   GDB_TRY(arena(ctx, A_OUT, out_bytes, (void**)&dout));
   GDB_TRY(pinned(ctx, std::max<size_t>(out_bytes, 256), (void**)&host));
@@ -596,10 +596,10 @@ int gdb_q2(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q2_params* p, gdb_q2_r
         psel, snat, mn);
   GDB_HIP(hipGetLastError());
   GDB_HIP(hipEventRecord(ctx->ev[2], s));
-  GDB_TRY(topk_run(ctx,
-                   Q2Src{(const int32_t*)qp->dev, (const int32_t*)qs->dev,
-                         (const int64_t*)qc->dev, psel, snat, sbal, mn, rank},
-                   nps, k, win, fin, (unsigned long long*)dout));
+  GDB_TRY(thk_run(ctx,
+                  Q2Src{(const int32_t*)qp->dev, (const int32_t*)qs->dev,
+                        (const int64_t*)qc->dev, psel, snat, sbal, mn, rank},
+                  nps, k, thk, tcand, fin, (unsigned long long*)dout));
   GDB_HIP(hipMemcpyAsync(host, dout, out_bytes, hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
   // SQL columns: part key, size (8 B), type, mfgr (2 B); supplier key, nation (8 B),

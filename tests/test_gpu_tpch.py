@@ -238,6 +238,26 @@ def test_long_orders_run_past_the_owner_window(engine):
         d.free()
 
 
+def test_top_k_ties_past_the_candidate_list(engine):
+    """Q18 with every o_totalprice equal and quantity 0: all 75K orders tie in
+    one value bucket, more than the 8192 candidates the top-k collects, so its
+    ranking workgroup scans every candidate; the ties resolve by orderdate, then
+    orderkey. Both the clustered (candidate list) and shuffled-lineitem paths."""
+    import numpy as np
+    tabs = tpch.generate(0.05, 6)
+    tp = np.full_like(tabs["orders"]["o_totalprice"], 12345678)
+    tabs = dict(tabs, orders=dict(tabs["orders"], o_totalprice=tp))
+    perm = np.random.default_rng(4).permutation(len(tabs["lineitem"]["l_orderkey"]))
+    shuffled = dict(tabs, lineitem={k: v[perm] for k, v in tabs["lineitem"].items()})
+    for t in (tabs, shuffled):
+        d = TpchDb(engine, t)
+        try:
+            assert L(d.q18(0, 100)) == L(T.q18(tabs, 0, 100))
+            assert L(d.q18(0, 1)) == L(T.q18(tabs, 0, 1))
+        finally:
+            d.free()
+
+
 def test_streamed_lineitem_gives_the_same_results(engine):
     """Every pipeline with lineitem in host memory, streamed through the GPU in
     64K-row chunks (two passes for Q17 and Q21, the general paths for Q18 and
