@@ -674,30 +674,29 @@ struct QtySt {
   __device__ QtySt read(int l) const { return QtySt{read_lane(v, l)}; }
 };
 
-// Q21: per order, a supplier (+1) of its lines and of its late lines, the late
-// line count, and whether a second supplier / second late supplier appears.
+// Q21: per order, the least and greatest supplier (+1) over its lines and over
+// its late lines, and the late line count. A second (late) supplier appears iff
+// the least and greatest differ. Merging is min / max / add, so the empty state
+// (least ~0, greatest 0, count 0) is the identity.
 struct SuppSt {
-  static constexpr uint32_t COUNT = (1u << 30) - 1, MULTI = 1u << 30, LMULTI = 1u << 31;
-  uint32_t first = 0, lfirst = 0, lc = 0;  // lc: late count | MULTI | LMULTI
+  uint32_t lo = ~0u, hi = 0u, llo = ~0u, lhi = 0u, lc = 0u;
   __device__ static SuppSt row(int s, int c, int r) {
     const uint32_t s1 = (uint32_t)s + 1u;
     const bool late = r > c;
-    return SuppSt{s1, late ? s1 : 0u, late ? 1u : 0u};
+    return SuppSt{s1, s1, late ? s1 : ~0u, late ? s1 : 0u, late ? 1u : 0u};
   }
   __device__ static SuppSt merge(const SuppSt& a, const SuppSt& b) {
-    const uint32_t multi = ((a.lc | b.lc) & MULTI) |
-                           (a.first && b.first && a.first != b.first ? MULTI : 0u);
-    const uint32_t lmulti = ((a.lc | b.lc) & LMULTI) |
-                            (a.lfirst && b.lfirst && a.lfirst != b.lfirst ? LMULTI : 0u);
-    return SuppSt{a.first ? a.first : b.first, a.lfirst ? a.lfirst : b.lfirst,
-                  ((a.lc & COUNT) + (b.lc & COUNT)) | multi | lmulti};
+    return SuppSt{min(a.lo, b.lo), max(a.hi, b.hi), min(a.llo, b.llo), max(a.lhi, b.lhi),
+                  a.lc + b.lc};
   }
   __device__ SuppSt shfl_down(int d) const {
-    return SuppSt{(uint32_t)__shfl_down((int)first, d, 64),
-                  (uint32_t)__shfl_down((int)lfirst, d, 64), (uint32_t)__shfl_down((int)lc, d, 64)};
+    return SuppSt{(uint32_t)__shfl_down((int)lo, d, 64), (uint32_t)__shfl_down((int)hi, d, 64),
+                  (uint32_t)__shfl_down((int)llo, d, 64), (uint32_t)__shfl_down((int)lhi, d, 64),
+                  (uint32_t)__shfl_down((int)lc, d, 64)};
   }
   __device__ SuppSt read(int l) const {
-    return SuppSt{(uint32_t)read_lane((int)first, l), (uint32_t)read_lane((int)lfirst, l),
+    return SuppSt{(uint32_t)read_lane((int)lo, l), (uint32_t)read_lane((int)hi, l),
+                  (uint32_t)read_lane((int)llo, l), (uint32_t)read_lane((int)lhi, l),
                   (uint32_t)read_lane((int)lc, l)};
   }
 };
@@ -745,10 +744,9 @@ __global__ __launch_bounds__(256) void q21_owner(const int32_t* __restrict__ ok,
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const SuppSt t = lr.e[j] ? SuppSt::merge(lr.r[j], carry) : lr.r[j];
-      late[j] = t.lc & SuppSt::COUNT;
-      want[j] = ((lr.head >> j) & 1) && f[j] && (t.lc & SuppSt::MULTI) && late[j] &&
-                !(t.lc & SuppSt::LMULTI);
-      supp[j] = want[j] ? t.lfirst - 1 : 0u;
+      late[j] = t.lc;
+      want[j] = ((lr.head >> j) & 1) && f[j] && t.lo != t.hi && late[j] && t.llo == t.lhi;
+      supp[j] = want[j] ? t.llo - 1 : 0u;
     }
     // Supplier nations of the candidates, loaded at once.
     const uint8_t sn[4] = {smap[supp[0]], smap[supp[1]], smap[supp[2]], smap[supp[3]]};
