@@ -57,7 +57,9 @@ class Table:
             descs[i].type = typ
             descs[i].width = width
             descs[i].host_data = arr.ctypes.data_as(C.c_void_p)
-        if nrows is not None:
+        if nrows is not None and columns and nrows != n:
+            raise GdbError(abi.GDB_E_INVALID, f"nrows={nrows} but the columns hold {n} rows")
+        if n is None:
             n = nrows
         self.nrows = n or 0
         self.names = list(columns)
@@ -164,6 +166,14 @@ class Engine:
         r, c = C.c_double(), C.c_double()
         abi.check(lib().gdb_bandwidth_probe(self._ctx, nbytes, reps, C.byref(r), C.byref(c)))
         return r.value, c.value
+
+    def h2d_probe(self, host_array, reps=5):
+        """Host-to-device copy GB/s of `host_array` (pinned: pinned_empty) on
+        the stream streamed tables are copied on."""
+        g = C.c_double()
+        abi.check(lib().gdb_h2d_probe(self._ctx, host_array.ctypes.data_as(C.c_void_p),
+                                      host_array.nbytes, reps, C.byref(g)))
+        return g.value
 
     def table(self, columns):
         return Table(self, columns)

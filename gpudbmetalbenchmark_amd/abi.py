@@ -132,7 +132,8 @@ class ServeStats(C.Structure):
                 ("batches", C.c_uint64), ("qps", C.c_double), ("p50_ms", C.c_double),
                 ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
                 ("mean_batch", C.c_double), ("inflight_peak_bytes", C.c_uint64),
-                ("graph_captures", C.c_uint64), ("graph_replays", C.c_uint64)]
+                ("graph_captures", C.c_uint64), ("graph_replays", C.c_uint64),
+                ("index_bytes", C.c_uint64), ("index_build_ms", C.c_double)]
 
 
 REQ_Q6, REQ_Q1, REQ_Q3 = 0, 1, 2
@@ -164,6 +165,7 @@ PROTOTYPES = {
     "gdb_debug_fail_allocs": (None, [P, C.c_int]),
     "gdb_bandwidth_probe": (C.c_int, [P, C.c_uint64, C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double)]),
+    "gdb_h2d_probe": (C.c_int, [P, P, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
     "gdb_table_create": (C.c_int, [P, C.POINTER(ColDesc), C.c_int, C.c_uint64,
                                    C.POINTER(C.c_void_p)]),
     "gdb_table_free": (None, [P]),

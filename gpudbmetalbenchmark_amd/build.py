@@ -2,6 +2,7 @@
 
     python -m gpudbmetalbenchmark_amd.build          # engine + oracle + driver
     python -m gpudbmetalbenchmark_amd.build --engine # engine only
+    ... --incremental                                # only changed sources
 
 The engine is every csrc/*.hip and csrc/*.cpp compiled by hipcc for gfx950 and
 linked into gpudbmetalbenchmark_amd/libgdb.so next to this file, so it travels
@@ -45,14 +46,26 @@ def _sources(exts):
                   if os.path.splitext(f)[1] in exts and not f.startswith("driver"))
 
 
-def build_engine(jobs=8):
+def _headers_mtime():
+    hs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    hs += [os.path.join(INCLUDE, f) for f in os.listdir(INCLUDE) if f.endswith(".h")]
+    return max(os.path.getmtime(h) for h in hs)
+
+
+def build_engine(jobs=8, incremental=False):
+    """Compiles every source (incremental=True: only sources newer than their
+    object or than any header) and links libgdb.so."""
     os.makedirs(BUILD, exist_ok=True)
     srcs = _sources((".hip", ".cpp"))
+    hdr = _headers_mtime() if incremental else None
     objs = []
     cmds = []
     for s in srcs:
         o = os.path.join(BUILD, os.path.basename(s) + ".o")
         objs.append(o)
+        if incremental and os.path.exists(o) and \
+                os.path.getmtime(o) > max(os.path.getmtime(s), hdr):
+            continue
         lang = ["-x", "hip"] if s.endswith(".hip") else []
         cmds.append([HIPCC] + HIP_FLAGS + lang + ["-c", s, "-o", o])
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
@@ -79,8 +92,8 @@ def build_oracle():
     return ORACLE_LIB
 
 
-def build_all(jobs=8):
-    out = [build_engine(jobs), build_oracle()]
+def build_all(jobs=8, incremental=False):
+    out = [build_engine(jobs, incremental), build_oracle()]
     d = build_driver()
     if d:
         out.append(d)
@@ -91,10 +104,12 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--engine", action="store_true")
     ap.add_argument("-j", type=int, default=8)
+    ap.add_argument("--incremental", action="store_true",
+                    help="recompile only sources changed since their object (or any header)")
     a = ap.parse_args()
     if a.engine:
-        print(build_engine(a.j))
+        print(build_engine(a.j, a.incremental))
     else:
-        for p in build_all(a.j):
+        for p in build_all(a.j, a.incremental):
             print(p)
     sys.exit(0)

@@ -80,3 +80,29 @@ extern "C" int gdb_bandwidth_probe(gdb_ctx* ctx, uint64_t bytes, int reps, doubl
   if (copy_gbps) *copy_gbps = 2.0 * (double)bytes * reps / (tc / 1000.0) / 1e9;
   return GDB_OK;
 }
+
+// Host-to-device copy rate of `bytes` from caller memory (pinned from
+// gdb_host_alloc for the full PCIe rate), on the copy stream the streamed
+// pipelines use; rep 0 warms up. The ceiling a streamed pipeline can reach.
+extern "C" int gdb_h2d_probe(gdb_ctx* ctx, const void* host, uint64_t bytes, int reps,
+                             double* gbps) {
+  if (!ctx || !host || reps < 1 || bytes < 4096 || !gbps)
+    return fail(GDB_E_INVALID, "bad probe arguments");
+  GDB_HIP(hipSetDevice(ctx->device));
+  char* d;
+  GDB_TRY(arena(ctx, A_STREAM0, bytes, (void**)&d));
+  hipStream_t s = ctx->copy_stream;
+  GDB_HIP(hipStreamSynchronize(ctx->stream));
+  float total = 0.f;
+  for (int r = 0; r <= reps; ++r) {
+    GDB_HIP(hipEventRecord(ctx->ev[3], s));
+    GDB_HIP(hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, s));
+    GDB_HIP(hipEventRecord(ctx->ev[4], s));
+    GDB_HIP(hipEventSynchronize(ctx->ev[4]));
+    float x = 0.f;
+    GDB_HIP(hipEventElapsedTime(&x, ctx->ev[3], ctx->ev[4]));
+    if (r) total += x;
+  }
+  *gbps = (double)bytes * reps / (total / 1000.0) / 1e9;
+  return GDB_OK;
+}

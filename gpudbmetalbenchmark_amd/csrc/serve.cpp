@@ -80,6 +80,7 @@ struct gdb_server {
   bool q3_busy = false;
   uint64_t next_id = 1;
   uint64_t reserved = 0, peak = 0;
+  double index_build_ms = 0;  // Q3 join indexes: build time at start
   int fatal = GDB_OK;
   // statistics
   std::vector<double> lat;
@@ -353,8 +354,10 @@ int gdb_serve_start(gdb_ctx* ctx, const gdb_serve_cfg* cfg, gdb_server** out) {
       s.flat.graphs = cfg->graphs != 0;
     }
     if (cfg->customer && cfg->orders && cfg->lineitem) {
+      const double t0 = now_ms();
       int rc = q3_slot_create(ctx, cfg->customer, cfg->orders, cfg->lineitem, &S->q3);
       if (rc != GDB_OK) return bail(rc);
+      S->index_build_ms = now_ms() - t0;
       q3_slot_use_graphs(S->q3, cfg->graphs != 0);
     }
   }
@@ -464,6 +467,8 @@ int gdb_serve_stats_get(gdb_server* S, gdb_serve_stats* out) {
   fill_stats(S->lat, S->submitted, S->completed, S->rejected, S->batches, S->batch_reqs, S->t_first,
              S->t_last, S->peak, out);
   graph_counts(S, &out->graph_captures, &out->graph_replays);
+  out->index_bytes = q3_slot_bytes(S->q3);
+  out->index_build_ms = S->index_build_ms;
   return S->fatal;
 }
 
@@ -569,6 +574,8 @@ int gdb_serve_loadgen(gdb_server* S, double rate, double seconds, uint64_t seed,
   {
     std::lock_guard<std::mutex> g(S->mu);
     graph_counts(S, &stats->graph_captures, &stats->graph_replays);
+    stats->index_bytes = q3_slot_bytes(S->q3);
+    stats->index_build_ms = S->index_build_ms;
   }
   if (got + failed < ids.size()) return fail(GDB_E_TIMEOUT, "responses missing after 120 s");
   return GDB_OK;

@@ -75,6 +75,11 @@ uint64_t gdb_hbm_used(gdb_ctx* ctx);
  * the calibration kernel for counter-based traffic measurement. */
 int gdb_bandwidth_probe(gdb_ctx* ctx, uint64_t bytes, int reps, double* read_gbps,
                         double* copy_gbps);
+/* Host-to-device copy rate (GB/s) of `bytes` from `host` (pinned memory from
+ * gdb_host_alloc gives the full PCIe rate) on the stream that streamed tables
+ * are copied on; `reps` timed copies after one warm-up. The ceiling of a
+ * streamed pipeline (the reference's SF100 paths, src/infra.h:816-869). */
+int gdb_h2d_probe(gdb_ctx* ctx, const void* host, uint64_t bytes, int reps, double* gbps);
 /* Test hook: the next `count` device allocations fail with GDB_E_NOMEM. */
 void gdb_debug_fail_allocs(gdb_ctx* ctx, int count);
 
@@ -270,6 +275,8 @@ typedef struct {
   uint64_t inflight_peak_bytes;
   uint64_t graph_captures;         /* launch shapes captured as graphs */
   uint64_t graph_replays;          /* batches launched as graph replays */
+  uint64_t index_bytes;            /* HBM held by the Q3 join indexes built at start */
+  double index_build_ms;           /* wall time gdb_serve_start spent building them */
 } gdb_serve_stats;
 int gdb_serve_start(gdb_ctx* ctx, const gdb_serve_cfg* cfg, gdb_server** out);
 /* Thread-safe; GDB_E_ADMISSION if the request alone exceeds the budget. */

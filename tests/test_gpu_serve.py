@@ -131,3 +131,32 @@ def test_streamed_flat_table_serving(engine):
         engine.set_chunk_rows(0)
     for i, (k, p) in enumerate(reqs):
         assert same(k, got[i]["result"], expected[i]), (i, k, p)
+
+
+def test_admission_defers_until_a_slot_frees(engine, tables):
+    """With a budget that holds one single-request batch, requests whose
+    boundaries differ cannot share a batch and a second batch cannot start
+    while one is in flight: each waits for the budget to free (deferral, not
+    refusal), and every response is still exact."""
+    flat, t = tables
+    with Server(engine, flat, slots=2) as srv:
+        srv.submit(request("q1", cutoff=1500))
+        while not srv.poll(16, 1000):
+            pass
+        one = srv.stats()["inflight_peak_bytes"]
+    assert one > 0
+    cutoffs = [1000 + 37 * i for i in range(24)]
+    expected = [engine.q1(flat, c) for c in cutoffs]
+    with Server(engine, flat, slots=2, hbm_budget=one) as srv:
+        ids = {srv.submit(request("q1", cutoff=c)): i for i, c in enumerate(cutoffs)}
+        got = {}
+        while len(got) < len(ids):
+            for r in srv.poll(256, 1000):
+                got[ids[r["id"]]] = r
+        st = srv.stats()
+    assert st["rejected"] == 0 and st["completed"] == len(cutoffs)
+    assert st["inflight_peak_bytes"] <= one
+    assert st["batches"] == len(cutoffs)  # one request per batch, one batch at a time
+    for i in range(len(cutoffs)):
+        assert got[i]["batch_size"] == 1
+        assert same("q1", got[i]["result"], expected[i]), i

@@ -1,6 +1,7 @@
 """GPU: host-resident tables streamed in chunks (the reference's SF100
-chunkedStreamLoop path) give bit-identical results to the same tables resident
-in HBM, for every pipeline; misuse is rejected with an error code."""
+chunkedStreamLoop path, src/infra.h:816-869) give bit-identical results to the
+same tables resident in HBM and to the oracle, for every pipeline; misuse is
+rejected with an error code."""
 import numpy as np
 import pytest
 
@@ -54,6 +55,8 @@ def test_q3_q9_streamed_fact(engine, mid):
             a = engine.q3(t["customer"], t["orders"], hli, seg, date, 20)
             b = engine.q3(t["customer"], t["orders"], t["lineitem"], seg, date, 20)
             assert a == b
+            assert b == O.q3(mid["customer"], mid["orders"], mid["lineitem"], seg, date, 20,
+                             nthreads=8)
         a = engine.q9(t["part"], t["supplier"], t["partsupp"], t["orders"], hli, "green")
         b = engine.q9(t["part"], t["supplier"], t["partsupp"], t["orders"], t["lineitem"], "green")
         assert a == b
@@ -69,9 +72,10 @@ def test_q13_streamed_orders(engine, mid):
     ho = engine.host_table({k: mid["orders"][k] for k in ("o_custkey", "o_comment")})
     engine.set_chunk_rows(300 * 1024)
     try:
-        assert engine.q13(c, ho) == engine.q13(c, o)
+        a = engine.q13(c, ho)
     finally:
         engine.set_chunk_rows(0)
+    assert a == engine.q13(c, o) == O.q13(mid["customer"], mid["orders"], nthreads=8)
 
 
 def test_host_tables_rejected_where_residency_is_required(engine, mid):
