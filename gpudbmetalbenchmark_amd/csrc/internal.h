@@ -74,9 +74,6 @@ struct gdb_ctx {
   size_t host_pinned_bytes = 0;
   gdb::Tuning tune[16];
   uint64_t chunk_rows = 0;         // streamed chunk size, 0 = adaptive
-  uint8_t q3_epoch = 0;            // tag of the current Q3 run in its orderkey byte map
-  void* q3_map = nullptr;          // the byte map the tags refer to
-  size_t q3_map_bytes = 0;
   hipEvent_t sev[4] = {};          // copy-done / slot-free events of the two slots
   std::vector<hipEvent_t> chunk_ev;  // per-chunk compute start/end
 };

@@ -2,26 +2,34 @@
 // host src/q3.cpp:8-173, CPU top-10 src/infra.h:538-564, SQL sql/q3.sql).
 //
 // Plan (all on the engine stream, no host round trip until the k results):
-//   clear     zero the customer bitmap, the counters and the revenue histogram
-//   customer  bit[custkey] = (segment == s)                        (188 KB bitmap)
-//   orders    qualifying = orderdate < D && bit[custkey]: append (key, date,
-//             prio) to a list in row order, tag[orderkey] = this run's epoch,
-//             slot[orderkey] = list position (plain stores, no atomics)
-//   acc_clear zero the per-list-entry revenue and hit flags
-//   probe     fact rows with shipdate > D and tag[orderkey] == epoch add
-//             extprice*(100-discount) into their order's list entry; discount
-//             and price are loaded only for lanes with a surviving row, and a
-//             lane's consecutive rows of one order are added once
-//   topk      a 4096-bucket log-linear histogram of the groups' revenues finds
-//             the bucket holding the k-th best; only groups at or above it are
-//             collected, and one block ranks them (revenue desc, orderdate asc,
+//   clear     zero the customer bitmap, the order-key words, the counters and
+//             the revenue histograms
+//   customer  bit[custkey - min custkey] = (segment == s)
+//   orders    qualifying = orderdate < D && customer bit: set the order's bit
+//             in the order-key words (one 64-bit word per 32 key values)
+//   rank      every order-key word gets, in its high half, the number of
+//             qualifying orders with a smaller key (a prefix count over the
+//             bits), so rank(key) = high + popcount(bits below the key's bit);
+//             the revenue slots [0, qualifying) are zeroed
+//   probe     fact rows with shipdate > D whose order bit is set add
+//             extprice*(100-discount) into slot rank(l_orderkey); discount and
+//             price are loaded only for 4-row groups with a surviving row, and
+//             a lane's consecutive rows of one order are added once
+//   topk      a 4096-bucket log-linear histogram of the slots' revenues finds
+//             the bucket holding the k-th best; an orders scan collects the
+//             qualifying orders at or above it with their orderdate and
+//             shippriority; one block ranks them (revenue desc, orderdate asc,
 //             orderkey asc) with the LDS top-k of topk.h
-// The reference's direct map orders_map[orderkey] (kernels/q3.metal:59-78) and
-// its reset before every run become the byte tags: a run only trusts entries
-// tagged with its own epoch, so neither map is cleared between runs (the
-// byte map is zeroed once every 255 runs). Device-scope global atomics run at
-// ~10 G/s on MI355X, so the build uses none; only the probe's revenue adds are
-// atomic (one per run of rows of one order, ~10^5 per run).
+//
+// The reference maps orderkey -> order row with a direct int32 map
+// orders_map[max_orderkey + 1] reset to -1 before every run (kernels/q3.metal:59-78,
+// src/q3.cpp:59-66), or with an open-addressing hash table once that map would
+// exceed 30 % of the working set (src/q3.cpp:243-246, kernels/q3.metal:83-117).
+// Here the order side costs 2 bits per key value (8 bytes per 32 keys; 15 MB for
+// the 60M-key span of 15M TPC-H orders, 1 GiB for the whole int32 range), so one
+// structure serves every key distribution: there is no per-key row map to clear
+// or to scatter into, only the bits the qualifying orders set. Orderkeys are
+// unique (the SQL's primary key), as the reference's map assumes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,143 +46,224 @@ namespace {
 
 constexpr int TOPK_BUF = 2048;
 constexpr uint32_t CAND_CAP = 65536;  // candidate list entries
-constexpr int OTILE_PER_THREAD = 16;
+constexpr int HIST_COPIES = 8;        // revenue histograms, merged by the threshold step
+constexpr int RANK_WORDS = 512;       // order-key words per block of the rank kernels (2 per thread)
 
 struct Q3Scratch {
-  uint32_t* cbm;      // customer bitmap
-  uint32_t* counters; // [0] qualifying orders, [1] groups, [2] threshold, [3] candidates
-  uint32_t* hist;     // 4096 revenue buckets
-  int32_t* qkey;      // list of qualifying orders
-  int32_t* qdate;
-  int32_t* qprio;
-  int64_t* rev;       // per list entry
-  uint8_t* hit;
-  uint8_t* tag;       // [maxok+1] epoch of the run that listed the key
-  int32_t* slot;      // [maxok+1] list position (valid where tag == epoch)
-  TopEntry* cand;     // CAND_CAP candidates
-  TopEntry* out;      // k final
+  uint32_t* cbm;       // customer bitmap
+  uint32_t* counters;  // [0] qualifying orders, [1] groups, [2] threshold, [3] candidates,
+                       // [6] workgroups done (overflow ranking)
+  uint32_t* hist;      // HIST_COPIES x 4096 revenue buckets
+  uint64_t* okw;       // order-key words: low 32 bits = qualifying keys, high = rank of the first
+  uint32_t* bsum;      // per rank block: qualifying orders
+  int64_t* rev;        // revenue by rank
+  uint8_t* hit;        // 1 where a fact row reached the slot
+  TopEntry* cand;      // CAND_CAP candidates
+  TopEntry* out;       // k final
 };
 
-__global__ __launch_bounds__(256) void q3_clear(uint32_t* __restrict__ a, uint32_t na,
-                                                uint32_t* __restrict__ b, uint32_t nb) {
+// Zeroes three regions, each a whole number of 16-byte units.
+__global__ __launch_bounds__(256) void q3_clear(uint4* __restrict__ a, uint32_t na,
+                                                uint4* __restrict__ b, uint32_t nb,
+                                                uint4* __restrict__ c, uint32_t nc) {
+  const uint4 z = make_uint4(0u, 0u, 0u, 0u);
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na + nb; i += stride) {
-    if (i < na) a[i] = 0u;
-    else b[i - na] = 0u;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na + nb + nc; i += stride) {
+    if (i < na) a[i] = z;
+    else if (i < na + nb) b[i - na] = z;
+    else c[i - na - nb] = z;
   }
 }
 
-__global__ __launch_bounds__(256) void q3_fill_bytes(uint32_t* __restrict__ p, uint32_t nwords) {
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += stride) p[i] = 0u;
+// ORs one lane's bits of up to four keys (offsets off[j], valid where q[j])
+// into 32-bit words WSTRIDE words apart. Each run of keys in one word but the
+// lane's last is ORed at once; the last runs are combined across the wave by a
+// doubling OR over neighbouring lanes with the same word (keys ascending in
+// row order give long runs), after which the first lane of each run ORs it.
+// Every lane of the wave must call it.
+template <int WSTRIDE>
+__device__ __forceinline__ void or_bits4(uint32_t* __restrict__ words, const uint32_t off[4],
+                                         const bool q[4]) {
+  const int lane = threadIdx.x & 63;
+  uint32_t word = 0xFFFFFFFFu, mask = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (!q[j]) continue;
+    const uint32_t w = off[j] >> 5;
+    if (mask && w != word) {
+      atomicOr(&words[(size_t)word * WSTRIDE], mask);
+      mask = 0;
+    }
+    word = w;
+    mask |= 1u << (off[j] & 31);
+  }
+  const uint32_t mine = word;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t w2 = (uint32_t)__shfl_down((int)mine, d, 64);
+    const uint32_t m2 = (uint32_t)__shfl_down((int)mask, d, 64);
+    if (lane + d < 64 && w2 == mine) mask |= m2;
+  }
+  const uint32_t before = (uint32_t)__shfl_up((int)mine, 1, 64);
+  if (mask && (lane == 0 || before != mine)) atomicOr(&words[(size_t)mine * WSTRIDE], mask);
 }
 
+// bit[custkey - lo] for customers of the segment; 4 rows per lane.
 __global__ __launch_bounds__(256) void q3_customer(const int32_t* __restrict__ custkey,
                                                    const uint8_t* __restrict__ seg, uint32_t n,
-                                                   int32_t segment, uint32_t maxck,
+                                                   int32_t segment, int32_t lo, uint32_t last,
                                                    uint32_t* __restrict__ cbm) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t ck = (uint32_t)custkey[i];
-  if (seg[i] == (uint8_t)segment && ck <= maxck) atomicOr(&cbm[ck >> 5], 1u << (ck & 31));
+  const uint32_t nq = (n + 3) / 4, stride = gridDim.x * blockDim.x;
+  for (uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; gw < nq; gw += stride) {
+    const uint32_t g = gw + (threadIdx.x & 63);
+    int4 k4 = make_int4(0, 0, 0, 0);
+    uint32_t s4 = 0;
+    if (g < nq) {
+      k4 = ld4(custkey, g);
+      s4 = ld_u8x4(seg, g);
+    }
+    const int32_t ks[4] = {k4.x, k4.y, k4.z, k4.w};
+    uint32_t off[4];
+    bool q[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      off[j] = (uint32_t)((int64_t)ks[j] - lo);
+      q[j] = 4 * g + j < n && ((s4 >> (8 * j)) & 0xFFu) == (uint32_t)(uint8_t)segment &&
+             off[j] <= last;
+    }
+    or_bits4<1>(cbm, off, q);
+  }
 }
 
-// Tiles of 16 x 256 rows: thread t reads rows 4t..4t+3 of each of the tile's
-// four 1024-row quarters (coalesced dwordx4 loads, all twelve issued before
-// the first use), then the 16 customer-bitmap lookups together. Qualifying
-// rows are appended with one atomic per tile and a block-wide scan for the
-// slots, written from the values still held in registers. The list order
-// (tile, thread, quarter, row) is deterministic; nothing downstream depends on
-// it.
+// Orders with orderdate < D whose customer bit is set: their key bits. U
+// vector groups of 4 rows per lane and trip; all column loads are issued
+// first, then the customer-bit gathers of the rows that pass the date test.
+template <int U>
 __global__ __launch_bounds__(256) void q3_orders(const int32_t* __restrict__ okey,
                                                  const int32_t* __restrict__ ocust,
-                                                 const int32_t* __restrict__ odate,
-                                                 const int32_t* __restrict__ oprio, uint32_t n,
-                                                 int32_t date, uint32_t maxck, uint32_t maxok,
-                                                 uint8_t epoch, Q3Scratch s) {
-  constexpr int R = OTILE_PER_THREAD;
-  constexpr int V = R / 4;
-  __shared__ uint32_t scratch[17];
-  __shared__ uint32_t base_sh;
-  const uint32_t tile_rows = R * blockDim.x;
-  const uint32_t ntiles = (n + tile_rows - 1) / tile_rows;
-  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    int32_t ks[R], cs[R], ds[R];
-    uint32_t rows[V];
+                                                 const int32_t* __restrict__ odate, uint32_t n,
+                                                 int32_t date, int32_t clo, uint32_t clast,
+                                                 const uint32_t* __restrict__ cbm, int32_t olo,
+                                                 uint32_t olast, uint32_t* __restrict__ okw32) {
+  const uint32_t nq = (n + 3) / 4, nthr = gridDim.x * blockDim.x;
+  for (uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; gw < nq; gw += U * nthr) {
+    int4 k4[U], c4[U], d4[U];
 #pragma unroll
-    for (int v = 0; v < V; ++v) rows[v] = t * tile_rows + v * 4 * blockDim.x + 4 * threadIdx.x;
-    if (t * tile_rows + tile_rows <= n) {
-      int4 k4[V], c4[V], d4[V];
-#pragma unroll
-      for (int v = 0; v < V; ++v) {
-        k4[v] = ld4(okey, rows[v] / 4);
-        c4[v] = ld4(ocust, rows[v] / 4);
-        d4[v] = ld4(odate, rows[v] / 4);
-      }
-#pragma unroll
-      for (int v = 0; v < V; ++v) {
-        ks[4 * v] = k4[v].x; ks[4 * v + 1] = k4[v].y; ks[4 * v + 2] = k4[v].z; ks[4 * v + 3] = k4[v].w;
-        cs[4 * v] = c4[v].x; cs[4 * v + 1] = c4[v].y; cs[4 * v + 2] = c4[v].z; cs[4 * v + 3] = c4[v].w;
-        ds[4 * v] = d4[v].x; ds[4 * v + 1] = d4[v].y; ds[4 * v + 2] = d4[v].z; ds[4 * v + 3] = d4[v].w;
-      }
-    } else {
-#pragma unroll
-      for (int v = 0; v < V; ++v) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const uint32_t r = rows[v] + j;
-          const bool in = r < n;
-          ks[4 * v + j] = in ? okey[r] : 0;
-          cs[4 * v + j] = in ? ocust[r] : 0;
-          ds[4 * v + j] = in ? odate[r] : date;  // fails the date test
-        }
+    for (int u = 0; u < U; ++u) {
+      const uint32_t g = gw + u * nthr + (threadIdx.x & 63);
+      if (g < nq) {
+        k4[u] = ld4nt(okey, g);
+        c4[u] = ld4nt(ocust, g);
+        d4[u] = ld4nt(odate, g);
+      } else {
+        k4[u] = c4[u] = make_int4(0, 0, 0, 0);
+        d4[u] = make_int4(date, date, date, date);  // fails the date test
       }
     }
-    // Customer bits only for orders that pass the date test.
-    uint32_t words[R];
+    uint32_t cw[U][4], coff[U][4];
+    bool dq[U][4];
 #pragma unroll
-    for (int j = 0; j < R; ++j) words[j] = ds[j] < date ? s.cbm[min((uint32_t)cs[j], maxck) >> 5] : 0u;
-    uint32_t mask = 0;
+    for (int u = 0; u < U; ++u) {
+      const uint32_t g = gw + u * nthr + (threadIdx.x & 63);
+      const int32_t cs[4] = {c4[u].x, c4[u].y, c4[u].z, c4[u].w};
+      const int32_t ds[4] = {d4[u].x, d4[u].y, d4[u].z, d4[u].w};
 #pragma unroll
-    for (int j = 0; j < R; ++j) {
-      const uint32_t ck = (uint32_t)cs[j];
-      const bool q = ds[j] < date && ck <= maxck && (uint32_t)ks[j] <= maxok &&
-                     ((words[j] >> (ck & 31)) & 1u);
-      mask |= (q ? 1u : 0u) << j;
-    }
-    uint32_t total;
-    uint32_t pos = block_excl_scan((uint32_t)__popc(mask), scratch, &total);
-    if (total == 0) continue;
-    if (threadIdx.x == 0) base_sh = atomicAdd(&s.counters[0], total);
-    __syncthreads();
-    pos += base_sh;
-    if (mask) {
-#pragma unroll
-      for (int j = 0; j < R; ++j) {
-        if ((mask >> j) & 1u) {
-          const uint32_t k = (uint32_t)ks[j];
-          s.qkey[pos] = ks[j];
-          s.qdate[pos] = ds[j];
-          s.qprio[pos] = oprio[rows[j / 4] + (j & 3)];
-          s.tag[k] = epoch;
-          s.slot[k] = (int32_t)pos;
-          ++pos;
-        }
+      for (int j = 0; j < 4; ++j) {
+        coff[u][j] = (uint32_t)((int64_t)cs[j] - clo);
+        dq[u][j] = 4 * g + j < n && ds[j] < date && coff[u][j] <= clast;
       }
     }
-    __syncthreads();  // base_sh is rewritten by the next tile
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) cw[u][j] = dq[u][j] ? cbm[coff[u][j] >> 5] : 0u;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int32_t ks[4] = {k4[u].x, k4[u].y, k4[u].z, k4[u].w};
+      uint32_t off[4];
+      bool q[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        off[j] = (uint32_t)((int64_t)ks[j] - olo);
+        q[j] = dq[u][j] && ((cw[u][j] >> (coff[u][j] & 31)) & 1u) && off[j] <= olast;
+      }
+      or_bits4<2>(okw32, off, q);
+    }
   }
 }
 
-__global__ __launch_bounds__(256) void q3_acc_clear(const uint32_t* __restrict__ counters,
-                                                    int64_t* __restrict__ rev,
-                                                    uint8_t* __restrict__ hit) {
-  const uint32_t count = counters[0];
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += stride) {
-    rev[j] = 0;
-    hit[j] = 0;
+// True in every thread of the workgroup that finishes last among the grid's
+// workgroups (counted on `done`, zero before the launch); all earlier
+// workgroups' global writes are visible to it afterwards.
+__device__ __forceinline__ bool last_block(uint32_t* done) {
+  __shared__ uint32_t last_sh;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last_sh = atomicAdd(done, 1u) == gridDim.x - 1;
+  __syncthreads();
+  const bool last = last_sh != 0;
+  if (last) __threadfence();
+  return last;
+}
+
+// Rank, step 1: qualifying orders per block of RANK_WORDS order-key words
+// (two words per thread).
+__global__ __launch_bounds__(256) void q3_rank_sums(const uint64_t* __restrict__ okw,
+                                                    uint32_t words, uint32_t* __restrict__ bsum) {
+  __shared__ int64_t red[4];
+  const uint32_t w = blockIdx.x * RANK_WORDS + 2 * threadIdx.x;
+  uint32_t c = 0;
+  if (w + 1 < words) {
+    const ulonglong2 v = reinterpret_cast<const ulonglong2*>(okw)[w / 2];
+    c = __popc((uint32_t)v.x) + __popc((uint32_t)v.y);
+  } else if (w < words) {
+    c = __popc((uint32_t)okw[w]);
   }
+  const int64_t sum = block_sum(c, red);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = (uint32_t)sum;
+}
+
+// Rank, step 2: the workgroup's base is the sum of the earlier blocks'
+// counts; each thread's two words get the count of qualifying orders before
+// them in their high halves; the workgroup zeroes the revenue slots its ranks
+// cover, and the last one records the number of qualifying orders.
+__global__ __launch_bounds__(256) void q3_rank_write(uint64_t* __restrict__ okw, uint32_t words,
+                                                     const uint32_t* __restrict__ bsum,
+                                                     Q3Scratch s) {
+  __shared__ uint32_t scratch[17];
+  __shared__ int64_t red[4];
+  __shared__ uint32_t base_sh;
+  int64_t pre = 0;
+  for (uint32_t b = threadIdx.x; b < blockIdx.x; b += blockDim.x) pre += bsum[b];
+  pre = block_sum(pre, red);
+  if (threadIdx.x == 0) base_sh = (uint32_t)pre;
+  __syncthreads();
+  const uint32_t base = base_sh;
+  const uint32_t w = blockIdx.x * RANK_WORDS + 2 * threadIdx.x;
+  uint32_t a = 0, b = 0;
+  const bool pair = w + 1 < words;
+  if (pair) {
+    const ulonglong2 v = reinterpret_cast<const ulonglong2*>(okw)[w / 2];
+    a = (uint32_t)v.x;
+    b = (uint32_t)v.y;
+  } else if (w < words) {
+    a = (uint32_t)okw[w];
+  }
+  uint32_t total;
+  const uint32_t r = base + block_excl_scan(__popc(a) + __popc(b), scratch, &total);
+  if (pair) {
+    ulonglong2 v;
+    v.x = ((uint64_t)r << 32) | a;
+    v.y = ((uint64_t)(r + __popc(a)) << 32) | b;
+    reinterpret_cast<ulonglong2*>(okw)[w / 2] = v;
+  } else if (w < words) {
+    okw[w] = ((uint64_t)r << 32) | a;
+  }
+  for (uint32_t j = threadIdx.x; j < total; j += blockDim.x) {
+    s.rev[base + j] = 0;
+    s.hit[base + j] = 0;
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) s.counters[0] = base + total;
 }
 
 struct Q3Fact {
@@ -182,25 +271,64 @@ struct Q3Fact {
   const int64_t* price;
 };
 
-__device__ __forceinline__ void q3_flush(int32_t key, int64_t r, const Q3Scratch& s) {
-  const int32_t j = s.slot[(uint32_t)key];
-  atomicAdd((unsigned long long*)&s.rev[j], (unsigned long long)r);
-  s.hit[j] = 1;
+__device__ __forceinline__ void q3_flush(uint32_t rank, int64_t r, const Q3Scratch& s) {
+  atomicAdd((unsigned long long*)&s.rev[rank], (unsigned long long)r);
+  s.hit[rank] = 1;
 }
 
-// Rows of one 4-row group whose keys and ship dates are loaded and whose
-// order tags have been fetched: discount and price are loaded only if a row
-// survives, and a lane's consecutive rows of one order are added once.
-__device__ __forceinline__ void q3_finish(const Q3Fact& f, uint32_t g, int4 k4, int4 s4,
-                                          const uint8_t* tg, int32_t date, uint32_t maxok,
-                                          uint8_t epoch, const Q3Scratch& s) {
-  const int32_t ks[4] = {k4.x, k4.y, k4.z, k4.w};
-  const int32_t ss[4] = {s4.x, s4.y, s4.z, s4.w};
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, uint32_t src) {
+  const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, (int)src, 64);
+  const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), (int)src, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Order-key words of a wave's 4-row groups. Rows in key order touch few
+// words, so each lane loads one word of a 64-word window starting at the word
+// of lane 0's first row (one coalesced load per wave), and every lane takes
+// the words of its row 0 and the next from the window by two shuffles. A
+// needed row (need[j]) whose word is neither falls back to a gather. Every
+// lane of the wave must call it.
+__device__ __forceinline__ void q3_words(const uint64_t* __restrict__ okw, uint32_t wlast,
+                                         const uint32_t off[4], const bool need[4],
+                                         uint64_t v[4]) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t w0 = __builtin_amdgcn_readfirstlane(off[0] >> 5);
+  const uint64_t win = okw[min(w0 + lane, wlast)];
+  const uint32_t wa = off[0] >> 5, rel = wa - w0;
+  const bool in = rel < 63u;
+  const uint32_t src = in ? rel : 0u;
+  const uint64_t va = shfl64(win, src), vb = shfl64(win, src + 1);
+  bool miss = false;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t wj = off[j] >> 5;
+    v[j] = (in && wj == wa) ? va : vb;
+    const bool hitwin = in && (wj == wa || wj == wa + 1);
+    miss |= need[j] && !hitwin;
+  }
+  if (miss) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t wj = off[j] >> 5;
+      if (need[j] && !(in && (wj == wa || wj == wa + 1))) v[j] = okw[wj];
+    }
+  }
+}
+
+// Rows of one 4-row group whose words are known: survivors' slots get their
+// revenue; discount and price are loaded only if a row survives.
+__device__ __forceinline__ void q3_finish(const Q3Fact& f, uint32_t g, const uint32_t off[4],
+                                          const bool need[4], const uint64_t v[4],
+                                          const Q3Scratch& s) {
   bool pass[4];
+  uint32_t rank[4];
   bool any = false;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    pass[j] = ss[j] > date && (uint32_t)ks[j] <= maxok && tg[j] == epoch;
+    const uint32_t b = off[j] & 31;
+    const uint32_t lo = (uint32_t)v[j];
+    pass[j] = need[j] && ((lo >> b) & 1u);
+    rank[j] = (uint32_t)(v[j] >> 32) + __popc(lo & ((1u << b) - 1u));
     any |= pass[j];
   }
   if (!any) return;
@@ -208,162 +336,294 @@ __device__ __forceinline__ void q3_finish(const Q3Fact& f, uint32_t g, int4 k4, 
   const longlong2 pa = ld2(f.price, 2 * g), pb = ld2(f.price, 2 * g + 1);
   const int32_t ds[4] = {d4.x, d4.y, d4.z, d4.w};
   const int64_t ps[4] = {pa.x, pa.y, pb.x, pb.y};
-  int32_t run_key = 0;
+  uint32_t run = 0;
   int64_t run_rev = 0;
   bool open = false;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     if (!pass[j]) continue;
     const int64_t r = ps[j] * (int64_t)(100 - ds[j]);
-    if (open && ks[j] == run_key) {
+    if (open && rank[j] == run) {
       run_rev += r;
     } else {
-      if (open) q3_flush(run_key, run_rev, s);
-      run_key = ks[j];
+      if (open) q3_flush(run, run_rev, s);
+      run = rank[j];
       run_rev = r;
       open = true;
     }
   }
-  if (open) q3_flush(run_key, run_rev, s);
+  if (open) q3_flush(run, run_rev, s);
 }
 
-// U groups of 4 rows per thread and step: all key and ship-date loads are
-// issued first, then all order-tag gathers (keys are clamped into the map; a
-// clamped key fails the key test), then the groups are finished.
+// Fact rows in 4-row vector groups; a wave takes 64 consecutive groups per
+// step and U steps per trip (all key and ship-date loads first). Loops are
+// uniform per wave, as the window shuffles require.
 template <int U>
-__global__ __launch_bounds__(256) void q3_probe(Q3Fact f, uint32_t n, int32_t date, uint32_t maxok,
-                                                uint8_t epoch, Q3Scratch s) {
-  const uint32_t nvec = n >> 2;
+__global__ __launch_bounds__(256) void q3_probe(Q3Fact f, uint32_t n, int32_t date, int32_t olo,
+                                                uint32_t olast, const uint64_t* __restrict__ okw,
+                                                Q3Scratch s) {
+  const uint32_t nvec = n >> 2, wlast = olast >> 5;
   const uint32_t nthr = gridDim.x * blockDim.x;
-  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  for (; g + (U - 1) * nthr < nvec; g += U * nthr) {
+  const uint32_t lane = threadIdx.x & 63;
+  uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
+  for (; gw + (U - 1) * nthr + 64 <= nvec; gw += U * nthr) {
     int4 k4[U], s4[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      k4[u] = ld4nt(f.okey, g + u * nthr);
-      s4[u] = ld4nt(f.ship, g + u * nthr);
+      k4[u] = ld4nt(f.okey, gw + u * nthr + lane);
+      s4[u] = ld4nt(f.ship, gw + u * nthr + lane);
     }
-    uint8_t tg[U][4];
+    uint32_t off[U][4];
+    bool need[U][4];
+    uint64_t v[U][4];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      tg[u][0] = s.tag[min((uint32_t)k4[u].x, maxok)];
-      tg[u][1] = s.tag[min((uint32_t)k4[u].y, maxok)];
-      tg[u][2] = s.tag[min((uint32_t)k4[u].z, maxok)];
-      tg[u][3] = s.tag[min((uint32_t)k4[u].w, maxok)];
+      const int32_t ks[4] = {k4[u].x, k4[u].y, k4[u].z, k4[u].w};
+      const int32_t ss[4] = {s4[u].x, s4[u].y, s4[u].z, s4[u].w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        off[u][j] = (uint32_t)((int64_t)ks[j] - olo);
+        need[u][j] = ss[j] > date && off[u][j] <= olast;
+      }
+      q3_words(okw, wlast, off[u], need[u], v[u]);
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) q3_finish(f, g + u * nthr, k4[u], s4[u], tg[u], date, maxok, epoch, s);
+    for (int u = 0; u < U; ++u) q3_finish(f, gw + u * nthr + lane, off[u], need[u], v[u], s);
   }
-  for (; g < nvec; g += nthr) {
-    const int4 k4 = ld4nt(f.okey, g), s4 = ld4nt(f.ship, g);
-    const uint8_t tg[4] = {s.tag[min((uint32_t)k4.x, maxok)], s.tag[min((uint32_t)k4.y, maxok)],
-                           s.tag[min((uint32_t)k4.z, maxok)], s.tag[min((uint32_t)k4.w, maxok)]};
-    q3_finish(f, g, k4, s4, tg, date, maxok, epoch, s);
+  for (; gw < nvec; gw += nthr) {
+    const uint32_t g = gw + lane;
+    const bool valid = g < nvec;
+    int4 k4 = make_int4(0, 0, 0, 0), s4 = make_int4(date, date, date, date);
+    if (valid) {
+      k4 = ld4nt(f.okey, g);
+      s4 = ld4nt(f.ship, g);
+    }
+    const int32_t ks[4] = {k4.x, k4.y, k4.z, k4.w};
+    const int32_t ss[4] = {s4.x, s4.y, s4.z, s4.w};
+    uint32_t off[4];
+    bool need[4];
+    uint64_t v[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      off[j] = (uint32_t)((int64_t)ks[j] - olo);
+      need[j] = valid && ss[j] > date && off[j] <= olast;
+    }
+    q3_words(okw, wlast, off, need, v);
+    if (valid) q3_finish(f, g, off, need, v, s);
   }
   if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
     const uint32_t r = nvec * 4 + threadIdx.x;
-    const int32_t k = f.okey[r];
-    if (f.ship[r] > date && (uint32_t)k <= maxok && s.tag[(uint32_t)k] == epoch)
-      q3_flush(k, f.price[r] * (int64_t)(100 - f.disc[r]), s);
+    const uint32_t off = (uint32_t)((int64_t)f.okey[r] - olo);
+    if (f.ship[r] > date && off <= olast) {
+      const uint64_t w = okw[off >> 5];
+      const uint32_t b = off & 31, lo = (uint32_t)w;
+      if ((lo >> b) & 1u)
+        q3_flush((uint32_t)(w >> 32) + __popc(lo & ((1u << b) - 1u)),
+                 f.price[r] * (int64_t)(100 - f.disc[r]), s);
+    }
   }
 }
 
 // Top-k, step 1: histogram of the groups' revenue buckets (LDS per block,
-// merged by global atomics). Its total is the group count (step 2 records it).
+// added into one of HIST_COPIES global copies). Its total is the group count.
 __global__ __launch_bounds__(256) void q3_topk_hist(Q3Scratch s) {
   __shared__ uint32_t lh[4096];
   for (int i = threadIdx.x; i < 4096; i += blockDim.x) lh[i] = 0u;
   __syncthreads();
   const uint32_t count = s.counters[0];
+  const uint32_t nq = (count + 3) / 4;
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += stride)
-    if (s.hit[j]) atomicAdd(&lh[value_bucket(s.rev[j])], 1u);
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += stride) {
+    uint32_t h = ld_u8x4(s.hit, q);
+    if (4 * q + 4 > count) h &= (1u << (8 * (count - 4 * q))) - 1u;  // bytes past count are stale
+    if (!h) continue;
+    const longlong2 a = ld2(s.rev, 2 * q), b = ld2(s.rev, 2 * q + 1);
+    const int64_t r[4] = {a.x, a.y, b.x, b.y};
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if ((h >> (8 * j)) & 0xFFu) atomicAdd(&lh[value_bucket(r[j])], 1u);
+  }
   __syncthreads();
+  uint32_t* gh = s.hist + (blockIdx.x % HIST_COPIES) * 4096;
   for (int i = threadIdx.x; i < 4096; i += blockDim.x)
-    if (lh[i]) atomicAdd(&s.hist[i], lh[i]);
+    if (lh[i]) atomicAdd(&gh[i], lh[i]);
 }
 
-// Step 2 (one block): the highest bucket b such that buckets >= b hold at
-// least k groups (0 when there are fewer than k groups), and the group count
-// (every group with revenue is in exactly one bucket).
+// Step 2 (one workgroup): merges the histogram copies and finds the highest
+// bucket b such that buckets >= b hold at least k groups (0 when there are
+// fewer than k groups); the total is the group count. Thread t owns buckets
+// [16t, 16t + 16); a block scan gives each thread the groups above its range.
 __global__ __launch_bounds__(256) void q3_topk_threshold(int k, Q3Scratch s) {
-  __shared__ uint32_t part[256];
-  uint32_t v = 0;
-  for (int i = 0; i < 16; ++i) v += s.hist[threadIdx.x * 16 + i];
-  part[threadIdx.x] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t acc = 0, b = 0;
-    for (int t = 255; t >= 0; --t) {
-      if (acc + part[t] >= (uint32_t)k) {
-        for (int i = 15; i >= 0; --i) {
-          acc += s.hist[t * 16 + i];
-          if (acc >= (uint32_t)k) {
-            b = (uint32_t)(t * 16 + i);
-            break;
-          }
-        }
+  __shared__ uint32_t scratch[17];
+  uint32_t c[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) c[i] = 0;
+#pragma unroll
+  for (int h = 0; h < HIST_COPIES; ++h) {
+    const uint4* src = reinterpret_cast<const uint4*>(s.hist + h * 4096 + threadIdx.x * 16);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint4 v = src[q];
+      c[4 * q] += v.x;
+      c[4 * q + 1] += v.y;
+      c[4 * q + 2] += v.z;
+      c[4 * q + 3] += v.w;
+    }
+  }
+  uint32_t part = 0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) part += c[i];
+  uint32_t total;
+  const uint32_t pre = block_excl_scan(part, scratch, &total);
+  // groups in buckets >= 16t: total - pre; in buckets >= 16(t+1): that - part
+  const uint32_t at = total - pre, above = at - part;
+  if (at >= (uint32_t)k && above < (uint32_t)k) {
+    uint32_t acc = above, b = threadIdx.x * 16;
+    for (int i = 15; i >= 0; --i) {
+      acc += c[i];
+      if (acc >= (uint32_t)k) {
+        b = threadIdx.x * 16 + i;
         break;
       }
-      acc += part[t];
     }
-    uint32_t groups = 0;
-    for (int t = 0; t < 256; ++t) groups += part[t];
-    s.counters[1] = groups;
     s.counters[2] = b;
+  }
+  if (threadIdx.x == 0) {
+    s.counters[1] = total;
     s.counters[3] = 0u;
+    if (total < (uint32_t)k) s.counters[2] = 0u;
   }
 }
 
-// Step 3: groups in buckets >= threshold -> candidate list (tile-aggregated append).
-__global__ __launch_bounds__(256) void q3_topk_collect(Q3Scratch s) {
-  __shared__ uint32_t scratch[17];
-  __shared__ uint32_t base_sh;
-  const uint32_t count = s.counters[0];
+struct Q3Orders {
+  const int32_t *okey, *odate, *oprio;
+  uint32_t n;
+  int32_t olo;
+  uint32_t olast;
+  const uint64_t* okw;
+};
+
+// Whether orders rows' keys (offsets off[j], rows valid[j]) belong to groups
+// at or above the threshold bucket; their revenue in rev[j].
+__device__ __forceinline__ void q3_candidates(const Q3Orders& o, const uint32_t off[4],
+                                              const bool valid[4], uint32_t thr,
+                                              const Q3Scratch& s, bool cand[4], int64_t rev[4]) {
+  bool need[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) need[j] = valid[j] && off[j] <= o.olast;
+  uint64_t v[4];
+  q3_words(o.okw, o.olast >> 5, off, need, v);
+  uint32_t rank[4];
+  bool q[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t b = off[j] & 31, lo = (uint32_t)v[j];
+    q[j] = need[j] && ((lo >> b) & 1u);
+    rank[j] = (uint32_t)(v[j] >> 32) + __popc(lo & ((1u << b) - 1u));
+  }
+  uint8_t h[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) h[j] = q[j] ? s.hit[rank[j]] : 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    rev[j] = h[j] ? s.rev[rank[j]] : 0;
+    cand[j] = h[j] && value_bucket(rev[j]) >= thr;
+  }
+}
+
+// Step 2: an orders scan collects the groups in buckets >= threshold with
+// their orderdate and shippriority, 4 rows per lane (one append atomic per
+// wave and row position with a hit).
+__global__ __launch_bounds__(256) void q3_topk_collect(Q3Orders o, Q3Scratch s) {
   const uint32_t thr = s.counters[2];
-  const uint32_t stride = gridDim.x * blockDim.x;
-  const uint32_t span = (count + stride - 1) / stride * stride;
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < span; j += stride) {
-    const bool q = j < count && s.hit[j] && value_bucket(s.rev[j]) >= thr;
-    uint32_t total;
-    const uint32_t pre = block_excl_scan(q ? 1u : 0u, scratch, &total);
-    if (total == 0) continue;
-    if (threadIdx.x == 0) base_sh = atomicAdd(&s.counters[3], total);
-    __syncthreads();
-    const uint32_t pos = base_sh + pre;
-    if (q && pos < CAND_CAP) s.cand[pos] = TopEntry{s.rev[j], s.qdate[j], s.qkey[j], s.qprio[j], 0};
-    __syncthreads();
+  const uint32_t nq = (o.n + 3) / 4, nthr = gridDim.x * blockDim.x;
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; gw < nq; gw += nthr) {
+    const uint32_t g = gw + lane;
+    const int4 k4 = g < nq ? ld4nt(o.okey, g) : make_int4(0, 0, 0, 0);
+    const int32_t ks[4] = {k4.x, k4.y, k4.z, k4.w};
+    uint32_t off[4];
+    bool valid[4], cand[4];
+    int64_t rev[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      off[j] = (uint32_t)((int64_t)ks[j] - o.olo);
+      valid[j] = 4 * g + j < o.n;
+    }
+    q3_candidates(o, off, valid, thr, s, cand, rev);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const unsigned long long m = __ballot(cand[j]);
+      if (!m) continue;
+      const int leader = __ffsll((long long)m) - 1;
+      uint32_t base = 0;
+      if ((int)lane == leader) base = atomicAdd(&s.counters[3], (uint32_t)__popcll(m));
+      base = (uint32_t)__shfl((int)base, leader, 64);
+      const uint32_t pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+      const uint32_t r = 4 * g + j;
+      if (cand[j] && pos < CAND_CAP) s.cand[pos] = TopEntry{rev[j], o.odate[r], ks[j], o.oprio[r], 0};
+    }
   }
 }
 
-// Step 4 (one block): rank the candidates; if more than fit the list (heavy
-// ties in the threshold bucket), rank the qualifying list entries directly.
-__global__ __launch_bounds__(256) void q3_topk_final(int k, Q3Scratch s) {
+// Step 3. Normally (at most CAND_CAP candidates) workgroup 0 ranks the
+// candidate list and writes the k best. When the candidates overflowed the
+// list (more than CAND_CAP groups tie into the threshold bucket or above),
+// every workgroup ranks its share of the orders and leaves its k best, and
+// the workgroup that finishes last ranks those.
+__global__ __launch_bounds__(256) void q3_topk_rank(int k, Q3Orders o, Q3Scratch s) {
   __shared__ TopEntry buf[TOPK_BUF];
   __shared__ int nbuf;
-  BlockTopK<TOPK_BUF> tk{buf, &nbuf, k};
-  tk.init();
   const uint32_t nc = s.counters[3];
+  BlockTopK<TOPK_BUF> tk{buf, &nbuf, k};
   if (nc <= CAND_CAP) {
+    if (blockIdx.x != 0) return;
+    tk.init();
     for (uint32_t base = 0; base < nc; base += blockDim.x) {
       const uint32_t i = base + threadIdx.x;
-      TopEntry e;
-      const bool valid = i < nc;
-      if (valid) e = s.cand[i];
-      tk.push(valid, e);
+      TopEntry e = worst_entry();
+      if (i < nc) e = s.cand[i];
+      tk.push(i < nc, e);
     }
-  } else {
-    const uint32_t count = s.counters[0], thr = s.counters[2];
-    for (uint32_t base = 0; base < count; base += blockDim.x) {
-      const uint32_t j = base + threadIdx.x;
-      TopEntry e;
-      bool valid = false;
-      if (j < count && s.hit[j] && value_bucket(s.rev[j]) >= thr) {
-        e = TopEntry{s.rev[j], s.qdate[j], s.qkey[j], s.qprio[j], 0};
-        valid = true;
-      }
-      if (__syncthreads_or(valid)) tk.push(valid, e);
+    tk.finish(s.out);
+    return;
+  }
+  tk.init();
+  const uint32_t thr = s.counters[2];
+  const uint32_t nq = (o.n + 3) / 4;
+  const uint32_t per = (nq + gridDim.x - 1) / gridDim.x;
+  const uint32_t q0 = blockIdx.x * per, q1 = min(nq, q0 + per);
+  for (uint32_t base = q0; base < q1; base += blockDim.x) {
+    const uint32_t g = base + threadIdx.x;
+    const bool in = g < q1;
+    const int4 k4 = in ? ld4(o.okey, g) : make_int4(0, 0, 0, 0);
+    const int32_t ks[4] = {k4.x, k4.y, k4.z, k4.w};
+    uint32_t off[4];
+    bool valid[4], cand[4];
+    int64_t rev[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      off[j] = (uint32_t)((int64_t)ks[j] - o.olo);
+      valid[j] = in && 4 * g + j < o.n;
     }
+    q3_candidates(o, off, valid, thr, s, cand, rev);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      TopEntry e = worst_entry();
+      if (cand[j]) e = TopEntry{rev[j], o.odate[4 * g + j], ks[j], o.oprio[4 * g + j], 0};
+      if (__syncthreads_or(cand[j])) tk.push(cand[j], e);
+    }
+  }
+  tk.finish(s.cand + (size_t)blockIdx.x * k);
+  if (!last_block(&s.counters[6])) return;
+  tk.init();
+  const uint32_t m = gridDim.x * (uint32_t)k;
+  for (uint32_t base = 0; base < m; base += blockDim.x) {
+    const uint32_t i = base + threadIdx.x;
+    TopEntry e = worst_entry();
+    if (i < m) e = s.cand[i];
+    // a workgroup's list is padded with worst entries when it had fewer than k
+    tk.push(i < m && !(e.rev == INT64_MIN && e.key == INT32_MAX), e);
   }
   tk.finish(s.out);
 }
@@ -389,18 +649,22 @@ extern "C" int gdb_q3(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* 
   GDB_TRY(column(lineitem, "l_extendedprice", GDB_I64, &clp));
   GDB_TRY(require_resident(customer, "Q3 customer"));
   GDB_TRY(require_resident(orders, "Q3 orders"));
-  if ((customer->nrows && cck->minv < 0) || (orders->nrows && cok->minv < 0))
-    return fail(GDB_E_UNSUPPORTED, "Q3 needs non-negative custkey and orderkey values");
+  if (orders->nrows >= (1ull << 31)) return fail(GDB_E_UNSUPPORTED, "Q3 orders beyond 2^31 rows");
   GDB_HIP(hipSetDevice(ctx->device));
   const uint32_t nc = (uint32_t)customer->nrows, no = (uint32_t)orders->nrows;
   const uint64_t nl = lineitem->nrows;
-  const uint32_t maxck = (uint32_t)std::max<int64_t>(0, cck->maxv);
-  const uint32_t maxok = (uint32_t)std::max<int64_t>(0, cok->maxv);
-  const uint32_t ncw = maxck / 32 + 1;
+  // Key ranges: offsets key - lo in [0, last]; an empty table gets one word.
+  const int32_t clo = nc ? (int32_t)cck->minv : 0, olo = no ? (int32_t)cok->minv : 0;
+  const uint32_t clast = nc ? (uint32_t)(cck->maxv - cck->minv) : 0u;
+  const uint32_t olast = no ? (uint32_t)(cok->maxv - cok->minv) : 0u;
+  const uint32_t cwords = clast / 32 + 1, owords = olast / 32 + 1;
+  const uint32_t rblocks = (owords + RANK_WORDS - 1) / RANK_WORDS;
+  const uint32_t oblocks = ctx->info.cu_count;  // workgroups of the overflow ranking
   const int k = p->top_k;
-  const uint32_t nlist = std::max<uint32_t>(no, 1);
+  const uint32_t nslots = std::max<uint32_t>(no, 1);
 
   Q3Scratch s{};
+  size_t cbm_bytes, cnt_bytes, okw_bytes;
   {
     char* b;
     size_t off = 0;
@@ -409,25 +673,25 @@ extern "C" int gdb_q3(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* 
       off += (bytes + 255) & ~size_t(255);
       return o;
     };
-    const size_t o_cbm = take(ncw * 4ull), o_cnt = take((16 + 4096) * 4), o_qk = take(nlist * 4ull),
-                 o_qd = take(nlist * 4ull), o_qp = take(nlist * 4ull), o_rev = take(nlist * 8ull),
-                 o_hit = take(nlist);
+    cbm_bytes = ((size_t)cwords * 4 + 15) & ~size_t(15);
+    cnt_bytes = (size_t)(16 + HIST_COPIES * 4096) * 4;
+    // whole 16-byte pairs of words, so the rank kernels' pair loads stay inside
+    okw_bytes = ((size_t)owords * 8 + 15) & ~size_t(15);
+    const size_t o_cbm = take(cbm_bytes), o_cnt = take(cnt_bytes), o_bs = take(rblocks * 4ull),
+                 o_rev = take(nslots * 8ull + 32), o_hit = take(nslots + 4ull);
     GDB_TRY(arena(ctx, A_BITMAP, off, (void**)&b));
     s.cbm = (uint32_t*)(b + o_cbm);
     s.counters = (uint32_t*)(b + o_cnt);
     s.hist = s.counters + 16;
-    s.qkey = (int32_t*)(b + o_qk);
-    s.qdate = (int32_t*)(b + o_qd);
-    s.qprio = (int32_t*)(b + o_qp);
+    s.bsum = (uint32_t*)(b + o_bs);
     s.rev = (int64_t*)(b + o_rev);
     s.hit = (uint8_t*)(b + o_hit);
-    const size_t tag_bytes = ((size_t)maxok + 1 + 255) & ~size_t(255);
-    GDB_TRY(arena(ctx, A_Q3BYTE, tag_bytes, (void**)&s.tag));
-    GDB_TRY(arena(ctx, A_Q3MAP, ((size_t)maxok + 1) * 4, (void**)&s.slot));
+    GDB_TRY(arena(ctx, A_Q3MAP, okw_bytes, (void**)&s.okw));
     char* a;
-    GDB_TRY(arena(ctx, A_AUX, sizeof(TopEntry) * ((size_t)CAND_CAP + k) + 64, (void**)&a));
+    const size_t ncand = std::max<size_t>(CAND_CAP, (size_t)oblocks * k);
+    GDB_TRY(arena(ctx, A_AUX, sizeof(TopEntry) * (ncand + k) + 64, (void**)&a));
     s.cand = (TopEntry*)a;
-    s.out = s.cand + CAND_CAP;
+    s.out = s.cand + ncand;
   }
   TopEntry* host;
   GDB_TRY(pinned(ctx, sizeof(TopEntry) * k + 64, (void**)&host));
@@ -437,34 +701,26 @@ extern "C" int gdb_q3(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* 
   const int B = 256;
   const int fill_blocks = ctx->info.cu_count * 4;
   GDB_HIP(hipEventRecord(ctx->ev[0], st));
-  // Epoch of this run in the tag map; a new or reused-after-wrap map is zeroed.
-  const size_t tag_bytes = ((size_t)maxok + 1 + 255) & ~size_t(255);
-  if (ctx->q3_map != (void*)s.tag || ctx->q3_map_bytes < tag_bytes || ctx->q3_epoch == 255) {
-    const size_t alloc_bytes = ctx->arenas[A_Q3BYTE].bytes;
-    q3_fill_bytes<<<fill_blocks, B, 0, st>>>((uint32_t*)s.tag, (uint32_t)(alloc_bytes / 4));
-    ctx->q3_map = s.tag;
-    ctx->q3_map_bytes = alloc_bytes;
-    ctx->q3_epoch = 0;
-  }
-  const uint8_t epoch = ++ctx->q3_epoch;
-  q3_clear<<<std::min<uint32_t>(2048, (ncw + 16 + 4096 + B - 1) / B + 1), B, 0, st>>>(
-      s.cbm, ncw, s.counters, 16 + 4096);
+  q3_clear<<<fill_blocks, B, 0, st>>>((uint4*)s.cbm, (uint32_t)(cbm_bytes / 16), (uint4*)s.counters,
+                                      (uint32_t)(cnt_bytes / 16), (uint4*)s.okw,
+                                      (uint32_t)(okw_bytes / 16));
   if (nc)
-    q3_customer<<<(nc + B - 1) / B, B, 0, st>>>((const int32_t*)cck->dev, (const uint8_t*)cseg->dev,
-                                                 nc, p->segment, maxck, s.cbm);
+    q3_customer<<<std::min<uint32_t>(fill_blocks, ((nc + 3) / 4 + B - 1) / B), B, 0, st>>>(
+        (const int32_t*)cck->dev, (const uint8_t*)cseg->dev, nc, p->segment, clo, clast, s.cbm);
   if (no)
-    q3_orders<<<std::min<uint32_t>(ctx->info.cu_count * 8, (no + 16 * B - 1) / (16 * B)), B, 0, st>>>(
-        (const int32_t*)cok->dev, (const int32_t*)coc->dev, (const int32_t*)cod->dev,
-        (const int32_t*)cop->dev, no, p->date, maxck, maxok, epoch, s);
-  q3_acc_clear<<<fill_blocks, B, 0, st>>>(s.counters, s.rev, s.hit);
+    q3_orders<2><<<((no + 3) / 4 + 2 * B - 1) / (2 * B), B, 0, st>>>((const int32_t*)cok->dev, (const int32_t*)coc->dev,
+                               (const int32_t*)cod->dev, no, p->date, clo, clast, s.cbm, olo, olast,
+                               (uint32_t*)s.okw);
+  q3_rank_sums<<<rblocks, B, 0, st>>>(s.okw, owords, s.bsum);
+  q3_rank_write<<<rblocks, B, 0, st>>>(s.okw, owords, s.bsum, s);
   GDB_HIP(hipGetLastError());
   const int president =
-      resident_blocks(ctx, (const void*)q3_probe<4>, B, 0, ctx->tune[T_Q3].blocks_per_cu);
+      resident_blocks(ctx, (const void*)q3_probe<2>, B, 0, ctx->tune[T_Q3].blocks_per_cu);
   auto consume = [&](const void* const* d, uint32_t rows) -> int {
     const Q3Fact f{(const int32_t*)d[0], (const int32_t*)d[1], (const int32_t*)d[2],
                    (const int64_t*)d[3]};
     const int pblocks = std::max(1, std::min<int>(president, (int)((rows / 4 + B - 1) / B)));
-    q3_probe<4><<<pblocks, B, 0, st>>>(f, rows, p->date, maxok, epoch, s);
+    q3_probe<2><<<pblocks, B, 0, st>>>(f, rows, p->date, olo, olast, s.okw, s);
     GDB_HIP(hipGetLastError());
     return GDB_OK;
   };
@@ -476,10 +732,12 @@ extern "C" int gdb_q3(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* 
     GDB_HIP(hipEventRecord(ctx->ev[1], st));
     GDB_HIP(hipEventRecord(ctx->ev[2], st));
   }
+  const Q3Orders o{(const int32_t*)cok->dev, (const int32_t*)cod->dev, (const int32_t*)cop->dev,
+                   no, olo, olast, s.okw};
   q3_topk_hist<<<ctx->info.cu_count, B, 0, st>>>(s);  // one per CU: fewer bin merges
   q3_topk_threshold<<<1, B, 0, st>>>(k, s);
-  q3_topk_collect<<<fill_blocks, B, 0, st>>>(s);
-  q3_topk_final<<<1, B, 0, st>>>(k, s);
+  if (no) q3_topk_collect<<<((no + 3) / 4 + B - 1) / B, B, 0, st>>>(o, s);
+  q3_topk_rank<<<oblocks, B, 0, st>>>(k, o, s);
   GDB_HIP(hipGetLastError());
   GDB_HIP(hipMemcpyAsync(host, s.out, sizeof(TopEntry) * k, hipMemcpyDeviceToHost, st));
   GDB_HIP(hipMemcpyAsync(host_cnt, s.counters, 8, hipMemcpyDeviceToHost, st));
@@ -487,10 +745,10 @@ extern "C" int gdb_q3(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* 
   // Pipeline bytes: every column the query names, read once (customer 5 B,
   // orders 16 B, fact 20 B per row). Probe bytes: what the probe must read
   // whatever the data, l_orderkey and l_shipdate of every row plus one pass
-  // over the orderkey tag map; discount and price are read only for 4-row
+  // over the order-key words; discount and price are read only for 4-row
   // groups holding a surviving row and are not counted.
   const uint64_t bytes = 5ull * nc + 16ull * no + 20ull * nl;
-  const uint64_t probe_bytes = 8ull * nl + (nl ? (uint64_t)maxok + 1 : 0);
+  const uint64_t probe_bytes = 8ull * nl + (nl ? (uint64_t)owords * 8 : 0);
   GDB_TRY(timing_finish(ctx, timing, bytes, probe_bytes, &rs));
   memset(out, 0, sizeof(*out));
   out->groups = host_cnt[1];
