@@ -269,7 +269,7 @@ int gdb_open(int device, uint64_t hbm_budget_bytes, gdb_ctx** out) {
   ctx->tune[T_Q9] = Tuning{256, 0, 1};
   // The Q13 matcher is instruction-issue bound: more than 4 workgroups per CU
   // only add contention (335 us at 4, 358 us at 7 for 15M comments).
-  ctx->tune[T_Q13] = Tuning{256, 4, 1};
+  ctx->tune[T_Q13] = Tuning{256, 0, 1};
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
     (void)hipGetLastError();

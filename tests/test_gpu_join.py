@@ -155,6 +155,39 @@ def test_q13_words_bytes_after_nul_and_long_patterns(engine):
         assert engine.q13(c, o, w1, w2) == O.q13(cust, od, w1, w2), (w1, w2)
 
 
+@pytest.mark.parametrize("layout", ["dense", "dense_from_0", "permuted", "sparse", "gaps"])
+def test_q13_customer_layouts(engine, layout):
+    """Customer keys dense and ascending (counts turned into the histogram per
+    bucket), starting at 0, in random row order, sparse, or with gaps (counts
+    per custkey, then a histogram over customer rows); several custkey buckets,
+    orders whose custkey has no customer, and customers with more than 1024
+    counted orders (the overflow list)."""
+    rng = np.random.default_rng(17)
+    nc = 60_000
+    keys = {"dense": np.arange(1, nc + 1), "dense_from_0": np.arange(nc),
+            "permuted": rng.permutation(np.arange(1, nc + 1)),
+            "sparse": np.arange(1, nc + 1) * 3 + 2,
+            "gaps": np.sort(rng.choice(np.arange(1, 2 * nc), nc, replace=False))}[layout]
+    keys = keys.astype(np.int32)
+    n = 300_000
+    ock = rng.choice(keys, n)
+    ock[:5000] = keys[7]          # a customer with 5000 orders
+    ock[5000:6500] = keys[-1]     # and one with 1500
+    ock[6500:7000] = keys.max() + 11  # orders of no customer
+    com = np.zeros((n, 80), np.uint8)
+    pick = rng.random(n)
+    for i, t in enumerate((b"very special requests indeed", b"special pending", b"requests special")):
+        m = (pick >= i / 3) & (pick < (i + 1) / 3)
+        com[m, :len(t)] = np.frombuffer(t, np.uint8)
+    cust = dict(c_custkey=keys)
+    od = dict(o_custkey=ock.astype(np.int32), o_comment=com)
+    c, o = engine.table(cust), engine.table(od)
+    got = engine.q13(c, o)
+    assert got == O.q13(cust, od)
+    assert sum(r["custdist"] for r in got) == nc
+    assert max(r["c_count"] for r in got) > 1024
+
+
 # -------------------------------------------------------------- join primitive
 def test_join_count_golden(engine, golden, small):
     for c in golden["primitives"]["join"]:
