@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "internal.h"
@@ -333,11 +334,15 @@ struct Q9Queue {
 // U groups are issued first; then, row slot by row slot, lanes whose row's part
 // bit is set push the row's values onto the wave's queue, and whenever 64 are
 // queued one full wave of entries is processed.
-// The part test reads the folded filter in LDS (up to 2^20 bits, 128 KiB):
-// random 4-byte gathers from the 250 KB global bitmap cost a vector-cache line
-// per lane and took as long as streaming all six columns. Only keys whose
-// folded bit is set read the global bitmap (none when the fold is exact).
-constexpr int PROBE_THREADS = 512;
+// The part test reads the folded filter in LDS: random 4-byte gathers from
+// the 250 KB global bitmap cost a vector-cache line per lane and took as long
+// as streaming all six columns. Only keys whose folded bit is set read the
+// global bitmap (none when the fold is exact). One workgroup of 1024 threads
+// per CU shares one fold (2^19 bits next to sixteen wave queues): sixteen
+// waves per CU stream the columns, against eight with 512-thread workgroups
+// and a 2^20-bit fold (466 -> 412 us at 60M fact rows; the global bitmap reads
+// for the extra fold hits cost nothing measurable).
+constexpr int PROBE_THREADS = 1024;
 template <int U>
 __global__ __launch_bounds__(PROBE_THREADS) void q9_probe(Q9Fact f, uint32_t n, Q9Dims dm,
                                                           Q9Scratch s) {
@@ -390,37 +395,43 @@ __global__ __launch_bounds__(PROBE_THREADS) void q9_probe(Q9Fact f, uint32_t n, 
   };
   const uint32_t nvec = n >> 2;
   const uint32_t nthr = gridDim.x * blockDim.x;
-  // Wave-uniform trip count: every lane of a wave runs the same iterations so
-  // the ballots and queue counters stay in step; lanes past the end load nothing.
-  const uint32_t g0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
-  for (uint32_t gb = g0; gb < nvec; gb += U * nthr) {
-    int4 p4[U], s4[U], o4[U], q4[U], d4[U];
-    longlong2 pa[U], pb[U];
+  // Two register buffers of U groups each: the loads of the next trip are in
+  // flight while the current trip's rows are tested, queued and drained.
+  // Trips are wave-uniform: every lane of a wave runs the same iterations so
+  // the ballots and queue counters stay in step; lanes past the end load
+  // nothing (partkey -1 fails the test).
+  int4 p4[2][U], s4[2][U], o4[2][U], q4[2][U], d4[2][U];
+  longlong2 pa[2][U], pb[2][U];
+  auto load = [&](uint32_t gb, auto B) {
+    constexpr int b = decltype(B)::value;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t g = gb + u * nthr + lane;
       if (g < nvec) {
-        p4[u] = ld4nt(f.pk, g);
-        s4[u] = ld4nt(f.sk, g);
-        o4[u] = ld4nt(f.ok, g);
-        q4[u] = ld4nt(f.qty, g);
-        d4[u] = ld4nt(f.disc, g);
-        pa[u] = ld2nt(f.price, 2 * g);
-        pb[u] = ld2nt(f.price, 2 * g + 1);
+        p4[b][u] = ld4nt(f.pk, g);
+        s4[b][u] = ld4nt(f.sk, g);
+        o4[b][u] = ld4nt(f.ok, g);
+        q4[b][u] = ld4nt(f.qty, g);
+        d4[b][u] = ld4nt(f.disc, g);
+        pa[b][u] = ld2nt(f.price, 2 * g);
+        pb[b][u] = ld2nt(f.price, 2 * g + 1);
       } else {
-        p4[u] = int4{-1, -1, -1, -1};
-        s4[u] = o4[u] = q4[u] = d4[u] = int4{0, 0, 0, 0};
-        pa[u].x = pa[u].y = pb[u].x = pb[u].y = 0;
+        p4[b][u] = int4{-1, -1, -1, -1};
+        s4[b][u] = o4[b][u] = q4[b][u] = d4[b][u] = int4{0, 0, 0, 0};
+        pa[b][u].x = pa[b][u].y = pb[b][u].x = pb[b][u].y = 0;
       }
     }
+  };
+  auto process = [&](auto B) {
+    constexpr int b = decltype(B)::value;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int32_t ps[4] = {p4[u].x, p4[u].y, p4[u].z, p4[u].w};
-      const int32_t ss[4] = {s4[u].x, s4[u].y, s4[u].z, s4[u].w};
-      const int32_t os[4] = {o4[u].x, o4[u].y, o4[u].z, o4[u].w};
-      const int32_t qs[4] = {q4[u].x, q4[u].y, q4[u].z, q4[u].w};
-      const int32_t ds[4] = {d4[u].x, d4[u].y, d4[u].z, d4[u].w};
-      const int64_t pr[4] = {pa[u].x, pa[u].y, pb[u].x, pb[u].y};
+      const int32_t ps[4] = {p4[b][u].x, p4[b][u].y, p4[b][u].z, p4[b][u].w};
+      const int32_t ss[4] = {s4[b][u].x, s4[b][u].y, s4[b][u].z, s4[b][u].w};
+      const int32_t os[4] = {o4[b][u].x, o4[b][u].y, o4[b][u].z, o4[b][u].w};
+      const int32_t qs[4] = {q4[b][u].x, q4[b][u].y, q4[b][u].z, q4[b][u].w};
+      const int32_t ds[4] = {d4[b][u].x, d4[b][u].y, d4[b][u].z, d4[b][u].w};
+      const int64_t pr[4] = {pa[b][u].x, pa[b][u].y, pb[b][u].x, pb[b][u].y};
       bool pass[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) pass[j] = test(ps[j]);
@@ -430,6 +441,19 @@ __global__ __launch_bounds__(PROBE_THREADS) void q9_probe(Q9Fact f, uint32_t n, 
         if (tail - head >= 64) drain(false);
       }
     }
+  };
+  constexpr std::integral_constant<int, 0> B0{};
+  constexpr std::integral_constant<int, 1> B1{};
+  const uint32_t g0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+  const uint32_t step = U * nthr;
+  if (g0 < nvec) load(g0, B0);
+  for (uint32_t gb = g0; gb < nvec; gb += 2 * step) {
+    const bool more = gb + step < nvec;
+    if (more) load(gb + step, B1);
+    process(B0);
+    if (!more) break;
+    if (gb + 2 * step < nvec) load(gb + 2 * step, B0);
+    process(B1);
   }
   if (blockIdx.x == 0 && w == 0) {  // wave-uniform: the last n % 4 rows
     const uint32_t r = nvec * 4 + lane;
