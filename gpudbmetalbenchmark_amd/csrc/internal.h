@@ -74,6 +74,7 @@ struct gdb_ctx {
   size_t host_pinned_bytes = 0;
   gdb::Tuning tune[16];
   uint64_t chunk_rows = 0;         // streamed chunk size, 0 = adaptive
+  uint64_t q20_pairs = 0;          // Q20: (part, supplier) pairs of the last run, sizes the next table
   hipEvent_t sev[4] = {};          // copy-done / slot-free events of the two slots
   std::vector<hipEvent_t> chunk_ev;  // per-chunk compute start/end
 };
@@ -98,6 +99,19 @@ int fill(void* p, int byte, size_t bytes, hipStream_t s);
 int pinned(gdb_ctx* ctx, size_t bytes, void** out);
 // Column lookup with type check; sets the error message on failure.
 int column(const gdb_table* t, const char* name, int type, const Column** out);
+
+// Largest magnitude of an integer column's values (1 for an empty column).
+inline uint64_t magnitude(const Column* c) {
+  if (c->maxv < c->minv) return 1;
+  const uint64_t a = c->minv < 0 ? (uint64_t)0 - (uint64_t)c->minv : (uint64_t)c->minv;
+  const uint64_t b = c->maxv < 0 ? (uint64_t)0 - (uint64_t)c->maxv : (uint64_t)c->maxv;
+  return a > b ? a : b;
+}
+// GDB_OK if n terms each of magnitude at most `term` (a product of column
+// magnitudes, given as a 128-bit value) sum without leaving int64; else a
+// GDB_E_UNSUPPORTED naming `what`. The pipelines sum money in int64, so a
+// table whose bound fails is refused instead of returning a wrapped sum.
+int sum_fits(uint64_t n, unsigned __int128 term, const char* what);
 // Number of workgroups that keep every CU busy for `kernel` at `block` threads
 // and `lds` bytes of dynamic LDS.
 int resident_blocks(gdb_ctx* ctx, const void* kernel, int block, size_t lds, int per_cu_cap);

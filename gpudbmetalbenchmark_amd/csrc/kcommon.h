@@ -109,6 +109,22 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* scratc
   return r;
 }
 
+// True in every thread of the workgroup that finishes last among the grid's
+// workgroups (counted on `done`, zero before the launch); all earlier
+// workgroups' global writes are visible to it afterwards. Every workgroup
+// passes through one atomic on `done`, so this is for grids of a few hundred
+// workgroups at most (same-address atomics serialise at tens of ns each).
+__device__ __forceinline__ bool last_block(uint32_t* done) {
+  __shared__ uint32_t last_sh;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last_sh = atomicAdd(done, 1u) == gridDim.x - 1;
+  __syncthreads();
+  const bool last = last_sh != 0;
+  if (last) __threadfence();
+  return last;
+}
+
 // Monotone bucket of an int64 in [0, 4096): log-linear with 32 sub-buckets per
 // power of two (about 3% wide), negatives mirrored below the non-negatives.
 __device__ __forceinline__ uint32_t log_bucket_mag(uint64_t x) {

@@ -18,7 +18,9 @@
 // Packing is exact while a lane sees < 2^14 rows and every row has
 // qty <= 127, disc <= 100; prices must be < 2^32 and tax <= 100. The kernel
 // counts rows outside that domain and the host rejects such tables rather
-// than return a wrong sum. At block end the lane tables are unpacked and
+// than return a wrong sum; before the launch the host also refuses a table
+// whose rows x max(price) x 100 x max(tax) reaches 2^63, since the sums of
+// price*(100-disc) and price*(100-disc)*tax are int64. At block end the lane tables are unpacked and
 // reduced to 36 per-block partials [bin][cnt, qty, disc, base, dp, dpt], k-major.
 #include <hip/hip_runtime.h>
 
@@ -162,8 +164,12 @@ extern "C" int gdb_q1(gdb_ctx* ctx, const gdb_table* li, const gdb_q1_params* p,
   GDB_TRY(column(li, "l_extendedprice", GDB_I64, &c[4]));
   GDB_TRY(column(li, "l_returnflag", GDB_U8, &c[5]));
   GDB_TRY(column(li, "l_linestatus", GDB_U8, &c[6]));
-  GDB_HIP(hipSetDevice(ctx->device));
   const uint64_t n = li->nrows;
+  // Rows in the per-row domain still sum into int64: the largest term is
+  // price*(100-disc)*tax (disc >= 0 there), which bounds the other sums.
+  GDB_TRY(sum_fits(n, (unsigned __int128)magnitude(c[4]) * 100u * std::max<uint64_t>(1, magnitude(c[3])),
+                   "Q1 SUM(extprice*(100-disc)*tax)"));
+  GDB_HIP(hipSetDevice(ctx->device));
   const Tuning& tu = ctx->tune[T_Q1];
   const int block = (tu.block == 64 || tu.block == 128 || tu.block == 256) ? tu.block : 256;
   const size_t lds = (size_t)(block / 64) * NBIN * NVAL * WAVE * sizeof(unsigned long long);

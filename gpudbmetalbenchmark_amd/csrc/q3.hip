@@ -192,20 +192,6 @@ __global__ __launch_bounds__(256) void q3_orders(const int32_t* __restrict__ oke
   }
 }
 
-// True in every thread of the workgroup that finishes last among the grid's
-// workgroups (counted on `done`, zero before the launch); all earlier
-// workgroups' global writes are visible to it afterwards.
-__device__ __forceinline__ bool last_block(uint32_t* done) {
-  __shared__ uint32_t last_sh;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) last_sh = atomicAdd(done, 1u) == gridDim.x - 1;
-  __syncthreads();
-  const bool last = last_sh != 0;
-  if (last) __threadfence();
-  return last;
-}
-
 // Rank, step 1: qualifying orders per block of RANK_WORDS order-key words
 // (two words per thread).
 __global__ __launch_bounds__(256) void q3_rank_sums(const uint64_t* __restrict__ okw,
@@ -650,6 +636,8 @@ extern "C" int gdb_q3(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* 
   GDB_TRY(require_resident(customer, "Q3 customer"));
   GDB_TRY(require_resident(orders, "Q3 orders"));
   if (orders->nrows >= (1ull << 31)) return fail(GDB_E_UNSUPPORTED, "Q3 orders beyond 2^31 rows");
+  GDB_TRY(sum_fits(lineitem->nrows, (unsigned __int128)magnitude(clp) * (100u + magnitude(cld)),
+                   "Q3 SUM(extprice*(100-discount))"));
   GDB_HIP(hipSetDevice(ctx->device));
   const uint32_t nc = (uint32_t)customer->nrows, no = (uint32_t)orders->nrows;
   const uint64_t nl = lineitem->nrows;

@@ -521,6 +521,10 @@ extern "C" int gdb_q9(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supp
   GDB_TRY(column(lineitem, "l_discount", GDB_I32, &ld));
   GDB_TRY(column(lineitem, "l_extendedprice", GDB_I64, &lp));
   if (pnm->width != NW) return fail(GDB_E_UNSUPPORTED, "p_name must be a 55-byte field");
+  GDB_TRY(sum_fits(lineitem->nrows,
+                   (unsigned __int128)magnitude(lp) * (100u + magnitude(ld)) +
+                       (unsigned __int128)magnitude(psc) * magnitude(lq) * 100u,
+                   "Q9 SUM(amount)"));
   GDB_TRY(require_resident(part, "Q9 part"));
   GDB_TRY(require_resident(supplier, "Q9 supplier"));
   GDB_TRY(require_resident(partsupp, "Q9 partsupp"));

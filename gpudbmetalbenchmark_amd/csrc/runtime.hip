@@ -82,6 +82,14 @@ __global__ __launch_bounds__(256) void fill_bytes(uint8_t* __restrict__ p, uint3
   if (tid < bytes - tail0) p[tail0 + tid] = (uint8_t)word;
 }
 
+int sum_fits(uint64_t n, unsigned __int128 term, const char* what) {
+  const unsigned __int128 lim = (unsigned __int128)1 << 63;
+  if (n && term && (term >= lim || (unsigned __int128)n * term >= lim))
+    return fail(GDB_E_UNSUPPORTED, std::string(what) + " could exceed int64 for this table "
+                                   "(rows x the largest term must stay below 2^63)");
+  return GDB_OK;
+}
+
 int fill(void* ptr, int byte, size_t bytes, hipStream_t s) {
   if (!bytes) return GDB_OK;
   uint8_t* p = (uint8_t*)ptr;

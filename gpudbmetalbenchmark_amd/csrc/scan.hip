@@ -206,6 +206,8 @@ int gdb_q6(gdb_ctx* ctx, const gdb_table* li, const gdb_q6_params* p, gdb_q6_res
   GDB_TRY(column(li, "l_discount", GDB_I32, &c[1]));
   GDB_TRY(column(li, "l_quantity", GDB_I32, &c[2]));
   GDB_TRY(column(li, "l_extendedprice", GDB_I64, &c[3]));
+  GDB_TRY(sum_fits(li->nrows, (unsigned __int128)magnitude(c[3]) * magnitude(c[1]),
+                   "Q6 SUM(extprice*discount)"));
   GDB_HIP(hipSetDevice(ctx->device));
   const uint64_t n = li->nrows;
   const Tuning& tu = ctx->tune[T_Q6];

@@ -323,6 +323,20 @@ int gdb_serve_start(gdb_ctx* ctx, const gdb_serve_cfg* cfg, gdb_server** out) {
   if (cfg->max_batch < 1 || cfg->max_batch > Q3B_MAX || cfg->slots < 1 || cfg->slots > 4 ||
       (cfg->mode != GDB_SERVE_BATCHED && cfg->mode != GDB_SERVE_SERIAL))
     return fail(GDB_E_INVALID, "serve config out of range (max_batch 1..64, slots 1..4)");
+  if (cfg->lineitem_flat) {
+    // The batched passes sum the same int64 terms as gdb_q6 and gdb_q1.
+    const gdb_table* t = cfg->lineitem_flat;
+    const Column *price, *disc, *tax;
+    GDB_TRY(column(t, "l_extendedprice", GDB_I64, &price));
+    GDB_TRY(column(t, "l_discount", GDB_I32, &disc));
+    GDB_TRY(sum_fits(t->nrows, (unsigned __int128)magnitude(price) * magnitude(disc),
+                     "Q6 SUM(extprice*discount)"));
+    if (column(t, "l_tax", GDB_I32, &tax) == GDB_OK)
+      GDB_TRY(sum_fits(t->nrows,
+                       (unsigned __int128)magnitude(price) * (100u + magnitude(disc)) *
+                           std::max<uint64_t>(1, magnitude(tax)),
+                       "Q1 SUM(extprice*(100-disc)*tax)"));
+  }
   GDB_HIP(hipSetDevice(ctx->device));
   gdb_server* S = new gdb_server();
   S->ctx = ctx;

@@ -235,12 +235,14 @@ __device__ __forceinline__ void block_add(int64_t v, int64_t* red, int64_t* out)
 // monotone in the revenue), the bucket b the k-th best falls in, then only the
 // candidates in buckets >= b are collected and ranked by one workgroup. The
 // histogram's total is the number of valid candidates. If heavy ties in bucket b
-// overflow the candidate list, the ranking workgroup scans every candidate.
+// overflow the candidate list, every workgroup of the ranking grid ranks its
+// share of the candidates and the last one to finish ranks their winners.
 constexpr uint32_t THK_BUCKETS = 4096, THK_CAP = 8192;
 struct ThkState {   // device memory, cleared before thk_hist
   uint32_t hist[THK_BUCKETS];
   uint32_t thr, ncand;
   unsigned long long total;
+  uint32_t done;    // ranking workgroups finished (overflow path)
 };
 
 template <class Src>
@@ -263,31 +265,34 @@ __global__ __launch_bounds__(256) void thk_hist(Src src, uint64_t n, const uint3
 namespace {
 // One workgroup of 256: the highest bucket b with at least k candidates in
 // buckets >= b (0 when there are fewer than k), and the candidate total.
+// Thread t owns buckets [16t, 16t + 16); a block scan gives each thread the
+// candidates above its range, so one thread finds b without a serial walk.
 __global__ __launch_bounds__(256) void thk_threshold(int k, ThkState* __restrict__ st,
                                                      unsigned long long* __restrict__ count) {
-  __shared__ uint32_t part[256];
-  uint32_t v = 0;
-  for (int i = 0; i < 16; ++i) v += st->hist[threadIdx.x * 16 + i];
-  part[threadIdx.x] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t acc = 0, b = 0;
-    for (int t = 255; t >= 0; --t) {
-      if (acc + part[t] >= (uint32_t)k) {
-        for (int i = 15; i >= 0; --i) {
-          acc += st->hist[t * 16 + i];
-          if (acc >= (uint32_t)k) {
-            b = (uint32_t)(t * 16 + i);
-            break;
-          }
-        }
+  __shared__ uint32_t scratch[17];
+  uint32_t c[16];
+  uint32_t part = 0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    c[i] = st->hist[threadIdx.x * 16 + i];
+    part += c[i];
+  }
+  uint32_t total;
+  const uint32_t pre = gdbk::block_excl_scan(part, scratch, &total);
+  const uint32_t at = total - pre, above = at - part;  // in buckets >= 16t, >= 16(t+1)
+  if (at >= (uint32_t)k && above < (uint32_t)k) {
+    uint32_t acc = above, b = threadIdx.x * 16;
+    for (int i = 15; i >= 0; --i) {
+      acc += c[i];
+      if (acc >= (uint32_t)k) {
+        b = threadIdx.x * 16 + i;
         break;
       }
-      acc += part[t];
     }
-    unsigned long long total = 0;
-    for (int t = 0; t < 256; ++t) total += part[t];
     st->thr = b;
+  }
+  if (threadIdx.x == 0) {
+    if (total < (uint32_t)k) st->thr = 0u;
     st->ncand = 0u;
     st->total = total;
     if (count) *count = total;
@@ -320,34 +325,53 @@ __global__ __launch_bounds__(256) void thk_collect(Src src, uint64_t n,
   }
 }
 
-// One workgroup ranks the candidates (or, after an overflow, every candidate in
-// buckets >= thr) into out[0, k).
+// Normally workgroup 0 ranks the candidate list into out[0, k) and the
+// others return. After an overflow each workgroup ranks its share of the
+// candidates in buckets >= thr, leaves its k best in cand[b*k, (b+1)*k) (the
+// grid is at most THK_CAP / k workgroups), and the workgroup that finishes
+// last ranks those into out[0, k).
 template <class Src>
 __global__ __launch_bounds__(256) void thk_final(Src src, uint64_t n, const uint32_t* __restrict__ dn,
-                                                 int k, const ThkState* __restrict__ st,
-                                                 const gdbk::TopEntry* __restrict__ cand,
+                                                 int k, ThkState* __restrict__ st,
+                                                 gdbk::TopEntry* __restrict__ cand,
                                                  gdbk::TopEntry* __restrict__ out) {
   __shared__ gdbk::TopEntry buf[1024];
   __shared__ gdbk::TopEntry thr;
   __shared__ int nbuf, have;
   gdbk::BlockTopK<1024> tk{buf, &nbuf, k, &thr, &have};
-  tk.init();
   if (dn) n = n < *dn ? n : *dn;
   const uint32_t nc = st->ncand;
   if (nc <= THK_CAP) {
+    if (blockIdx.x != 0) return;
+    tk.init();
     for (uint32_t b0 = 0; b0 < nc; b0 += blockDim.x) {
       const uint32_t i = b0 + threadIdx.x;
       const bool v = i < nc;
       tk.push(v, v ? cand[i] : gdbk::worst_entry());
     }
-  } else {
-    const uint32_t tb = st->thr;
-    for (uint64_t b0 = 0; b0 < n; b0 += blockDim.x) {
-      const uint64_t i = b0 + threadIdx.x;
-      gdbk::TopEntry e = gdbk::worst_entry();
-      const bool v = i < n && src(i, e) && gdbk::value_bucket(e.rev) >= tb;
-      tk.push(v, e);
-    }
+    tk.finish(out);
+    return;
+  }
+  tk.init();
+  const uint32_t tb = st->thr;
+  const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
+  const uint64_t lo = (uint64_t)blockIdx.x * per, hi = lo + per < n ? lo + per : n;
+  for (uint64_t b0 = lo; b0 < hi; b0 += blockDim.x) {
+    const uint64_t i = b0 + threadIdx.x;
+    gdbk::TopEntry e = gdbk::worst_entry();
+    const bool v = i < hi && src(i, e) && gdbk::value_bucket(e.rev) >= tb;
+    tk.push(v, e);
+  }
+  tk.finish(cand + (size_t)blockIdx.x * k);
+  if (!gdbk::last_block(&st->done)) return;
+  tk.init();
+  const uint32_t m = gridDim.x * (uint32_t)k;
+  for (uint32_t b0 = 0; b0 < m; b0 += blockDim.x) {
+    const uint32_t i = b0 + threadIdx.x;
+    gdbk::TopEntry e = gdbk::worst_entry();
+    if (i < m) e = cand[i];
+    // a workgroup's list is padded with worst entries when it had fewer than k
+    tk.push(i < m && !(e.rev == INT64_MIN && e.key == INT32_MAX), e);
   }
   tk.finish(out);
 }
@@ -378,7 +402,8 @@ int thk_run(gdb_ctx* ctx, Src src, uint64_t n, int k, ThkState* state, gdbk::Top
   thk_threshold<<<1, 256, 0, s>>>(std::max(k, 1), state, count);
   if (k > 0) {
     thk_collect<Src><<<hb * 4, BLOCK, 0, s>>>(src, n, dn, state, cand);
-    thk_final<Src><<<1, BLOCK, 0, s>>>(src, n, dn, k, state, cand, final_out);
+    const int fb = std::max(1, std::min<int>(ctx->info.cu_count, (int)(THK_CAP / (uint32_t)k)));
+    thk_final<Src><<<fb, BLOCK, 0, s>>>(src, n, dn, k, state, cand, final_out);
   }
   GDB_HIP(hipGetLastError());
   return GDB_OK;

@@ -356,12 +356,16 @@ __global__ __launch_bounds__(256) void q20_part(const int32_t* __restrict__ pk,
 
 // The pair table's size is set on the device from the pair count (at least
 // twice the pairs, a power of two), so the host does not wait for the count.
-__global__ void q20_size(const unsigned long long* __restrict__ count,
-                         uint64_t* __restrict__ slots) {
+// The host allocated `cap` slots from an estimate; if the pairs need more,
+// *slots = 0 turns the build, scan and pair kernels into no-ops and *over = 1
+// tells the host to run the query again with a table sized from the count.
+__global__ void q20_size(const unsigned long long* __restrict__ count, uint64_t cap,
+                         uint64_t* __restrict__ slots, uint64_t* __restrict__ over) {
   const uint64_t want = 2 * (*count > 0 ? *count : 1ull);
   uint64_t s = 1;
   while (s < want) s <<= 1;
-  *slots = s;
+  *slots = s <= cap ? s : 0ull;
+  *over = s <= cap ? 0ull : 1ull;
 }
 
 // Clears the first *slots entries of the pair table.
@@ -385,6 +389,7 @@ __global__ __launch_bounds__(256) void q20_build(const int32_t* __restrict__ pk,
                                                  unsigned long long* __restrict__ keys,
                                                  const uint64_t* __restrict__ slots,
                                                  uint32_t* __restrict__ row) {
+  if (*slots == 0) return;  // the table is too small: the host reruns
   const uint64_t mask = *slots - 1;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int32_t p = pk[i];
@@ -417,6 +422,7 @@ __global__ __launch_bounds__(256) void q20_scan(const int32_t* __restrict__ ship
                                                 const uint64_t* __restrict__ slots,
                                                 int64_t* __restrict__ sum,
                                                 uint32_t* __restrict__ lines) {
+  if (*slots == 0) return;  // the table is too small: the host reruns
   const uint64_t mask = *slots - 1;
   walk2(quads(n), [&](uint32_t g) { return ld4nt(ship, g); }, [&](uint32_t g, const int4& d) {
     uint32_t m = 0;
@@ -823,8 +829,24 @@ int gdb_q16(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q16_params* p, gdb_q1
   return GDB_OK;
 }
 
+static int q20_run(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q20_params* p, int32_t* suppkeys,
+                   int64_t cap, int64_t* nrows, gdb_timing* timing, bool* again);
+
 int gdb_q20(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q20_params* p, int32_t* suppkeys,
             int64_t cap, int64_t* nrows, gdb_timing* timing) {
+  bool again = false;
+  GDB_TRY(q20_run(ctx, db, p, suppkeys, cap, nrows, timing, &again));
+  if (again) GDB_TRY(q20_run(ctx, db, p, suppkeys, cap, nrows, timing, &again));
+  return again ? fail(GDB_E_INVALID, "Q20 pair table sizing did not converge") : GDB_OK;
+}
+
+// One run of Q20. The pair table is sized from the last run's pair count
+// (ctx->q20_pairs), or from 1/32 of partsupp on the first run; when the
+// device finds more pairs than that table holds, the run produces nothing,
+// records the count and sets *again, and the caller runs it once more.
+static int q20_run(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q20_params* p, int32_t* suppkeys,
+                   int64_t cap, int64_t* nrows, gdb_timing* timing, bool* again) {
+  *again = false;
   GDB_TRY(args_ok(ctx, db, p, nrows));
   if (cap < 0 || (cap > 0 && !suppkeys) || p->frac_den <= 0 || p->frac_num < 0)
     return fail(GDB_E_INVALID, "Q20 needs cap >= 0 with suppkeys, frac_num >= 0, frac_den > 0");
@@ -863,7 +885,7 @@ int gdb_q20(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q20_params* p, int32_
   GDB_TRY(arena(ctx, A_BITMAP, words * 4, (void**)&bits));
   GDB_TRY(arena(ctx, A_AUX2, sspan, (void**)&good));
   GDB_TRY(arena(ctx, A_OUT, ACC_BYTES, (void**)&dout));
-  GDB_TRY(pinned(ctx, 16 + std::max<uint64_t>(ns, 1) * 4, (void**)&hall));
+  GDB_TRY(pinned(ctx, 32 + std::max<uint64_t>(ns, 1) * 4, (void**)&hall));
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
   GDB_TRY(fill(bits, 0, words * 4, s));
@@ -877,11 +899,13 @@ int gdb_q20(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q20_params* p, int32_
         (const int32_t*)qp->dev, (uint32_t)nps, bits, (unsigned long long*)dout);
   GDB_HIP(hipGetLastError());
   GDB_TRY(acc_finish(ctx, (int64_t*)dout, 0, 1));
-  // The pair table is allocated for every partsupp row being a pair; its used
-  // size comes from the count on the device (dout[2]).
-  const uint64_t smax = pow2_at_least(2 * std::max<uint64_t>(nps, 1));
+  // The pair table holds twice the expected pairs (the last run's count, or
+  // 1/32 of partsupp); its used size comes from the count on the device
+  // (dout[2]), and dout[3] flags a count beyond the table.
+  const uint64_t expect = ctx->q20_pairs ? ctx->q20_pairs : std::max<uint64_t>(nps / 32, 1);
+  const uint64_t smax = pow2_at_least(2 * expect);
   uint64_t* dslots = dout + 2;
-  q20_size<<<1, 1, 0, s>>>((const unsigned long long*)dout, dslots);
+  q20_size<<<1, 1, 0, s>>>((const unsigned long long*)dout, smax, dslots, dout + 3);
   unsigned long long* keys;
   uint32_t *row, *lines;
   int64_t* sum;
@@ -912,19 +936,24 @@ int gdb_q20(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q20_params* p, int32_
         (const int32_t*)sk->dev, (const int32_t*)sn->dev, (uint32_t)ns, p->nation, good,
         out_keys, (unsigned long long*)(dout + 1));
   GDB_HIP(hipGetLastError());
-  // The count and every possible supplier key (at most one per supplier) come back
-  // in one copy into pinned memory: no wait for the count before the keys.
-  GDB_HIP(hipMemcpyAsync(hall, dout, 16, hipMemcpyDeviceToHost, s));
+  // The counts and every possible supplier key (at most one per supplier) come
+  // back in one copy into pinned memory: no wait for the count before the keys.
+  GDB_HIP(hipMemcpyAsync(hall, dout, 32, hipMemcpyDeviceToHost, s));
   if (ns)
-    GDB_HIP(hipMemcpyAsync(hall + 2, out_keys, ns * 4, hipMemcpyDeviceToHost, s));
+    GDB_HIP(hipMemcpyAsync(hall + 4, out_keys, ns * 4, hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
   // SQL columns: lineitem shipdate, partkey, suppkey, quantity (16 B); partsupp key
   // pair + availqty (12 B); part key + name; supplier key + nation. Mandatory in
   // q20_scan: l_shipdate, 4 B per line.
   GDB_TRY(timing_finish(ctx, timing, 16 * nl + 12 * nps + (4 + pn->width) * np + 8 * ns,
                         4 * nl, &rs));
+  ctx->q20_pairs = std::max<uint64_t>(hall[0], 1);  // pairs found: sizes the next table
+  if (hall[3]) {
+    *again = true;
+    return GDB_OK;
+  }
   const uint64_t n = hall[1];
-  int32_t* got = (int32_t*)(hall + 2);
+  int32_t* got = (int32_t*)(hall + 4);
   std::sort(got, got + n);
   *nrows = (int64_t)n;
   for (int64_t i = 0; i < std::min<int64_t>(cap, (int64_t)n); ++i) suppkeys[i] = got[i];

@@ -178,10 +178,23 @@ static void q1_range(void* p, uint64_t lo, uint64_t hi, int tid) {
   memcpy(a->charge[tid], ch, sizeof(ch));
 }
 
-void oracle_q1(uint64_t n, const int32_t* shipdate, const int32_t* quantity,
-               const int32_t* discount, const int32_t* tax, const int64_t* extprice,
-               const uint8_t* returnflag, const uint8_t* linestatus, int32_t cutoff,
-               int nthreads, int64_t* out, uint64_t* charge_lo, int64_t* charge_hi) {
+int oracle_q1(uint64_t n, const int32_t* shipdate, const int32_t* quantity,
+              const int32_t* discount, const int32_t* tax, const int64_t* extprice,
+              const uint8_t* returnflag, const uint8_t* linestatus, int32_t cutoff,
+              int nthreads, int64_t* out, uint64_t* charge_lo, int64_t* charge_hi) {
+  /* The sums of price and price*(100-disc) are int64 (the charge is int128):
+   * refuse (-1) a table where rows x max|price| x (100 + max|disc|) reaches
+   * 2^63 instead of returning a wrapped sum. */
+  unsigned __int128 mp = 0, md = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    const unsigned __int128 p = extprice[i] < 0 ? (unsigned __int128)(-(__int128)extprice[i])
+                                                : (unsigned __int128)extprice[i];
+    const unsigned __int128 d = discount[i] < 0 ? (unsigned __int128)(-(__int128)discount[i])
+                                                : (unsigned __int128)discount[i];
+    if (p > mp) mp = p;
+    if (d > md) md = d;
+  }
+  if ((unsigned __int128)n * mp * (100 + md) >= ((unsigned __int128)1 << 63)) return -1;
   q1_arg* a = (q1_arg*)calloc(1, sizeof(q1_arg));
   a->ship = shipdate; a->qty = quantity; a->disc = discount; a->tax = tax;
   a->price = extprice; a->rf = returnflag; a->ls = linestatus; a->cutoff = cutoff;
@@ -197,6 +210,7 @@ void oracle_q1(uint64_t n, const int32_t* shipdate, const int32_t* quantity,
     charge_hi[g] = (int64_t)(ch[g] >> 64);
   }
   free(a);
+  return 0;
 }
 
 /* --------------------------------------------------------------------- Q3 */

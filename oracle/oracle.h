@@ -29,11 +29,13 @@ void oracle_q6(uint64_t n, const int32_t* shipdate, const int32_t* discount,
 /* Q1: per group g = rf*2+ls (rf A,N,R = 0,1,2; ls F,O = 0,1), out[g*5 + k]
  * for k = sum_qty, sum_base, sum_disc_price, sum_disc, count; the 128-bit
  * sum_charge in charge_lo[g] (low 64 bits) and charge_hi[g] (high 64 bits). */
-void oracle_q1(uint64_t n, const int32_t* shipdate, const int32_t* quantity,
-               const int32_t* discount, const int32_t* tax, const int64_t* extprice,
-               const uint8_t* returnflag, const uint8_t* linestatus, int32_t cutoff,
-               int nthreads, int64_t* out /* [30] */, uint64_t* charge_lo /* [6] */,
-               int64_t* charge_hi /* [6] */);
+/* Returns 0, or -1 (nothing written) when rows x max|price| x (100 + max|disc|)
+ * reaches 2^63, where its int64 sums could wrap. */
+int oracle_q1(uint64_t n, const int32_t* shipdate, const int32_t* quantity,
+              const int32_t* discount, const int32_t* tax, const int64_t* extprice,
+              const uint8_t* returnflag, const uint8_t* linestatus, int32_t cutoff,
+              int nthreads, int64_t* out /* [30] */, uint64_t* charge_lo /* [6] */,
+              int64_t* charge_hi /* [6] */);
 
 /* Q3: top-k rows (orderkey, orderdate, shippriority, revenue) by revenue desc,
  * orderdate asc, orderkey asc; returns the number written, *groups = number of

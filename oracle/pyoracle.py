@@ -23,7 +23,7 @@ def lib():
             raise RuntimeError(f"{LIB} missing: run make -C oracle")
         _lib = C.CDLL(LIB)
         _lib.oracle_q6.restype = None
-        _lib.oracle_q1.restype = None
+        _lib.oracle_q1.restype = C.c_int
         _lib.oracle_q3.restype = C.c_int
         _lib.oracle_q9.restype = C.c_int
         _lib.oracle_q13.restype = C.c_int
@@ -63,9 +63,11 @@ def q1(li, cutoff=2435, nthreads=1):
     out = np.zeros(30, np.int64)
     lo = np.zeros(6, np.uint64)
     hi = np.zeros(6, np.int64)
-    lib().oracle_q1(C.c_uint64(len(price)), *[_p(x, C.c_int32) for x in cols],
-                    _p(price, C.c_int64), _p(rf, C.c_uint8), _p(ls, C.c_uint8), cutoff, nthreads,
-                    _p(out, C.c_int64), _p(lo, C.c_uint64), _p(hi, C.c_int64))
+    rc = lib().oracle_q1(C.c_uint64(len(price)), *[_p(x, C.c_int32) for x in cols],
+                         _p(price, C.c_int64), _p(rf, C.c_uint8), _p(ls, C.c_uint8), cutoff,
+                         nthreads, _p(out, C.c_int64), _p(lo, C.c_uint64), _p(hi, C.c_int64))
+    if rc != 0:
+        raise OverflowError("Q1 sums could exceed int64 for this table")
     rows = []
     for g in range(6):
         v = out[g * 5:(g + 1) * 5]

@@ -138,3 +138,38 @@ def test_select_sizes(engine, n):
     assert np.array_equal(r["bitmap"], ref["bitmap"])
     assert np.array_equal(r["rowids"], np.flatnonzero(col < 17))
     assert engine.sum(t, "x") == int(col.astype(np.int64).sum())
+
+
+def test_q1_q6_sums_near_the_int64_limit(engine):
+    """Q1 over rows at the edge of its exact domain (prices near 2^32, tax
+    100) with rows x max(price) x 100 x max(tax) just under 2^63: every sum,
+    including the 128-bit charge, equals Python's integer arithmetic. A table
+    past the bound is refused (GDB_E_UNSUPPORTED), as is a Q6 table whose
+    SUM(extprice*discount) could leave int64."""
+    from test_oracle_golden import extreme_q1_table, q1_bigint
+    n = 200_000  # 200000 * (2^32 - 1) * 100 * 100 < 2^63
+    li = extreme_q1_table(n)
+    exp = q1_bigint(li, 1500)
+    t = table(engine, li, Q1_COLS)
+    got = engine.q1(t, 1500)
+    assert len(got) == len(exp)
+    for r in got:
+        e = exp[(r["returnflag"], r["linestatus"])]
+        assert [r["sum_qty"], r["sum_base_price"], r["sum_disc_price"], r["sum_charge"],
+                r["sum_disc"], r["count"]] == e
+    q1_equal(got, O.q1(li, 1500, nthreads=8))
+    t.free()
+    li2 = extreme_q1_table(220_000)  # past the bound
+    t2 = table(engine, li2, Q1_COLS)
+    with pytest.raises(GdbError) as e:
+        engine.q1(t2, 1500)
+    assert e.value.code == GDB_E_UNSUPPORTED
+    t2.free()
+    m = 1 << 20
+    q6 = dict(l_shipdate=np.zeros(m, np.int32), l_discount=np.full(m, 10, np.int32),
+              l_quantity=np.ones(m, np.int32), l_extendedprice=np.full(m, 1 << 40, np.int64))
+    t3 = table(engine, q6, Q6_COLS)
+    with pytest.raises(GdbError) as e:
+        engine.q6(t3, 0, 1, 0, 10, 2)
+    assert e.value.code == GDB_E_UNSUPPORTED
+    t3.free()

@@ -289,3 +289,19 @@ def test_streamed_lineitem_gives_the_same_results(engine):
     finally:
         d.free()
         engine.set_chunk_rows(0)
+
+
+def test_q20_pair_table_grows_past_its_estimate():
+    """Q20's pair table is sized from the last run's pair count (1/32 of
+    partsupp on a fresh context). Prefixes that select more parts than that
+    make the device flag the overflow and the query rerun with a table sized
+    from the count: the answers stay exact, in either order of prefixes."""
+    tabs = tpch.generate(0.02, 9)
+    with Engine(0) as e:
+        d = TpchDb(e, tabs)
+        try:
+            for pre in ("a", "forest", "b", "a", "lemon"):
+                assert d.q20(pre, 0, 2600, "JAPAN", 1, 10) == \
+                    T.q20(tabs, pre.encode(), 0, 2600, 12, 1, 10), pre
+        finally:
+            d.free()

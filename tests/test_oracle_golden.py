@@ -229,3 +229,51 @@ def test_substring_helpers_against_python():
             junk[n + 1:] = np.frombuffer(b"splaaspecla" * 3, np.uint8)[:29 - n]
             assert O.contains(junk, "spec") == ("spec" in t)
             assert O.like_two(junk, "sp", "la") == bool(re.search("sp.*la", t))
+
+
+def extreme_q1_table(n, seed=3):
+    """Rows at the edge of Q1's exact domain: extendedprice up to 2^32-1,
+    tax up to 100, discount 0..100, quantity up to 127, every group."""
+    rng = np.random.default_rng(seed)
+    return dict(
+        l_shipdate=rng.integers(0, 2000, n).astype(np.int32),
+        l_quantity=rng.integers(120, 128, n).astype(np.int32),
+        l_discount=rng.choice(np.array([0, 0, 0, 100, 37], np.int32), n),
+        l_tax=np.where(rng.random(n) < 0.5, 100, rng.integers(0, 101, n)).astype(np.int32),
+        l_extendedprice=(2**32 - 1 - rng.integers(0, 1000, n)).astype(np.int64),
+        l_returnflag=rng.choice(np.frombuffer(b"ANR", np.uint8), n),
+        l_linestatus=rng.choice(np.frombuffer(b"FO", np.uint8), n))
+
+
+def q1_bigint(li, cutoff):
+    """Q1 sums with Python integers (no overflow possible)."""
+    out = {}
+    for i in np.nonzero(li["l_shipdate"] <= cutoff)[0]:
+        key = (chr(li["l_returnflag"][i]), chr(li["l_linestatus"][i]))
+        p, d, t = int(li["l_extendedprice"][i]), int(li["l_discount"][i]), int(li["l_tax"][i])
+        g = out.setdefault(key, [0, 0, 0, 0, 0, 0])
+        g[0] += int(li["l_quantity"][i])
+        g[1] += p
+        g[2] += p * (100 - d)
+        g[3] += p * (100 - d) * (100 + t)
+        g[4] += d
+        g[5] += 1
+    return out
+
+
+def test_oracle_q1_extreme_values_exact_or_refused():
+    """Near the int64 limit the oracle's sums must still equal Python's
+    integer arithmetic; past its bound (rows x max|price| x (100 + max|disc|)
+    >= 2^63) it must refuse rather than wrap."""
+    li = extreme_q1_table(40_000)
+    exp = q1_bigint(li, 1500)
+    got = O.q1(li, 1500, nthreads=4)
+    assert len(got) == len(exp)
+    for r in got:
+        e = exp[(r["returnflag"], r["linestatus"])]
+        assert [r["sum_qty"], r["sum_base_price"], r["sum_disc_price"], r["sum_charge"],
+                r["sum_disc"], r["count"]] == e
+    big = dict(li)
+    big["l_extendedprice"] = np.full(len(li["l_shipdate"]), 2**55, np.int64)
+    with pytest.raises(OverflowError):
+        O.q1({k: v[:3] for k, v in big.items()}, 1500)
