@@ -187,6 +187,15 @@ class Engine:
                                abi.tref(timing)))
         return dict(revenue=r.revenue, rows=r.rows)
 
+    def q6_noalloc(self, lineitem, date_lo=731, date_hi=1096, disc_lo=5, disc_hi=7, qty_lt=24,
+                   timing=None):
+        """gdb_q6_noalloc: Q6 with no allocation in the call (fixed device scratch)."""
+        p = Q6Params(date_lo, date_hi, disc_lo, disc_hi, qty_lt)
+        r = Q6Result()
+        abi.check(lib().gdb_q6_noalloc(self._ctx, lineitem.handle, C.byref(p), C.byref(r),
+                                       abi.tref(timing)))
+        return dict(revenue=r.revenue, rows=r.rows)
+
     def q1(self, lineitem, cutoff=2435, timing=None):
         p = Q1Params(cutoff)
         r = Q1Result()

@@ -178,6 +178,8 @@ PROTOTYPES = {
     "gdb_table_is_host": (C.c_int, [P]),
     "gdb_set_chunk_rows": (C.c_int, [P, C.c_uint64]),
     "gdb_q6": (C.c_int, [P, P, C.POINTER(Q6Params), C.POINTER(Q6Result), C.POINTER(Timing)]),
+    "gdb_q6_noalloc": (C.c_int, [P, P, C.POINTER(Q6Params), C.POINTER(Q6Result),
+                                 C.POINTER(Timing)]),
     "gdb_q1": (C.c_int, [P, P, C.POINTER(Q1Params), C.POINTER(Q1Result), C.POINTER(Timing)]),
     "gdb_q3": (C.c_int, [P, P, P, P, C.POINTER(Q3Params), C.POINTER(Q3Result),
                          C.POINTER(Timing)]),

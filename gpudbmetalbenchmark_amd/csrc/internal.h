@@ -141,6 +141,9 @@ int join_derived_bits(uint64_t nb);
 
 int run_over(gdb_ctx* ctx, const gdb_table* t, const Column* const* cols, int ncols,
              const Consume& fn, RunStats* rs);
+// Looks up, once per device (that device current), the device address of the
+// scratch array gdb_q6_noalloc uses instead of allocations.
+int noalloc_scratch_init(int device);
 // Rejects host tables where a pipeline needs the table resident.
 int require_resident(const gdb_table* t, const char* what);
 

@@ -275,8 +275,8 @@ int gdb_open(int device, uint64_t hbm_budget_bytes, gdb_ctx** out) {
   ctx->tune[T_Q1] = Tuning{128, 4, 2};
   // Q9's probe: one 4-row group in flight per lane measured fastest.
   ctx->tune[T_Q9] = Tuning{256, 0, 1};
-  // The Q13 matcher is instruction-issue bound: more than 4 workgroups per CU
-  // only add contention (335 us at 4, 358 us at 7 for 15M comments).
+  // The Q13 matcher runs best at its occupancy limit (5 waves per SIMD):
+  // 253 us for 15M comments, against 278 us at 4 workgroups per CU.
   ctx->tune[T_Q13] = Tuning{256, 0, 1};
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
@@ -290,6 +290,9 @@ int gdb_open(int device, uint64_t hbm_budget_bytes, gdb_ctx** out) {
     delete ctx;
     return fail(GDB_E_HIP, "stream creation failed");
   }
+  // The allocation-free entry point's scratch address on this device; a
+  // failure here is reported by that entry point, which retries the lookup.
+  if (noalloc_scratch_init(device) != GDB_OK) (void)hipGetLastError();
   for (auto& e : ctx->ev) {
     if (hipEventCreate(&e) != hipSuccess) {
       delete ctx;

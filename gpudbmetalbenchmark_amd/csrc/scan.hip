@@ -10,6 +10,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "internal.h"
 #include "kcommon.h"
@@ -253,6 +255,112 @@ int gdb_q6(gdb_ctx* ctx, const gdb_table* li, const gdb_q6_params* p, gdb_q6_res
   GDB_TRY(timing_finish(ctx, timing, 20ull * n, 20ull * n, &rs));
   out->revenue = host[0];
   out->rows = (uint64_t)host[1];
+  return GDB_OK;
+}
+
+// ---- Q6 without allocation --------------------------------------------------
+// gdb_q6_noalloc runs the same kernels as gdb_q6 with their scratch (the
+// per-workgroup partials and the two totals) in `scratch`, a fixed array in
+// device memory that every GPU the library runs on holds until the process
+// ends, instead of the context's arenas and pinned staging. Its address on a
+// device comes from hipGetSymbolAddress once per device (gdb_open calls
+// noalloc_scratch_init with the device current); host code never uses the
+// array's name as an address. Every call on a GPU shares that GPU's array:
+// calls that overlap on one GPU race.
+}  // extern "C"
+
+namespace {
+constexpr size_t NOALLOC_SCRATCH_BYTES = 1u << 20;  // 1 MiB per device
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+int64_t* g_scratch[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+static __device__ __align__(256) int64_t scratch[NOALLOC_SCRATCH_BYTES / sizeof(int64_t)];
+
+namespace gdb {
+int noalloc_scratch_init(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+    return fail(GDB_E_UNSUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (g_scratch[device]) return GDB_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    (void)hipGetLastError();
+    return fail(GDB_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  }
+  g_scratch[device] = (int64_t*)p;
+  return GDB_OK;
+}
+}  // namespace gdb
+
+extern "C" {
+
+int gdb_q6_noalloc(gdb_ctx* ctx, const gdb_table* li, const gdb_q6_params* p, gdb_q6_result* out,
+                   gdb_timing* timing) {
+  if (!ctx || !li || !p || !out) return fail(GDB_E_INVALID, "null argument");
+  const Column* c[4];
+  GDB_TRY(column(li, "l_shipdate", GDB_I32, &c[0]));
+  GDB_TRY(column(li, "l_discount", GDB_I32, &c[1]));
+  GDB_TRY(column(li, "l_quantity", GDB_I32, &c[2]));
+  GDB_TRY(column(li, "l_extendedprice", GDB_I64, &c[3]));
+  GDB_TRY(require_resident(li, "the lineitem table of gdb_q6_noalloc"));  // streaming allocates
+  GDB_TRY(sum_fits(li->nrows, (unsigned __int128)magnitude(c[3]) * magnitude(c[1]),
+                   "Q6 SUM(extprice*discount)"));
+  if (li->nrows >= (1ull << 32)) return fail(GDB_E_UNSUPPORTED, "gdb_q6_noalloc takes below 2^32 rows");
+  GDB_HIP(hipSetDevice(ctx->device));
+  // Set up by gdb_open; retried here (device current) if that failed.
+  GDB_TRY(noalloc_scratch_init(ctx->device));
+  int64_t* base = nullptr;
+  {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    base = g_scratch[ctx->device];
+  }
+  const uint32_t n = (uint32_t)li->nrows;
+  const Tuning& tu = ctx->tune[T_Q6];
+  const int block = 256;
+  const int resident = resident_blocks(ctx, (const void*)q6_scan<2>, block, 0, tu.blocks_per_cu);
+  const int blocks = std::max(1, std::min<int>(resident, (int)((n / 4 + block - 1) / block)));
+  // partials [2][blocks], then the two totals at a 256-byte boundary
+  const size_t part_words = ((size_t)2 * blocks + 31) & ~size_t(31);
+  const size_t need = (part_words + 2) * sizeof(int64_t);
+  if (need > NOALLOC_SCRATCH_BYTES)
+    return fail(GDB_E_UNSUPPORTED, "gdb_q6_noalloc: this launch shape needs " + std::to_string(need) +
+                                       " bytes of scratch, above its " +
+                                       std::to_string(NOALLOC_SCRATCH_BYTES) + "-byte maximum");
+  int64_t* partials = base;
+  int64_t* dout = base + part_words;
+  const Q6P kp{p->date_lo, p->date_hi, p->disc_lo, p->disc_hi, p->qty_lt};
+  hipStream_t s = ctx->stream;
+  const int32_t *sp = (const int32_t*)c[0]->dev, *dp = (const int32_t*)c[1]->dev,
+                *qp = (const int32_t*)c[2]->dev;
+  const int64_t* pp = (const int64_t*)c[3]->dev;
+  GDB_HIP(hipEventRecord(ctx->ev[0], s));
+  GDB_HIP(hipEventRecord(ctx->ev[1], s));
+  if (n == 0) {
+    GDB_TRY(fill(dout, 0, 2 * sizeof(int64_t), s));  // a fill kernel, not a memset node
+  } else {
+    // Every workgroup writes both of its partials and the finaliser
+    // overwrites both totals, so nothing in the scratch needs clearing.
+    if (tu.unroll >= 4)
+      q6_scan<4><<<blocks, block, 0, s>>>(sp, dp, qp, pp, n, kp, partials);
+    else if (tu.unroll == 1)
+      q6_scan<1><<<blocks, block, 0, s>>>(sp, dp, qp, pp, n, kp, partials);
+    else
+      q6_scan<2><<<blocks, block, 0, s>>>(sp, dp, qp, pp, n, kp, partials);
+    GDB_HIP(hipGetLastError());
+    sum_columns<0><<<2, 256, 0, s>>>(partials, (uint32_t)blocks, dout);
+    GDB_HIP(hipGetLastError());
+  }
+  GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  int64_t h[2] = {0, 0};  // the caller's stack, not pinned staging
+  GDB_HIP(hipMemcpyAsync(h, dout, sizeof(h), hipMemcpyDeviceToHost, s));
+  GDB_HIP(hipEventRecord(ctx->ev[5], s));
+  GDB_HIP(hipStreamSynchronize(s));
+  GDB_TRY(timing_finish(ctx, timing, 20ull * n, 20ull * n, nullptr));
+  out->revenue = h[0];
+  out->rows = (uint64_t)h[1];
   return GDB_OK;
 }
 

@@ -144,6 +144,19 @@ typedef struct { int64_t revenue; uint64_t rows; } gdb_q6_result;
 int gdb_q6(gdb_ctx* ctx, const gdb_table* lineitem, const gdb_q6_params* p, gdb_q6_result* out,
            gdb_timing* timing);
 
+/* Q6 without allocation: no counterpart in the reference. For callers that
+ * cannot allocate device or pinned memory during a query (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * same kernels and result as gdb_q6 on a resident table, with their scratch
+ * (per-workgroup partials and totals) in a fixed 1 MiB array in device memory
+ * that each GPU holds until the process ends; no hipMalloc, hipFree or pinned
+ * allocation happens in the call. A launch shape needing more than 1 MiB of
+ * scratch fails with GDB_E_UNSUPPORTED, as does a host (streamed) table.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (from several threads or contexts) race; serialise them. */
+int gdb_q6_noalloc(gdb_ctx* ctx, const gdb_table* lineitem, const gdb_q6_params* p,
+                   gdb_q6_result* out, gdb_timing* timing);
+
 /* Q1 grouped aggregation (reference src/q1.cpp:8-173, kernels/q1.metal:39-121,
  * sql/q1.sql). Columns: l_shipdate, l_quantity, l_discount, l_tax (int32),
  * l_extendedprice (int64), l_returnflag, l_linestatus (uint8 chars).
