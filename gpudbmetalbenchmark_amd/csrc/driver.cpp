@@ -15,6 +15,7 @@
 // timed runs; the median GPU time is reported (the reference times one run).
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -110,6 +111,13 @@ uint64_t load_tbl(const std::string& file, const std::vector<TblCol>& cols, bool
   return n;
 }
 
+// Latency of the timed runs of the last `timed` call: nearest-rank p50 and
+// p99 of GPU + CPU post time, and the serial rate they imply.
+struct RunStatsLine {
+  int runs = 0;
+  double p50 = 0, p99 = 0, qps = 0;
+} g_last;
+
 void print_timing(const char* q, uint64_t rows, double parse_ms, double gpu_ms, double post_ms,
                   uint64_t bytes) {
   printf("\n%s | %llu rows\n", q, (unsigned long long)rows);
@@ -118,9 +126,13 @@ void print_timing(const char* q, uint64_t rows, double parse_ms, double gpu_ms, 
   printf("  CPU Post Process:   %10.2f ms\n", post_ms);
   printf("  Total Execution:    %10.2f ms  (GPU + CPU post)\n", gpu_ms + post_ms);
   printf("  Bandwidth:          %10.2f GB/s\n", bytes / (gpu_ms / 1000.0) / 1e9);
+  if (g_last.runs)
+    printf("  Latency p50 / p99:  %10.3f / %.3f ms  (%d runs, %.1f queries/s one at a time)\n",
+           g_last.p50, g_last.p99, g_last.runs, g_last.qps);
 }
 
-// Runs fn 2 + 10 times; returns the median timing of the timed runs.
+// Runs fn 2 + 10 times; returns the median timing of the timed runs and
+// records their latency percentiles in g_last.
 gdb_timing timed(const std::function<void(gdb_timing*)>& fn) {
   std::vector<gdb_timing> ts;
   for (int i = 0; i < 12; ++i) {
@@ -128,6 +140,18 @@ gdb_timing timed(const std::function<void(gdb_timing*)>& fn) {
     fn(&t);
     if (i >= 2) ts.push_back(t);
   }
+  std::vector<double> lat;
+  double sum = 0;
+  for (const gdb_timing& t : ts) {
+    lat.push_back(t.gpu_ms + t.post_ms);
+    sum += t.gpu_ms + t.post_ms;
+  }
+  std::sort(lat.begin(), lat.end());
+  auto rank = [&](double q) { return lat[std::min(lat.size() - 1, (size_t)std::ceil(q * lat.size()) - 1)]; };
+  g_last.runs = (int)lat.size();
+  g_last.p50 = rank(0.50);
+  g_last.p99 = rank(0.99);
+  g_last.qps = sum > 0 ? 1000.0 * lat.size() / sum : 0.0;
   std::sort(ts.begin(), ts.end(), [](const gdb_timing& a, const gdb_timing& b) { return a.gpu_ms < b.gpu_ms; });
   return ts[ts.size() / 2];
 }

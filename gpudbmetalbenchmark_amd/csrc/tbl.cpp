@@ -154,8 +154,8 @@ bool parse_date(const char* s, const char* e, int32_t* out) {
   const char* p = s;
   for (; k < 3; ++k) {
     const char* q = p;
-    while (p < e && *p >= '0' && *p <= '9') v[k] = v[k] * 10 + (*p++ - '0');
-    if (p == q) return false;
+    while (p < e && *p >= '0' && *p <= '9' && p - q < 5) v[k] = v[k] * 10 + (*p++ - '0');
+    if (p == q || p - q > 4) return false;  // at most four digits per part
     if (k < 2) {
       if (p == e || *p != '-') return false;
       ++p;

@@ -9,8 +9,10 @@ appends one row per query to results/gpu_results.csv with the header
 `timestamp,scale_factor,query,gpu_exec_ms,cpu_post_ms,total_exec_ms,gpu_name,memory_gb`
 (scripts/benchmark_gpu.sh:35, :47-68), plus a per-query log. This does the same
 over gdb_bench, whose output keeps those lines, and appends two columns the
-reference lacks: `gbps` (the pipeline's algorithmic bytes over its GPU time)
-and `pct_hbm_peak`.
+reference lacks: `gbps` (the pipeline's algorithmic bytes over its GPU time),
+`pct_hbm_peak`, and the latency of the timed runs gdb_bench makes per query:
+`p50_ms` and `p99_ms` (nearest rank, GPU + CPU post time) and `qps` (queries
+per second one at a time).
 """
 import argparse
 import datetime
@@ -22,7 +24,7 @@ import sys
 PKG = os.path.dirname(os.path.abspath(__file__))
 EXE = os.path.join(PKG, "gdb_bench")
 HEADER = ("timestamp,scale_factor,query,gpu_exec_ms,cpu_post_ms,total_exec_ms,gpu_name,memory_gb,"
-          "gbps,pct_hbm_peak")
+          "gbps,pct_hbm_peak,p50_ms,p99_ms,qps")
 HBM_PEAK_GBPS = 8000.0
 SF_LABEL = {"sf1": "SF-1", "sf10": "SF-10", "sf100": "SF-100"}
 
@@ -39,13 +41,18 @@ def parse_output(text):
     dev = re.search(r"^Device: (.*?) \((.*?)\), (\d+) CUs, ([0-9.]+) GB HBM", text, re.M)
     recs = []
     for m in re.finditer(r"^(?:SF100 )?(Q\d+|selection|aggregation|join) \| (\d+) rows\n(.*?Total "
-                         r"Execution:[^\n]*)(?:\n\s+Bandwidth:\s+([0-9.]+) GB/s)?", text, re.M | re.S):
+                         r"Execution:[^\n]*)(?:\n\s+Bandwidth:\s+([0-9.]+) GB/s)?"
+                         r"(?:\n\s+Latency p50 / p99:\s+([0-9.]+) / ([0-9.]+) ms\s+\((\d+) runs, "
+                         r"([0-9.]+) queries/s one at a time\))?", text, re.M | re.S):
         block = m.group(3)
         recs.append(dict(query=m.group(1), rows=int(m.group(2)),
                          gpu_exec_ms=_metric(block, "GPU Execution"),
                          cpu_post_ms=_metric(block, "CPU Post Process") or 0.0,
                          total_exec_ms=_metric(block, "Total Execution") or 0.0,
-                         gbps=float(m.group(4)) if m.group(4) else None))
+                         gbps=float(m.group(4)) if m.group(4) else None,
+                         p50_ms=float(m.group(5)) if m.group(5) else None,
+                         p99_ms=float(m.group(6)) if m.group(6) else None,
+                         qps=float(m.group(8)) if m.group(8) else None))
     return dict(gpu_name=dev.group(1) if dev else "unknown",
                 memory_gb=float(dev.group(4)) if dev else 0.0, queries=recs)
 
@@ -54,9 +61,11 @@ def csv_rows(parsed, sf_label, timestamp):
     out = []
     for r in parsed["queries"]:
         gbps = r["gbps"] if r["gbps"] is not None else 0.0
+        lat = (f"{r['p50_ms']:.3f},{r['p99_ms']:.3f},{r['qps']:.1f}" if r.get("p50_ms") is not None
+               else ",,")
         out.append(f"{timestamp},{sf_label},{r['query']},{r['gpu_exec_ms']:.2f},"
                    f"{r['cpu_post_ms']:.2f},{r['total_exec_ms']:.2f},{parsed['gpu_name']},"
-                   f"{parsed['memory_gb']:.0f},{gbps:.2f},{100.0 * gbps / HBM_PEAK_GBPS:.2f}")
+                   f"{parsed['memory_gb']:.0f},{gbps:.2f},{100.0 * gbps / HBM_PEAK_GBPS:.2f},{lat}")
     return out
 
 
