@@ -138,6 +138,10 @@ class Engine:
     def hbm_used(self):
         return lib().gdb_hbm_used(self._ctx)
 
+    def release_scratch(self):
+        """Frees the scratch arenas the pipelines grew (tables stay)."""
+        abi.check(lib().gdb_release_scratch(self._ctx))
+
     def autotune(self, pipeline, tables, reps=3):
         """Times the candidate launch shapes of `pipeline` ('q6', 'q1', 'q3',
         'q9', 'q13', 'join') on `tables` (argument order of the pipeline; join:

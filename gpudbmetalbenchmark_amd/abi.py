@@ -163,6 +163,8 @@ PROTOTYPES = {
     "gdb_device_info_get": (C.c_int, [P, C.POINTER(DeviceInfo)]),
     "gdb_hbm_used": (C.c_uint64, [P]),
     "gdb_debug_fail_allocs": (None, [P, C.c_int]),
+    "gdb_debug_fail_allocs_after": (None, [P, C.c_int, C.c_int]),
+    "gdb_release_scratch": (C.c_int, [P]),
     "gdb_bandwidth_probe": (C.c_int, [P, C.c_uint64, C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double)]),
     "gdb_h2d_probe": (C.c_int, [P, P, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),

@@ -67,7 +67,8 @@ struct gdb_ctx {
   gdb_device_info info{};
   uint64_t budget = 0;
   uint64_t used = 0;
-  int fail_allocs = 0;
+  int fail_allocs = 0;              // test hook: allocations still to fail
+  int fail_skip = 0;                // test hook: allocations to pass before those
   std::mutex mu;
   std::vector<gdb::Arena> arenas;  // indexed by purpose slot
   void* host_pinned = nullptr;     // small pinned staging for results

@@ -26,8 +26,12 @@ int dev_alloc(gdb_ctx* ctx, void** p, size_t bytes) {
   *p = nullptr;
   if (bytes == 0) bytes = 256;
   if (ctx->fail_allocs > 0) {
-    --ctx->fail_allocs;
-    return fail(GDB_E_NOMEM, "injected device allocation failure");
+    if (ctx->fail_skip > 0) {
+      --ctx->fail_skip;
+    } else {
+      --ctx->fail_allocs;
+      return fail(GDB_E_NOMEM, "injected device allocation failure");
+    }
   }
   if (ctx->budget && ctx->used + bytes > ctx->budget)
     return fail(GDB_E_NOMEM, "HBM budget exceeded: " + std::to_string(ctx->used + bytes) +
@@ -335,8 +339,25 @@ int gdb_device_info_get(gdb_ctx* ctx, gdb_device_info* out) {
 
 uint64_t gdb_hbm_used(gdb_ctx* ctx) { return ctx ? ctx->used : 0; }
 
-void gdb_debug_fail_allocs(gdb_ctx* ctx, int count) {
-  if (ctx) ctx->fail_allocs = count;
+void gdb_debug_fail_allocs(gdb_ctx* ctx, int count) { gdb_debug_fail_allocs_after(ctx, 0, count); }
+
+void gdb_debug_fail_allocs_after(gdb_ctx* ctx, int skip, int count) {
+  if (!ctx) return;
+  ctx->fail_skip = std::max(skip, 0);
+  ctx->fail_allocs = count;
+}
+
+int gdb_release_scratch(gdb_ctx* ctx) {
+  if (!ctx) return fail(GDB_E_INVALID, "null argument");
+  GDB_HIP(hipSetDevice(ctx->device));
+  GDB_HIP(hipStreamSynchronize(ctx->stream));
+  GDB_HIP(hipStreamSynchronize(ctx->copy_stream));
+  for (Arena& a : ctx->arenas) {
+    dev_free(ctx, a.ptr, a.bytes);
+    a.ptr = nullptr;
+    a.bytes = 0;
+  }
+  return GDB_OK;
 }
 
 int gdb_table_create(gdb_ctx* ctx, const gdb_col_desc* cols, int ncols, uint64_t nrows,

@@ -80,8 +80,17 @@ int gdb_bandwidth_probe(gdb_ctx* ctx, uint64_t bytes, int reps, double* read_gbp
  * are copied on; `reps` timed copies after one warm-up. The ceiling of a
  * streamed pipeline (the reference's SF100 paths, src/infra.h:816-869). */
 int gdb_h2d_probe(gdb_ctx* ctx, const void* host, uint64_t bytes, int reps, double* gbps);
+/* Gives the context's scratch arenas (partials, maps, hash tables, stream
+ * slots; they grow on demand and are otherwise kept between calls) back to
+ * the device after the work queued on the context has finished. Tables stay.
+ * No counterpart in the reference, whose Metal buffers live only as long as
+ * each query function. Like every call on a context, not concurrent with
+ * another call on it (a server running on the context included). */
+int gdb_release_scratch(gdb_ctx* ctx);
 /* Test hook: the next `count` device allocations fail with GDB_E_NOMEM. */
 void gdb_debug_fail_allocs(gdb_ctx* ctx, int count);
+/* Test hook: the next `skip` device allocations succeed, then `count` fail. */
+void gdb_debug_fail_allocs_after(gdb_ctx* ctx, int skip, int count);
 
 /* ---- resident columnar tables (replaces newBuffer(ptr, bytes, Shared),
  *      reference src/q6.cpp:42-47, and the per-query column vectors) -------- */
