@@ -213,6 +213,84 @@ int oracle_q1(uint64_t n, const int32_t* shipdate, const int32_t* quantity,
   return 0;
 }
 
+/* -------------------------------------------------------------- key index */
+/* Unique int32 keys (primary keys: custkey, orderkey, partkey, suppkey) of the
+ * rows that pass a filter -> an int32 value per key (a row number, a nation,
+ * a year). Any key layout: a direct array over [min, max] of the kept keys
+ * when that span is at most 4 x rows + 2^20 values, else the (key, value)
+ * pairs sorted by key and searched by bisection; both give the same answers,
+ * the direct array keeps benchmark-size runs fast. -1 = no such key. */
+typedef struct { int32_t key, val; } kv_t;
+typedef struct {
+  int32_t lo;
+  uint64_t span;   /* direct: values over [lo, lo + span) */
+  int32_t* direct;
+  kv_t* pairs;     /* sorted, when not direct */
+  uint64_t n;
+} keymap;
+
+static int kv_cmp(const void* x, const void* y) {
+  const kv_t *a = (const kv_t*)x, *b = (const kv_t*)y;
+  return (a->key > b->key) - (a->key < b->key);
+}
+
+/* keep: NULL = every row; vals: NULL = 1 for every kept key. */
+static void keymap_build(keymap* m, uint64_t n, const int32_t* keys, const int32_t* vals,
+                         const uint8_t* keep) {
+  memset(m, 0, sizeof(*m));
+  int64_t lo = INT64_MAX, hi = INT64_MIN;
+  uint64_t kept = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    if (keep && !keep[i]) continue;
+    if (keys[i] < lo) lo = keys[i];
+    if (keys[i] > hi) hi = keys[i];
+    ++kept;
+  }
+  m->n = kept;
+  if (!kept) return;
+  const uint64_t span = (uint64_t)(hi - lo) + 1;
+  if (span <= 4 * kept + (1u << 20)) {
+    m->lo = (int32_t)lo;
+    m->span = span;
+    m->direct = (int32_t*)malloc(span * sizeof(int32_t));
+    memset(m->direct, 0xff, span * sizeof(int32_t));
+    for (uint64_t i = 0; i < n; ++i)
+      if (!keep || keep[i]) m->direct[(int64_t)keys[i] - lo] = vals ? vals[i] : 1;
+    return;
+  }
+  m->pairs = (kv_t*)malloc(kept * sizeof(kv_t));
+  uint64_t j = 0;
+  for (uint64_t i = 0; i < n; ++i)
+    if (!keep || keep[i]) m->pairs[j++] = (kv_t){keys[i], vals ? vals[i] : 1};
+  qsort(m->pairs, kept, sizeof(kv_t), kv_cmp);
+}
+
+static int32_t keymap_get(const keymap* m, int32_t key) {
+  if (m->direct) {
+    const int64_t d = (int64_t)key - m->lo;
+    return d >= 0 && (uint64_t)d < m->span ? m->direct[d] : -1;
+  }
+  uint64_t lo = 0, hi = m->n;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) / 2;
+    if (m->pairs[mid].key < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo < m->n && m->pairs[lo].key == key ? m->pairs[lo].val : -1;
+}
+
+static void keymap_free(keymap* m) {
+  free(m->direct);
+  free(m->pairs);
+}
+
+/* 0..n-1 as int32 row numbers. */
+static int32_t* row_numbers(uint64_t n) {
+  int32_t* r = (int32_t*)calloc(n ? n : 1, sizeof(int32_t));
+  for (uint64_t i = 0; i < n; ++i) r[i] = (int32_t)i;
+  return r;
+}
+
 /* --------------------------------------------------------------------- Q3 */
 /* sql/q3.sql:4-26; filters of kernels/q3.metal:52 (segment), :70 (orderdate <
  * D), :167 (shipdate > D); order of src/infra.h:547-551 plus orderkey asc as
@@ -220,9 +298,9 @@ int oracle_q1(uint64_t n, const int32_t* shipdate, const int32_t* quantity,
 typedef struct {
   const int32_t *lok, *lship, *ldisc;
   const int64_t* lprice;
-  const int32_t* omap; /* orderkey -> orders row, or -1 */
-  int32_t maxok, date;
-  int64_t* rev;        /* per orderkey */
+  const keymap* omap;  /* orderkey of a qualifying order -> its orders row */
+  int32_t date;
+  int64_t* rev;        /* per orders row */
   uint8_t* hit;
 } q3_arg;
 
@@ -231,10 +309,10 @@ static void q3_range(void* p, uint64_t lo, uint64_t hi, int tid) {
   q3_arg* a = (q3_arg*)p;
   for (uint64_t i = lo; i < hi; ++i) {
     if (a->lship[i] <= a->date) continue;
-    int32_t k = a->lok[i];
-    if (k < 0 || k > a->maxok || a->omap[k] < 0) continue;
-    __atomic_fetch_add(&a->rev[k], a->lprice[i] * (int64_t)(100 - a->ldisc[i]), __ATOMIC_RELAXED);
-    a->hit[k] = 1;
+    const int32_t r = keymap_get(a->omap, a->lok[i]);
+    if (r < 0) continue;
+    __atomic_fetch_add(&a->rev[r], a->lprice[i] * (int64_t)(100 - a->ldisc[i]), __ATOMIC_RELAXED);
+    a->hit[r] = 1;
   }
 }
 
@@ -254,32 +332,23 @@ int oracle_q3(uint64_t nc, const int32_t* c_custkey, const uint8_t* c_segment, u
               int32_t segment, int32_t date, int k, int nthreads, int32_t* out_orderkey,
               int32_t* out_orderdate, int32_t* out_shippriority, int64_t* out_revenue,
               int64_t* groups) {
-  int32_t maxck = 0, maxok = 0;
-  for (uint64_t i = 0; i < nc; ++i) if (c_custkey[i] > maxck) maxck = c_custkey[i];
-  for (uint64_t i = 0; i < no; ++i) if (o_orderkey[i] > maxok) maxok = o_orderkey[i];
-  uint8_t* cseg = (uint8_t*)calloc((size_t)maxck + 1, 1);
-  for (uint64_t i = 0; i < nc; ++i)
-    if (c_custkey[i] >= 0 && c_segment[i] == (uint8_t)segment) cseg[c_custkey[i]] = 1;
-  int32_t* omap = (int32_t*)malloc(((size_t)maxok + 1) * sizeof(int32_t));
-  memset(omap, 0xff, ((size_t)maxok + 1) * sizeof(int32_t));
-  for (uint64_t i = 0; i < no; ++i) {
-    int32_t ck = o_custkey[i];
-    if (o_orderdate[i] < date && ck >= 0 && ck <= maxck && cseg[ck] && o_orderkey[i] >= 0)
-      omap[o_orderkey[i]] = (int32_t)i;
-  }
-  q3_arg a = {l_orderkey, l_shipdate, l_discount, l_extprice, omap, maxok, date,
-              (int64_t*)calloc((size_t)maxok + 1, sizeof(int64_t)),
-              (uint8_t*)calloc((size_t)maxok + 1, 1)};
+  uint8_t* keep = (uint8_t*)malloc((nc > no ? nc : no) + 1);
+  for (uint64_t i = 0; i < nc; ++i) keep[i] = c_segment[i] == (uint8_t)segment;
+  keymap cseg, omap;
+  keymap_build(&cseg, nc, c_custkey, NULL, keep);
+  for (uint64_t i = 0; i < no; ++i)
+    keep[i] = o_orderdate[i] < date && keymap_get(&cseg, o_custkey[i]) >= 0;
+  int32_t* orow = row_numbers(no);
+  keymap_build(&omap, no, o_orderkey, orow, keep);
+  q3_arg a = {l_orderkey, l_shipdate, l_discount, l_extprice, &omap, date,
+              (int64_t*)calloc(no + 1, sizeof(int64_t)), (uint8_t*)calloc(no + 1, 1)};
   par_for(nl, nthreads, q3_range, &a);
   int64_t ng = 0;
-  for (int32_t key = 0; key <= maxok; ++key) ng += a.hit[key];
+  for (uint64_t r = 0; r < no; ++r) ng += a.hit[r];
   q3_row* rows = (q3_row*)malloc((size_t)(ng ? ng : 1) * sizeof(q3_row));
   int64_t m = 0;
-  for (int32_t key = 0; key <= maxok; ++key) {
-    if (!a.hit[key]) continue;
-    int32_t r = omap[key];
-    rows[m++] = (q3_row){key, o_orderdate[r], o_shippriority[r], a.rev[key]};
-  }
+  for (uint64_t r = 0; r < no; ++r)
+    if (a.hit[r]) rows[m++] = (q3_row){o_orderkey[r], o_orderdate[r], o_shippriority[r], a.rev[r]};
   qsort(rows, (size_t)m, sizeof(q3_row), q3_cmp);
   int w = (int)(m < k ? m : k);
   for (int i = 0; i < w; ++i) {
@@ -289,7 +358,9 @@ int oracle_q3(uint64_t nc, const int32_t* c_custkey, const uint8_t* c_segment, u
     out_revenue[i] = rows[i].rev;
   }
   *groups = ng;
-  free(rows); free(a.rev); free(a.hit); free(omap); free(cseg);
+  free(rows); free(a.rev); free(a.hit); free(orow); free(keep);
+  keymap_free(&omap);
+  keymap_free(&cseg);
   return w;
 }
 
@@ -303,10 +374,9 @@ int oracle_q3(uint64_t nc, const int32_t* c_custkey, const uint8_t* c_segment, u
 typedef struct {
   const int32_t *lpk, *lsk, *lok, *lqty, *ldisc;
   const int64_t* lprice;
-  const uint8_t* green;  /* by partkey */
-  const int32_t* snat;   /* by suppkey, -1 none */
-  const int32_t* oyear;  /* by orderkey, -1 none */
-  int32_t maxpk, maxsk, maxok;
+  const keymap* green;   /* partkey of a part whose name has the pattern -> 1 */
+  const keymap* snat;    /* suppkey -> nation */
+  const keymap* oyear;   /* orderkey -> year */
   uint64_t* ht_key; int64_t* ht_cost; uint64_t ht_mask;
   int64_t (*acc)[25 * Q9_YEARS];
   uint8_t (*seen)[25 * Q9_YEARS];
@@ -322,10 +392,11 @@ static void q9_range(void* p, uint64_t lo, uint64_t hi, int tid) {
   int64_t* acc = a->acc[tid];
   uint8_t* seen = a->seen[tid];
   for (uint64_t i = lo; i < hi; ++i) {
-    int32_t pk = a->lpk[i], sk = a->lsk[i], ok = a->lok[i];
-    if (pk < 0 || pk > a->maxpk || !a->green[pk]) continue;
-    if (sk < 0 || sk > a->maxsk || a->snat[sk] < 0) continue;
-    if (ok < 0 || ok > a->maxok || a->oyear[ok] < 0) continue;
+    const int32_t pk = a->lpk[i], sk = a->lsk[i];
+    if (keymap_get(a->green, pk) < 0) continue;
+    const int32_t nat = keymap_get(a->snat, sk);
+    const int32_t year = keymap_get(a->oyear, a->lok[i]);
+    if (nat < 0 || year < 0) continue;
     uint64_t key = ((uint64_t)(uint32_t)pk << 32) | (uint32_t)sk;
     uint64_t h = q9_hash(key) & a->ht_mask;
     int64_t cost = -1;
@@ -335,7 +406,7 @@ static void q9_range(void* p, uint64_t lo, uint64_t hi, int tid) {
     }
     if (cost < 0) continue;
     int64_t amount = a->lprice[i] * (int64_t)(100 - a->ldisc[i]) - cost * (int64_t)a->lqty[i] * 100;
-    int slot = a->snat[sk] * Q9_YEARS + (a->oyear[ok] - 1970);
+    int slot = nat * Q9_YEARS + (year - 1970);
     acc[slot] += amount;
     seen[slot] = 1;
   }
@@ -363,21 +434,15 @@ int oracle_q9(uint64_t np, const int32_t* p_partkey, const uint8_t* p_name, int 
               const int32_t* l_quantity, const int32_t* l_discount, const int64_t* l_extprice,
               const char* pattern, int nthreads, int32_t* out_nation, int32_t* out_year,
               int64_t* out_profit, int max_rows) {
-  int32_t maxpk = 0, maxsk = 0, maxok = 0;
-  for (uint64_t i = 0; i < np; ++i) if (p_partkey[i] > maxpk) maxpk = p_partkey[i];
-  for (uint64_t i = 0; i < ns; ++i) if (s_suppkey[i] > maxsk) maxsk = s_suppkey[i];
-  for (uint64_t i = 0; i < no; ++i) if (o_orderkey[i] > maxok) maxok = o_orderkey[i];
-  uint8_t* green = (uint8_t*)calloc((size_t)maxpk + 1, 1);
+  uint8_t* named = (uint8_t*)malloc(np + 1);
   for (uint64_t i = 0; i < np; ++i)
-    if (p_partkey[i] >= 0 && oracle_contains(p_name + i * (uint64_t)name_width, name_width, pattern))
-      green[p_partkey[i]] = 1;
-  int32_t* snat = (int32_t*)malloc(((size_t)maxsk + 1) * sizeof(int32_t));
-  memset(snat, 0xff, ((size_t)maxsk + 1) * sizeof(int32_t));
-  for (uint64_t i = 0; i < ns; ++i) if (s_suppkey[i] >= 0) snat[s_suppkey[i]] = s_nationkey[i];
-  int32_t* oyear = (int32_t*)malloc(((size_t)maxok + 1) * sizeof(int32_t));
-  memset(oyear, 0xff, ((size_t)maxok + 1) * sizeof(int32_t));
-  for (uint64_t i = 0; i < no; ++i)
-    if (o_orderkey[i] >= 0) oyear[o_orderkey[i]] = oracle_year_of_day(o_orderdate[i]);
+    named[i] = (uint8_t)oracle_contains(p_name + i * (uint64_t)name_width, name_width, pattern);
+  int32_t* years = (int32_t*)malloc((no + 1) * sizeof(int32_t));
+  for (uint64_t i = 0; i < no; ++i) years[i] = oracle_year_of_day(o_orderdate[i]);
+  keymap green, snat, oyear;
+  keymap_build(&green, np, p_partkey, NULL, named);
+  keymap_build(&snat, ns, s_suppkey, s_nationkey, NULL);
+  keymap_build(&oyear, no, o_orderkey, years, NULL);
   uint64_t cap = 16;
   while (cap < 2 * nps + 16) cap <<= 1;
   uint64_t* hk = (uint64_t*)malloc(cap * sizeof(uint64_t));
@@ -385,7 +450,7 @@ int oracle_q9(uint64_t np, const int32_t* p_partkey, const uint8_t* p_name, int 
   memset(hk, 0xff, cap * sizeof(uint64_t));
   for (uint64_t i = 0; i < nps; ++i) {
     int32_t pk = ps_partkey[i];
-    if (pk < 0 || pk > maxpk || !green[pk]) continue;
+    if (keymap_get(&green, pk) < 0) continue;
     uint64_t key = ((uint64_t)(uint32_t)pk << 32) | (uint32_t)ps_suppkey[i];
     uint64_t h = q9_hash(key) & (cap - 1);
     while (hk[h] != UINT64_MAX && hk[h] != key) h = (h + 1) & (cap - 1);
@@ -393,8 +458,8 @@ int oracle_q9(uint64_t np, const int32_t* p_partkey, const uint8_t* p_name, int 
     hc[h] = ps_supplycost[i];
   }
   int T = nthreads < 1 ? 1 : (nthreads > 256 ? 256 : nthreads);
-  q9_arg a = {l_partkey, l_suppkey, l_orderkey, l_quantity, l_discount, l_extprice, green, snat,
-              oyear, maxpk, maxsk, maxok, hk, hc, cap - 1,
+  q9_arg a = {l_partkey, l_suppkey, l_orderkey, l_quantity, l_discount, l_extprice, &green, &snat,
+              &oyear, hk, hc, cap - 1,
               calloc((size_t)T, sizeof(int64_t[25 * Q9_YEARS])),
               calloc((size_t)T, sizeof(uint8_t[25 * Q9_YEARS]))};
   par_for(nl, T, q9_range, &a);
@@ -413,7 +478,10 @@ int oracle_q9(uint64_t np, const int32_t* p_partkey, const uint8_t* p_name, int 
     out_year[i] = rows[i].year;
     out_profit[i] = rows[i].profit;
   }
-  free(a.acc); free(a.seen); free(hk); free(hc); free(oyear); free(snat); free(green);
+  free(a.acc); free(a.seen); free(hk); free(hc); free(years); free(named);
+  keymap_free(&green);
+  keymap_free(&snat);
+  keymap_free(&oyear);
   return w;
 }
 
@@ -426,8 +494,7 @@ typedef struct {
   const uint8_t* com;
   int width;
   const char *w1, *w2;
-  const int32_t* crow; /* custkey -> customer row or -1 */
-  int32_t maxck;
+  const keymap* crow;  /* custkey -> customer row */
   uint32_t* cnt;       /* per customer row */
 } q13_arg;
 
@@ -435,10 +502,10 @@ static void q13_range(void* p, uint64_t lo, uint64_t hi, int tid) {
   (void)tid;
   q13_arg* a = (q13_arg*)p;
   for (uint64_t i = lo; i < hi; ++i) {
-    int32_t ck = a->ock[i];
-    if (ck < 0 || ck > a->maxck || a->crow[ck] < 0) continue;
+    const int32_t r = keymap_get(a->crow, a->ock[i]);
+    if (r < 0) continue;
     if (oracle_like_two(a->com + i * (uint64_t)a->width, a->width, a->w1, a->w2)) continue;
-    __atomic_fetch_add(&a->cnt[a->crow[ck]], 1u, __ATOMIC_RELAXED);
+    __atomic_fetch_add(&a->cnt[r], 1u, __ATOMIC_RELAXED);
   }
 }
 
@@ -452,12 +519,10 @@ static int q13_cmp(const void* x, const void* y) {
 int oracle_q13(uint64_t nc, const int32_t* c_custkey, uint64_t no, const int32_t* o_custkey,
                const uint8_t* o_comment, int comment_width, const char* word1, const char* word2,
                int nthreads, int32_t* out_count, int64_t* out_custdist, int max_rows) {
-  int32_t maxck = 0;
-  for (uint64_t i = 0; i < nc; ++i) if (c_custkey[i] > maxck) maxck = c_custkey[i];
-  int32_t* crow = (int32_t*)malloc(((size_t)maxck + 1) * sizeof(int32_t));
-  memset(crow, 0xff, ((size_t)maxck + 1) * sizeof(int32_t));
-  for (uint64_t i = 0; i < nc; ++i) if (c_custkey[i] >= 0) crow[c_custkey[i]] = (int32_t)i;
-  q13_arg a = {o_custkey, o_comment, comment_width, word1, word2, crow, maxck,
+  int32_t* rows_of = row_numbers(nc);
+  keymap crow;
+  keymap_build(&crow, nc, c_custkey, rows_of, NULL);
+  q13_arg a = {o_custkey, o_comment, comment_width, word1, word2, &crow,
                (uint32_t*)calloc(nc ? nc : 1, sizeof(uint32_t))};
   par_for(no, nthreads, q13_range, &a);
   uint32_t maxc = 0;
@@ -470,7 +535,8 @@ int oracle_q13(uint64_t nc, const int32_t* c_custkey, uint64_t no, const int32_t
   qsort(rows, (size_t)m, sizeof(q13_row), q13_cmp);
   int w = m < max_rows ? m : max_rows;
   for (int i = 0; i < w; ++i) { out_count[i] = rows[i].c; out_custdist[i] = rows[i].d; }
-  free(rows); free(hist); free(a.cnt); free(crow);
+  free(rows); free(hist); free(a.cnt); free(rows_of);
+  keymap_free(&crow);
   return w;
 }
 
