@@ -248,3 +248,43 @@ def test_join_full_config_all_strategies(engine, join_strategy, full):
     assert -2 <= got["radix_bits"] <= 14
     assert engine.get_tuning("join")["radix_bits"] == got["radix_bits"]
     assert engine.join_count(kb, "k", kp, "k") == exp
+
+
+def test_q9_parts_with_more_than_four_suppliers(engine):
+    """Parts with 1..7 suppliers: the direct partsupp table holds four per part,
+    so a fifth sends the query through the hash table; both must equal the
+    oracle (as must parts with fewer than four)."""
+    rng = np.random.default_rng(3)
+    np_, ns = 400, 12
+    name = np.zeros((np_, 55), np.uint8)
+    for i in range(np_):
+        t = (b"forest green" if i % 3 else b"lime")
+        name[i, :len(t)] = np.frombuffer(t, np.uint8)
+    pa = dict(p_partkey=np.arange(1, np_ + 1, dtype=np.int32), p_name=name)
+    su = dict(s_suppkey=np.arange(1, ns + 1, dtype=np.int32),
+              s_nationkey=rng.integers(0, 25, ns).astype(np.int32))
+    for most in (4, 7):
+        pk, sk = [], []
+        for p in range(1, np_ + 1):
+            k = 1 + (p * 5) % most
+            for s_ in rng.choice(np.arange(1, ns + 1), k, replace=False):
+                pk.append(p)
+                sk.append(int(s_))
+        ps = dict(ps_partkey=np.array(pk, np.int32), ps_suppkey=np.array(sk, np.int32),
+                  ps_supplycost=rng.integers(100, 100_000, len(pk)).astype(np.int64))
+        no = 3000
+        od = dict(o_orderkey=(np.arange(no) * 4 + 1).astype(np.int32),
+                  o_orderdate=rng.integers(0, 2406, no).astype(np.int32))
+        row = rng.integers(0, len(pk), 20000)
+        li = dict(l_partkey=ps["ps_partkey"][row], l_suppkey=ps["ps_suppkey"][row],
+                  l_orderkey=od["o_orderkey"][rng.integers(0, no, 20000)],
+                  l_quantity=rng.integers(1, 51, 20000).astype(np.int32),
+                  l_discount=rng.integers(0, 11, 20000).astype(np.int32),
+                  l_extendedprice=rng.integers(90_000, 10_500_000, 20000).astype(np.int64))
+        tabs = [engine.table(t) for t in (pa, su, ps, od, li)]
+        try:
+            for pat in ("green", "lime", "e"):
+                assert engine.q9(*tabs, pat) == O.q9(pa, su, ps, od, li, pat), (most, pat)
+        finally:
+            for t in tabs:
+                t.free()
