@@ -40,7 +40,7 @@ namespace {
 constexpr int TILE = 256;
 constexpr int CW = 80;         // comment width
 constexpr int NBINS = 1024;
-constexpr int BUCKET_SHIFT = 13;                 // 8192 custkeys per bucket (32 KiB of LDS counters)
+constexpr int BUCKET_SHIFT = 13;                 // 8192 custkeys per bucket (32 KiB of LDS counters); 4096 sped the count up by 9 us and slowed the scatter by 10
 constexpr int STILE = 8192;                      // keys per LDS-sorted tile of the scatter
 constexpr uint32_t MAX_BUCKETS = 4096;
 
@@ -265,9 +265,49 @@ __global__ __launch_bounds__(TILE) void q13_match(const uint8_t* __restrict__ co
 // LDS first (ranks from LDS atomics, a scan of the bucket counts), so each
 // bucket's keys of a tile leave as one contiguous run: key-by-key stores sent
 // every lane of a store to a different segment.
+// Per bucket (one workgroup each): the exclusive prefix over the match
+// blocks of the block's key count in the bucket (hist is bucket-major, so a
+// bucket's counts are contiguous) and the bucket's total. The buckets' start
+// positions, the prefix over these totals, are formed where they are used
+// (a few thousand values at most); one launch instead of a three-pass scan
+// of the whole histogram.
+__global__ __launch_bounds__(256) void q13_offsets(const uint32_t* __restrict__ hist, uint32_t G,
+                                                   uint32_t* __restrict__ offs,
+                                                   uint32_t* __restrict__ btot) {
+  __shared__ uint32_t scratch[17];
+  const uint32_t k = blockIdx.x;
+  uint32_t run = 0;
+  for (uint32_t g0 = 0; g0 < G; g0 += 256) {
+    const uint32_t g = g0 + threadIdx.x;
+    const uint32_t c = g < G ? hist[(size_t)k * G + g] : 0u;
+    uint32_t tot;
+    const uint32_t pre = block_excl_scan(c, scratch, &tot);
+    if (g < G) offs[(size_t)k * G + g] = run + pre;
+    run += tot;
+  }
+  if (threadIdx.x == 0) btot[k] = run;
+}
+
+// Start of every bucket (exclusive prefix of btot) into base[0 .. nb), and
+// the overall count; a workgroup-wide pass over the nb bucket totals.
+__device__ __forceinline__ uint32_t bucket_starts(const uint32_t* __restrict__ btot, uint32_t nb,
+                                                  uint32_t* base, uint32_t* scratch) {
+  uint32_t run = 0;
+  for (uint32_t k0 = 0; k0 < nb; k0 += blockDim.x) {
+    const uint32_t k = k0 + threadIdx.x;
+    const uint32_t c = k < nb ? btot[k] : 0u;
+    uint32_t tot;
+    const uint32_t pre = block_excl_scan(c, scratch, &tot);
+    if (k < nb) base[k] = run + pre;
+    run += tot;
+  }
+  return run;
+}
+
 __global__ __launch_bounds__(TILE) void q13_scatter(const uint32_t* __restrict__ keys, uint32_t tpb,
                                                     const uint32_t* __restrict__ region_count,
-                                                    const uint32_t* __restrict__ offs, uint32_t nb,
+                                                    const uint32_t* __restrict__ offs,
+                                                    const uint32_t* __restrict__ btot, uint32_t nb,
                                                     uint32_t* __restrict__ sorted) {
   __shared__ uint32_t base[MAX_BUCKETS];   // next global position per bucket
   __shared__ uint32_t cnt[MAX_BUCKETS];    // tile count, then tile start per bucket
@@ -275,7 +315,9 @@ __global__ __launch_bounds__(TILE) void q13_scatter(const uint32_t* __restrict__
   __shared__ uint32_t rc[TILE / WROWS + 1];
   __shared__ uint32_t scratch[17];
   constexpr int PT = STILE / TILE;
-  for (uint32_t k = threadIdx.x; k < nb; k += TILE) base[k] = offs[(size_t)k * gridDim.x + blockIdx.x];
+  bucket_starts(btot, nb, base, scratch);
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < nb; k += TILE) base[k] += offs[(size_t)k * gridDim.x + blockIdx.x];
   if (threadIdx.x == 0) {
     uint32_t run = 0;
     for (int w = 0; w < TILE / WROWS; ++w) {
@@ -340,9 +382,7 @@ __global__ __launch_bounds__(TILE) void q13_scatter(const uint32_t* __restrict__
 // the bucket's per-custkey counts for q13_histogram.
 template <bool DENSE>
 __global__ __launch_bounds__(1024) void q13_bucket_count(const uint32_t* __restrict__ sorted,
-                                                         const uint32_t* __restrict__ offs,
-                                                         uint32_t nblocks_a,
-                                                         const uint32_t* __restrict__ total,
+                                                         const uint32_t* __restrict__ btot,
                                                          uint32_t nb, uint32_t minck,
                                                          uint32_t maxck,
                                                          uint32_t* __restrict__ counts,
@@ -355,8 +395,15 @@ __global__ __launch_bounds__(1024) void q13_bucket_count(const uint32_t* __restr
     for (uint32_t i = threadIdx.x; i < NBINS; i += blockDim.x) lh[i] = 0u;
   __syncthreads();
   const uint32_t k = blockIdx.x;
-  const uint32_t begin = offs[(size_t)k * nblocks_a];
-  const uint32_t end = k + 1 < nb ? offs[(size_t)(k + 1) * nblocks_a] : *total;
+  __shared__ uint32_t before;  // keys of the buckets below k
+  if (threadIdx.x == 0) before = 0u;
+  __syncthreads();
+  uint32_t mine = 0;
+  for (uint32_t j = threadIdx.x; j < k; j += blockDim.x) mine += btot[j];
+  if (mine) atomicAdd(&before, mine);
+  __syncthreads();
+  const uint32_t begin = before;
+  const uint32_t end = begin + btot[k];
   for (uint32_t i = begin + threadIdx.x; i < end; i += blockDim.x)
     atomicAdd(&c[sorted[i] & ((1u << BUCKET_SHIFT) - 1)], 1u);
   __syncthreads();
@@ -477,16 +524,15 @@ extern "C" int gdb_q13(gdb_ctx* ctx, const gdb_table* customer, const gdb_table*
     const uint32_t tpb = std::max<uint32_t>(1, (ntiles + G - 1) / G);
     G = std::max<uint32_t>(1, (ntiles + tpb - 1) / tpb);
     const uint32_t nhist = nb * G;
-    uint32_t *keys, *sorted, *rcount, *bh, *tsum, *total;
+    uint32_t *keys, *sorted, *rcount, *bh, *offs, *btot;
     char* b;
     GDB_TRY(arena(ctx, A_AUX2, (size_t)G * tpb * TILE * 4 + 256, (void**)&keys));
     GDB_TRY(arena(ctx, A_HT, (size_t)std::max<uint32_t>(n, 1) * 4 + 256, (void**)&sorted));
-    const uint32_t ntsum = scan_tile_sums_needed(nhist);
-    GDB_TRY(arena(ctx, A_PARTIALS, ((size_t)4 * G + nhist + ntsum + 64) * 4, (void**)&b));
+    GDB_TRY(arena(ctx, A_PARTIALS, ((size_t)4 * G + 2 * (size_t)nhist + nb + 64) * 4, (void**)&b));
     rcount = (uint32_t*)b;
     bh = rcount + 4 * G;
-    tsum = bh + nhist;
-    total = tsum + ntsum;
+    offs = bh + nhist;
+    btot = offs + nhist;
     q13_clear<<<5, 256, 0, st>>>(counts, 0, hist);
     GDB_HIP(hipEventRecord(ctx->ev[1], st));
     const bool heads_full = w1.len >= 4 && w2.len >= 4;
@@ -507,13 +553,13 @@ extern "C" int gdb_q13(gdb_ctx* ctx, const gdb_table* customer, const gdb_table*
     }
     GDB_HIP(hipGetLastError());
     GDB_HIP(hipEventRecord(ctx->ev[2], st));
-    GDB_TRY(exclusive_scan_u32(ctx, bh, nhist, bh, total, tsum, st));
-    q13_scatter<<<G, TILE, 0, st>>>(keys, tpb, rcount, bh, nb, sorted);
+    q13_offsets<<<nb, 256, 0, st>>>(bh, G, offs, btot);
+    q13_scatter<<<G, TILE, 0, st>>>(keys, tpb, rcount, offs, btot, nb, sorted);
     if (dense)
-      q13_bucket_count<true><<<nb, 1024, 0, st>>>(sorted, bh, G, total, nb, minck, maxck, counts,
+      q13_bucket_count<true><<<nb, 1024, 0, st>>>(sorted, btot, nb, minck, maxck, counts,
                                                   hist, overflow);
     else
-      q13_bucket_count<false><<<nb, 1024, 0, st>>>(sorted, bh, G, total, nb, minck, maxck, counts,
+      q13_bucket_count<false><<<nb, 1024, 0, st>>>(sorted, btot, nb, minck, maxck, counts,
                                                    hist, overflow);
     GDB_HIP(hipGetLastError());
   } else {
