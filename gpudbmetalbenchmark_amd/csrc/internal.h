@@ -3,8 +3,10 @@
 #include <hip/hip_runtime.h>
 
 #include <functional>
+#include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "gdb.h"
@@ -78,6 +80,10 @@ struct gdb_ctx {
   uint64_t q20_pairs = 0;          // Q20: (part, supplier) pairs of the last run, sizes the next table
   hipEvent_t sev[4] = {};          // copy-done / slot-free events of the two slots
   std::vector<hipEvent_t> chunk_ev;  // per-chunk compute start/end
+  // Occupancy query results by (kernel, threads, dynamic LDS): the runtime's
+  // query costs microseconds and pipelines ask on every call.
+  std::mutex occ_mu;
+  std::map<std::tuple<const void*, int, size_t>, int> occ;
 };
 
 struct gdb_table {

@@ -557,6 +557,16 @@ __global__ __launch_bounds__(256) void q3_topk_collect(Q3Orders o, Q3Scratch s) 
 // list (more than CAND_CAP groups tie into the threshold bucket or above),
 // every workgroup ranks its share of the orders and leaves its k best, and
 // the workgroup that finishes last ranks those.
+// The qualifying orders and group count go right after the k result rows, so
+// the host reads the result back in one copy.
+__device__ __forceinline__ void q3_result_counts(int k, const Q3Scratch& s) {
+  if (threadIdx.x == 0) {
+    uint32_t* tail = reinterpret_cast<uint32_t*>(s.out + k);
+    tail[0] = s.counters[0];
+    tail[1] = s.counters[1];
+  }
+}
+
 __global__ __launch_bounds__(256) void q3_topk_rank(int k, Q3Orders o, Q3Scratch s) {
   __shared__ TopEntry buf[TOPK_BUF];
   __shared__ int nbuf;
@@ -572,6 +582,7 @@ __global__ __launch_bounds__(256) void q3_topk_rank(int k, Q3Orders o, Q3Scratch
       tk.push(i < nc, e);
     }
     tk.finish(s.out);
+    q3_result_counts(k, s);
     return;
   }
   tk.init();
@@ -612,6 +623,7 @@ __global__ __launch_bounds__(256) void q3_topk_rank(int k, Q3Orders o, Q3Scratch
     tk.push(i < m && !(e.rev == INT64_MIN && e.key == INT32_MAX), e);
   }
   tk.finish(s.out);
+  q3_result_counts(k, s);
 }
 
 }  // namespace
@@ -727,8 +739,7 @@ extern "C" int gdb_q3(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* 
   if (no) q3_topk_collect<<<((no + 3) / 4 + B - 1) / B, B, 0, st>>>(o, s);
   q3_topk_rank<<<oblocks, B, 0, st>>>(k, o, s);
   GDB_HIP(hipGetLastError());
-  GDB_HIP(hipMemcpyAsync(host, s.out, sizeof(TopEntry) * k, hipMemcpyDeviceToHost, st));
-  GDB_HIP(hipMemcpyAsync(host_cnt, s.counters, 8, hipMemcpyDeviceToHost, st));
+  GDB_HIP(hipMemcpyAsync(host, s.out, sizeof(TopEntry) * k + 8, hipMemcpyDeviceToHost, st));
   GDB_HIP(hipEventRecord(ctx->ev[5], st));
   // Pipeline bytes: every column the query names, read once (customer 5 B,
   // orders 16 B, fact 20 B per row). Probe bytes: what the probe must read

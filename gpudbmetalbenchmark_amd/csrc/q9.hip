@@ -7,8 +7,14 @@
 //              loads; bit[partkey] if the name contains the pattern
 //              (strmatch.h; the reference's SWAR 'g' scan, kernels/q9.metal:58-83,
 //              generalised to any pattern)
+//   rank       each bitmap word gets the number of passing parts before it
+//              (its high half), so a passing part's rank is that plus a
+//              popcount; the bitmap is ORed into the probe's folded filter
 //   supplier   nation[suppkey]
-//   partsupp   rows whose part passed -> compact list (partkey, suppkey, cost)
+//   partsupp   direct table: PS_WAYS {suppkey, cost} slots per passing part,
+//              indexed by the part's rank (about 3.5 MB at 2M parts, so the
+//              probe's row reads stay in L2); else rows whose part passed ->
+//              compact list (partkey, suppkey, cost) for the hash table:
 //   ps_clear / ps_insert
 //              open-addressing table of 16-byte {key, cost} slots keyed by
 //              (partkey, suppkey), capacity nextpow2(2*count) from the
@@ -49,6 +55,8 @@ constexpr int64_t EMPTY64 = -1;
 constexpr uint8_t NO_YEAR = 0xFF;
 constexpr int QCAP = 128;       // per-wave survivor queue: < 64 left over + 64 pushed
 constexpr int PS_WAYS = 4;      // direct partsupp table: supplier slots per part (TPC-H has 4)
+// Result block read back in one copy: profit, hit flags, then four counters.
+constexpr size_t RESULT_BYTES = (size_t)NNAT * NYEAR * 12 + 16;
 
 __device__ __forceinline__ uint32_t ps_cap(uint32_t count) {
   uint32_t c = 1024;
@@ -82,9 +90,20 @@ __device__ __forceinline__ uint32_t year_index(int32_t day, const Years& y) {
   return k;
 }
 
+// Part bitmap word: bits of 32 consecutive partkeys, and the number of set
+// bits in all earlier words (the rank of the word's first passing part).
+struct alignas(8) PartWord {
+  uint32_t bits, rank;
+};
+
+__device__ __forceinline__ uint32_t part_rank(PartWord w, uint32_t pk) {
+  return w.rank + (uint32_t)__popc(w.bits & ((1u << (pk & 31)) - 1u));
+}
+
 struct Q9Scratch {
-  uint32_t* pbm;       // part bitmap
+  PartWord* pbw;       // part bitmap with ranks
   uint32_t* pfold;     // part bitmap folded to the probe's LDS filter size
+  uint32_t* rsum;      // passing parts per rank block
   uint32_t* counters;  // [0] partsupp list count, [1] a part with more than PS_WAYS suppliers
   int32_t* snat;       // suppkey -> nation or -1
   uint8_t* oyear;      // orderkey -> year index or 0xFF
@@ -92,9 +111,9 @@ struct Q9Scratch {
   int32_t* lsk;
   int64_t* lcost;
   longlong2* ht;       // {(partkey, suppkey) or EMPTY64, cost}
-  int2* pst;           // direct table: PS_WAYS {suppkey or -1, cost} per partkey
-  uint32_t* pcnt;      // direct table: slots taken per partkey
-  int64_t* agg;        // [NNAT*NYEAR] profit
+  int2* pst;           // direct table: PS_WAYS {suppkey or -1, cost} per passing part's rank
+  uint32_t* pcnt;      // direct table: slots taken per rank
+  int64_t* agg;        // [NNAT*NYEAR] profit, then agg_hit and counters (one readback)
   uint32_t* agg_hit;   // [NNAT*NYEAR]
 };
 
@@ -106,7 +125,7 @@ __global__ __launch_bounds__(256) void q9_clear(Q9Scratch s, uint32_t npw, uint3
   const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t a = npw, b = a + nsupp, c = b + oywords, d = c + NNAT * NYEAR;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d; i += stride) {
-    if (i < a) s.pbm[i] = 0u;
+    if (i < a) s.pbw[i] = PartWord{0u, 0u};
     else if (i < b) s.snat[i - a] = -1;
     else if (i < c) reinterpret_cast<uint32_t*>(s.oyear)[i - b] = 0xFFFFFFFFu;
     else {
@@ -124,9 +143,7 @@ struct Q9DimArgs {
   uint32_t np;
   PatternW pat;
   uint32_t maxpk;
-  uint32_t* pbm;
-  int2* pst;       // direct partsupp table to set up, or null
-  uint32_t* pcnt;
+  PartWord* pbw;
   const int32_t* ok;
   const int32_t* od;
   uint32_t no;
@@ -145,8 +162,7 @@ struct Q9DimArgs {
 // loads (the next tile's chunks are loaded into registers while the current
 // one is searched), then lane t assembles name t (55 bytes at LDS offset 55t)
 // into 14 aligned dwords plus two zero pad dwords and searches it in
-// registers. Names with the pattern set their part's bit and, for the direct
-// partsupp table, empty their part's supplier slots.
+// registers. Names with the pattern set their part's bit.
 __device__ __forceinline__ void q9_part_tiles(const Q9DimArgs& a, uint32_t b0, uint32_t nb,
                                               int4* tile) {
   constexpr int TBYTES = TILE * NW;          // 14080
@@ -191,15 +207,7 @@ __device__ __forceinline__ void q9_part_tiles(const Q9DimArgs& a, uint32_t b0, u
     const int len = reg_len<14>(d);
     if (reg_find<14>(d, len, 0, a.pat) >= 0) {
       const uint32_t pk = (uint32_t)a.partkey[r];
-      if (pk <= a.maxpk) {
-        atomicOr(&a.pbm[pk >> 5], 1u << (pk & 31));
-        if (a.pst) {  // the part's supplier slots start empty
-          int4* row = reinterpret_cast<int4*>(a.pst + (size_t)pk * PS_WAYS);
-          row[0] = int4{-1, 0, -1, 0};
-          row[1] = int4{-1, 0, -1, 0};
-          a.pcnt[pk] = 0u;
-        }
-      }
+      if (pk <= a.maxpk) atomicOr(&a.pbw[pk >> 5].bits, 1u << (pk & 31));
     }
   }
 }
@@ -231,13 +239,69 @@ __global__ __launch_bounds__(256) void q9_supplier(Q9DimArgs a) {
 
 // The part bitmap folded onto fwords words: bit k of the fold is the OR of
 // the bits of every key congruent to k modulo 32 * fwords.
-__global__ __launch_bounds__(256) void q9_fold(const uint32_t* __restrict__ pbm, uint32_t npw,
+__global__ __launch_bounds__(256) void q9_fold(const PartWord* __restrict__ pbw, uint32_t npw,
                                                uint32_t fwords, uint32_t* __restrict__ fold) {
   const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= fwords) return;
   uint32_t v = 0;
-  for (uint32_t i = w; i < npw; i += fwords) v |= pbm[i];
+  for (uint32_t i = w; i < npw; i += fwords) v |= pbw[i].bits;
   fold[w] = v;
+}
+
+// Rank, step 1: passing parts per block of RANK_WORDS bitmap words.
+constexpr uint32_t RANK_WORDS = 512;  // two words per thread of a 256-thread block
+__global__ __launch_bounds__(256) void q9_rank_sums(const PartWord* __restrict__ pbw, uint32_t npw,
+                                                    uint32_t* __restrict__ rsum) {
+  __shared__ int64_t red[4];
+  const uint32_t w = blockIdx.x * RANK_WORDS + 2 * threadIdx.x;
+  uint32_t a = 0, b = 0;
+  if (w + 1 < npw) {
+    const uint4 v = reinterpret_cast<const uint4*>(pbw)[w / 2];
+    a = v.x;
+    b = v.z;
+  } else if (w < npw) {
+    a = pbw[w].bits;
+  }
+  const int64_t sum = block_sum(__popc(a) + __popc(b), red);
+  if (threadIdx.x == 0) rsum[blockIdx.x] = (uint32_t)sum;
+}
+
+// Rank, step 2: the block's base is the sum of the earlier blocks' counts;
+// each word gets the number of passing parts before it. With the direct
+// partsupp table, the block empties the supplier slots and slot counters of
+// the ranks it covers.
+__global__ __launch_bounds__(256) void q9_rank_write(PartWord* __restrict__ pbw, uint32_t npw,
+                                                     const uint32_t* __restrict__ rsum,
+                                                     int2* __restrict__ pst,
+                                                     uint32_t* __restrict__ pcnt) {
+  __shared__ uint32_t scratch[17];
+  __shared__ int64_t red[4];
+  __shared__ uint32_t base_sh;
+  int64_t pre = 0;
+  for (uint32_t b = threadIdx.x; b < blockIdx.x; b += blockDim.x) pre += rsum[b];
+  pre = block_sum(pre, red);
+  if (threadIdx.x == 0) base_sh = (uint32_t)pre;
+  __syncthreads();
+  const uint32_t base = base_sh;
+  const uint32_t w = blockIdx.x * RANK_WORDS + 2 * threadIdx.x;
+  uint32_t a = 0, b = 0;
+  const bool pair = w + 1 < npw;
+  if (pair) {
+    const uint4 v = reinterpret_cast<const uint4*>(pbw)[w / 2];
+    a = v.x;
+    b = v.z;
+  } else if (w < npw) {
+    a = pbw[w].bits;
+  }
+  uint32_t total;
+  const uint32_t r = base + block_excl_scan(__popc(a) + __popc(b), scratch, &total);
+  if (pair) reinterpret_cast<uint4*>(pbw)[w / 2] = uint4{a, r, b, r + __popc(a)};
+  else if (w < npw) pbw[w] = PartWord{a, r};
+  if (!pst) return;
+  int4* rows = reinterpret_cast<int4*>(pst);
+  for (uint32_t j = threadIdx.x; j < 2 * total; j += blockDim.x)
+    rows[2 * (size_t)base + j] = int4{-1, 0, -1, 0};
+  for (uint32_t j = threadIdx.x; j < total; j += blockDim.x) pcnt[base + j] = 0u;
 }
 
 // Partsupp rows whose part passed -> compact list in row order, one atomic per
@@ -246,7 +310,7 @@ __global__ __launch_bounds__(256) void q9_fold(const uint32_t* __restrict__ pbm,
 __global__ __launch_bounds__(256) void q9_partsupp(const int32_t* __restrict__ pk,
                                                    const int32_t* __restrict__ sk,
                                                    const int64_t* __restrict__ cost, uint32_t n,
-                                                   uint32_t maxpk, const uint32_t* __restrict__ pbm,
+                                                   uint32_t maxpk, const PartWord* __restrict__ pbw,
                                                    uint32_t* __restrict__ counters,
                                                    int32_t* __restrict__ lpk,
                                                    int32_t* __restrict__ lsk,
@@ -276,7 +340,7 @@ __global__ __launch_bounds__(256) void q9_partsupp(const int32_t* __restrict__ p
     }
     uint32_t words[16];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) words[j] = pbm[min((uint32_t)ks[j], maxpk) >> 5];
+    for (int j = 0; j < 16; ++j) words[j] = pbw[min((uint32_t)ks[j], maxpk) >> 5].bits;
     uint32_t mask = 0;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
@@ -303,7 +367,7 @@ __global__ __launch_bounds__(256) void q9_partsupp(const int32_t* __restrict__ p
 }
 
 // Direct table: each partsupp row whose part passed takes the next slot of
-// its part (a per-part counter: the adds of one thread's rows are
+// its part's rank (a per-rank counter: the adds of one thread's rows are
 // independent, so they are in flight together); a fifth supplier of a part
 // sets counters[1] and the query runs again through the hash table.
 // (Claiming slots by compare-and-swap of the suppkey took up to ten dependent
@@ -312,7 +376,7 @@ __global__ __launch_bounds__(256) void q9_partsupp_direct(const int32_t* __restr
                                                           const int32_t* __restrict__ sk,
                                                           const int64_t* __restrict__ cost,
                                                           uint32_t n, uint32_t maxpk,
-                                                          const uint32_t* __restrict__ pbm,
+                                                          const PartWord* __restrict__ pbw,
                                                           int2* __restrict__ pst,
                                                           uint32_t* __restrict__ pcnt,
                                                           uint32_t* __restrict__ counters) {
@@ -326,23 +390,24 @@ __global__ __launch_bounds__(256) void q9_partsupp_direct(const int32_t* __restr
 #pragma unroll
       for (int j = 0; j < 4; ++j) ks[j] = 4 * g + j < n ? pk[4 * g + j] : -1;
     }
-    uint32_t words[4];
+    PartWord words[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) words[j] = pbm[min((uint32_t)ks[j], maxpk) >> 5];
+    for (int j = 0; j < 4; ++j) words[j] = pbw[min((uint32_t)ks[j], maxpk) >> 5];
     bool green[4];
-    uint32_t slot[4];
+    uint32_t rank[4], slot[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const uint32_t p = (uint32_t)ks[j];
-      green[j] = p <= maxpk && ((words[j] >> (p & 31)) & 1u);
-      slot[j] = green[j] ? atomicAdd(&pcnt[p], 1u) : 0u;
+      green[j] = p <= maxpk && ((words[j].bits >> (p & 31)) & 1u);
+      rank[j] = part_rank(words[j], p);
+      slot[j] = green[j] ? atomicAdd(&pcnt[rank[j]], 1u) : 0u;
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       if (!green[j]) continue;
       const uint32_t r = 4 * g + j;
       if (slot[j] < PS_WAYS)
-        pst[(size_t)(uint32_t)ks[j] * PS_WAYS + slot[j]] = int2{sk[r], (int32_t)cost[r]};
+        pst[(size_t)rank[j] * PS_WAYS + slot[j]] = int2{sk[r], (int32_t)cost[r]};
       else
         atomicOr(&counters[1], 1u);
     }
@@ -396,12 +461,12 @@ struct Q9Dims {
   int32_t first;     // global year slot of LDS year 0
   int32_t nslots;    // nations x ny
   uint32_t fwords;   // words of the folded part filter in LDS (a power of two)
-  int32_t exact;     // 1: every partkey is below 32 * fwords, the fold is the bitmap
   int32_t direct;    // 1: partsupp costs in the direct table, else the hash table
 };
 
 // One queued row: nation, year and the first partsupp slot are loaded
-// together (none depends on another), then the rare further probe steps.
+// together (none depends on another), then the rare further probe steps. pk
+// is the part's rank with the direct table, else its partkey.
 __device__ __forceinline__ void q9_entry(int32_t pk, int32_t sk, int32_t ok, int32_t qty,
                                          int64_t amt, uint32_t cap, const Q9Dims& dm,
                                          const Q9Scratch& s, int64_t* ltab, uint32_t* lhit) {
@@ -409,7 +474,7 @@ __device__ __forceinline__ void q9_entry(int32_t pk, int32_t sk, int32_t ok, int
   const int32_t nat = s.snat[inr ? (uint32_t)sk : 0u];
   const uint8_t y = s.oyear[inr ? (uint32_t)ok : 0u];
   int64_t cost = -1;
-  if (dm.direct) {  // the part passed the exact bitmap, so its slots were set up
+  if (dm.direct) {  // the part passed the bitmap, so its rank's slots were set up
     const int4* row = reinterpret_cast<const int4*>(s.pst + (size_t)(uint32_t)pk * PS_WAYS);
     const int4 a = row[0], b = row[1];
     cost = a.x == sk ? a.y : a.z == sk ? a.w : b.x == sk ? b.y : b.z == sk ? b.w : -1;
@@ -434,25 +499,43 @@ __device__ __forceinline__ void q9_entry(int32_t pk, int32_t sk, int32_t ok, int
   lhit[slot] = 1u;
 }
 
+// A wave's queue of rows whose part passed: {key, suppkey, orderkey,
+// quantity} and the amount, one 16-byte and one 8-byte LDS access per row.
 struct Q9Queue {
-  int32_t pk[QCAP], sk[QCAP], ok[QCAP], qty[QCAP];
+  int4 e[QCAP];
   int64_t amt[QCAP];
 };
 
-// U groups of 4 rows per lane and step: the loads of all six columns for the
-// U groups are issued first; then, row slot by row slot, lanes whose row's part
-// bit is set push the row's values onto the wave's queue, and whenever 64 are
-// queued one full wave of entries is processed.
+// The probe is software-pipelined over trips of U groups of 4 rows per lane.
+// A wave's vector loads complete in issue order, so a wait for any load also
+// waits for every load issued before it. Each trip therefore issues its
+// dependent loads before the next trip's column loads, and uses their results
+// one trip later, after the wait for its own columns:
+//   A  push the previous trip's rows whose part bit is set (their bitmap
+//      words arrived with this trip's columns) onto the wave's LDS queue
+//   B  finish the previous trip's queue batch: cost from the partsupp row,
+//      amount into the LDS (nation, year) table
+//   C  wait for this trip's columns; test the keys against the folded filter
+//      in LDS and load the global bitmap word of every fold hit
+//   D  with 64 rows queued, take them off the queue and load their nation,
+//      year and partsupp row (direct table)
+//   E  load the next trip's six columns
+// The wait in C is the only wait on HBM: its columns were issued one trip
+// earlier, and the dependent loads in flight meanwhile were issued before
+// them. (With the dependent loads issued after the prefetch, each of their
+// waits also waited for the next trip's columns: 400 us at 60M fact rows.)
 // The part test reads the folded filter in LDS: random 4-byte gathers from
 // the 250 KB global bitmap cost a vector-cache line per lane and took as long
 // as streaming all six columns. Only keys whose folded bit is set read the
-// global bitmap (none when the fold is exact). One workgroup of 1024 threads
+// global bitmap word, which also gives a passing part's rank: the queue
+// carries the rank to the direct table's row. One workgroup of 1024 threads
 // per CU shares one fold (2^19 bits next to sixteen wave queues): sixteen
 // waves per CU stream the columns, against eight with 512-thread workgroups
-// and a 2^20-bit fold (466 -> 412 us at 60M fact rows; the global bitmap reads
-// for the extra fold hits cost nothing measurable).
-constexpr int PROBE_THREADS = 1024;
-template <int U>
+// and a 2^20-bit fold. With the hash table (no direct table) queued rows are
+// processed in place, 64 at a time, by q9_entry.
+constexpr int PROBE_THREADS = 512;
+constexpr int PROBE_U = 2;  // 4-row groups per lane and trip
+template <int U, bool DIRECT>
 __global__ __launch_bounds__(PROBE_THREADS) void q9_probe(Q9Fact f, uint32_t n, Q9Dims dm,
                                                           Q9Scratch s) {
   extern __shared__ uint32_t dyn[];  // folded filter, then nslots sums and nslots hit flags
@@ -472,67 +555,154 @@ __global__ __launch_bounds__(PROBE_THREADS) void q9_probe(Q9Fact f, uint32_t n, 
   const uint64_t below = (1ull << lane) - 1;
   Q9Queue& q = queues[w];
   uint32_t head = 0, tail = 0;  // wave-uniform
-  auto push = [&](bool pass, int32_t pk, int32_t sk, int32_t ok, int32_t qty, int64_t amt) {
-    const uint64_t b = __ballot(pass);
-    if (pass) {
-      const uint32_t i = (tail + (uint32_t)__popcll(b & below)) & (QCAP - 1);
-      q.pk[i] = pk;
-      q.sk[i] = sk;
-      q.ok[i] = ok;
-      q.qty[i] = qty;
-      q.amt[i] = amt;
-    }
-    tail += (uint32_t)__popcll(b);
-  };
-  auto drain = [&](bool all) {
+  // Processes queued rows now (64 per pass, waiting for each) until at most
+  // `keep` are left.
+  auto drain_now = [&](uint32_t keep) {
     wave_lds_sync();
-    while (tail - head >= 64 || (all && tail != head)) {
+    while (tail - head > keep) {
       const uint32_t avail = min(64u, tail - head);
       if (lane < avail) {
         const uint32_t i = (head + lane) & (QCAP - 1);
-        q9_entry(q.pk[i], q.sk[i], q.ok[i], q.qty[i], q.amt[i], cap, dm, s, ltab, lhit);
+        const int4 e = q.e[i];
+        q9_entry(e.x, e.y, e.z, e.w, q.amt[i], cap, dm, s, ltab, lhit);
       }
       head += avail;
     }
     wave_lds_sync();
   };
-  auto test = [&](int32_t pk) {
-    const uint32_t k = (uint32_t)pk, kf = k & fmask;
-    bool pass = k <= dm.maxpk && ((lf[kf >> 5] >> (kf & 31)) & 1u);
-    if (!dm.exact && pass) pass = (s.pbm[k >> 5] >> (k & 31)) & 1u;
-    return pass;
+  // Queues the rows with pass set; a full queue is processed first.
+  auto push = [&](bool pass, int32_t key, int32_t sk, int32_t ok, int32_t qty, int64_t amt) {
+    const uint64_t b = __ballot(pass);
+    const uint32_t cnt = (uint32_t)__popcll(b);
+    if (cnt == 0) return;
+    if (tail - head + cnt > QCAP) drain_now(0);
+    if (pass) {
+      const uint32_t i = (tail + (uint32_t)__popcll(b & below)) & (QCAP - 1);
+      q.e[i] = int4{key, sk, ok, qty};
+      q.amt[i] = amt;
+    }
+    tail += cnt;
+  };
+  // Batch of 64 queued rows (direct table): taken off the queue and its loads
+  // issued in D, finished in B of the next trip. The loads are issued on
+  // every trip, from clamped addresses when no batch is due, so that no value
+  // merges with an older one while the loads are in flight (a merge is a
+  // register copy, and the copy waits for the load).
+  bool gpend = false;  // wave-uniform
+  bool g_in = false;
+  int32_t g_sk = 0, g_qty = 0, g_nat = -1;
+  int64_t g_amt = 0;
+  uint8_t g_y = NO_YEAR;
+  int4 g_ra{-1, 0, -1, 0}, g_rb{-1, 0, -1, 0};
+  auto batch_issue = [&]() {
+    const bool due = tail - head >= 64;
+    wave_lds_sync();
+    const uint32_t i = (head + lane) & (QCAP - 1);
+    const int4 e = q.e[i];
+    const int32_t key = due ? e.x : 0, ok = e.z;
+    g_sk = e.y;
+    g_qty = e.w;
+    g_amt = q.amt[i];
+    g_in = due && (uint32_t)g_sk <= dm.maxsk && (uint32_t)ok <= dm.maxok;
+    g_nat = s.snat[g_in ? (uint32_t)g_sk : 0u];
+    g_y = s.oyear[g_in ? (uint32_t)ok : 0u];
+    const int4* row = reinterpret_cast<const int4*>(s.pst + (size_t)(uint32_t)key * PS_WAYS);
+    g_ra = row[0];
+    g_rb = row[1];
+    head += due ? 64u : 0u;
+    gpend = due;
+  };
+  auto batch_finish = [&]() {
+    int64_t cost = g_ra.x == g_sk ? g_ra.y : g_ra.z == g_sk ? g_ra.w
+                 : g_rb.x == g_sk ? g_rb.y : g_rb.z == g_sk ? g_rb.w : -1;
+    if (g_sk < 0) cost = -1;
+    if (g_in && g_nat >= 0 && g_y != NO_YEAR && cost >= 0) {
+      const int slot = g_nat * dm.ny + g_y;
+      atomicAdd((unsigned long long*)&ltab[slot],
+                (unsigned long long)(g_amt - cost * (int64_t)g_qty * 100));
+      lhit[slot] = 1u;
+    }
+    gpend = false;
   };
   const uint32_t nvec = n >> 2;
   const uint32_t nthr = gridDim.x * blockDim.x;
-  // Two register buffers of U groups each: the loads of the next trip are in
-  // flight while the current trip's rows are tested, queued and drained.
-  // Trips are wave-uniform: every lane of a wave runs the same iterations so
-  // the ballots and queue counters stay in step; lanes past the end load
-  // nothing (partkey -1 fails the test).
+  // Two register buffers of U groups each (this trip's and the other one);
+  // the bitmap words loaded for the last tested trip (word 0 where the folded
+  // filter missed) and its fold hits, one bit per row.
+  // The column loads are issued unconditionally (lanes past the end reload
+  // the last vector group and are masked by their buffer's base gbuf[b] in
+  // the test): a load under a branch leaves a path where it was not issued,
+  // and every wait after the join then has to count for that path too.
   int4 p4[2][U], s4[2][U], o4[2][U], q4[2][U], d4[2][U];
   longlong2 pa[2][U], pb[2][U];
+  uint32_t gbuf[2] = {0u, 0u};
+  PartWord pw[U][4];
+  uint32_t hits = 0;
   auto load = [&](uint32_t gb, auto B) {
     constexpr int b = decltype(B)::value;
+    gbuf[b] = gb;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const uint32_t g = gb + u * nthr + lane;
-      if (g < nvec) {
-        p4[b][u] = ld4nt(f.pk, g);
-        s4[b][u] = ld4nt(f.sk, g);
-        o4[b][u] = ld4nt(f.ok, g);
-        q4[b][u] = ld4nt(f.qty, g);
-        d4[b][u] = ld4nt(f.disc, g);
-        pa[b][u] = ld2nt(f.price, 2 * g);
-        pb[b][u] = ld2nt(f.price, 2 * g + 1);
-      } else {
-        p4[b][u] = int4{-1, -1, -1, -1};
-        s4[b][u] = o4[b][u] = q4[b][u] = d4[b][u] = int4{0, 0, 0, 0};
-        pa[b][u].x = pa[b][u].y = pb[b][u].x = pb[b][u].y = 0;
+      const uint32_t g = min(gb + u * nthr + lane, nvec - 1);
+      p4[b][u] = ld4nt(f.pk, g);
+      s4[b][u] = ld4nt(f.sk, g);
+      o4[b][u] = ld4nt(f.ok, g);
+      q4[b][u] = ld4nt(f.qty, g);
+      d4[b][u] = ld4nt(f.disc, g);
+      pa[b][u] = ld2nt(f.price, 2 * g);
+      pb[b][u] = ld2nt(f.price, 2 * g + 1);
+    }
+  };
+  // C: fold test of buffer b's keys; bitmap words for the fold hits.
+  auto test = [&](auto B) {
+    constexpr int b = decltype(B)::value;
+    hits = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t ks[4] = {(uint32_t)p4[b][u].x, (uint32_t)p4[b][u].y, (uint32_t)p4[b][u].z,
+                              (uint32_t)p4[b][u].w};
+      const bool valid = gbuf[b] + u * nthr + lane < nvec;
+      // the four folded words are read before any is tested (kf >> 5 is
+      // always inside the fold), so the LDS reads overlap
+      uint32_t fw[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) fw[j] = lf[(ks[j] & fmask) >> 5];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t bit = (fw[j] >> (ks[j] & 31)) & 1u;
+        const uint32_t hit = (valid ? 1u : 0u) & (ks[j] <= dm.maxpk ? 1u : 0u) & bit;
+        hits |= hit << (4 * u + j);
+        pw[u][j] = s.pbw[hit ? ks[j] >> 5 : 0u];
       }
     }
   };
-  auto process = [&](auto B) {
+  // A: queue buffer b's rows whose part bit is set. Each lane's place in the
+  // queue is the wave's exclusive prefix of its pass count, formed from one
+  // ballot per bit of the count; then row slot by row slot the passing lanes
+  // write their row. When the trip's rows might not fit, they are pushed slot
+  // by slot, processing the queue as it fills.
+  auto push_trip = [&](auto B) {
     constexpr int b = decltype(B)::value;
+    uint32_t pmask = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t ks[4] = {(uint32_t)p4[b][u].x, (uint32_t)p4[b][u].y, (uint32_t)p4[b][u].z,
+                              (uint32_t)p4[b][u].w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        pmask |= (((hits >> (4 * u + j)) & (pw[u][j].bits >> (ks[j] & 31))) & 1u) << (4 * u + j);
+    }
+    const uint32_t c = (uint32_t)__popc(pmask);
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int bit = 0; (1 << bit) <= 4 * U; ++bit) {
+      const uint64_t m = __ballot((c >> bit) & 1u);
+      pre += (uint32_t)__popcll(m & below) << bit;
+      tot += (uint32_t)__popcll(m) << bit;
+    }
+    if (tot == 0) return;
+    const bool fits = tail - head + tot <= QCAP;
+    uint32_t pos = tail + pre;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int32_t ps[4] = {p4[b][u].x, p4[b][u].y, p4[b][u].z, p4[b][u].w};
@@ -541,39 +711,71 @@ __global__ __launch_bounds__(PROBE_THREADS) void q9_probe(Q9Fact f, uint32_t n, 
       const int32_t qs[4] = {q4[b][u].x, q4[b][u].y, q4[b][u].z, q4[b][u].w};
       const int32_t ds[4] = {d4[b][u].x, d4[b][u].y, d4[b][u].z, d4[b][u].w};
       const int64_t pr[4] = {pa[b][u].x, pa[b][u].y, pb[b][u].x, pb[b][u].y};
-      bool pass[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) pass[j] = test(ps[j]);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        push(pass[j], ps[j], ss[j], os[j], qs[j], pr[j] * (int64_t)(100 - ds[j]));
-        if (tail - head >= 64) drain(false);
+        const bool pass = (pmask >> (4 * u + j)) & 1u;
+        const int32_t key =
+            DIRECT ? (int32_t)part_rank(pw[u][j], (uint32_t)ps[j]) : ps[j];
+        const int64_t amt = pr[j] * (int64_t)(100 - ds[j]);
+        if (fits) {
+          if (pass) {
+            const uint32_t i = pos & (QCAP - 1);
+            q.e[i] = int4{key, ss[j], os[j], qs[j]};
+            q.amt[i] = amt;
+            ++pos;
+          }
+        } else {
+          push(pass, key, ss[j], os[j], qs[j], amt);
+        }
       }
     }
+    if (fits) tail += tot;
+  };
+  // One trip: buffer b holds this trip's columns (in flight), buffer o the
+  // previous trip's (tested, not yet queued) if `prev`.
+  bool prev = false;  // wave-uniform
+  auto trip = [&](uint32_t gb, auto B, auto O) {
+    if (DIRECT && gpend) batch_finish();             // B
+    if (prev) push_trip(O);                          // A
+    test(B);                                         // C
+    prev = true;
+    if (DIRECT) batch_issue();                       // D
+    else if (tail - head >= 64) drain_now(63);
+    __builtin_amdgcn_sched_barrier(0);
+    load(gb + U * nthr, O);                          // E (past the end: masked)
   };
   constexpr std::integral_constant<int, 0> B0{};
   constexpr std::integral_constant<int, 1> B1{};
   const uint32_t g0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
   const uint32_t step = U * nthr;
+  bool last1 = false;  // the last tested trip is in buffer 1
   if (g0 < nvec) load(g0, B0);
   for (uint32_t gb = g0; gb < nvec; gb += 2 * step) {
-    const bool more = gb + step < nvec;
-    if (more) load(gb + step, B1);
-    process(B0);
-    if (!more) break;
-    if (gb + 2 * step < nvec) load(gb + 2 * step, B0);
-    process(B1);
+    trip(gb, B0, B1);
+    if (gb + step >= nvec) break;
+    trip(gb + step, B1, B0);
+    last1 = true;
+    if (gb + 2 * step >= nvec) break;
+    last1 = false;
+  }
+  if (DIRECT && gpend) batch_finish();
+  if (prev) {
+    if (last1) push_trip(B1);
+    else push_trip(B0);
   }
   if (blockIdx.x == 0 && w == 0) {  // wave-uniform: the last n % 4 rows
     const uint32_t r = nvec * 4 + lane;
     const bool in = lane < (n & 3);
-    const int32_t pk = in ? f.pk[r] : -1;
-    const bool pass = in && test(pk);
-    if (tail - head > QCAP - 64) drain(false);
-    push(pass, pk, pass ? f.sk[r] : 0, pass ? f.ok[r] : 0, pass ? f.qty[r] : 0,
+    const uint32_t k = in ? (uint32_t)f.pk[r] : 0xFFFFFFFFu;
+    const uint32_t kf = k & fmask;
+    PartWord wd{0u, 0u};
+    if (k <= dm.maxpk && ((lf[kf >> 5] >> (kf & 31)) & 1u)) wd = s.pbw[k >> 5];
+    const bool pass = (wd.bits >> (k & 31)) & 1u;
+    push(pass, DIRECT ? (int32_t)part_rank(wd, k) : (int32_t)k, pass ? f.sk[r] : 0,
+         pass ? f.ok[r] : 0, pass ? f.qty[r] : 0,
          pass ? f.price[r] * (int64_t)(100 - f.disc[r]) : 0);
   }
-  drain(true);
+  drain_now(0);
   __syncthreads();
   for (int i = threadIdx.x; i < dm.nslots; i += blockDim.x) {
     if (lhit[i]) {
@@ -679,9 +881,10 @@ static int q9_run(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supplier
                            sizeof(Q9Queue) * (PROBE_THREADS / 64) - 1024;
     while (fwords > 32 && (size_t)fwords * 4 + (size_t)nnat * years.ny * 12 > lds_max) fwords >>= 1;
   }
-  const bool exact = npw <= fwords;
-  const bool direct = try_direct && nps > 0 && psc->minv >= INT32_MIN && psc->maxv <= INT32_MAX &&
-                      (uint64_t)(maxpk + 1ull) * PS_WAYS * 8 <= (16ull << 30);
+  // Passing parts (ranks) are at most the part rows and the partkey values.
+  const uint64_t nrank = std::max<uint64_t>(1, std::min<uint64_t>(np, maxpk + 1ull));
+  const uint32_t rblocks = (npw + RANK_WORDS - 1) / RANK_WORDS;
+  const bool direct = try_direct && nps > 0 && psc->minv >= INT32_MIN && psc->maxv <= INT32_MAX;
   uint64_t capmax = 1024;
   while (!direct && capmax < 2ull * nps) capmax <<= 1;
   const uint32_t nlist = direct ? 1u : std::max<uint32_t>(nps, 1);
@@ -701,26 +904,27 @@ static int q9_run(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supplier
       off += (bytes + 255) & ~size_t(255);
       return o;
     };
-    const size_t o_pbm = take(npw * 4ull), o_fold = take(fwords * 4ull), o_cnt = take(16),
-                 o_snat = take(nsupp * 4ull),
-                 o_oy = take((nord + 3) / 4 * 4ull), o_agg = take(NNAT * NYEAR * 8),
-                 o_hit = take(NNAT * NYEAR * 4), o_lpk = take(nlist * 4ull),
-                 o_lsk = take(nlist * 4ull), o_lc = take(nlist * 8ull);
+    // agg, agg_hit and counters are adjacent: the result is one readback.
+    const size_t o_pbw = take(npw * 8ull), o_fold = take(fwords * 4ull),
+                 o_rsum = take(rblocks * 4ull), o_snat = take(nsupp * 4ull),
+                 o_oy = take((nord + 3) / 4 * 4ull), o_agg = take(RESULT_BYTES),
+                 o_lpk = take(nlist * 4ull), o_lsk = take(nlist * 4ull), o_lc = take(nlist * 8ull);
     GDB_TRY(arena(ctx, A_MAP, off, (void**)&b));
-    s.pbm = (uint32_t*)(b + o_pbm);
+    s.pbw = (PartWord*)(b + o_pbw);
     s.pfold = (uint32_t*)(b + o_fold);
-    s.counters = (uint32_t*)(b + o_cnt);
+    s.rsum = (uint32_t*)(b + o_rsum);
     s.snat = (int32_t*)(b + o_snat);
     s.oyear = (uint8_t*)(b + o_oy);
     s.agg = (int64_t*)(b + o_agg);
-    s.agg_hit = (uint32_t*)(b + o_hit);
+    s.agg_hit = (uint32_t*)(b + o_agg + NNAT * NYEAR * 8);
+    s.counters = (uint32_t*)(b + o_agg + NNAT * NYEAR * 12);
     s.lpk = (int32_t*)(b + o_lpk);
     s.lsk = (int32_t*)(b + o_lsk);
     s.lcost = (int64_t*)(b + o_lc);
     if (direct) {
-      const size_t tab = ((size_t)(maxpk + 1ull) * PS_WAYS * 8 + 255) & ~size_t(255);
+      const size_t tab = (nrank * PS_WAYS * 8 + 255) & ~size_t(255);
       char* t;
-      GDB_TRY(arena(ctx, A_HT, tab + (maxpk + 1ull) * 4 + 256, (void**)&t));
+      GDB_TRY(arena(ctx, A_HT, tab + nrank * 4 + 256, (void**)&t));
       s.pst = (int2*)t;
       s.pcnt = (uint32_t*)(t + tab);
     } else {
@@ -728,37 +932,36 @@ static int q9_run(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supplier
     }
   }
   char* host;
-  GDB_TRY(pinned(ctx, NNAT * NYEAR * 12 + 64, (void**)&host));
+  GDB_TRY(pinned(ctx, RESULT_BYTES, (void**)&host));
   hipStream_t st = ctx->stream;
   const int B = 256;
-  const uint32_t clear_items = npw + snat_clear + oy_words + NNAT * NYEAR;
   GDB_HIP(hipEventRecord(ctx->ev[0], st));
-  q9_clear<<<std::min<uint32_t>(8192, (clear_items + B - 1) / B), B, 0, st>>>(s, npw, snat_clear,
-                                                                              oy_words);
-  {
-    const Q9DimArgs da{(const uint8_t*)pnm->dev, (const int32_t*)ppk->dev, np, pat, maxpk, s.pbm,
-                       direct ? s.pst : nullptr, s.pcnt, (const int32_t*)ook->dev,
-                       (const int32_t*)ood->dev, no, maxok, years, s.oyear,
-                       (const int32_t*)ssk->dev, (const int32_t*)snk->dev, ns, maxsk, s.snat};
-    if (np) {
-      const int pb = std::max(1, std::min<int>(resident_blocks(ctx, (const void*)q9_part, TILE, 0, 0),
-                                               (int)((np + TILE - 1) / TILE)));
-      q9_part<<<pb, TILE, 0, st>>>(da);
-    }
-    if (ns) q9_supplier<<<(ns + B - 1) / B, B, 0, st>>>(da);
-    if (no) q9_orders<<<((no + 3) / 4 + B - 1) / B, B, 0, st>>>(da);
+  q9_clear<<<std::min<uint32_t>(8192, (npw + snat_clear + oy_words + NNAT * NYEAR + B - 1) / B), B,
+             0, st>>>(s, npw, snat_clear, oy_words);
+  const Q9DimArgs da{(const uint8_t*)pnm->dev, (const int32_t*)ppk->dev, np, pat, maxpk, s.pbw,
+                     (const int32_t*)ook->dev,
+                     (const int32_t*)ood->dev, no, maxok, years, s.oyear,
+                     (const int32_t*)ssk->dev, (const int32_t*)snk->dev, ns, maxsk, s.snat};
+  if (np) {
+    const int pb = std::max(1, std::min<int>(resident_blocks(ctx, (const void*)q9_part, TILE, 0, 0),
+                                             (int)((np + TILE - 1) / TILE)));
+    q9_part<<<pb, TILE, 0, st>>>(da);
   }
-  q9_fold<<<(fwords + B - 1) / B, B, 0, st>>>(s.pbm, npw, fwords, s.pfold);
+  if (ns) q9_supplier<<<(ns + B - 1) / B, B, 0, st>>>(da);
+  if (no) q9_orders<<<((no + 3) / 4 + B - 1) / B, B, 0, st>>>(da);
+  q9_fold<<<(fwords + B - 1) / B, B, 0, st>>>(s.pbw, npw, fwords, s.pfold);
+  q9_rank_sums<<<rblocks, B, 0, st>>>(s.pbw, npw, s.rsum);
+  q9_rank_write<<<rblocks, B, 0, st>>>(s.pbw, npw, s.rsum, direct ? s.pst : nullptr, s.pcnt);
   if (direct) {
     q9_partsupp_direct<<<std::min<uint32_t>(ctx->info.cu_count * 8, ((nps + 3) / 4 + B - 1) / B), B, 0,
                          st>>>((const int32_t*)pspk->dev, (const int32_t*)pssk->dev,
-                               (const int64_t*)psc->dev, nps, maxpk, s.pbm, s.pst, s.pcnt,
+                               (const int64_t*)psc->dev, nps, maxpk, s.pbw, s.pst, s.pcnt,
                                s.counters);
   } else {
     if (nps)
       q9_partsupp<<<std::min<uint32_t>(ctx->info.cu_count * 8, (nps + 16 * B - 1) / (16 * B)), B, 0,
                     st>>>((const int32_t*)pspk->dev, (const int32_t*)pssk->dev,
-                          (const int64_t*)psc->dev, nps, maxpk, s.pbm, s.counters, s.lpk, s.lsk,
+                          (const int64_t*)psc->dev, nps, maxpk, s.pbw, s.counters, s.lpk, s.lsk,
                           s.lcost);
     const int fill_blocks = ctx->info.cu_count * 4;
     q9_ps_clear<<<fill_blocks, B, 0, st>>>(s.counters, s.ht);
@@ -766,12 +969,12 @@ static int q9_run(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supplier
   }
   GDB_HIP(hipGetLastError());
   const Q9Dims dm{maxpk, maxsk, maxok, years.ny, years.first, nnat * years.ny, fwords,
-                  exact ? 1 : 0, direct ? 1 : 0};
+                  direct ? 1 : 0};
   const size_t tab_lds = (size_t)fwords * 4 + (size_t)dm.nslots * 12;
-  // 4-row groups in flight per lane: the tuning entry's unroll (1, 2 or 4)
-  const int U = ctx->tune[T_Q9].unroll;
-  const void* kfn = U >= 4 ? (const void*)q9_probe<4> : U == 1 ? (const void*)q9_probe<1>
-                                                                : (const void*)q9_probe<2>;
+  // One 4-row group per lane and trip: the pipelined probe holds two trips'
+  // columns plus the dependent loads in flight in registers (116 VGPRs at
+  // one group; two groups spill).
+  const void* kfn = direct ? (const void*)q9_probe<PROBE_U, true> : (const void*)q9_probe<PROBE_U, false>;
   GDB_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)tab_lds));
   const int PB = PROBE_THREADS;
   const int president = resident_blocks(ctx, kfn, PB, tab_lds, ctx->tune[T_Q9].blocks_per_cu);
@@ -779,9 +982,8 @@ static int q9_run(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supplier
     const Q9Fact f{(const int32_t*)d[0], (const int32_t*)d[1], (const int32_t*)d[2],
                    (const int32_t*)d[3], (const int32_t*)d[4], (const int64_t*)d[5]};
     const int pblocks = std::max(1, std::min<int>(president, (int)((rows / 4 + PB - 1) / PB)));
-    if (U >= 4) q9_probe<4><<<pblocks, PB, tab_lds, st>>>(f, rows, dm, s);
-    else if (U == 1) q9_probe<1><<<pblocks, PB, tab_lds, st>>>(f, rows, dm, s);
-    else q9_probe<2><<<pblocks, PB, tab_lds, st>>>(f, rows, dm, s);
+    if (direct) q9_probe<PROBE_U, true><<<pblocks, PB, tab_lds, st>>>(f, rows, dm, s);
+    else q9_probe<PROBE_U, false><<<pblocks, PB, tab_lds, st>>>(f, rows, dm, s);
     GDB_HIP(hipGetLastError());
     return GDB_OK;
   };
@@ -793,14 +995,11 @@ static int q9_run(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supplier
     GDB_HIP(hipEventRecord(ctx->ev[1], st));
     GDB_HIP(hipEventRecord(ctx->ev[2], st));
   }
-  GDB_HIP(hipMemcpyAsync(host, s.agg, NNAT * NYEAR * 8, hipMemcpyDeviceToHost, st));
-  GDB_HIP(hipMemcpyAsync(host + NNAT * NYEAR * 8, s.agg_hit, NNAT * NYEAR * 4,
-                         hipMemcpyDeviceToHost, st));
-  GDB_HIP(hipMemcpyAsync(host + NNAT * NYEAR * 12, s.counters + 1, 4, hipMemcpyDeviceToHost, st));
+  GDB_HIP(hipMemcpyAsync(host, s.agg, RESULT_BYTES, hipMemcpyDeviceToHost, st));
   GDB_HIP(hipEventRecord(ctx->ev[5], st));
   const uint64_t bytes = 28ull * nl + 59ull * np + 16ull * nps + 8ull * ns + 8ull * no;
   GDB_TRY(timing_finish(ctx, timing, bytes, 28ull * nl, &rs));
-  if (direct && *(const uint32_t*)(host + NNAT * NYEAR * 12)) {
+  if (direct && ((const uint32_t*)(host + NNAT * NYEAR * 12))[1]) {
     *redo = true;  // a part with more than PS_WAYS suppliers
     return GDB_OK;
   }

@@ -143,10 +143,20 @@ int column(const gdb_table* t, const char* name, int type, const Column** out) {
 
 int resident_blocks(gdb_ctx* ctx, const void* kernel, int block, size_t lds, int per_cu_cap) {
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds) != hipSuccess ||
-      per_cu <= 0) {
-    (void)hipGetLastError();
-    per_cu = 2;
+  const auto key = std::make_tuple(kernel, block, lds);
+  {
+    std::lock_guard<std::mutex> g(ctx->occ_mu);
+    auto it = ctx->occ.find(key);
+    if (it != ctx->occ.end()) per_cu = it->second;
+  }
+  if (per_cu <= 0) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds) != hipSuccess ||
+        per_cu <= 0) {
+      (void)hipGetLastError();
+      per_cu = 2;
+    }
+    std::lock_guard<std::mutex> g(ctx->occ_mu);
+    ctx->occ[key] = per_cu;
   }
   if (per_cu_cap > 0) per_cu = std::min(per_cu, per_cu_cap);
   return std::max(1, per_cu * ctx->info.cu_count);
@@ -277,8 +287,9 @@ int gdb_open(int device, uint64_t hbm_budget_bytes, gdb_ctx** out) {
   // on 60M rows is four 128-thread workgroups per CU (233 us, against 243 us
   // for three of 256).
   ctx->tune[T_Q1] = Tuning{128, 4, 2};
-  // Q9's probe: one 4-row group in flight per lane measured fastest.
-  ctx->tune[T_Q9] = Tuning{256, 0, 1};
+  // Q9's probe: its launch shape is fixed (one 512-thread workgroup per CU,
+  // two 4-row groups per lane and trip).
+  ctx->tune[T_Q9] = Tuning{256, 0, 2};
   // The Q13 matcher runs best at its occupancy limit (5 waves per SIMD):
   // 253 us for 15M comments, against 278 us at 4 workgroups per CU.
   ctx->tune[T_Q13] = Tuning{256, 0, 1};
@@ -291,7 +302,7 @@ int gdb_open(int device, uint64_t hbm_budget_bytes, gdb_ctx** out) {
   ctx->budget = hbm_budget_bytes ? hbm_budget_bytes : (free_b > reserve ? free_b - reserve : free_b);
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking) != hipSuccess) {
-    delete ctx;
+    gdb_close(ctx);
     return fail(GDB_E_HIP, "stream creation failed");
   }
   // The allocation-free entry point's scratch address on this device; a
@@ -299,13 +310,13 @@ int gdb_open(int device, uint64_t hbm_budget_bytes, gdb_ctx** out) {
   if (noalloc_scratch_init(device) != GDB_OK) (void)hipGetLastError();
   for (auto& e : ctx->ev) {
     if (hipEventCreate(&e) != hipSuccess) {
-      delete ctx;
+      gdb_close(ctx);
       return fail(GDB_E_HIP, "event creation failed");
     }
   }
   for (auto& e : ctx->sev) {
     if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
-      delete ctx;
+      gdb_close(ctx);
       return fail(GDB_E_HIP, "event creation failed");
     }
   }

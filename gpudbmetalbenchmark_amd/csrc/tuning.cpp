@@ -31,9 +31,9 @@ std::vector<Shape> candidates(int pipeline, const gdb_table* const* t, int n) {
       for (int bpc : {1, 2, 3, 4, 6, 8, 0}) c.push_back({256, bpc, 2, 0});
       break;
     case GDB_PIPE_Q9:
-      // the probe's LDS filter fixes one 512-thread workgroup per CU; what
-      // varies is the number of 4-row groups in flight per lane
-      for (int u : {1, 2, 4}) c.push_back({256, 0, u, 0});
+      // the probe's LDS filter fixes one 512-thread workgroup per CU and its
+      // pipelined loop two 4-row groups per lane: a single shape
+      c.push_back({256, 0, 2, 0});
       break;
     case GDB_PIPE_JOIN: {
       // one global table, the derived shape, and radix bits around the derived
