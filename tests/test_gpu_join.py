@@ -288,3 +288,50 @@ def test_q9_parts_with_more_than_four_suppliers(engine):
         finally:
             for t in tabs:
                 t.free()
+
+
+def test_q9_lineitem_layouts(engine):
+    """Q9 over clustered and shuffled lineitem, and with orphan orderkeys
+    (negative, past the largest orderkey, in the gaps between sparse orders)
+    placed at the start of 256-row groups and at random rows: each must equal
+    the oracle (an orphan joins no order)."""
+    rng = np.random.default_rng(21)
+    np_, ns, no = 3000, 40, 60_000
+    part = generate.part(5, np_)
+    ps = generate.partsupp(5, np_, ns)
+    su = generate.supplier(5, ns)
+    ok = np.sort(rng.choice(np.arange(1, 8 * no), no, replace=False)).astype(np.int32)
+    od = dict(o_orderkey=ok, o_orderdate=rng.integers(0, 2406, no).astype(np.int32))
+    lines = rng.integers(1, 8, no)
+    owner = np.repeat(np.arange(no), lines)
+    m = len(owner)
+    row = rng.integers(0, len(ps["ps_partkey"]), m)
+    base = dict(l_partkey=np.asarray(ps["ps_partkey"])[row].astype(np.int32),
+                l_suppkey=np.asarray(ps["ps_suppkey"])[row].astype(np.int32),
+                l_orderkey=ok[owner].astype(np.int32),
+                l_quantity=rng.integers(1, 51, m).astype(np.int32),
+                l_discount=rng.integers(0, 11, m).astype(np.int32),
+                l_extendedprice=rng.integers(90_000, 10_500_000, m).astype(np.int64))
+    orphan = base["l_orderkey"].copy()
+    starts = np.arange(0, m, 256)
+    orphan[starts[::3]] = np.int32(-2**31)          # negative key first in a group
+    orphan[starts[1::3]] = np.int32(int(ok[-1]) + 3)  # past the last order
+    gaps = np.setdiff1d(np.arange(1, int(ok[-1])), ok)[:2000]
+    orphan[rng.integers(0, m, len(gaps))] = gaps    # between orders
+    perm = rng.permutation(m)
+    layouts = {"clustered": base,
+               "shuffled": {k: v[perm] for k, v in base.items()},
+               "orphans": dict(base, l_orderkey=orphan)}
+    pt, st_, pst, ot = (engine.table(t) for t in (part, su, ps, od))
+    try:
+        for name, li in layouts.items():
+            lt = engine.table(li)
+            try:
+                for pat in ("green", "e"):
+                    got = engine.q9(pt, st_, pst, ot, lt, pat)
+                    assert got == O.q9(part, su, ps, od, li, pat), (name, pat)
+            finally:
+                lt.free()
+    finally:
+        for t in (pt, st_, pst, ot):
+            t.free()
