@@ -528,11 +528,11 @@ struct Q9Queue {
 // the 250 KB global bitmap cost a vector-cache line per lane and took as long
 // as streaming all six columns. Only keys whose folded bit is set read the
 // global bitmap word, which also gives a passing part's rank: the queue
-// carries the rank to the direct table's row. One workgroup of 1024 threads
-// per CU shares one fold (2^19 bits next to sixteen wave queues): sixteen
-// waves per CU stream the columns, against eight with 512-thread workgroups
-// and a 2^20-bit fold. With the hash table (no direct table) queued rows are
-// processed in place, 64 at a time, by q9_entry.
+// carries the rank to the direct table's row. One workgroup of 512 threads
+// per CU shares one fold (2^20 bits next to eight wave queues); two trips'
+// columns take 167 VGPRs per lane, which eight waves per CU allow (sixteen
+// waves with one group per trip spilled). With the hash table (no direct
+// table) queued rows are processed in place, 64 at a time, by q9_entry.
 constexpr int PROBE_THREADS = 512;
 constexpr int PROBE_U = 2;  // 4-row groups per lane and trip
 template <int U, bool DIRECT>
