@@ -165,6 +165,12 @@ class Engine:
         t = abi.Tuning(abi.PIPE_IDS[pipeline], block, blocks_per_cu, unroll, 0.0, 0, radix_bits)
         abi.check(lib().gdb_set_tuning(self._ctx, C.byref(t)))
 
+    def set_kernel_timing(self, on):
+        """On (the default): pipelines time their dominant kernel with events
+        (Timing.main_kernel_ms). Off: no events between a pipeline's kernels,
+        main_kernel_ms = -1."""
+        abi.check(lib().gdb_set_kernel_timing(self._ctx, 1 if on else 0))
+
     def bandwidth_probe(self, nbytes=1 << 30, reps=10):
         """(read GB/s, copy GB/s) of plain streaming kernels over nbytes."""
         r, c = C.c_double(), C.c_double()

@@ -197,6 +197,7 @@ PROTOTYPES = {
                                C.POINTER(Tuning)]),
     "gdb_get_tuning": (C.c_int, [P, C.c_int, C.POINTER(Tuning)]),
     "gdb_set_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
+    "gdb_set_kernel_timing": (C.c_int, [P, C.c_int]),
     "gdb_serve_start": (C.c_int, [P, C.POINTER(ServeCfg), C.POINTER(C.c_void_p)]),
     "gdb_submit": (C.c_int, [P, C.POINTER(Request), U64P]),
     "gdb_poll": (C.c_int, [P, C.POINTER(Response), C.c_int, C.c_int]),

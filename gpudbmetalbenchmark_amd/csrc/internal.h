@@ -66,6 +66,10 @@ struct gdb_ctx {
   hipStream_t stream = nullptr;
   hipStream_t copy_stream = nullptr;
   hipEvent_t ev[6] = {};
+  // ev[1] / ev[2] bracket a pipeline's dominant kernel only while this is
+  // set: each timing event recorded between two kernels leaves the GPU idle
+  // for a few microseconds.
+  bool kernel_timing = true;
   gdb_device_info info{};
   uint64_t budget = 0;
   uint64_t used = 0;
@@ -153,6 +157,12 @@ int run_over(gdb_ctx* ctx, const gdb_table* t, const Column* const* cols, int nc
 int noalloc_scratch_init(int device);
 // Rejects host tables where a pipeline needs the table resident.
 int require_resident(const gdb_table* t, const char* what);
+
+// Records ev[i] (1 or 2, around a pipeline's dominant kernel) on stream s
+// when kernel timing is on.
+inline hipError_t kernel_mark(gdb_ctx* ctx, int i, hipStream_t s) {
+  return ctx->kernel_timing ? hipEventRecord(ctx->ev[i], s) : hipSuccess;
+}
 
 // Event helpers around a pipeline (ev[0], ev[5]) and its dominant kernel (ev[1], ev[2]).
 int timing_finish(gdb_ctx* ctx, gdb_timing* t, uint64_t bytes, uint64_t main_bytes,

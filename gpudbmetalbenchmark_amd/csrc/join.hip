@@ -620,12 +620,12 @@ extern "C" int gdb_join_count(gdb_ctx* ctx, const gdb_table* build, const char* 
       jd_counts<<<fill, 256, 0, st>>>(rk, nb, lo, cnt);
       GDB_HIP(hipGetLastError());
     }
-    GDB_HIP(hipEventRecord(ctx->ev[1], st));
+    GDB_HIP(kernel_mark(ctx, 1, st));
     if (counts)
       jd_probe<true><<<blocks, 256, 0, st>>>(sk, np, lo, span, cnt, partials);
     else
       jd_probe<false><<<blocks, 256, 0, st>>>(sk, np, lo, span, table, partials);
-    GDB_HIP(hipEventRecord(ctx->ev[2], st));
+    GDB_HIP(kernel_mark(ctx, 2, st));
   } else if (global) {
     uint32_t cap = 1024, lg = 10;
     while (cap < 2ull * nb) {
@@ -645,18 +645,18 @@ extern "C" int gdb_join_count(gdb_ctx* ctx, const gdb_table* build, const char* 
     GDB_HIP(hipEventRecord(ctx->ev[0], st));
     join_clear<<<fill, 256, 0, st>>>(keys, mult, cap);
     if (nb) join_build<<<fill, 256, 0, st>>>(rk, nb, keys, mult, cap, shift);
-    GDB_HIP(hipEventRecord(ctx->ev[1], st));
+    GDB_HIP(kernel_mark(ctx, 1, st));
     join_probe<<<blocks, 256, 0, st>>>(sk, np, keys, mult, cap, shift, partials);
-    GDB_HIP(hipEventRecord(ctx->ev[2], st));
+    GDB_HIP(kernel_mark(ctx, 2, st));
   } else if (broadcast) {
     const int per_cu = std::max(1, ctx->info.lds_bytes_per_cu / (int)(LDS_SLOTS * 8 + 64));
     nparts = (uint32_t)std::max<uint64_t>(
         1, std::min<uint64_t>((uint64_t)per_cu * ctx->info.cu_count, ((uint64_t)np + 8191) / 8192));
     GDB_TRY(arena(ctx, A_PARTIALS, sizeof(int64_t) * nparts, (void**)&partials));
     GDB_HIP(hipEventRecord(ctx->ev[0], st));
-    GDB_HIP(hipEventRecord(ctx->ev[1], st));
+    GDB_HIP(kernel_mark(ctx, 1, st));
     rj_broadcast<<<nparts, JT, 0, st>>>(rk, nb, sk, np, partials);
-    GDB_HIP(hipEventRecord(ctx->ev[2], st));
+    GDB_HIP(kernel_mark(ctx, 2, st));
   } else {
     const uint32_t P = 1u << bits;
     const int b1 = bits > 7 ? bits / 2 : bits;  // first pass (or the only one)
@@ -703,10 +703,10 @@ extern "C" int gdb_join_count(gdb_ctx* ctx, const gdb_table* build, const char* 
       GDB_TRY(run_pass(ctx, r1, rG1, (uint32_t)o_r, roffs, totals + 0, tsum, st));
       GDB_TRY(run_pass(ctx, s1, sG1, (uint32_t)o_s, soffs, totals + 1, tsum, st));
     }
-    GDB_HIP(hipEventRecord(ctx->ev[1], st));
+    GDB_HIP(kernel_mark(ctx, 1, st));
     rj_join<<<P, JT, 0, st>>>(r_b, roffs, rG, totals + 0, s_b, soffs, sG, totals + 1, P,
                               partials);
-    GDB_HIP(hipEventRecord(ctx->ev[2], st));
+    GDB_HIP(kernel_mark(ctx, 2, st));
   }
   GDB_HIP(hipGetLastError());
   sum_columns<<<1, 256, 0, st>>>(partials, nparts, dout);

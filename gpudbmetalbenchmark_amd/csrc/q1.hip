@@ -215,8 +215,8 @@ extern "C" int gdb_q1(gdb_ctx* ctx, const gdb_table* li, const gdb_q1_params* p,
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
   if (n == 0) {
     GDB_TRY(fill(dout, 0, sizeof(int64_t) * K, s));
-    GDB_HIP(hipEventRecord(ctx->ev[1], s));
-    GDB_HIP(hipEventRecord(ctx->ev[2], s));
+    GDB_HIP(kernel_mark(ctx, 1, s));
+    GDB_HIP(kernel_mark(ctx, 2, s));
   } else {
     GDB_TRY(run_over(ctx, li, c, 7, consume, &rs));
   }

@@ -304,13 +304,17 @@ __device__ __forceinline__ uint32_t bucket_starts(const uint32_t* __restrict__ b
   return run;
 }
 
+// The per-bucket arrays are dynamic LDS sized to the bucket count (184 at
+// 1.5M customers), so more workgroups fit per CU than with arrays sized for
+// the most buckets.
 __global__ __launch_bounds__(TILE) void q13_scatter(const uint32_t* __restrict__ keys, uint32_t tpb,
                                                     const uint32_t* __restrict__ region_count,
                                                     const uint32_t* __restrict__ offs,
                                                     const uint32_t* __restrict__ btot, uint32_t nb,
                                                     uint32_t* __restrict__ sorted) {
-  __shared__ uint32_t base[MAX_BUCKETS];   // next global position per bucket
-  __shared__ uint32_t cnt[MAX_BUCKETS];    // tile count, then tile start per bucket
+  extern __shared__ uint32_t bucket_lds[];
+  uint32_t* base = bucket_lds;       // next global position per bucket
+  uint32_t* cnt = bucket_lds + nb;   // tile count, then tile start per bucket
   __shared__ uint32_t stage[STILE];
   __shared__ uint32_t rc[TILE / WROWS + 1];
   __shared__ uint32_t scratch[17];
@@ -534,7 +538,7 @@ extern "C" int gdb_q13(gdb_ctx* ctx, const gdb_table* customer, const gdb_table*
     offs = bh + nhist;
     btot = offs + nhist;
     q13_clear<<<5, 256, 0, st>>>(counts, 0, hist);
-    GDB_HIP(hipEventRecord(ctx->ev[1], st));
+    GDB_HIP(kernel_mark(ctx, 1, st));
     const bool heads_full = w1.len >= 4 && w2.len >= 4;
     const uint32_t ha = pattern_dword(w1, 0) & w1.m0, hb = pattern_dword(w2, 0) & w2.m0;
     auto launch = [&](auto match) {
@@ -552,9 +556,9 @@ extern "C" int gdb_q13(gdb_ctx* ctx, const gdb_table* customer, const gdb_table*
       else launch(GenericMatch<false>{w1, w2});
     }
     GDB_HIP(hipGetLastError());
-    GDB_HIP(hipEventRecord(ctx->ev[2], st));
+    GDB_HIP(kernel_mark(ctx, 2, st));
     q13_offsets<<<nb, 256, 0, st>>>(bh, G, offs, btot);
-    q13_scatter<<<G, TILE, 0, st>>>(keys, tpb, rcount, offs, btot, nb, sorted);
+    q13_scatter<<<G, TILE, (size_t)nb * 8, st>>>(keys, tpb, rcount, offs, btot, nb, sorted);
     if (dense)
       q13_bucket_count<true><<<nb, 1024, 0, st>>>(sorted, btot, nb, minck, maxck, counts,
                                                   hist, overflow);
@@ -583,8 +587,8 @@ extern "C" int gdb_q13(gdb_ctx* ctx, const gdb_table* customer, const gdb_table*
     if (no) {
       GDB_TRY(run_over(ctx, orders, oc, 2, consume, &rs));
     } else {
-      GDB_HIP(hipEventRecord(ctx->ev[1], st));
-      GDB_HIP(hipEventRecord(ctx->ev[2], st));
+      GDB_HIP(kernel_mark(ctx, 1, st));
+      GDB_HIP(kernel_mark(ctx, 2, st));
     }
   }
   if (nc && !(partitioned && dense))

@@ -729,8 +729,8 @@ extern "C" int gdb_q3(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* 
   if (nl) {
     GDB_TRY(run_over(ctx, lineitem, fc, 4, consume, &rs));
   } else {
-    GDB_HIP(hipEventRecord(ctx->ev[1], st));
-    GDB_HIP(hipEventRecord(ctx->ev[2], st));
+    GDB_HIP(kernel_mark(ctx, 1, st));
+    GDB_HIP(kernel_mark(ctx, 2, st));
   }
   const Q3Orders o{(const int32_t*)cok->dev, (const int32_t*)cod->dev, (const int32_t*)cop->dev,
                    no, olo, olast, s.okw};

@@ -992,8 +992,8 @@ static int q9_run(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supplier
   if (nl) {
     GDB_TRY(run_over(ctx, lineitem, fc, 6, consume, &rs));
   } else {
-    GDB_HIP(hipEventRecord(ctx->ev[1], st));
-    GDB_HIP(hipEventRecord(ctx->ev[2], st));
+    GDB_HIP(kernel_mark(ctx, 1, st));
+    GDB_HIP(kernel_mark(ctx, 2, st));
   }
   GDB_HIP(hipMemcpyAsync(host, s.agg, RESULT_BYTES, hipMemcpyDeviceToHost, st));
   GDB_HIP(hipEventRecord(ctx->ev[5], st));

@@ -170,8 +170,10 @@ int timing_finish(gdb_ctx* ctx, gdb_timing* t, uint64_t bytes, uint64_t main_byt
     GDB_HIP(hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[5]));
     if (rs && rs->chunks > 0) {
       b = (float)rs->kernel_ms;
-    } else {
+    } else if (ctx->kernel_timing) {
       GDB_HIP(hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]));
+    } else {
+      b = -1.f;  // not measured
     }
     t->gpu_ms = a;
     t->main_kernel_ms = b;
@@ -356,6 +358,12 @@ void gdb_debug_fail_allocs_after(gdb_ctx* ctx, int skip, int count) {
   if (!ctx) return;
   ctx->fail_skip = std::max(skip, 0);
   ctx->fail_allocs = count;
+}
+
+int gdb_set_kernel_timing(gdb_ctx* ctx, int on) {
+  if (!ctx) return fail(GDB_E_INVALID, "null argument");
+  ctx->kernel_timing = on != 0;
+  return GDB_OK;
 }
 
 int gdb_release_scratch(gdb_ctx* ctx) {

@@ -245,8 +245,8 @@ int gdb_q6(gdb_ctx* ctx, const gdb_table* li, const gdb_q6_params* p, gdb_q6_res
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
   if (n == 0) {
     GDB_TRY(fill(dout, 0, 16, s));
-    GDB_HIP(hipEventRecord(ctx->ev[1], s));
-    GDB_HIP(hipEventRecord(ctx->ev[2], s));
+    GDB_HIP(kernel_mark(ctx, 1, s));
+    GDB_HIP(kernel_mark(ctx, 2, s));
   } else {
     GDB_TRY(run_over(ctx, li, c, 4, consume, &rs));
   }
@@ -337,7 +337,7 @@ int gdb_q6_noalloc(gdb_ctx* ctx, const gdb_table* li, const gdb_q6_params* p, gd
                 *qp = (const int32_t*)c[2]->dev;
   const int64_t* pp = (const int64_t*)c[3]->dev;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_HIP(hipEventRecord(ctx->ev[1], s));
+  GDB_HIP(kernel_mark(ctx, 1, s));
   if (n == 0) {
     GDB_TRY(fill(dout, 0, 2 * sizeof(int64_t), s));  // a fill kernel, not a memset node
   } else {
@@ -353,7 +353,7 @@ int gdb_q6_noalloc(gdb_ctx* ctx, const gdb_table* li, const gdb_q6_params* p, gd
     sum_columns<0><<<2, 256, 0, s>>>(partials, (uint32_t)blocks, dout);
     GDB_HIP(hipGetLastError());
   }
-  GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  GDB_HIP(kernel_mark(ctx, 2, s));
   int64_t h[2] = {0, 0};  // the caller's stack, not pinned staging
   GDB_HIP(hipMemcpyAsync(h, dout, sizeof(h), hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
@@ -383,13 +383,13 @@ int gdb_sum(gdb_ctx* ctx, const gdb_table* t, const char* name, int64_t* sum, gd
   GDB_TRY(pinned(ctx, 64, (void**)&host));
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_HIP(hipEventRecord(ctx->ev[1], s));
+  GDB_HIP(kernel_mark(ctx, 1, s));
   if (c->type == GDB_I32)
     sum_scan<int32_t><<<blocks, block, 0, s>>>((const int32_t*)c->dev, n, partials);
   else
     sum_scan<int64_t><<<blocks, block, 0, s>>>((const int64_t*)c->dev, n, partials);
   GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  GDB_HIP(kernel_mark(ctx, 2, s));
   sum_columns<<<1, 256, 0, s>>>(partials, (uint32_t)blocks, dout);
   GDB_HIP(hipMemcpyAsync(host, dout, sizeof(int64_t), hipMemcpyDeviceToHost, s));
   GDB_HIP(hipEventRecord(ctx->ev[5], s));
@@ -424,10 +424,10 @@ int gdb_select_lt(gdb_ctx* ctx, const gdb_table* t, const char* name, int32_t va
   GDB_TRY(pinned(ctx, 64, (void**)&host));
   hipStream_t s = ctx->stream;
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
-  GDB_HIP(hipEventRecord(ctx->ev[1], s));
+  GDB_HIP(kernel_mark(ctx, 1, s));
   select_lt_bitmap<<<nblocks, block, 0, s>>>((const int32_t*)c->dev, n, value, chunk, dbm, dcounts);
   GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  GDB_HIP(kernel_mark(ctx, 2, s));
   scan_counts<<<1, 256, 0, s>>>(dcounts, nblocks, doffs, dtotal);
   GDB_HIP(hipMemcpyAsync(host, dtotal, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   GDB_HIP(hipStreamSynchronize(s));

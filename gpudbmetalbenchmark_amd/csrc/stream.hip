@@ -40,9 +40,9 @@ int run_over(gdb_ctx* ctx, const gdb_table* t, const Column* const* cols, int nc
   std::vector<const void*> dev(ncols);
   if (!t->host) {
     for (int i = 0; i < ncols; ++i) dev[i] = cols[i]->dev;
-    GDB_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+    GDB_HIP(kernel_mark(ctx, 1, ctx->stream));
     GDB_TRY(fn(dev.data(), (uint32_t)t->nrows));
-    GDB_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
+    GDB_HIP(kernel_mark(ctx, 2, ctx->stream));
     return GDB_OK;
   }
   const uint64_t n = t->nrows;

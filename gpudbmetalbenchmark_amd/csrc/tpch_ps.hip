@@ -595,13 +595,13 @@ int gdb_q2(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q2_params* p, gdb_q2_r
         (const int32_t*)sk->dev, (const int32_t*)sn->dev, (const int64_t*)sb->dev, (uint32_t)ns,
         nreg, p->region, snat, sbal);
   GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[1], s));
+  GDB_HIP(kernel_mark(ctx, 1, s));
   if (nps)
     q2_min<<<blocks_for(ctx, q2_min, nps), BLOCK, 0, s>>>(
         (const int32_t*)qp->dev, (const int32_t*)qs->dev, (const int64_t*)qc->dev, (uint32_t)nps,
         psel, snat, mn);
   GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  GDB_HIP(kernel_mark(ctx, 2, s));
   GDB_TRY(thk_run(ctx,
                   Q2Src{(const int32_t*)qp->dev, (const int32_t*)qs->dev,
                         (const int64_t*)qc->dev, psel, snat, sbal, mn, rank},
@@ -669,13 +669,13 @@ int gdb_q11(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q11_params* p, gdb_q1
     supp_nation_bits<<<blocks_for(ctx, supp_nation_bits, ns), BLOCK, 0, s>>>(
         (const int32_t*)sk->dev, (const int32_t*)sn->dev, (uint32_t)ns, p->nation, sbits);
   GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[1], s));
+  GDB_HIP(kernel_mark(ctx, 1, s));
   if (nps)
     q11_scan<<<blocks_for(ctx, q11_scan, quads(nps)), BLOCK, 0, s>>>(
         (const int32_t*)qp->dev, (const int32_t*)qs->dev, (const int32_t*)qa->dev,
         (const int64_t*)qc->dev, (uint32_t)nps, sbits, val, seen, dout);
   GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  GDB_HIP(kernel_mark(ctx, 2, s));
   GDB_TRY(acc_finish(ctx, dout, 0, 1));  // q11_collect reads the total
   q11_collect<<<blocks_for(ctx, q11_collect, pspan), BLOCK, 0, s>>>(
       val, seen, (uint32_t)pspan, dout, p->frac_num, p->frac_den, kv,
@@ -785,13 +785,13 @@ int gdb_q16(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q16_params* p, gdb_q1
         (const int32_t*)pp->dev, (const uint8_t*)pb->dev, (const uint8_t*)pt->dev,
         (const int32_t*)pz->dev, (uint32_t)np, qpart, gid);
   GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[1], s));
+  GDB_HIP(kernel_mark(ctx, 1, s));
   if (nps)
     q16_scan<<<blocks_for(ctx, q16_scan, quads(nps)), BLOCK, 0, s>>>(
         (const int32_t*)qp->dev, (const int32_t*)qs->dev, (uint32_t)nps, gid, bad, keys,
         slots - 1, cnt);
   GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  GDB_HIP(kernel_mark(ctx, 2, s));
   q16_block_counts<<<nblk, 256, 0, s>>>(cnt, dord, boff);
   q16_block_offsets<<<1, 1024, 0, s>>>(boff, (unsigned long long*)dout);
   q16_emit<<<nblk, 256, 0, s>>>(cnt, dord, boff, grp);

@@ -834,12 +834,12 @@ int gdb_q22(gdb_ctx* ctx, const gdb_tpch_db* db, const gdb_q22_params* p, gdb_q2
   GDB_HIP(hipEventRecord(ctx->ev[0], s));
   GDB_TRY(fill(has, 0, span, s));
   GDB_TRY(fill(dout, 0, ACC_BYTES, s));
-  GDB_HIP(hipEventRecord(ctx->ev[1], s));
+  GDB_HIP(kernel_mark(ctx, 1, s));
   if (no)
     mark_bytes<<<blocks_for(ctx, mark_bytes, quads(no)), BLOCK, 0, s>>>((const int32_t*)oc->dev,
                                                                          (uint32_t)no, has);
   GDB_HIP(hipGetLastError());
-  GDB_HIP(hipEventRecord(ctx->ev[2], s));
+  GDB_HIP(kernel_mark(ctx, 2, s));
   if (nc) {
     q22_avg<<<blocks_for(ctx, q22_avg, nc), BLOCK, 0, s>>>(
         (const int32_t*)cn->dev, (const int64_t*)cb->dev, (uint32_t)nc, codes, dout);

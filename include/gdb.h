@@ -122,6 +122,13 @@ typedef struct {
   int32_t pad;
 } gdb_timing;
 
+/* Kernel timing (on by default): while on, every pipeline brackets its
+ * dominant kernel with timing events and fills main_kernel_ms; off, it
+ * records no event between its kernels (each one leaves the GPU idle for a
+ * few microseconds) and sets main_kernel_ms to -1. No counterpart in the
+ * reference, which times whole command buffers (src/q6.cpp:92-94). */
+int gdb_set_kernel_timing(gdb_ctx* ctx, int on);
+
 /* ---- host-resident tables streamed in chunks (replaces chunkedStreamLoop
  *      and ChunkConfig, reference src/infra.h:37-57, :805-869, used by the
  *      runQ*BenchmarkSF100 entry points, src/infra.h:874-922) ----------------

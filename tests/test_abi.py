@@ -127,3 +127,8 @@ def test_struct_layouts_match_ctypes(tmp_path):
         cls = getattr(abi, LAYOUTS[cname])
         exp = C.sizeof(cls) if field == "__sizeof__" else getattr(cls, field).offset
         assert int(val) == exp, (cname, field, int(val), exp)
+
+
+def test_set_kernel_timing_null_context():
+    lib = abi.load()
+    assert lib.gdb_set_kernel_timing(None, 0) == abi.GDB_E_INVALID

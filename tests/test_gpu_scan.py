@@ -173,3 +173,28 @@ def test_q1_q6_sums_near_the_int64_limit(engine):
         engine.q6(t3, 0, 1, 0, 10, 2)
     assert e.value.code == GDB_E_UNSUPPORTED
     t3.free()
+
+
+def test_kernel_timing_switch(engine):
+    """With kernel timing off no event brackets the dominant kernel
+    (main_kernel_ms = -1) and every result is unchanged; back on, the kernel
+    is timed again."""
+    li = generate.lineitem_flat(9, 300_001)
+    t = engine.table(li)
+    try:
+        on, off = Timing(), Timing()
+        r_on = (engine.q6(t, timing=on), engine.q1(t, timing=on), engine.sum(t, "l_quantity"))
+        assert on.main_kernel_ms > 0
+        engine.set_kernel_timing(False)
+        try:
+            r_off = (engine.q6(t, timing=off), engine.q1(t, timing=off),
+                     engine.sum(t, "l_quantity", timing=off))
+            assert off.main_kernel_ms == -1 and off.gpu_ms > 0
+        finally:
+            engine.set_kernel_timing(True)
+        assert r_off == r_on
+        assert r_on[0] == O.q6(li)
+        engine.q6(t, timing=on)
+        assert on.main_kernel_ms > 0
+    finally:
+        t.free()
