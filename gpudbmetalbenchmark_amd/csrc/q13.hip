@@ -154,10 +154,15 @@ __global__ __launch_bounds__(TILE) void q13_match(const uint8_t* __restrict__ co
                                                   uint32_t tpb, uint32_t nb,
                                                   uint32_t* __restrict__ keys,
                                                   uint32_t* __restrict__ region_count,
-                                                  uint32_t* __restrict__ hist) {
+                                                  uint32_t* __restrict__ hist,
+                                                  uint32_t* __restrict__ out_hist) {
   extern __shared__ uint32_t lhist[];  // nb buckets (dynamic: small for small customer tables)
   __shared__ int4 stage[TILE / WROWS][WCHUNKS];
   __shared__ uint32_t mq[TILE / WROWS][MQCAP];
+  // the c_count histogram (and overflow counter) that q13_bucket_count fills
+  // starts at zero; clearing it here saves a launch
+  if (blockIdx.x == 0)
+    for (uint32_t i = threadIdx.x; i < NBINS + 4; i += TILE) out_hist[i] = 0u;
   for (uint32_t i = threadIdx.x; i < nb; i += TILE) lhist[i] = 0u;
   __syncthreads();
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -537,7 +542,6 @@ extern "C" int gdb_q13(gdb_ctx* ctx, const gdb_table* customer, const gdb_table*
     bh = rcount + 4 * G;
     offs = bh + nhist;
     btot = offs + nhist;
-    q13_clear<<<5, 256, 0, st>>>(counts, 0, hist);
     GDB_HIP(kernel_mark(ctx, 1, st));
     const bool heads_full = w1.len >= 4 && w2.len >= 4;
     const uint32_t ha = pattern_dword(w1, 0) & w1.m0, hb = pattern_dword(w2, 0) & w2.m0;
@@ -545,11 +549,11 @@ extern "C" int gdb_q13(gdb_ctx* ctx, const gdb_table* customer, const gdb_table*
       if (heads_full)
         q13_match<decltype(match), HeadPair<false>><<<G, TILE, hist_lds, st>>>(
             (const uint8_t*)ocom->dev, (const int32_t*)ock->dev, n, match,
-            HeadPair<false>{ha, hb, w1.m0, w2.m0}, maxck, tpb, nb, keys, rcount, bh);
+            HeadPair<false>{ha, hb, w1.m0, w2.m0}, maxck, tpb, nb, keys, rcount, bh, hist);
       else
         q13_match<decltype(match), HeadPair<true>><<<G, TILE, hist_lds, st>>>(
             (const uint8_t*)ocom->dev, (const int32_t*)ock->dev, n, match,
-            HeadPair<true>{ha, hb, w1.m0, w2.m0}, maxck, tpb, nb, keys, rcount, bh);
+            HeadPair<true>{ha, hb, w1.m0, w2.m0}, maxck, tpb, nb, keys, rcount, bh, hist);
     };
     if (!dispatch_fixed<4, 4>(w1, w2, launch)) {
       if (full0) launch(GenericMatch<true>{w1, w2});
