@@ -41,7 +41,7 @@ constexpr int TILE = 256;
 constexpr int CW = 80;         // comment width
 constexpr int NBINS = 1024;
 constexpr int BUCKET_SHIFT = 13;                 // 8192 custkeys per bucket (32 KiB of LDS counters); 4096 sped the count up by 9 us and slowed the scatter by 10
-constexpr int STILE = 8192;                      // keys per LDS-sorted tile of the scatter
+constexpr int STILE = 4096;                      // keys per LDS-sorted tile of the scatter (8192 needed 159 VGPRs against 78: half the waves)
 constexpr uint32_t MAX_BUCKETS = 4096;
 
 // Comment matchers: a functor on the 80-byte field held in 20 dwords followed
