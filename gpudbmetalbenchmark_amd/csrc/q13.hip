@@ -413,8 +413,31 @@ __global__ __launch_bounds__(1024) void q13_bucket_count(const uint32_t* __restr
   __syncthreads();
   const uint32_t begin = before;
   const uint32_t end = begin + btot[k];
-  for (uint32_t i = begin + threadIdx.x; i < end; i += blockDim.x)
-    atomicAdd(&c[sorted[i] & ((1u << BUCKET_SHIFT) - 1)], 1u);
+  // The segment as aligned 16-byte vectors of keys, two per thread and trip,
+  // with its unaligned head and tail key by key (one 4-byte load per key
+  // took twice as long: 29 -> 14 us at 15M orders).
+  constexpr uint32_t KM = (1u << BUCKET_SHIFT) - 1;
+  const uint32_t vb = (begin + 3) & ~3u, ve = max(vb, end & ~3u);
+  for (uint32_t i = begin + threadIdx.x; i < min(vb, end); i += blockDim.x)
+    atomicAdd(&c[sorted[i] & KM], 1u);
+  for (uint32_t i = ve + threadIdx.x; i < end; i += blockDim.x) atomicAdd(&c[sorted[i] & KM], 1u);
+  const uint4* sv = reinterpret_cast<const uint4*>(sorted);  // the arena is 256-byte aligned
+  const uint32_t v1 = ve / 4;
+  for (uint32_t v = vb / 4 + threadIdx.x; v < v1; v += 2 * blockDim.x) {
+    const uint4 x = sv[v];
+    const bool two = v + blockDim.x < v1;
+    const uint4 y = two ? sv[v + blockDim.x] : make_uint4(0u, 0u, 0u, 0u);
+    atomicAdd(&c[x.x & KM], 1u);
+    atomicAdd(&c[x.y & KM], 1u);
+    atomicAdd(&c[x.z & KM], 1u);
+    atomicAdd(&c[x.w & KM], 1u);
+    if (two) {
+      atomicAdd(&c[y.x & KM], 1u);
+      atomicAdd(&c[y.y & KM], 1u);
+      atomicAdd(&c[y.z & KM], 1u);
+      atomicAdd(&c[y.w & KM], 1u);
+    }
+  }
   __syncthreads();
   const uint32_t key0 = k << BUCKET_SHIFT;
   if (!DENSE) {
