@@ -79,6 +79,9 @@ struct gdb_ctx {
   std::vector<gdb::Arena> arenas;  // indexed by purpose slot
   void* host_pinned = nullptr;     // small pinned staging for results
   size_t host_pinned_bytes = 0;
+  void* host_result = nullptr;     // coherent pinned memory the final kernels write
+  void* host_result_dev = nullptr; // its address on the device
+  size_t host_result_bytes = 0;
   gdb::Tuning tune[16];
   uint64_t chunk_rows = 0;         // streamed chunk size, 0 = adaptive
   uint64_t q20_pairs = 0;          // Q20: (part, supplier) pairs of the last run, sizes the next table
@@ -108,6 +111,11 @@ int arena(gdb_ctx* ctx, int slot, size_t bytes, void** out);
 int fill(void* p, int byte, size_t bytes, hipStream_t s);
 // Pinned host staging of at least `bytes`.
 int pinned(gdb_ctx* ctx, size_t bytes, void** out);
+// Coherent pinned host memory of at least `bytes`, mapped into the device's
+// address space: a pipeline's final kernel stores its result there (at
+// `*dev`), and the host reads it (at `*host`) once the step's closing event
+// has completed, with no separate readback copy.
+int result_buf(gdb_ctx* ctx, size_t bytes, void** host, void** dev);
 // Column lookup with type check; sets the error message on failure.
 int column(const gdb_table* t, const char* name, int type, const Column** out);
 

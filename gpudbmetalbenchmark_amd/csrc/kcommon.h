@@ -140,17 +140,33 @@ __device__ __forceinline__ uint32_t value_bucket(int64_t v) {
 }
 
 // Finaliser: block k of the grid sums column k of a k-major partials array
-// ([K][nparts]) into out[k] (ACC = 1: adds to out[k], for streamed chunks).
+// ([K][nparts]) into out[k] (ACC = 1: adds to out[k], for streamed chunks),
+// and, when host_out is set, also stores it at host_out[k] (mapped pinned
+// host memory: the result reaches the host with no readback copy).
 // Launch with 256 threads.
 template <int ACC = 0>
 __global__ __launch_bounds__(256) void sum_columns(const int64_t* __restrict__ partials,
-                                                   uint32_t nparts, int64_t* __restrict__ out) {
+                                                   uint32_t nparts, int64_t* __restrict__ out,
+                                                   int64_t* host_out = nullptr) {
   __shared__ int64_t red[4];
   const int64_t* col = partials + (size_t)blockIdx.x * nparts;
   int64_t s = 0;
   for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) s += col[i];
   s = block_sum(s, red);
-  if (threadIdx.x == 0) out[blockIdx.x] = ACC ? out[blockIdx.x] + s : s;
+  if (threadIdx.x == 0) {
+    const int64_t v = ACC ? out[blockIdx.x] + s : s;
+    out[blockIdx.x] = v;
+    if (host_out) host_out[blockIdx.x] = v;
+  }
+}
+
+// Result readback without a copy engine: one workgroup copies `words` words
+// (W: 4 or 8 bytes) from device memory to mapped pinned host memory.
+// Launch <<<1, 256>>>.
+template <class W>
+__global__ __launch_bounds__(256) void publish_words(const W* __restrict__ src, uint32_t words,
+                                                     W* dst) {
+  for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
 }
 
 }  // namespace gdbk

@@ -532,7 +532,7 @@ extern "C" int gdb_q13(gdb_ctx* ctx, const gdb_table* customer, const gdb_table*
   const uint32_t minck = nc ? (uint32_t)cck->minv : 1u;
   const bool dense = nc && cck->strict && (uint64_t)(cck->maxv - cck->minv) + 1 == nc;
   const bool full0 = w1.len >= 4 && w2.len >= 4;
-  uint32_t *counts, *hist, *overflow, *host;
+  uint32_t *counts, *hist, *overflow, *host, *hdev;
   {
     char* b;
     GDB_TRY(arena(ctx, A_MAP, (size_t)ncounts * 4 + 256, (void**)&b));
@@ -542,7 +542,7 @@ extern "C" int gdb_q13(gdb_ctx* ctx, const gdb_table* customer, const gdb_table*
     GDB_TRY(arena(ctx, A_AUX, (size_t)std::max<uint32_t>(nc, 1) * 4, (void**)&b));
     overflow = (uint32_t*)b;
   }
-  GDB_TRY(pinned(ctx, (NBINS + 4) * 4, (void**)&host));
+  GDB_TRY(result_buf(ctx, (NBINS + 1) * 4, (void**)&host, (void**)&hdev));
   hipStream_t st = ctx->stream;
   RunStats rs;
   GDB_HIP(hipEventRecord(ctx->ev[0], st));
@@ -621,8 +621,11 @@ extern "C" int gdb_q13(gdb_ctx* ctx, const gdb_table* customer, const gdb_table*
   if (nc && !(partitioned && dense))
     q13_histogram<<<std::min<uint32_t>(1024, (nc + 255) / 256), 256, 0, st>>>(
         (const int32_t*)cck->dev, nc, counts, hist, overflow);
+  // The histogram and overflow count into the mapped result buffer. (A
+  // last-workgroup ticket in the counting kernel instead cost more than this
+  // launch: every workgroup's device-scope fence flushed its XCD's L2.)
+  publish_words<<<1, 256, 0, st>>>(hist, NBINS + 1, hdev);
   GDB_HIP(hipGetLastError());
-  GDB_HIP(hipMemcpyAsync(host, hist, (NBINS + 1) * 4, hipMemcpyDeviceToHost, st));
   GDB_HIP(hipEventRecord(ctx->ev[5], st));
   GDB_TRY(timing_finish(ctx, timing, 84ull * no + 4ull * nc, 84ull * no, &rs));
   std::map<uint32_t, int64_t> h;

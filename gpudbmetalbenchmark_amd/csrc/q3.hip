@@ -60,7 +60,7 @@ struct Q3Scratch {
   int64_t* rev;        // revenue by rank
   uint8_t* hit;        // 1 where a fact row reached the slot
   TopEntry* cand;      // CAND_CAP candidates
-  TopEntry* out;       // k final
+  TopEntry* out;       // k final rows and two counts (mapped host memory)
 };
 
 // Zeroes three regions, each a whole number of 16-byte units.
@@ -600,8 +600,8 @@ __global__ __launch_bounds__(256) void q3_topk_collect(Q3Orders o, Q3Scratch s) 
 // list (more than CAND_CAP groups tie into the threshold bucket or above),
 // every workgroup ranks its share of the orders and leaves its k best, and
 // the workgroup that finishes last ranks those.
-// The qualifying orders and group count go right after the k result rows, so
-// the host reads the result back in one copy.
+// The qualifying orders and group count go right after the k result rows, in
+// the host's mapped result buffer.
 __device__ __forceinline__ void q3_result_counts(int k, const Q3Scratch& s) {
   if (threadIdx.x == 0) {
     uint32_t* tail = reinterpret_cast<uint32_t*>(s.out + k);
@@ -732,12 +732,15 @@ extern "C" int gdb_q3(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* 
     GDB_TRY(arena(ctx, A_Q3MAP, okw_bytes, (void**)&s.okw));
     char* a;
     const size_t ncand = std::max<size_t>(CAND_CAP, (size_t)oblocks * k);
-    GDB_TRY(arena(ctx, A_AUX, sizeof(TopEntry) * (ncand + k) + 64, (void**)&a));
+    GDB_TRY(arena(ctx, A_AUX, sizeof(TopEntry) * ncand + 64, (void**)&a));
     s.cand = (TopEntry*)a;
-    s.out = s.cand + ncand;
   }
+  // The ranking workgroup writes the k rows and the two counts straight into
+  // the mapped host result buffer: no readback copy.
   TopEntry* host;
-  GDB_TRY(pinned(ctx, sizeof(TopEntry) * k + 64, (void**)&host));
+  void* hdev;
+  GDB_TRY(result_buf(ctx, sizeof(TopEntry) * k + 64, (void**)&host, &hdev));
+  s.out = (TopEntry*)hdev;
   uint32_t* host_cnt = (uint32_t*)(host + k);
 
   hipStream_t st = ctx->stream;
@@ -791,7 +794,6 @@ extern "C" int gdb_q3(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* 
   if (no) q3_topk_collect<<<((no + 3) / 4 + B - 1) / B, B, 0, st>>>(o, s);
   q3_topk_rank<<<oblocks, B, 0, st>>>(k, o, s);
   GDB_HIP(hipGetLastError());
-  GDB_HIP(hipMemcpyAsync(host, s.out, sizeof(TopEntry) * k + 8, hipMemcpyDeviceToHost, st));
   GDB_HIP(hipEventRecord(ctx->ev[5], st));
   // Pipeline bytes: every column the query names, read once (customer 5 B,
   // orders 16 B, fact 20 B per row). Probe bytes: what the probe must read

@@ -55,7 +55,8 @@ constexpr int64_t EMPTY64 = -1;
 constexpr uint8_t NO_YEAR = 0xFF;
 constexpr int QCAP = 128;       // per-wave survivor queue: < 64 left over + 64 pushed
 constexpr int PS_WAYS = 4;      // direct partsupp table: supplier slots per part (TPC-H has 4)
-// Result block read back in one copy: profit, hit flags, then four counters.
+// Result block, copied to the host by one publish launch: profit, hit flags,
+// then four counters.
 constexpr size_t RESULT_BYTES = (size_t)NNAT * NYEAR * 12 + 16;
 
 __device__ __forceinline__ uint32_t ps_cap(uint32_t count) {
@@ -931,8 +932,8 @@ static int q9_run(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supplier
       GDB_TRY(arena(ctx, A_HT, capmax * 16 + 256, (void**)&s.ht));
     }
   }
-  char* host;
-  GDB_TRY(pinned(ctx, RESULT_BYTES, (void**)&host));
+  char *host, *hdev;
+  GDB_TRY(result_buf(ctx, RESULT_BYTES, (void**)&host, (void**)&hdev));
   hipStream_t st = ctx->stream;
   const int B = 256;
   GDB_HIP(hipEventRecord(ctx->ev[0], st));
@@ -995,7 +996,10 @@ static int q9_run(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supplier
     GDB_HIP(kernel_mark(ctx, 1, st));
     GDB_HIP(kernel_mark(ctx, 2, st));
   }
-  GDB_HIP(hipMemcpyAsync(host, s.agg, RESULT_BYTES, hipMemcpyDeviceToHost, st));
+  static_assert(RESULT_BYTES % 4 == 0, "result block is copied in 4-byte words");
+  publish_words<<<1, 256, 0, st>>>((const uint32_t*)s.agg, (uint32_t)(RESULT_BYTES / 4),
+                                   (uint32_t*)hdev);
+  GDB_HIP(hipGetLastError());
   GDB_HIP(hipEventRecord(ctx->ev[5], st));
   const uint64_t bytes = 28ull * nl + 59ull * np + 16ull * nps + 8ull * ns + 8ull * no;
   GDB_TRY(timing_finish(ctx, timing, bytes, 28ull * nl, &rs));

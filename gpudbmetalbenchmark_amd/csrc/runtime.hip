@@ -127,6 +127,36 @@ int pinned(gdb_ctx* ctx, size_t bytes, void** out) {
   return GDB_OK;
 }
 
+int result_buf(gdb_ctx* ctx, size_t bytes, void** host, void** dev) {
+  if (ctx->host_result_bytes < bytes) {
+    if (ctx->host_result) {
+      GDB_HIP(hipStreamSynchronize(ctx->stream));
+      (void)hipHostFree(ctx->host_result);
+    }
+    ctx->host_result = nullptr;
+    ctx->host_result_dev = nullptr;
+    ctx->host_result_bytes = 0;
+    const size_t want = std::max<size_t>(bytes, 16384);
+    void* h = nullptr;
+    if (hipHostMalloc(&h, want, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(GDB_E_NOMEM, "hipHostMalloc (coherent result buffer) failed");
+    }
+    void* d = nullptr;
+    if (hipHostGetDevicePointer(&d, h, 0) != hipSuccess || !d) {
+      (void)hipGetLastError();
+      (void)hipHostFree(h);
+      return fail(GDB_E_HIP, "hipHostGetDevicePointer failed for the result buffer");
+    }
+    ctx->host_result = h;
+    ctx->host_result_dev = d;
+    ctx->host_result_bytes = want;
+  }
+  *host = ctx->host_result;
+  *dev = ctx->host_result_dev;
+  return GDB_OK;
+}
+
 int column(const gdb_table* t, const char* name, int type, const Column** out) {
   if (!t) return fail(GDB_E_INVALID, "null table");
   if (!name) return fail(GDB_E_INVALID, "null column name");
@@ -333,6 +363,7 @@ void gdb_close(gdb_ctx* ctx) {
   if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
   for (Arena& a : ctx->arenas) dev_free(ctx, a.ptr, a.bytes);
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
+  if (ctx->host_result) (void)hipHostFree(ctx->host_result);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : ctx->sev)
