@@ -26,6 +26,21 @@ def lib():
     return abi.load()
 
 
+# numpy views of the integer result rows: one frombuffer + tolist per column
+# instead of a ctypes attribute read per field (Q9's 175 rows: 43 -> 29 us).
+_Q9_DT = np.dtype({"names": ["nationkey", "year", "profit"], "formats": ["<i4", "<i4", "<i8"],
+                   "offsets": [abi.Q9Row.nationkey.offset, abi.Q9Row.year.offset,
+                               abi.Q9Row.profit.offset], "itemsize": C.sizeof(abi.Q9Row)})
+_Q13_DT = np.dtype({"names": ["c_count", "custdist"], "formats": ["<i4", "<i8"],
+                    "offsets": [abi.Q13Row.c_count.offset, abi.Q13Row.custdist.offset],
+                    "itemsize": C.sizeof(abi.Q13Row)})
+
+
+def _columns(res, dt, names):
+    a = np.frombuffer(res, dtype=dt, count=res.nrows, offset=type(res).rows.offset)
+    return [a[n].tolist() for n in names]
+
+
 class Table:
     """A columnar table resident in HBM (gdb_table_create), or, with host=True,
     a host-resident table streamed in chunks by the pipelines
@@ -228,15 +243,16 @@ class Engine:
         abi.check(lib().gdb_q9(self._ctx, part.handle, supplier.handle, partsupp.handle,
                                orders.handle, lineitem.handle, C.byref(p), C.byref(r),
                                abi.tref(timing)))
-        return [dict(nationkey=x.nationkey, year=x.year, profit=x.profit)
-                for x in r.rows[:r.nrows]]
+        nk, yr, pr = _columns(r, _Q9_DT, ("nationkey", "year", "profit"))
+        return [{"nationkey": a, "year": b, "profit": c} for a, b, c in zip(nk, yr, pr)]
 
     def q13(self, customer, orders, word1="special", word2="requests", timing=None):
         p = Q13Params(word1.encode(), word2.encode())
         r = Q13Result()
         abi.check(lib().gdb_q13(self._ctx, customer.handle, orders.handle, C.byref(p),
                                 C.byref(r), abi.tref(timing)))
-        return [dict(c_count=x.c_count, custdist=x.custdist) for x in r.rows[:r.nrows]]
+        cc, cd = _columns(r, _Q13_DT, ("c_count", "custdist"))
+        return [{"c_count": a, "custdist": b} for a, b in zip(cc, cd)]
 
     # ---- primitives --------------------------------------------------------
     def select_lt(self, table, column, value, bitmap=True, rowids=True, timing=None):

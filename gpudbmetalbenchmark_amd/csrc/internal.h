@@ -91,6 +91,7 @@ struct gdb_ctx {
   // query costs microseconds and pipelines ask on every call.
   std::mutex occ_mu;
   std::map<std::tuple<const void*, int, size_t>, int> occ;
+  std::map<const void*, size_t> lds_attr;  // dynamic LDS granted per kernel (occ_mu)
 };
 
 struct gdb_table {
@@ -134,6 +135,9 @@ int sum_fits(uint64_t n, unsigned __int128 term, const char* what);
 // Number of workgroups that keep every CU busy for `kernel` at `block` threads
 // and `lds` bytes of dynamic LDS.
 int resident_blocks(gdb_ctx* ctx, const void* kernel, int block, size_t lds, int per_cu_cap);
+// Lets `kernel` launch with `bytes` of dynamic LDS: hipFuncSetAttribute only
+// when the context has not already granted that much to it.
+int allow_dynamic_lds(gdb_ctx* ctx, const void* kernel, size_t bytes);
 
 enum ArenaSlot {
   A_PARTIALS = 0, A_BITMAP = 1, A_MAP = 2, A_HT = 3, A_AGG = 4, A_AUX = 5, A_OUT = 6,

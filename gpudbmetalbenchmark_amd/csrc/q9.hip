@@ -976,7 +976,7 @@ static int q9_run(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supplier
   // columns plus the dependent loads in flight in registers (116 VGPRs at
   // one group; two groups spill).
   const void* kfn = direct ? (const void*)q9_probe<PROBE_U, true> : (const void*)q9_probe<PROBE_U, false>;
-  GDB_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)tab_lds));
+  GDB_TRY(allow_dynamic_lds(ctx, kfn, tab_lds));
   const int PB = PROBE_THREADS;
   const int president = resident_blocks(ctx, kfn, PB, tab_lds, ctx->tune[T_Q9].blocks_per_cu);
   auto consume = [&](const void* const* d, uint32_t rows) -> int {

@@ -171,6 +171,15 @@ int column(const gdb_table* t, const char* name, int type, const Column** out) {
   return fail(GDB_E_INVALID, std::string("table has no column ") + name);
 }
 
+int allow_dynamic_lds(gdb_ctx* ctx, const void* kernel, size_t bytes) {
+  std::lock_guard<std::mutex> g(ctx->occ_mu);
+  size_t& granted = ctx->lds_attr[kernel];
+  if (granted >= bytes) return GDB_OK;
+  GDB_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  granted = bytes;
+  return GDB_OK;
+}
+
 int resident_blocks(gdb_ctx* ctx, const void* kernel, int block, size_t lds, int per_cu_cap) {
   int per_cu = 0;
   const auto key = std::make_tuple(kernel, block, lds);
