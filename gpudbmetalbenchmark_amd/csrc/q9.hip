@@ -55,8 +55,8 @@ constexpr int64_t EMPTY64 = -1;
 constexpr uint8_t NO_YEAR = 0xFF;
 constexpr int QCAP = 128;       // per-wave survivor queue: < 64 left over + 64 pushed
 constexpr int PS_WAYS = 4;      // direct partsupp table: supplier slots per part (TPC-H has 4)
-// Result block, copied to the host by one publish launch: profit, hit flags,
-// then four counters.
+// Device result block: profit and hit flags per (nation, year) slot, then
+// four counters (q9_publish copies the query's slots to the host).
 constexpr size_t RESULT_BYTES = (size_t)NNAT * NYEAR * 12 + 16;
 
 __device__ __forceinline__ uint32_t ps_cap(uint32_t count) {
@@ -91,6 +91,19 @@ __device__ __forceinline__ uint32_t year_index(int32_t day, const Years& y) {
   return k;
 }
 
+// The year map, orderkey -> year index. NIB: 4 bits per key value, eight to
+// a 32-bit word, 15 = no order (for at most 15 years and unique keys: half
+// the footprint of the probe's gathers); else one byte per key value, 0xFF
+// = no order. Both start as all ones.
+template <bool NIB>
+__device__ __forceinline__ uint8_t year_at(const uint8_t* oyear, uint32_t ok) {
+  if (NIB) {
+    const uint32_t v = (reinterpret_cast<const uint32_t*>(oyear)[ok >> 3] >> ((ok & 7u) * 4u)) & 15u;
+    return v == 15u ? NO_YEAR : (uint8_t)v;
+  }
+  return oyear[ok];
+}
+
 // Part bitmap word: bits of 32 consecutive partkeys, and the number of set
 // bits in all earlier words (the rank of the word's first passing part).
 struct alignas(8) PartWord {
@@ -107,7 +120,7 @@ struct Q9Scratch {
   uint32_t* rsum;      // passing parts per rank block
   uint32_t* counters;  // [0] partsupp list count, [1] a part with more than PS_WAYS suppliers
   int32_t* snat;       // suppkey -> nation or -1
-  uint8_t* oyear;      // orderkey -> year index or 0xFF
+  uint8_t* oyear;      // orderkey -> year index (bytes, or 4-bit codes)
   int32_t* lpk;        // compact partsupp list
   int32_t* lsk;
   int64_t* lcost;
@@ -222,15 +235,39 @@ __global__ __launch_bounds__(TILE) void q9_part(Q9DimArgs a) {
 // loads of key and date). (Running the part filter, this and the supplier map
 // as one grid, workgroup ranges per task, took longer than the three
 // launches: each is latency-bound and got fewer workgroups.)
+// NIB: a key's 4-bit code is set by ANDing its word with the code in place
+// of the key's 1111 (one non-returning atomicAnd per word the thread's four
+// orders touch); exact because the keys are unique.
+template <bool NIB>
 __global__ __launch_bounds__(256) void q9_orders(Q9DimArgs a) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= (a.no + 3) / 4) return;
   const int4 k = ld4nt(a.ok, g), d = ld4nt(a.od, g);
   const int32_t ks[4] = {k.x, k.y, k.z, k.w}, ds[4] = {d.x, d.y, d.z, d.w};
+  if (!NIB) {
 #pragma unroll
-  for (int j = 0; j < 4; ++j)
-    if (4 * g + j < a.no && (uint32_t)ks[j] <= a.maxok)
-      a.oyear[ks[j]] = (uint8_t)year_index(ds[j], a.years);
+    for (int j = 0; j < 4; ++j)
+      if (4 * g + j < a.no && (uint32_t)ks[j] <= a.maxok)
+        a.oyear[ks[j]] = (uint8_t)year_index(ds[j], a.years);
+    return;
+  }
+  uint32_t* w32 = reinterpret_cast<uint32_t*>(a.oyear);
+  uint32_t cur = 0xFFFFFFFFu, mask = 0xFFFFFFFFu;  // keys are < 2^31: word < 2^28
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (4 * g + j < a.no && (uint32_t)ks[j] <= a.maxok) {
+      const uint32_t key = (uint32_t)ks[j], wi = key >> 3;
+      const uint32_t m = ~((15u ^ year_index(ds[j], a.years)) << ((key & 7u) * 4u));
+      if (wi != cur) {
+        if (cur != 0xFFFFFFFFu) atomicAnd(&w32[cur], mask);
+        cur = wi;
+        mask = m;
+      } else {
+        mask &= m;
+      }
+    }
+  }
+  if (cur != 0xFFFFFFFFu) atomicAnd(&w32[cur], mask);
 }
 
 __global__ __launch_bounds__(256) void q9_supplier(Q9DimArgs a) {
@@ -465,15 +502,32 @@ struct Q9Dims {
   int32_t direct;    // 1: partsupp costs in the direct table, else the hash table
 };
 
+// The result into mapped host memory, compacted to the nnat x ny slots the
+// query can fill (2.1 KB for 25 nations over 7 years, against 48 KB for the
+// whole 64 x 64 table, which a workgroup's stores to host memory take 13 us
+// to write): profit[nslots], hit[nslots], then the four counters.
+__global__ __launch_bounds__(256) void q9_publish(Q9Scratch s, int32_t ny, int32_t first,
+                                                  int32_t nslots, char* out) {
+  int64_t* profit = reinterpret_cast<int64_t*>(out);
+  uint32_t* hit = reinterpret_cast<uint32_t*>(out + (size_t)nslots * 8);
+  for (int i = threadIdx.x; i < nslots; i += blockDim.x) {
+    const int g = (i / ny) * NYEAR + first + i % ny;
+    profit[i] = s.agg[g];
+    hit[i] = s.agg_hit[g];
+  }
+  if (threadIdx.x < 4) hit[nslots + threadIdx.x] = s.counters[threadIdx.x];
+}
+
 // One queued row: nation, year and the first partsupp slot are loaded
 // together (none depends on another), then the rare further probe steps. pk
 // is the part's rank with the direct table, else its partkey.
+template <bool NIB>
 __device__ __forceinline__ void q9_entry(int32_t pk, int32_t sk, int32_t ok, int32_t qty,
                                          int64_t amt, uint32_t cap, const Q9Dims& dm,
                                          const Q9Scratch& s, int64_t* ltab, uint32_t* lhit) {
   const bool inr = (uint32_t)sk <= dm.maxsk && (uint32_t)ok <= dm.maxok;
   const int32_t nat = s.snat[inr ? (uint32_t)sk : 0u];
-  const uint8_t y = s.oyear[inr ? (uint32_t)ok : 0u];
+  const uint8_t y = year_at<NIB>(s.oyear, inr ? (uint32_t)ok : 0u);
   int64_t cost = -1;
   if (dm.direct) {  // the part passed the bitmap, so its rank's slots were set up
     const int4* row = reinterpret_cast<const int4*>(s.pst + (size_t)(uint32_t)pk * PS_WAYS);
@@ -536,7 +590,7 @@ struct Q9Queue {
 // table) queued rows are processed in place, 64 at a time, by q9_entry.
 constexpr int PROBE_THREADS = 512;
 constexpr int PROBE_U = 2;  // 4-row groups per lane and trip
-template <int U, bool DIRECT>
+template <int U, bool DIRECT, bool NIB>
 __global__ __launch_bounds__(PROBE_THREADS) void q9_probe(Q9Fact f, uint32_t n, Q9Dims dm,
                                                           Q9Scratch s) {
   extern __shared__ uint32_t dyn[];  // folded filter, then nslots sums and nslots hit flags
@@ -565,7 +619,7 @@ __global__ __launch_bounds__(PROBE_THREADS) void q9_probe(Q9Fact f, uint32_t n, 
       if (lane < avail) {
         const uint32_t i = (head + lane) & (QCAP - 1);
         const int4 e = q.e[i];
-        q9_entry(e.x, e.y, e.z, e.w, q.amt[i], cap, dm, s, ltab, lhit);
+        q9_entry<NIB>(e.x, e.y, e.z, e.w, q.amt[i], cap, dm, s, ltab, lhit);
       }
       head += avail;
     }
@@ -606,7 +660,7 @@ __global__ __launch_bounds__(PROBE_THREADS) void q9_probe(Q9Fact f, uint32_t n, 
     g_amt = q.amt[i];
     g_in = due && (uint32_t)g_sk <= dm.maxsk && (uint32_t)ok <= dm.maxok;
     g_nat = s.snat[g_in ? (uint32_t)g_sk : 0u];
-    g_y = s.oyear[g_in ? (uint32_t)ok : 0u];
+    g_y = year_at<NIB>(s.oyear, g_in ? (uint32_t)ok : 0u);
     const int4* row = reinterpret_cast<const int4*>(s.pst + (size_t)(uint32_t)key * PS_WAYS);
     g_ra = row[0];
     g_rb = row[1];
@@ -895,7 +949,10 @@ static int q9_run(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supplier
   // longer than the clear.)
   const bool sup_dense = ns && ssk->strict && (uint64_t)(ssk->maxv - ssk->minv) + 1 == ns;
   const uint32_t snat_clear = sup_dense ? (uint32_t)ssk->minv : nsupp;
-  const uint32_t oy_words = (nord + 3) / 4;
+  // 4-bit year codes for at most 15 years and unique (strictly ascending)
+  // orderkeys; else a byte per key value.
+  const bool nib = years.ny <= 15 && (no == 0 || ook->strict);
+  const uint32_t oy_words = nib ? (nord + 7) / 8 : (nord + 3) / 4;
   Q9Scratch s{};
   {
     char* b;
@@ -908,7 +965,7 @@ static int q9_run(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supplier
     // agg, agg_hit and counters are adjacent: the result is one readback.
     const size_t o_pbw = take(npw * 8ull), o_fold = take(fwords * 4ull),
                  o_rsum = take(rblocks * 4ull), o_snat = take(nsupp * 4ull),
-                 o_oy = take((nord + 3) / 4 * 4ull), o_agg = take(RESULT_BYTES),
+                 o_oy = take(oy_words * 4ull), o_agg = take(RESULT_BYTES),
                  o_lpk = take(nlist * 4ull), o_lsk = take(nlist * 4ull), o_lc = take(nlist * 8ull);
     GDB_TRY(arena(ctx, A_MAP, off, (void**)&b));
     s.pbw = (PartWord*)(b + o_pbw);
@@ -933,7 +990,8 @@ static int q9_run(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supplier
     }
   }
   char *host, *hdev;
-  GDB_TRY(result_buf(ctx, RESULT_BYTES, (void**)&host, (void**)&hdev));
+  const int nslots = nnat * years.ny;
+  GDB_TRY(result_buf(ctx, (size_t)nslots * 12 + 16, (void**)&host, (void**)&hdev));
   hipStream_t st = ctx->stream;
   const int B = 256;
   GDB_HIP(hipEventRecord(ctx->ev[0], st));
@@ -949,7 +1007,10 @@ static int q9_run(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supplier
     q9_part<<<pb, TILE, 0, st>>>(da);
   }
   if (ns) q9_supplier<<<(ns + B - 1) / B, B, 0, st>>>(da);
-  if (no) q9_orders<<<((no + 3) / 4 + B - 1) / B, B, 0, st>>>(da);
+  if (no) {
+    if (nib) q9_orders<true><<<((no + 3) / 4 + B - 1) / B, B, 0, st>>>(da);
+    else q9_orders<false><<<((no + 3) / 4 + B - 1) / B, B, 0, st>>>(da);
+  }
   q9_fold<<<(fwords + B - 1) / B, B, 0, st>>>(s.pbw, npw, fwords, s.pfold);
   q9_rank_sums<<<rblocks, B, 0, st>>>(s.pbw, npw, s.rsum);
   q9_rank_write<<<rblocks, B, 0, st>>>(s.pbw, npw, s.rsum, direct ? s.pst : nullptr, s.pcnt);
@@ -975,7 +1036,10 @@ static int q9_run(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supplier
   // One 4-row group per lane and trip: the pipelined probe holds two trips'
   // columns plus the dependent loads in flight in registers (116 VGPRs at
   // one group; two groups spill).
-  const void* kfn = direct ? (const void*)q9_probe<PROBE_U, true> : (const void*)q9_probe<PROBE_U, false>;
+  const void* kfn = direct ? (nib ? (const void*)q9_probe<PROBE_U, true, true>
+                                  : (const void*)q9_probe<PROBE_U, true, false>)
+                           : (nib ? (const void*)q9_probe<PROBE_U, false, true>
+                                  : (const void*)q9_probe<PROBE_U, false, false>);
   GDB_TRY(allow_dynamic_lds(ctx, kfn, tab_lds));
   const int PB = PROBE_THREADS;
   const int president = resident_blocks(ctx, kfn, PB, tab_lds, ctx->tune[T_Q9].blocks_per_cu);
@@ -983,8 +1047,10 @@ static int q9_run(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supplier
     const Q9Fact f{(const int32_t*)d[0], (const int32_t*)d[1], (const int32_t*)d[2],
                    (const int32_t*)d[3], (const int32_t*)d[4], (const int64_t*)d[5]};
     const int pblocks = std::max(1, std::min<int>(president, (int)((rows / 4 + PB - 1) / PB)));
-    if (direct) q9_probe<PROBE_U, true><<<pblocks, PB, tab_lds, st>>>(f, rows, dm, s);
-    else q9_probe<PROBE_U, false><<<pblocks, PB, tab_lds, st>>>(f, rows, dm, s);
+    if (direct && nib) q9_probe<PROBE_U, true, true><<<pblocks, PB, tab_lds, st>>>(f, rows, dm, s);
+    else if (direct) q9_probe<PROBE_U, true, false><<<pblocks, PB, tab_lds, st>>>(f, rows, dm, s);
+    else if (nib) q9_probe<PROBE_U, false, true><<<pblocks, PB, tab_lds, st>>>(f, rows, dm, s);
+    else q9_probe<PROBE_U, false, false><<<pblocks, PB, tab_lds, st>>>(f, rows, dm, s);
     GDB_HIP(hipGetLastError());
     return GDB_OK;
   };
@@ -996,22 +1062,21 @@ static int q9_run(gdb_ctx* ctx, const gdb_table* part, const gdb_table* supplier
     GDB_HIP(kernel_mark(ctx, 1, st));
     GDB_HIP(kernel_mark(ctx, 2, st));
   }
-  static_assert(RESULT_BYTES % 4 == 0, "result block is copied in 4-byte words");
-  publish_words<<<1, 256, 0, st>>>((const uint32_t*)s.agg, (uint32_t)(RESULT_BYTES / 4),
-                                   (uint32_t*)hdev);
+  q9_publish<<<1, 256, 0, st>>>(s, years.ny, years.first, nslots, hdev);
   GDB_HIP(hipGetLastError());
   GDB_HIP(hipEventRecord(ctx->ev[5], st));
   const uint64_t bytes = 28ull * nl + 59ull * np + 16ull * nps + 8ull * ns + 8ull * no;
   GDB_TRY(timing_finish(ctx, timing, bytes, 28ull * nl, &rs));
-  if (direct && ((const uint32_t*)(host + NNAT * NYEAR * 12))[1]) {
+  const int64_t* agg = (const int64_t*)host;
+  const uint32_t* hit = (const uint32_t*)(host + (size_t)nslots * 8);
+  if (direct && hit[nslots + 1]) {
     *redo = true;  // a part with more than PS_WAYS suppliers
     return GDB_OK;
   }
-  const int64_t* agg = (const int64_t*)host;
-  const uint32_t* hit = (const uint32_t*)(host + NNAT * NYEAR * 8);
   std::vector<gdb_q9_row> rows;
-  for (int i = 0; i < NNAT * NYEAR; ++i)
-    if (hit[i]) rows.push_back(gdb_q9_row{i / NYEAR, YEAR0 + i % NYEAR, agg[i]});
+  for (int i = 0; i < nslots; ++i)
+    if (hit[i])
+      rows.push_back(gdb_q9_row{i / years.ny, YEAR0 + years.first + i % years.ny, agg[i]});
   std::sort(rows.begin(), rows.end(), [](const gdb_q9_row& a, const gdb_q9_row& b) {
     const int c = strcmp(gdb_nation_name(a.nationkey), gdb_nation_name(b.nationkey));
     if (c != 0) return c < 0;

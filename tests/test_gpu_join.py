@@ -290,6 +290,57 @@ def test_q9_parts_with_more_than_four_suppliers(engine):
                 t.free()
 
 
+def test_q9_order_year_spans(engine):
+    """Q9's year map takes 4-bit codes for orders spanning at most 15 years with
+    strictly ascending orderkeys, else a byte per key value: 15 years (codes),
+    16 years and 1970-2033 (bytes; suppliers in 6 nations, so the 384 groups
+    fit the 400-row result), and 7 years with permuted keys (bytes), each with
+    both end years present, must equal the oracle."""
+    import datetime as dt
+
+    def day(y):
+        return (dt.date(y, 1, 1) - dt.date(1992, 1, 1)).days
+
+    rng = np.random.default_rng(33)
+    np_, ns, no = 2000, 30, 40_000
+    part = generate.part(7, np_)
+    ps = generate.partsupp(7, np_, ns)
+    su = generate.supplier(7, ns)
+    su6 = dict(su, s_nationkey=(np.asarray(su["s_nationkey"]) % 6).astype(np.int32))
+    sorted_keys = (np.arange(no, dtype=np.int64) // 8 * 32 + np.arange(no) % 8 + 1).astype(np.int32)
+    cases = {"15 years": (1992, 2007, sorted_keys), "16 years": (1992, 2008, sorted_keys),
+             "1970-2033": (1970, 2034, sorted_keys),
+             "7 years, permuted keys": (1992, 1999, rng.permutation(sorted_keys))}
+    pt, st_, st6, pst = (engine.table(t) for t in (part, su, su6, ps))
+    try:
+        for name, (y0, y1, ok) in cases.items():
+            sup, stab = (su6, st6) if y0 == 1970 else (su, st_)
+            odate = rng.integers(day(y0), day(y1), no).astype(np.int32)
+            odate[0], odate[-1] = day(y0), day(y1) - 1
+            od = dict(o_orderkey=ok, o_orderdate=odate)
+            owner = rng.integers(0, no, 5 * no)
+            m = len(owner)
+            row = rng.integers(0, len(ps["ps_partkey"]), m)
+            li = dict(l_partkey=np.asarray(ps["ps_partkey"])[row].astype(np.int32),
+                      l_suppkey=np.asarray(ps["ps_suppkey"])[row].astype(np.int32),
+                      l_orderkey=ok[owner].astype(np.int32),
+                      l_quantity=rng.integers(1, 51, m).astype(np.int32),
+                      l_discount=rng.integers(0, 11, m).astype(np.int32),
+                      l_extendedprice=rng.integers(90_000, 10_500_000, m).astype(np.int64))
+            ot, lt = engine.table(od), engine.table(li)
+            try:
+                for pat in ("green", "e"):
+                    want = O.q9(part, sup, ps, od, li, pat)
+                    assert {r["year"] for r in want} >= ({y0, y1 - 1} if pat == "e" else set())
+                    assert engine.q9(pt, stab, pst, ot, lt, pat) == want, (name, pat)
+            finally:
+                lt.free()
+                ot.free()
+    finally:
+        for t in (pt, st_, st6, pst):
+            t.free()
+
+
 def test_q9_lineitem_layouts(engine):
     """Q9 over clustered and shuffled lineitem, and with orphan orderkeys
     (negative, past the largest orderkey, in the gaps between sparse orders)
