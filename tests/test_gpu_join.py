@@ -294,8 +294,10 @@ def test_q9_order_year_spans(engine):
     """Q9's year map takes 4-bit codes for orders spanning at most 15 years with
     strictly ascending orderkeys, else a byte per key value: 15 years (codes),
     16 years and 1970-2033 (bytes; suppliers in 6 nations, so the 384 groups
-    fit the 400-row result), and 7 years with permuted keys (bytes), each with
-    both end years present, must equal the oracle."""
+    fit the 400-row result), 7 years with permuted keys (bytes) and 7 years
+    with dense keys 1..n (codes; every 4096-order slice of the map builder
+    then shares its first word with the slice before), each with both end
+    years present, must equal the oracle."""
     import datetime as dt
 
     def day(y):
@@ -310,7 +312,8 @@ def test_q9_order_year_spans(engine):
     sorted_keys = (np.arange(no, dtype=np.int64) // 8 * 32 + np.arange(no) % 8 + 1).astype(np.int32)
     cases = {"15 years": (1992, 2007, sorted_keys), "16 years": (1992, 2008, sorted_keys),
              "1970-2033": (1970, 2034, sorted_keys),
-             "7 years, permuted keys": (1992, 1999, rng.permutation(sorted_keys))}
+             "7 years, permuted keys": (1992, 1999, rng.permutation(sorted_keys)),
+             "7 years, dense keys": (1992, 1999, np.arange(1, no + 1, dtype=np.int32))}
     pt, st_, st6, pst = (engine.table(t) for t in (part, su, su6, ps))
     try:
         for name, (y0, y1, ok) in cases.items():
