@@ -293,9 +293,22 @@ __global__ __launch_bounds__(256) void q9_orders_owned(Q9DimArgs a, uint32_t nwo
   const uint32_t w0 = blockIdx.x == 0 ? 0u : (uint32_t)a.ok[o0] >> 3;
   const uint32_t w1 = o1 < a.no ? (uint32_t)a.ok[o1] >> 3 : nwords;
   const uint32_t nq = (o1 - o0 + 3) / 4;  // 4-order groups (the columns are padded)
-  for (uint32_t q = threadIdx.x; q < nq; q += 256) {
-    const int4 k = ld4nt(a.ok, o0 / 4 + q), d = ld4nt(a.od, o0 / 4 + q);
-    const int32_t ks[4] = {k.x, k.y, k.z, k.w}, ds[4] = {d.x, d.y, d.z, d.w};
+  // All of the thread's key and date loads are issued before any is used
+  // (clamped addresses; groups past the slice are masked afterwards).
+  constexpr uint32_t QPT = OY_SLICE / 4 / 256;
+  int4 kk[QPT], dd[QPT];
+#pragma unroll
+  for (uint32_t u = 0; u < QPT; ++u) {
+    const uint32_t q = min(threadIdx.x + u * 256, nq - 1);
+    kk[u] = ld4nt(a.ok, o0 / 4 + q);
+    dd[u] = ld4nt(a.od, o0 / 4 + q);
+  }
+#pragma unroll
+  for (uint32_t u = 0; u < QPT; ++u) {
+    const uint32_t q = threadIdx.x + u * 256;
+    if (q >= nq) continue;
+    const int32_t ks[4] = {kk[u].x, kk[u].y, kk[u].z, kk[u].w};
+    const int32_t ds[4] = {dd[u].x, dd[u].y, dd[u].z, dd[u].w};
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const uint32_t i = 4 * q + j;
