@@ -475,9 +475,9 @@ def run_default(args, info, O):
     # ---- config 1 (headline) and config 0 -------------------------------------
     wl = build_workloads(eng, ["q1"], "sf10", O)
     say("c1 q1 60M")
-    head = measure(wl["q1"], args.steps, args.warmup, args.tune,
-                   0.0 if args.no_cpu else args.cpu_seconds)
-    del wl
+    # CPU baselines run after the GPU timings of their phase: an all-core oracle
+    # run just before a timed GPU step slowed that step's host side.
+    head = measure(wl["q1"], args.steps, args.warmup, args.tune)
     say("c0 q6 6M")
     li6 = generate.lineitem_flat(42, FLAT_ROWS["sf1"])
     q6t = eng.table({k: li6[k] for k in ("l_shipdate", "l_discount", "l_quantity",
@@ -489,12 +489,16 @@ def run_default(args, info, O):
                   FLAT_ROWS["sf1"], lambda tm: eng.q6(q6t, timing=tm),
                   lambda: O.q6(li6, nthreads=th), 20 * FLAT_ROWS["sf1"])
     w6.eng = eng
-    c0 = measure(w6, side["steps"] * 2, side["warmup"], cpu_seconds=cpu_side,
-                 flush=lambda: eng.sum(flusher, "x"))
+    c0 = measure(w6, side["steps"] * 2, side["warmup"], flush=lambda: eng.sum(flusher, "x"))
     res = measure(w6, side["steps"] * 2, side["warmup"])
     c0["cache_resident"] = {"value": res["value"], "ms_per_step": res["ms_per_step"],
                             "kernel_ms_avg": res["roofline"]["kernel_ms_avg"],
                             "note": "back-to-back steps, columns left in the Infinity Cache"}
+    if not args.no_cpu:
+        say("c1/c0 cpu baselines")
+        head["cpu_baseline"] = cpu_baseline(wl["q1"], args.cpu_seconds)
+        c0["cpu_baseline"] = cpu_baseline(w6, cpu_side)
+    del wl
     configs["c0_q6_6m"] = c0
     flusher.free()
     q6t.free()
@@ -502,9 +506,14 @@ def run_default(args, info, O):
     # ---- configs 2 and 3 at the 60M-row scale --------------------------------
     say("c2/c3 tables")
     wl = build_workloads(eng, ["q3", "q9", "q13"], "sf10", O)
-    for key, name in (("c2_q3", "q3"), ("c3_q9", "q9"), ("c3_q13", "q13")):
+    c23 = (("c2_q3", "q3"), ("c3_q9", "q9"), ("c3_q13", "q13"))
+    for key, name in c23:
         say(f"{key}")
-        configs[key] = measure(wl[name], side["steps"], side["warmup"], args.tune, cpu_side)
+        configs[key] = measure(wl[name], side["steps"], side["warmup"], args.tune)
+    if cpu_side > 0:
+        say("c2/c3 cpu baselines")
+        for key, name in c23:
+            configs[key]["cpu_baseline"] = cpu_baseline(wl[name], cpu_side)
     del wl
     eng.close()
     # ---- config 3, the 600M-row streamed variant -----------------------------

@@ -628,19 +628,24 @@ extern "C" int gdb_q13(gdb_ctx* ctx, const gdb_table* customer, const gdb_table*
   GDB_HIP(hipGetLastError());
   GDB_HIP(hipEventRecord(ctx->ev[5], st));
   GDB_TRY(timing_finish(ctx, timing, 84ull * no + 4ull * nc, 84ull * no, &rs));
-  std::map<uint32_t, int64_t> h;
+  // (c_count, custdist) rows: the bins in c_count order, then, only when some
+  // customer counted NBINS or more orders, the overflow values merged in.
+  std::vector<gdb_q13_row> rows;
+  rows.reserve(64);
   for (int i = 0; i < NBINS; ++i)
-    if (host[i]) h[i] += host[i];
+    if (host[i]) rows.push_back(gdb_q13_row{i, 0, (int64_t)host[i]});
   const uint32_t nover = host[NBINS];
   if (nover) {
     std::vector<uint32_t> ov(nover);
     GDB_HIP(hipMemcpy(ov.data(), overflow, nover * 4ull, hipMemcpyDeviceToHost));
+    std::map<uint32_t, int64_t> h;
+    for (const gdb_q13_row& r : rows) h[(uint32_t)r.c_count] += r.custdist;
     for (uint32_t c : ov) h[c] += 1;
+    rows.clear();
+    for (auto& kv : h) rows.push_back(gdb_q13_row{(int32_t)kv.first, 0, kv.second});
   }
-  if (h.size() > sizeof(out->rows) / sizeof(out->rows[0]))
+  if (rows.size() > sizeof(out->rows) / sizeof(out->rows[0]))
     return fail(GDB_E_UNSUPPORTED, "more distinct c_count values than the result holds");
-  std::vector<gdb_q13_row> rows;
-  for (auto& kv : h) rows.push_back(gdb_q13_row{(int32_t)kv.first, 0, kv.second});
   std::sort(rows.begin(), rows.end(), [](const gdb_q13_row& a, const gdb_q13_row& b) {
     if (a.custdist != b.custdist) return a.custdist > b.custdist;
     return a.c_count > b.c_count;
