@@ -72,6 +72,33 @@ def test_sparse_orderkeys_64x_with_noise(engine, mid):
                                                  seg, date, k, nthreads=8)
 
 
+def test_orderkeys_share_slice_words(engine, mid):
+    """Orderkeys remapped by k -> k + R[k // 32], R a random non-decreasing
+    sequence (strictly increasing on every key): the order-key words are built
+    per 4096-order slice, and with the TPC-H layout (8 of every 32 key
+    values) every slice boundary falls on a word boundary; shifted this way
+    many slices share their first 32-key word with the slice before, whose
+    last orders must then land in the later slice's word. Checked against the
+    oracle and through the remapping."""
+    top = int(max(mid["orders"]["o_orderkey"].max(), mid["lineitem"]["l_orderkey"].max())) // 32 + 2
+    shift = np.cumsum(np.random.default_rng(7).integers(0, 32, top)).astype(np.int64)
+    f = lambda k: k + shift[k // 32]  # noqa: E731  order-preserving, unique
+    js = remap(mid, okey=f)
+    ok = js["orders"]["o_orderkey"].astype(np.int64)
+    assert np.all(np.diff(ok) > 0)
+    n = len(ok)
+    shared = sum(1 for b in range(4096, n, 4096) if (ok[b] - ok[0]) >> 5 == (ok[b - 1] - ok[0]) >> 5)
+    assert shared >= 5, shared
+    for seg, date, k in CASES:
+        base = run(engine, mid, seg, date, k)
+        got = run(engine, js, seg, date, k)
+        assert got["groups"] == base["groups"]
+        assert [r["orderkey"] for r in got["rows"]] == [int(f(np.int64(r["orderkey"])))
+                                                       for r in base["rows"]]
+        assert got == O.q3(js["customer"], js["orders"], js["lineitem"], seg, date, k,
+                           nthreads=8), (seg, date, k)
+
+
 @pytest.mark.parametrize("name,f", [
     ("near_int32_max", lambda k, m: 2147483647 - 61 * (m - k)),
     ("negative", lambda k, m: k - (1 << 30) - 7 * m),
