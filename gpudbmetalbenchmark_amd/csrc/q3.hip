@@ -655,31 +655,42 @@ __global__ __launch_bounds__(256) void q3_topk_collect(Q3Orders o, Q3Scratch s) 
   const uint32_t thr = s.counters[2];
   const uint32_t nq = (o.n + 3) / 4, nthr = gridDim.x * blockDim.x;
   const uint32_t lane = threadIdx.x & 63;
-  for (uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; gw < nq; gw += nthr) {
-    const uint32_t g = gw + lane;
-    const int4 k4 = g < nq ? ld4nt(o.okey, g) : make_int4(0, 0, 0, 0);
-    const int32_t ks[4] = {k4.x, k4.y, k4.z, k4.w};
-    uint32_t off[4];
-    bool valid[4], cand[4];
-    int64_t rev[4];
+  // Two groups per lane and trip (g and g + nthr): the two groups' chains of
+  // dependent loads (order-key word, hit flag, revenue) are in flight together.
+  for (uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; gw < nq; gw += 2 * nthr) {
+    uint32_t gs[2];
+    int32_t ks[2][4];
+    uint32_t off[2][4];
+    bool valid[2][4], cand[2][4];
+    int64_t rev[2][4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      off[j] = (uint32_t)((int64_t)ks[j] - o.olo);
-      valid[j] = 4 * g + j < o.n;
-    }
-    q3_candidates(o, off, valid, thr, s, cand, rev);
+    for (int u = 0; u < 2; ++u) {
+      gs[u] = gw + u * nthr + lane;
+      const int4 k4 = ld4nt(o.okey, min(gs[u], nq - 1));
+      ks[u][0] = k4.x; ks[u][1] = k4.y; ks[u][2] = k4.z; ks[u][3] = k4.w;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const unsigned long long m = __ballot(cand[j]);
-      if (!m) continue;
-      const int leader = __ffsll((long long)m) - 1;
-      uint32_t base = 0;
-      if ((int)lane == leader) base = atomicAdd(&s.counters[3], (uint32_t)__popcll(m));
-      base = (uint32_t)__shfl((int)base, leader, 64);
-      const uint32_t pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-      const uint32_t r = 4 * g + j;
-      if (cand[j] && pos < CAND_CAP) s.cand[pos] = TopEntry{rev[j], o.odate[r], ks[j], o.oprio[r], 0};
+      for (int j = 0; j < 4; ++j) {
+        off[u][j] = (uint32_t)((int64_t)ks[u][j] - o.olo);
+        valid[u][j] = gs[u] < nq && 4 * gs[u] + j < o.n;
+      }
     }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) q3_candidates(o, off[u], valid[u], thr, s, cand[u], rev[u]);
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const unsigned long long m = __ballot(cand[u][j]);
+        if (!m) continue;
+        const int leader = __ffsll((long long)m) - 1;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&s.counters[3], (uint32_t)__popcll(m));
+        base = (uint32_t)__shfl((int)base, leader, 64);
+        const uint32_t pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        const uint32_t r = 4 * gs[u] + j;
+        if (cand[u][j] && pos < CAND_CAP)
+          s.cand[pos] = TopEntry{rev[u][j], o.odate[r], ks[u][j], o.oprio[r], 0};
+      }
   }
 }
 
@@ -887,7 +898,7 @@ extern "C" int gdb_q3(gdb_ctx* ctx, const gdb_table* customer, const gdb_table* 
                    no, olo, olast, s.okw};
   q3_topk_hist<<<ctx->info.cu_count, B, 0, st>>>(s);  // one per CU: fewer bin merges
   q3_topk_threshold<<<1, B, 0, st>>>(k, s);
-  if (no) q3_topk_collect<<<((no + 3) / 4 + B - 1) / B, B, 0, st>>>(o, s);
+  if (no) q3_topk_collect<<<((no + 3) / 4 + 2 * B - 1) / (2 * B), B, 0, st>>>(o, s);
   q3_topk_rank<<<oblocks, B, 0, st>>>(k, o, s);
   GDB_HIP(hipGetLastError());
   GDB_HIP(hipEventRecord(ctx->ev[5], st));
