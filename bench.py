@@ -529,7 +529,7 @@ def run_default(args, info, O):
         flat, t = serve_tables(eng)
         r = serve_sweep(eng, flat, t, 0.6 if quick else 1.0,
                         [1000, 2000, 3000, 4000, 5000],
-                        (2, 4, 8, 12, 16, 20, 24))
+                        (2, 4, 8, 12, 14, 16, 18, 20, 22, 24))
         b, s = r["batched_best"], r["serial_best"]
         configs["c4_serving"] = {
             "workload": SERVE_WORKLOAD, "value": b["qps"] if b else 0.0, "unit": "req/s",
